@@ -1,0 +1,700 @@
+/*
+ * ORACLE -- test infrastructure only (see kpz_oracle.h for the pinning
+ * status of every function).  Never linked into the product.
+ *
+ * Plain C restatement of the reference's CPU hot path.  Reference paths are
+ * relative to the CudaKpz source tree.
+ */
+#include "kpz_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ======================================================================
+ * dSFMT-19937, restated from the published algorithm (Saito & Matsumoto)
+ * with the parameters of inc/dSFMT/dSFMT-params19937.h:6-16 and the
+ * generic recursion of inc/dSFMT/dSFMT.cpp:153-167 (non-SIMD variant),
+ * gen_rand_all :506-521, init_gen_rand :622-639, initial_mask :428-436,
+ * period_certification :442-476, genrand_close_open dSFMT.h:230-265.
+ * ====================================================================== */
+#define DS_N 191
+#define DS_N64 (DS_N * 2)
+#define DS_POS1 117
+#define DS_SL1 19
+#define DS_SR 12
+#define DS_MSK1 0x000ffafffffffb3full
+#define DS_MSK2 0x000ffdfffc90fffdull
+#define DS_FIX1 0x90014964b32f4329ull
+#define DS_FIX2 0x3b8d12ac548a7c7aull
+#define DS_PCV1 0x3d84e1ac0dc82880ull
+#define DS_PCV2 0x0000000000000001ull
+#define DS_LOW_MASK 0x000FFFFFFFFFFFFFull
+#define DS_HIGH_CONST 0x3FF0000000000000ull
+
+int kor_dsfmt_size(void) { return (int)sizeof(kor_dsfmt); }
+
+static void ds_rec(uint64_t* r, const uint64_t* a, const uint64_t* b, uint64_t* lung)
+{
+	const uint64_t t0 = a[0], t1 = a[1], L0 = lung[0], L1 = lung[1];
+	lung[0] = (t0 << DS_SL1) ^ (L1 >> 32) ^ (L1 << 32) ^ b[0];
+	lung[1] = (t1 << DS_SL1) ^ (L0 >> 32) ^ (L0 << 32) ^ b[1];
+	r[0] = (lung[0] >> DS_SR) ^ (lung[0] & DS_MSK1) ^ t0;
+	r[1] = (lung[1] >> DS_SR) ^ (lung[1] & DS_MSK2) ^ t1;
+}
+
+static void ds_gen_all(kor_dsfmt* d)
+{
+	uint64_t* s = d->st.u64;
+	uint64_t lung[2] = {s[2 * DS_N], s[2 * DS_N + 1]};
+	int i;
+	ds_rec(&s[0], &s[0], &s[2 * DS_POS1], lung);
+	for(i = 1; i < DS_N - DS_POS1; ++i)
+		ds_rec(&s[2 * i], &s[2 * i], &s[2 * (i + DS_POS1)], lung);
+	for(; i < DS_N; ++i)
+		ds_rec(&s[2 * i], &s[2 * i], &s[2 * (i + DS_POS1 - DS_N)], lung);
+	s[2 * DS_N] = lung[0];
+	s[2 * DS_N + 1] = lung[1];
+}
+
+void kor_dsfmt_init(kor_dsfmt* d, uint32_t seed)
+{
+	uint32_t* p = d->st.u32;
+	p[0] = seed;
+	for(int i = 1; i < (DS_N + 1) * 4; ++i)
+		p[i] = 1812433253u * (p[i - 1] ^ (p[i - 1] >> 30)) + (uint32_t)i;
+	for(int i = 0; i < DS_N * 2; ++i)
+		d->st.u64[i] = (d->st.u64[i] & DS_LOW_MASK) | DS_HIGH_CONST;
+	/* period certification */
+	uint64_t t0 = d->st.u64[2 * DS_N] ^ DS_FIX1;
+	uint64_t t1 = d->st.u64[2 * DS_N + 1] ^ DS_FIX2;
+	uint64_t inner = (t0 & DS_PCV1) ^ (t1 & DS_PCV2);
+	for(int i = 32; i > 0; i >>= 1)
+		inner ^= inner >> i;
+	if(!(inner & 1))
+		d->st.u64[2 * DS_N + 1] ^= 1; /* PCV2 has bit 0 set */
+	d->idx = DS_N64;
+}
+
+double kor_dsfmt_close_open(kor_dsfmt* d)
+{
+	if(d->idx >= DS_N64)
+	{
+		ds_gen_all(d);
+		d->idx = 0;
+	}
+	double r;
+	memcpy(&r, &d->st.u64[d->idx++], sizeof r);
+	return r - 1.0;
+}
+
+/* kpz/schedulerService.cpp:460-470 */
+void kor_reseed(kor_dsfmt* d, uint32_t seed)
+{
+	kor_dsfmt_init(d, seed);
+	for(int a = 0; a < 100000; ++a)
+		kor_dsfmt_close_open(d);
+}
+
+uint32_t kor_seed32(uint64_t seed) { return (uint32_t)(seed ^ (seed >> 32)); }
+
+/* Kmc::Shuffle<4>::shuffle(dsfmt_t*), inc/shuffle.h:76-88: N=4, L=2, one
+ * 32-bit draw covers all four RC4-key-schedule swaps. */
+void kor_shuffle4(kor_dsfmt* d, int seq[4])
+{
+	for(int a = 0; a < 4; ++a)
+		seq[a] = a;
+	unsigned int rnd = (unsigned int)(kor_dsfmt_close_open(d) * 4294967296.0);
+	for(int a = 0; a < 4; ++a)
+	{
+		const int i = rnd & 3;
+		const int v = seq[i];
+		seq[i] = seq[a];
+		seq[a] = v;
+		rnd >>= 2;
+	}
+}
+
+/* ======================================================================
+ * SLCG64, inc/kmcRandom.h:59-60 (A, C), :74-117 (skip form), kpz/kpzRandom.cu:
+ * 62-68 (output fold, randomFloat), kpz/GPURandom.cu:42-54 (stream seeding).
+ * ====================================================================== */
+
+/* skip form of the reference: A_S = A^S, C_S = C * sum_{k=1..S} A^k */
+void kor_slcg_skip(uint64_t S, uint64_t* a, uint64_t* c)
+{
+	uint64_t ak = 1, cs = 0;
+	for(uint64_t k = 0; k < S; ++k)
+	{
+		ak *= KOR_SLCG_A;
+		cs += ak * KOR_SLCG_C;
+	}
+	*a = ak;
+	*c = cs;
+}
+
+/* k-fold composition of x -> a*x + c, by squaring */
+void kor_slcg_jump(uint64_t a, uint64_t c, uint64_t k, uint64_t* ak, uint64_t* ck)
+{
+	uint64_t ra = 1, rc = 0; /* identity */
+	uint64_t ba = a, bc = c;
+	while(k)
+	{
+		if(k & 1)
+		{
+			rc = ba * rc + bc;
+			ra = ba * ra;
+		}
+		bc = ba * bc + bc;
+		ba = ba * ba;
+		k >>= 1;
+	}
+	*ak = ra;
+	*ck = rc;
+}
+
+/* (contract) skip length: the reference constant 30721 (kpz/kpzConst.h:85)
+ * whenever T < 30721 (kpz/GPURandom.cu:44 asserts exactly that), otherwise the
+ * smallest odd S > T, so the multiplier stays = 5 mod 8 and C_S stays odd. */
+uint64_t kor_default_skip(uint64_t T)
+{
+	if(T < 30721)
+		return 30721;
+	return T + 1 + (T & 1);
+}
+
+/* SLCG64::randomize, kpz/GPURandom.cu:42-54; (contract) the first master
+ * draw is the low half. */
+void kor_streams_randomize(kor_dsfmt* master, uint64_t* states, int64_t T)
+{
+	const double m1 = kor_dsfmt_close_open(master);
+	const double m2 = kor_dsfmt_close_open(master);
+	uint64_t s = (uint64_t)(m1 * 4294967295.0) ^ ((uint64_t)(m2 * 4294967295.0) << 32);
+	for(int64_t t = 0; t < T; ++t)
+	{
+		states[t] = s;
+		s = s * KOR_SLCG_A + KOR_SLCG_C;
+	}
+}
+
+uint32_t kor_slcg_draw(uint64_t* s, uint64_t a, uint64_t c)
+{
+	*s = a * *s + c;
+	return (uint32_t)(*s ^ (*s >> 32));
+}
+
+/* ======================================================================
+ * lattice encodings, kpz/systemSize.h:94-165, kpz/lattice.h:29-96
+ * ====================================================================== */
+static inline uint32_t loc_index(int lx, int x, int y) { return ((uint32_t)(y >> 2) << (lx - 2)) + (uint32_t)(x >> 2); }
+static inline int loc_shift(int x, int y) { return ((x & 3) << 1) | ((y & 3) << 3); }
+
+static inline uint64_t size_w(int lx, int ly) { return 1ull << (lx + ly - 4); }
+
+static inline uint64_t cb_index(int lx, int ly, int x, int y, int dir)
+{
+	const uint64_t sw = size_w(lx, ly);
+	const int sub = (x ^ y) & 1;
+	uint64_t w = ((uint64_t)y << (lx - 6)) + (uint64_t)(x >> 6);
+	if(sub)
+		w += sw >> 1;
+	if(dir)
+		w += sw >> 2;
+	return w;
+}
+
+int kor_get_slope(const uint32_t* lat, int enc, int lx, int ly, int x, int y, int dir)
+{
+	if(enc == 0)
+		return (lat[loc_index(lx, x, y)] >> (loc_shift(x, y) | dir)) & 1;
+	return (lat[cb_index(lx, ly, x, y, dir)] >> ((x >> 1) & 31)) & 1;
+}
+
+static inline void flip_slope(uint32_t* lat, int enc, int lx, int ly, int x, int y, int dir)
+{
+	if(enc == 0)
+		lat[loc_index(lx, x, y)] ^= 1u << (loc_shift(x, y) | dir);
+	else
+		lat[cb_index(lx, ly, x, y, dir)] ^= 1u << ((x >> 1) & 31);
+}
+
+/* SchedulerService::initHomogeneous, kpz/schedulerService.cpp:156-175 */
+void kor_init_flat_local(uint32_t* lat, int lx, int ly)
+{
+	const uint64_t sw = size_w(lx, ly);
+	for(uint64_t i = 0; i < sw; ++i)
+		lat[i] = 0x33CC33CCu;
+}
+
+void kor_init_flat_cbbit(uint32_t* lat, int lx, int ly)
+{
+	const uint64_t sw = size_w(lx, ly);
+	memset(lat, 0, (size_t)(sw << 1));
+	memset(lat + (sw >> 1), 0xFF, (size_t)(sw << 1));
+}
+
+void kor_local_to_cbbit(const uint32_t* in, uint32_t* out, int lx, int ly)
+{
+	memset(out, 0, (size_t)(size_w(lx, ly) * 4));
+	for(int y = 0; y < (1 << ly); ++y)
+		for(int x = 0; x < (1 << lx); ++x)
+			for(int d = 0; d < 2; ++d)
+				if(kor_get_slope(in, 0, lx, ly, x, y, d))
+					flip_slope(out, 1, lx, ly, x, y, d);
+}
+
+void kor_cbbit_to_local(const uint32_t* in, uint32_t* out, int lx, int ly)
+{
+	memset(out, 0, (size_t)(size_w(lx, ly) * 4));
+	for(int y = 0; y < (1 << ly); ++y)
+		for(int x = 0; x < (1 << lx); ++x)
+			for(int d = 0; d < 2; ++d)
+				if(kor_get_slope(in, 1, lx, ly, x, y, d))
+					flip_slope(out, 0, lx, ly, x, y, d);
+}
+
+/* ======================================================================
+ * heights and reductions, kpz/correlator.cpp:37-106 (Correlator::set height
+ * convention), :127-164 (correlate: slope overlap), :450-455 (ch, cs),
+ * kpz/roughness.cpp:38-43 (w2)
+ * ====================================================================== */
+void kor_heights(const uint32_t* lat, int enc, int lx, int ly, int32_t* h)
+{
+	const int X = 1 << lx, Y = 1 << ly;
+	int hh = 0;
+	for(int y = 0; y < Y; ++y)
+	{
+		hh += kor_get_slope(lat, enc, lx, ly, 0, y, 1) ? 1 : -1;
+		for(int x = 1; x < X; ++x)
+		{
+			hh += kor_get_slope(lat, enc, lx, ly, x, y, 0) ? 1 : -1;
+			h[(size_t)y * X + x] = hh;
+		}
+		hh += kor_get_slope(lat, enc, lx, ly, 0, y, 0) ? 1 : -1;
+		h[(size_t)y * X] = hh;
+	}
+}
+
+void kor_roughness(const uint32_t* lat, int enc, int lx, int ly, int64_t out[3])
+{
+	const size_t N = (size_t)1 << (lx + ly);
+	int32_t* h = (int32_t*)malloc(N * sizeof(int32_t));
+	kor_heights(lat, enc, lx, ly, h);
+	int64_t s1 = 0;
+	__int128 s2 = 0;
+	for(size_t i = 0; i < N; ++i)
+	{
+		s1 += h[i];
+		s2 += (int64_t)h[i] * h[i];
+	}
+	free(h);
+	out[0] = s1;
+	out[1] = (int64_t)(uint64_t)s2;
+	out[2] = (int64_t)(s2 >> 64);
+}
+
+void kor_correlate(const uint32_t* sys, const uint32_t* snap, int enc, int lx, int ly, int64_t out[7])
+{
+	const size_t N = (size_t)1 << (lx + ly);
+	int32_t* h = (int32_t*)malloc(N * sizeof(int32_t));
+	int32_t* g = (int32_t*)malloc(N * sizeof(int32_t));
+	kor_heights(sys, enc, lx, ly, h);
+	kor_heights(snap, enc, lx, ly, g);
+	int64_t s1 = 0, t1 = 0, tcs = 0;
+	__int128 s2 = 0, s11 = 0;
+	for(size_t i = 0; i < N; ++i)
+	{
+		s1 += h[i];
+		t1 += g[i];
+		s2 += (int64_t)h[i] * h[i];
+		s11 += (int64_t)h[i] * g[i];
+	}
+	for(int y = 0; y < (1 << ly); ++y)
+		for(int x = 0; x < (1 << lx); ++x)
+			for(int d = 0; d < 2; ++d)
+				tcs += (kor_get_slope(sys, enc, lx, ly, x, y, d) == kor_get_slope(snap, enc, lx, ly, x, y, d)) ? 1 : -1;
+	free(h);
+	free(g);
+	out[0] = s1;
+	out[1] = t1;
+	out[2] = (int64_t)(uint64_t)s2;
+	out[3] = (int64_t)(s2 >> 64);
+	out[4] = (int64_t)(uint64_t)s11;
+	out[5] = (int64_t)(s11 >> 64);
+	out[6] = tcs;
+}
+
+/* SchedulerService::consitencyCheck, kpz/schedulerService.cpp:721-854:
+ * row sums of x slopes, column sums of y slopes, plaquette closure */
+int kor_consistent(const uint32_t* lat, int enc, int lx, int ly)
+{
+	const int X = 1 << lx, Y = 1 << ly;
+	for(int y = 0; y < Y; ++y)
+	{
+		int s = 0;
+		for(int x = 0; x < X; ++x)
+			s += kor_get_slope(lat, enc, lx, ly, x, y, 0) ? 1 : -1;
+		if(s)
+			return 0;
+	}
+	for(int x = 0; x < X; ++x)
+	{
+		int s = 0;
+		for(int y = 0; y < Y; ++y)
+			s += kor_get_slope(lat, enc, lx, ly, x, y, 1) ? 1 : -1;
+		if(s)
+			return 0;
+	}
+	for(int y = 0; y < Y; ++y)
+		for(int x = 0; x < X; ++x)
+		{
+			const int x1 = (x + 1) & (X - 1), y1 = (y + 1) & (Y - 1);
+			/* h(x+1,y+1)-h(x,y) along right-then-up and up-then-right */
+			const int a = (kor_get_slope(lat, enc, lx, ly, x1, y, 0) ? 1 : -1) + (kor_get_slope(lat, enc, lx, ly, x1, y1, 1) ? 1 : -1);
+			const int b = (kor_get_slope(lat, enc, lx, ly, x, y1, 1) ? 1 : -1) + (kor_get_slope(lat, enc, lx, ly, x1, y1, 0) ? 1 : -1);
+			if(a != b)
+				return 0;
+		}
+	return 1;
+}
+
+/* ======================================================================
+ * update rules
+ * ====================================================================== */
+
+/* randomFloat() < p  <=>  (r & 0x7FFFFF) < ceil(p * 2^23), exact because
+ * randomFloat is r23 * 2^-23 exactly and the comparison is done in double
+ * (kpz/schedulerCPUBaseThreadedRng.inl:34-44). */
+static uint32_t thresh23(double p)
+{
+	if(!(p > 0.0))
+		return 0;
+	if(p >= 1.0)
+		return 1u << 23;
+	return (uint32_t)ceil(p * 8388608.0);
+}
+
+/* SchedulerCPU::pqOneUpdate, kpz/schedulerCPU.inl:21-33 (upd = KPZ) and
+ * Func_MCSBase::pqSyncRngUpdate, kpz/schedulerCPUBaseThreadedRng.inl:25-48
+ * (upd = PQ; r23 is the 23-bit randomFloat mantissa) */
+static inline void upd_local(uint32_t* lat, int lx, int ly, int x, int y, int upd, uint32_t r23, uint32_t tp, uint32_t tq)
+{
+	const uint32_t ic = loc_index(lx, x, y);
+	const int sc = loc_shift(x, y);
+	const int xr = (x + 1) & ((1 << lx) - 1);
+	const int yu = (y + 1) & ((1 << ly) - 1);
+	const uint32_t ir = loc_index(lx, xr, y);
+	const int sr = loc_shift(xr, y);
+	const uint32_t il = loc_index(lx, x, yu);
+	const int sl = loc_shift(x, yu) | 1;
+	const uint32_t c = (lat[ic] >> sc) & 3;
+	const uint32_t r = (lat[ir] >> sr) & 1;
+	const uint32_t l = (lat[il] >> sl) & 1;
+	if(upd == KOR_UPD_KPZ)
+	{
+		if(!(c == 0 && r && l))
+			return;
+	}
+	else
+	{
+		const uint32_t config = c | (r << 2) | (l << 3);
+		if(config == 0xC)
+		{
+			if(!(r23 < tp))
+				return;
+		}
+		else if(config == 0x3)
+		{
+			if(!(r23 < tq))
+				return;
+		}
+		else
+			return;
+	}
+	lat[ic] ^= 3u << sc;
+	lat[ir] ^= 1u << sr;
+	lat[il] ^= 1u << sl;
+}
+
+void kor_update_local(uint32_t* lat, int lx, int ly, int x, int y, int upd, float rnd, double p, double q)
+{
+	upd_local(lat, lx, ly, x, y, upd, (uint32_t)(rnd * 8388608.0f), thresh23(p), thresh23(q));
+}
+
+/* Func_MCSLocalBase::pqSyncRngUpdate on ENC_CBBIT,
+ * kpz/schedulerCPUCBBit_nested.inl:262-295 */
+static inline void upd_cbbit(uint32_t* lat, int lx, int ly, int x, int y, uint32_t r23, uint32_t tp, uint32_t tq)
+{
+	const int xr = (x + 1) & ((1 << lx) - 1);
+	const int yu = (y + 1) & ((1 << ly) - 1);
+	const uint32_t config = (uint32_t)kor_get_slope(lat, 1, lx, ly, x, y, 0)
+		| ((uint32_t)kor_get_slope(lat, 1, lx, ly, x, y, 1) << 1)
+		| ((uint32_t)kor_get_slope(lat, 1, lx, ly, xr, y, 0) << 2)
+		| ((uint32_t)kor_get_slope(lat, 1, lx, ly, x, yu, 1) << 3);
+	if(config == 0xC)
+	{
+		if(!(r23 < tp))
+			return;
+	}
+	else if(config == 0x3)
+	{
+		if(!(r23 < tq))
+			return;
+	}
+	else
+		return;
+	flip_slope(lat, 1, lx, ly, x, y, 0);
+	flip_slope(lat, 1, lx, ly, x, y, 1);
+	flip_slope(lat, 1, lx, ly, xr, y, 0);
+	flip_slope(lat, 1, lx, ly, x, yu, 1);
+}
+
+void kor_update_cbbit(uint32_t* lat, int lx, int ly, int x, int y, float rnd, double p, double q)
+{
+	upd_cbbit(lat, lx, ly, x, y, (uint32_t)(rnd * 8388608.0f), thresh23(p), thresh23(q));
+}
+
+/* ======================================================================
+ * double tiling: Kmc::DoubleTilingDD<2>::setBlock, inc/doubleTilingDD.h:71-78
+ * with the BlockDD layout of inc/blockDD.h:137-160
+ * ====================================================================== */
+void kor_dt_block_offset(int lx, int ly, int lbx, int lby, int domain_set,
+		const unsigned shift[2], int i, int off[2])
+{
+	const int lBlocks[2] = {lx - lbx, ly - lby};
+	const int lBlockDim[2] = {lbx, lby};
+	for(int a = 0; a < 2; ++a)
+	{
+		const int mBlocks = (1 << lBlocks[a]) - 1;
+		i <<= 1;
+		off[a] = (int)(shift[a] & ((1u << lBlockDim[a]) - 1))
+			+ (((i & mBlocks) + ((domain_set >> a) & 1)) << lBlockDim[a]);
+		i >>= lBlocks[a];
+	}
+}
+
+/* SchedulerCPUDT<Rng>::mcs, kpz/schedulerCPUDT.cpp:65-90 with
+ * Func_MCSBase::exec, kpz/schedulerCPUDT_nested.h:46-66 (random_origin = 0),
+ * and SchedulerCPUDTr<Rng>::doMcs / Func_MCS::exec,
+ * kpz/schedulerCPUDTr.cpp:66-98, :147-175 (random_origin = 1).
+ * Rng = SLCG64 stream t with close_open() = draw * 2^-32 (contract), so
+ * (int)(close_open()*2^b) = draw >> (32-b); randomFloat()*2^b = r23 >> (23-b).
+ * Blocks of one domain set are disjoint including their +1 halo, so the
+ * sequential walk t = 0..T-1 equals any thread interleaving. */
+int kor_dt_mcs(uint32_t* lat, int lx, int ly, int lbx, int lby,
+		uint64_t* states, int64_t T, uint64_t a, uint64_t c,
+		kor_dsfmt* master, int n, int random_origin, int upd, double p, double q)
+{
+	if(lbx < 2 || lby < 2 || lx - lbx < 1 || ly - lby < 1 || T < 1 || lbx > 23 || lby > 23)
+		return -1;
+	const int64_t nb = 1ll << (lx - lbx + ly - lby - 2);
+	const int64_t bs = 1ll << (lbx + lby);
+	const int mx = (1 << lx) - 1, my = (1 << ly) - 1;
+	const uint32_t tp = thresh23(p), tq = thresh23(q);
+	for(int m = 0; m < n; ++m)
+	{
+		int seq[4];
+		kor_shuffle4(master, seq);
+		unsigned shift[2] = {0, 0};
+		if(random_origin)
+		{
+			shift[0] = (unsigned)(kor_dsfmt_close_open(master) * (double)(1 << lbx));
+			shift[1] = (unsigned)(kor_dsfmt_close_open(master) * (double)(1 << lby));
+		}
+		for(int d = 0; d < 4; ++d)
+		{
+			for(int64_t t = 0; t < T; ++t)
+			{
+				uint64_t s = states[t];
+				for(int64_t i = t; i < nb; i += T)
+				{
+					int off[2];
+					kor_dt_block_offset(lx, ly, lbx, lby, seq[d], shift, (int)i, off);
+					for(int64_t k = 0; k < bs; ++k)
+					{
+						int x, y;
+						if(random_origin)
+						{
+							x = ((int)((kor_slcg_draw(&s, a, c) & 0x7FFFFF) >> (23 - lbx)) + off[0]) & mx;
+							y = ((int)((kor_slcg_draw(&s, a, c) & 0x7FFFFF) >> (23 - lby)) + off[1]) & my;
+						}
+						else
+						{
+							x = (int)(kor_slcg_draw(&s, a, c) >> (32 - lbx)) + off[0];
+							y = (int)(kor_slcg_draw(&s, a, c) >> (32 - lby)) + off[1];
+						}
+						uint32_t r23 = 0;
+						if(upd == KOR_UPD_PQ)
+							r23 = kor_slcg_draw(&s, a, c) & 0x7FFFFF;
+						upd_local(lat, lx, ly, x, y, upd, r23, tp, tq);
+					}
+				}
+				states[t] = s;
+			}
+		}
+	}
+	return 0;
+}
+
+/* ======================================================================
+ * checkerboard SCA on ENC_CBBIT: SchedulerCPUCBBit<Rng>::doMcs,
+ * kpz/schedulerCPUCBBit.cpp:43-79, Func_MCS::exec
+ * kpz/schedulerCPUCBBit_nested.h:80-95 (rows [yMin, yCenter) then
+ * [yCenter, ySup), i.e. ascending), pSyncRngUpdateVL3
+ * kpz/schedulerCPUCBBit_nested.inl:200-259, pqSyncRngUpdate :262-295.
+ * ====================================================================== */
+static inline uint64_t ld64(const uint32_t* p, uint64_t i)
+{
+	uint64_t v;
+	memcpy(&v, p + 2 * i, 8);
+	return v;
+}
+static inline void st64(uint32_t* p, uint64_t i, uint64_t v) { memcpy(p + 2 * i, &v, 8); }
+
+/* one row of the bit-parallel half-sweep.  wbits = 64 (VL3: two draws per
+ * word, first draw is the high half -- contract for the unsequenced
+ * (r()<<32)^r() of :240) or 32 (L = 64 only: one draw per u32 word --
+ * contract; the reference's 64-bit loop does nothing when L < 128). */
+static void cbbit_row_words(uint32_t* lat, int lx, int ly, int s, int y, int wbits,
+		uint64_t* st, uint64_t a, uint64_t c, const uint64_t* masks)
+{
+	const int lw = (wbits == 64) ? 7 : 6; /* log2 sites per word pair */
+	const uint64_t Q = size_w(lx, ly) >> (wbits == 64 ? 3 : 2);
+	const uint64_t W = (uint64_t)1 << (lx - lw);
+	const uint64_t MSB = (wbits == 64) ? (1ull << 63) : (1ull << 31);
+	const uint64_t FULL = (wbits == 64) ? ~0ull : 0xFFFFFFFFull;
+	const int parity = (s == (y & 1));
+	const uint64_t thisBase = s ? (Q << 1) : 0;
+	const uint64_t othBase = s ? 0 : (Q << 1);
+#define LD(i) ((wbits == 64) ? ld64(lat, (i)) : (uint64_t)lat[(i)])
+#define ST(i, v) do { if(wbits == 64) st64(lat, (i), (v)); else lat[(i)] = (uint32_t)(v); } while(0)
+	for(uint64_t xw = 0; xw < W; ++xw)
+	{
+		const uint64_t off = xw + (uint64_t)y * W;
+		const uint64_t tXi = thisBase + off, tYi = thisBase + Q + off;
+		const uint64_t tX = LD(tXi), tY = LD(tYi);
+		const uint64_t yIdx = othBase + Q + xw + (uint64_t)((y + 1) & ((1 << ly) - 1)) * W;
+		const uint64_t yY = LD(yIdx);
+		const uint64_t xIdx = othBase + off;
+		uint64_t xX = LD(xIdx);
+		uint64_t nIdx = xIdx;
+		if(!parity)
+		{
+			nIdx = (xw + 1 == W) ? xIdx - xw : xIdx + 1;
+			xX = ((xX >> 1) & ~MSB) | ((LD(nIdx) << (wbits - 1)) & MSB);
+		}
+		uint64_t m = (~(tX | tY)) & xX & yY & FULL;
+		uint64_t rnd;
+		if(masks)
+			rnd = masks[off];
+		else if(wbits == 64)
+		{
+			const uint64_t hi = kor_slcg_draw(st, a, c);
+			const uint64_t lo = kor_slcg_draw(st, a, c);
+			rnd = (hi << 32) ^ lo;
+		}
+		else
+			rnd = kor_slcg_draw(st, a, c);
+		m &= rnd;
+		uint64_t xm = m;
+		if(!parity)
+		{
+			ST(nIdx, LD(nIdx) ^ ((m >> (wbits - 1)) & 1));
+			xm = (m << 1) & FULL;
+		}
+		ST(xIdx, LD(xIdx) ^ xm);
+		ST(tXi, tX ^ m);
+		ST(tYi, tY ^ m);
+		ST(yIdx, yY ^ m);
+	}
+#undef LD
+#undef ST
+}
+
+void kor_cbbit_halfsweep_masked(uint32_t* lat, int lx, int ly, int signum, const uint64_t* masks)
+{
+	const int wbits = (lx >= 7) ? 64 : 32;
+	for(int y = 0; y < (1 << ly); ++y)
+		cbbit_row_words(lat, lx, ly, signum, y, wbits, NULL, 0, 0, masks);
+}
+
+static void cbbit_rows(int L, int64_t T, int64_t t, int* yMin, int* ySup)
+{
+	const int64_t per = L / T, rem = L % T;
+	const int64_t lo = t * per + (t < rem ? t : rem);
+	*yMin = (int)lo;
+	*ySup = (int)(lo + per + (t < rem ? 1 : 0));
+}
+
+int kor_cbbit_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
+		uint64_t a, uint64_t c, int n, int mode, double p, double q)
+{
+	if(lx < 6 || ly < 1 || T < 1)
+		return -1;
+	const int X = 1 << lx, Y = 1 << ly;
+	const uint32_t tp = thresh23(p), tq = thresh23(q);
+	const int wbits = (lx >= 7) ? 64 : 32;
+	for(int m = 0; m < n; ++m)
+		for(int s = 0; s < 2; ++s)
+			for(int64_t t = 0; t < T; ++t)
+			{
+				int y0, y1;
+				cbbit_rows(Y, T, t, &y0, &y1);
+				uint64_t st = states[t];
+				for(int y = y0; y < y1; ++y)
+				{
+					if(mode == 0)
+						cbbit_row_words(lat, lx, ly, s, y, wbits, &st, a, c, NULL);
+					else
+						for(int x = ((y & 1) ^ s); x < X; x += 2)
+							upd_cbbit(lat, lx, ly, x, y, kor_slcg_draw(&st, a, c) & 0x7FFFFF, tp, tq);
+				}
+				states[t] = st;
+			}
+	return 0;
+}
+
+/* ======================================================================
+ * checkerboard SCA on ENC_LOCAL: SchedulerCPUCB<Rng>::doMcs,
+ * kpz/schedulerCPUCB.cpp:43-80, Func_MCS::updateLoop :128-142 and exec
+ * :153-168 (strip split at xCenter, first half entirely before second).
+ * ====================================================================== */
+static void cb_loop(uint32_t* lat, int lx, int ly, int s, int xMin, int xSup,
+		uint64_t* st, uint64_t a, uint64_t c, uint32_t tp, uint32_t tq)
+{
+	for(int yW = 0; yW < (1 << ly); yW += 4)
+		for(int xW = xMin; xW < xSup; xW += 4)
+			for(int y = yW; y < yW + 4; ++y)
+			{
+				const int x = (xW + (y & 1)) ^ s;
+				upd_local(lat, lx, ly, x, y, KOR_UPD_PQ, kor_slcg_draw(st, a, c) & 0x7FFFFF, tp, tq);
+				upd_local(lat, lx, ly, x + 2, y, KOR_UPD_PQ, kor_slcg_draw(st, a, c) & 0x7FFFFF, tp, tq);
+			}
+}
+
+int kor_cb_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
+		uint64_t a, uint64_t c, int n, double p, double q)
+{
+	if(lx < 2 || ly < 2 || T < 1)
+		return -1;
+	const int dimXW = 1 << (lx - 2);
+	const int per = (int)(dimXW / T) << 2;
+	const int rem = (int)(dimXW % T);
+	const uint32_t tp = thresh23(p), tq = thresh23(q);
+	for(int m = 0; m < n; ++m)
+		for(int s = 0; s < 2; ++s)
+		{
+			int xSup = 0;
+			for(int64_t t = 0; t < T; ++t)
+			{
+				const int xMin = xSup;
+				xSup += per + (t < rem ? 4 : 0);
+				const int xCenter = xMin + (((xSup - xMin) / 2) & ~3);
+				uint64_t st = states[t];
+				cb_loop(lat, lx, ly, s, xMin, xCenter, &st, a, c, tp, tq);
+				cb_loop(lat, lx, ly, s, xCenter, xSup, &st, a, c, tp, tq);
+				states[t] = st;
+			}
+		}
+	return 0;
+}

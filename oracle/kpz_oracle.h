@@ -1,0 +1,109 @@
+/*
+ * ORACLE -- test infrastructure only.
+ *
+ * Plain-C restatement of the CPU counterparts of the CudaKpz hot path, used
+ * as the checker for the HIP engine.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load this library; the product
+ * (cudakpz_amd) never links or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Where the
+ * reference is ambiguous the contract choice is marked "(contract)".
+ *
+ * Pinning status (how each function is checked; see tests/test_oracle_*.py):
+ *   kor_dsfmt_*          golden vectors from the reference's own dSFMT.cpp
+ *                        compiled into oracle/_ref (tests/golden/dsfmt.json)
+ *   kor_shuffle4         golden vectors from the reference's Kmc::Shuffle<4>
+ *   kor_dt_block_offset  golden vectors from the reference's
+ *                        Kmc::DoubleTilingDD<2>::setBlock
+ *   kor_slcg_skip        known-answer constants of inc/helper/slcgInst.h and
+ *                        the reference's own SLCGskipA/SLCGskipC
+ *   kor_slcg_jump        closed form vs. step-by-step iteration
+ *   update rules         brute-force explicit-height octahedron simulator
+ *   half-sweeps          per-site scalar equivalence, order independence
+ *   heights/roughness    closed forms (flat IC), brute force on tiny lattices
+ *   DT/DTr/CBBit/CB MCS  composition of the pinned pieces above plus
+ *                        invariants (consistency, height-change accounting)
+ */
+#ifndef KPZ_ORACLE_H
+#define KPZ_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- master RNG: dSFMT-19937 ---------------- */
+#define KOR_DSFMT_N 191
+typedef struct kor_dsfmt {
+	union {
+		uint64_t u64[(KOR_DSFMT_N + 1) * 2];
+		uint32_t u32[(KOR_DSFMT_N + 1) * 4];
+	} st;
+	int idx;
+} kor_dsfmt;
+
+void kor_dsfmt_init(kor_dsfmt* d, uint32_t seed);
+double kor_dsfmt_close_open(kor_dsfmt* d);
+/* SchedulerService::reSeed: init + 100000 discarded draws */
+void kor_reseed(kor_dsfmt* d, uint32_t seed);
+/* (contract) 64-bit seed -> 32-bit dSFMT seed: lo ^ hi */
+uint32_t kor_seed32(uint64_t seed);
+void kor_shuffle4(kor_dsfmt* d, int seq[4]);
+int kor_dsfmt_size(void);
+
+/* ---------------- SLCG64 ---------------- */
+#define KOR_SLCG_A 2862933555777941757ull
+#define KOR_SLCG_C 1442695040888963407ull
+void kor_slcg_skip(uint64_t S, uint64_t* a, uint64_t* c);
+void kor_slcg_jump(uint64_t a, uint64_t c, uint64_t k, uint64_t* ak, uint64_t* ck);
+uint64_t kor_default_skip(uint64_t T);
+void kor_streams_randomize(kor_dsfmt* master, uint64_t* states, int64_t T);
+uint32_t kor_slcg_draw(uint64_t* s, uint64_t a, uint64_t c);
+
+/* ---------------- lattice ---------------- */
+void kor_init_flat_local(uint32_t* lat, int lx, int ly);
+void kor_init_flat_cbbit(uint32_t* lat, int lx, int ly);
+void kor_local_to_cbbit(const uint32_t* in, uint32_t* out, int lx, int ly);
+void kor_cbbit_to_local(const uint32_t* in, uint32_t* out, int lx, int ly);
+int kor_get_slope(const uint32_t* lat, int enc, int lx, int ly, int x, int y, int dir);
+
+/* heights following Correlator::set; h has 2^(lx+ly) entries, row-major */
+void kor_heights(const uint32_t* lat, int enc, int lx, int ly, int32_t* h);
+/* exact sums; out = {sum h, sum h^2 (lo), sum h^2 (hi)} as int64 words */
+void kor_roughness(const uint32_t* lat, int enc, int lx, int ly, int64_t out[3]);
+/* out = {sum h_sys, sum h_snap, sum h_sys^2 lo, hi, sum h_sys*h_snap lo, hi, tcs} */
+void kor_correlate(const uint32_t* sys, const uint32_t* snap, int enc, int lx, int ly, int64_t out[7]);
+/* 1 if every row/column slope sum is zero and every plaquette closes */
+int kor_consistent(const uint32_t* lat, int enc, int lx, int ly);
+
+/* ---------------- schedulers ---------------- */
+/* update kinds */
+#define KOR_UPD_KPZ 0   /* pqOneUpdate: p=1, q=0, two draws per update */
+#define KOR_UPD_PQ  1   /* pqSyncRngUpdate: one extra randomFloat per update */
+
+/* SchedulerCPUDT (origin_shift=0) and SchedulerCPUDTr (origin_shift=1) */
+int kor_dt_mcs(uint32_t* lat, int lx, int ly, int lbx, int lby,
+		uint64_t* states, int64_t T, uint64_t a, uint64_t c,
+		kor_dsfmt* master, int n, int random_origin, int upd, double p, double q);
+void kor_dt_block_offset(int lx, int ly, int lbx, int lby, int domain_set,
+		const unsigned shift[2], int i, int off[2]);
+
+/* SchedulerCPUCBBit: mode 0 = pSyncRngUpdateVL3 (p=.5, q=0), 1 = per-site pq */
+int kor_cbbit_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
+		uint64_t a, uint64_t c, int n, int mode, double p, double q);
+/* one half-sweep with an explicit per-site mask source, for equivalence tests:
+ * mask bits are taken from `masks` (one u64 per u64 word, row-major) */
+void kor_cbbit_halfsweep_masked(uint32_t* lat, int lx, int ly, int signum, const uint64_t* masks);
+/* SchedulerCPUCB (ENC_LOCAL checkerboard, per-site pq) */
+int kor_cb_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
+		uint64_t a, uint64_t c, int n, double p, double q);
+
+/* single updates (for brute-force checks) */
+void kor_update_local(uint32_t* lat, int lx, int ly, int x, int y, int upd, float rnd, double p, double q);
+void kor_update_cbbit(uint32_t* lat, int lx, int ly, int x, int y, float rnd, double p, double q);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,244 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes wrapper around liboracle_kpz.so, the plain-C restatement of the
+CudaKpz CPU hot path (kpz_oracle.c).  Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg may import this module; the product package
+cudakpz_amd never does.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+ENC_LOCAL, ENC_CBBIT = 0, 1
+UPD_KPZ, UPD_PQ = 0, 1
+
+
+def build(ref=False):
+    cmd = ["make", "-s", "-C", _HERE, "liboracle_kpz.so"]
+    subprocess.check_call(cmd)
+    if ref:
+        subprocess.check_call(["make", "-s", "-C", _HERE, "ref"])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle_kpz.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = C.CDLL(path)
+        _declare(_LIB)
+    return _LIB
+
+
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+
+def _declare(L):
+    vp = C.c_void_p
+    L.kor_dsfmt_size.restype = C.c_int
+    L.kor_dsfmt_init.argtypes = [vp, C.c_uint32]
+    L.kor_dsfmt_close_open.argtypes = [vp]
+    L.kor_dsfmt_close_open.restype = C.c_double
+    L.kor_reseed.argtypes = [vp, C.c_uint32]
+    L.kor_seed32.argtypes = [C.c_uint64]
+    L.kor_seed32.restype = C.c_uint32
+    L.kor_shuffle4.argtypes = [vp, C.POINTER(C.c_int)]
+    L.kor_slcg_skip.argtypes = [C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.kor_slcg_jump.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.kor_default_skip.argtypes = [C.c_uint64]
+    L.kor_default_skip.restype = C.c_uint64
+    L.kor_streams_randomize.argtypes = [vp, _u64p, C.c_int64]
+    L.kor_slcg_draw.argtypes = [C.POINTER(C.c_uint64), C.c_uint64, C.c_uint64]
+    L.kor_slcg_draw.restype = C.c_uint32
+    for f in ("kor_init_flat_local", "kor_init_flat_cbbit"):
+        getattr(L, f).argtypes = [_u32p, C.c_int, C.c_int]
+    for f in ("kor_local_to_cbbit", "kor_cbbit_to_local"):
+        getattr(L, f).argtypes = [_u32p, _u32p, C.c_int, C.c_int]
+    L.kor_get_slope.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.kor_heights.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, _i32p]
+    L.kor_roughness.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, _i64p]
+    L.kor_correlate.argtypes = [_u32p, _u32p, C.c_int, C.c_int, C.c_int, _i64p]
+    L.kor_consistent.argtypes = [_u32p, C.c_int, C.c_int, C.c_int]
+    L.kor_dt_mcs.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, _u64p, C.c_int64,
+                             C.c_uint64, C.c_uint64, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
+    L.kor_dt_block_offset.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(C.c_uint), C.c_int, C.POINTER(C.c_int)]
+    L.kor_cbbit_mcs.argtypes = [_u32p, C.c_int, C.c_int, _u64p, C.c_int64, C.c_uint64, C.c_uint64,
+                                C.c_int, C.c_int, C.c_double, C.c_double]
+    L.kor_cbbit_halfsweep_masked.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, _u64p]
+    L.kor_cb_mcs.argtypes = [_u32p, C.c_int, C.c_int, _u64p, C.c_int64, C.c_uint64, C.c_uint64,
+                             C.c_int, C.c_double, C.c_double]
+    L.kor_update_local.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                                   C.c_double, C.c_double]
+    L.kor_update_cbbit.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                                   C.c_double, C.c_double]
+
+
+class Dsfmt:
+    """dSFMT-19937 master generator (restated)."""
+
+    def __init__(self, seed=None, reseed=True):
+        self.buf = C.create_string_buffer(lib().kor_dsfmt_size())
+        if seed is not None:
+            if reseed:
+                lib().kor_reseed(self.buf, seed & 0xFFFFFFFF)
+            else:
+                lib().kor_dsfmt_init(self.buf, seed & 0xFFFFFFFF)
+
+    @property
+    def ptr(self):
+        return C.cast(self.buf, C.c_void_p)
+
+    def close_open(self):
+        return lib().kor_dsfmt_close_open(self.buf)
+
+    def shuffle4(self):
+        out = (C.c_int * 4)()
+        lib().kor_shuffle4(self.buf, out)
+        return list(out)
+
+    def copy(self):
+        d = Dsfmt()
+        C.memmove(d.buf, self.buf, len(self.buf))
+        return d
+
+
+def seed32(seed):
+    return lib().kor_seed32(seed)
+
+
+def slcg_skip(S):
+    a, c = C.c_uint64(), C.c_uint64()
+    lib().kor_slcg_skip(S, C.byref(a), C.byref(c))
+    return a.value, c.value
+
+
+def slcg_jump(a, c, k):
+    ak, ck = C.c_uint64(), C.c_uint64()
+    lib().kor_slcg_jump(a, c, k, C.byref(ak), C.byref(ck))
+    return ak.value, ck.value
+
+
+def default_skip(T):
+    return lib().kor_default_skip(T)
+
+
+class Streams:
+    """T SLCG64 streams with the reference skip form."""
+
+    def __init__(self, T, master, S=None):
+        self.T = int(T)
+        self.S = default_skip(self.T) if S is None else int(S)
+        self.a, self.c = slcg_skip(self.S)
+        self.states = np.zeros(self.T, np.uint64)
+        lib().kor_streams_randomize(master.ptr, self.states, self.T)
+
+    def draw(self, t):
+        s = C.c_uint64(int(self.states[t]))
+        r = lib().kor_slcg_draw(C.byref(s), self.a, self.c)
+        self.states[t] = s.value
+        return r
+
+
+def size_words(lx, ly):
+    return 1 << (lx + ly - 4)
+
+
+def flat(lx, ly, enc):
+    lat = np.zeros(size_words(lx, ly), np.uint32)
+    if enc == ENC_LOCAL:
+        lib().kor_init_flat_local(lat, lx, ly)
+    else:
+        lib().kor_init_flat_cbbit(lat, lx, ly)
+    return lat
+
+
+def local_to_cbbit(lat, lx, ly):
+    out = np.zeros_like(lat)
+    lib().kor_local_to_cbbit(np.ascontiguousarray(lat), out, lx, ly)
+    return out
+
+
+def cbbit_to_local(lat, lx, ly):
+    out = np.zeros_like(lat)
+    lib().kor_cbbit_to_local(np.ascontiguousarray(lat), out, lx, ly)
+    return out
+
+
+def slope(lat, enc, lx, ly, x, y, d):
+    return lib().kor_get_slope(lat, enc, lx, ly, x, y, d)
+
+
+def heights(lat, enc, lx, ly):
+    h = np.zeros((1 << ly, 1 << lx), np.int32)
+    lib().kor_heights(np.ascontiguousarray(lat), enc, lx, ly, h.reshape(-1))
+    return h
+
+
+def _i128(lo, hi):
+    return (int(hi) << 64) | (int(lo) & 0xFFFFFFFFFFFFFFFF)
+
+
+def roughness(lat, enc, lx, ly):
+    """(sum h, sum h^2) as exact Python ints."""
+    out = np.zeros(3, np.int64)
+    lib().kor_roughness(np.ascontiguousarray(lat), enc, lx, ly, out)
+    return int(out[0]), _i128(out[1], out[2])
+
+
+def correlate(sys, snap, enc, lx, ly):
+    """(sum h, sum h_snap, sum h^2, sum h*h_snap, tcs) as exact ints."""
+    out = np.zeros(7, np.int64)
+    lib().kor_correlate(np.ascontiguousarray(sys), np.ascontiguousarray(snap), enc, lx, ly, out)
+    return int(out[0]), int(out[1]), _i128(out[2], out[3]), _i128(out[4], out[5]), int(out[6])
+
+
+def consistent(lat, enc, lx, ly):
+    return bool(lib().kor_consistent(np.ascontiguousarray(lat), enc, lx, ly))
+
+
+def dt_block_offset(lx, ly, lbx, lby, ds, shift, i):
+    sh = (C.c_uint * 2)(*shift)
+    off = (C.c_int * 2)()
+    lib().kor_dt_block_offset(lx, ly, lbx, lby, ds, sh, i, off)
+    return off[0], off[1]
+
+
+def dt_mcs(lat, lx, ly, lbx, lby, streams, master, n, random_origin=False, upd=UPD_KPZ, p=1.0, q=0.0):
+    rc = lib().kor_dt_mcs(lat, lx, ly, lbx, lby, streams.states, streams.T, streams.a, streams.c,
+                          master.ptr, n, int(random_origin), upd, p, q)
+    if rc:
+        raise ValueError("invalid DT layout")
+
+
+def cbbit_mcs(lat, lx, ly, streams, n, mode, p=0.5, q=0.0):
+    rc = lib().kor_cbbit_mcs(lat, lx, ly, streams.states, streams.T, streams.a, streams.c, n, mode, p, q)
+    if rc:
+        raise ValueError("invalid CBBit size")
+
+
+def cbbit_halfsweep_masked(lat, lx, ly, signum, masks):
+    lib().kor_cbbit_halfsweep_masked(lat, lx, ly, signum, np.ascontiguousarray(masks, np.uint64))
+
+
+def cb_mcs(lat, lx, ly, streams, n, p=0.95, q=0.0):
+    rc = lib().kor_cb_mcs(lat, lx, ly, streams.states, streams.T, streams.a, streams.c, n, p, q)
+    if rc:
+        raise ValueError("invalid CB size")
+
+
+def update_local(lat, lx, ly, x, y, upd, rnd, p, q):
+    lib().kor_update_local(lat, lx, ly, x, y, upd, rnd, p, q)
+
+
+def update_cbbit(lat, lx, ly, x, y, rnd, p, q):
+    lib().kor_update_cbbit(lat, lx, ly, x, y, rnd, p, q)

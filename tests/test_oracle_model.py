@@ -1,0 +1,328 @@
+"""The oracle's lattice, update rules and MCS compositions against an
+independently written explicit-height model (tests/heightmodel.py) and
+closed forms.  Small sizes only: these run on the CPU in seconds."""
+import numpy as np
+import pytest
+
+from heightmodel import HeightLattice, normalized, thresh23
+from oracle import oracle as O
+
+ENCS = [O.ENC_LOCAL, O.ENC_CBBIT]
+
+
+def evolved_local(lx, ly, seed, mcs=3):
+    """a rough consistent ENC_LOCAL lattice from the oracle's SCA dynamics"""
+    m = O.Dsfmt(seed)
+    st = O.Streams(3, m)
+    lat = O.flat(lx, ly, O.ENC_LOCAL)
+    O.cb_mcs(lat, lx, ly, st, mcs, 0.7, 0.3)
+    return lat
+
+
+@pytest.mark.parametrize("lx,ly", [(6, 6), (7, 6), (8, 8), (6, 9)])
+@pytest.mark.parametrize("enc", ENCS)
+def test_flat_closed_form(lx, ly, enc):
+    lat = O.flat(lx, ly, enc)
+    N = 1 << (lx + ly)
+    h = O.heights(lat, enc, lx, ly)
+    # checkerboard of {-1, 0}: even sites are minima one below their neighbours
+    yy, xx = np.mgrid[0:1 << ly, 0:1 << lx]
+    assert np.array_equal(h, np.where((xx + yy) % 2 == 0, -1, 0))
+    s1, s2 = O.roughness(lat, enc, lx, ly)
+    assert s1 == -N // 2 and s2 == N // 2  # <h> = -1/2, w^2 = 1/4
+    assert O.consistent(lat, enc, lx, ly)
+
+
+def test_flat_encodings_agree():
+    for lx, ly in [(6, 6), (8, 7)]:
+        assert np.array_equal(O.local_to_cbbit(O.flat(lx, ly, 0), lx, ly), O.flat(lx, ly, 1))
+
+
+def test_one_full_halfsweep_lifts_every_minimum():
+    lx, ly = 8, 6
+    N = 1 << (lx + ly)
+    lat = O.flat(lx, ly, O.ENC_CBBIT)
+    masks = np.full(1 << (lx + ly - 7), ~np.uint64(0), np.uint64)
+    O.cbbit_halfsweep_masked(lat, lx, ly, 0, masks)
+    s1, s2 = O.roughness(lat, O.ENC_CBBIT, lx, ly)
+    assert s1 == N // 2 and s2 == N // 2  # heights {0, +1}: w^2 = 1/4
+    # the other sublattice now holds the minima; lifting it restores the flat
+    # shape two units higher, and because the convention anchors
+    # h(0, L-1) = 0 (a sublattice-1 site) the reported heights are {-1, 0}
+    O.cbbit_halfsweep_masked(lat, lx, ly, 1, masks)
+    s1b, s2b = O.roughness(lat, O.ENC_CBBIT, lx, ly)
+    assert s1b == -N // 2 and s2b == N // 2
+
+
+def test_heights_follow_slope_definition():
+    lx, ly = 7, 6
+    lat = evolved_local(lx, ly, 5)
+    assert O.consistent(lat, 0, lx, ly)
+    h = O.heights(lat, 0, lx, ly).astype(np.int64)
+    X, Y = 1 << lx, 1 << ly
+    for y in range(Y):
+        for x in range(X):
+            sx = 1 if O.slope(lat, 0, lx, ly, x, y, 0) else -1
+            sy = 1 if O.slope(lat, 0, lx, ly, x, y, 1) else -1
+            assert h[y, x] - h[y, (x - 1) % X] == sx
+            assert h[y, x] - h[(y - 1) % Y, x] == sy
+    # the convention anchors h(0, L-1) = 0 (kpz/correlator.cpp:50-64)
+    assert h[Y - 1, 0] == 0
+
+
+def test_encoding_roundtrip_and_reductions_agree():
+    lx, ly = 8, 7
+    lat = evolved_local(lx, ly, 11)
+    cb = O.local_to_cbbit(lat, lx, ly)
+    assert np.array_equal(O.cbbit_to_local(cb, lx, ly), lat)
+    assert np.array_equal(O.heights(lat, 0, lx, ly), O.heights(cb, 1, lx, ly))
+    assert O.roughness(lat, 0, lx, ly) == O.roughness(cb, 1, lx, ly)
+
+
+def test_correlate_brute_force():
+    lx, ly = 7, 7
+    a = evolved_local(lx, ly, 1, 2)
+    b = evolved_local(lx, ly, 2, 4)
+    ha = O.heights(a, 0, lx, ly).astype(np.int64)
+    hb = O.heights(b, 0, lx, ly).astype(np.int64)
+    s1, t1, s2, s11, tcs = O.correlate(a, b, 0, lx, ly)
+    assert s1 == ha.sum() and t1 == hb.sum()
+    assert s2 == int((ha * ha).sum()) and s11 == int((ha * hb).sum())
+    eq = 0
+    for y in range(1 << ly):
+        for x in range(1 << lx):
+            for d in range(2):
+                eq += 1 if O.slope(a, 0, lx, ly, x, y, d) == O.slope(b, 0, lx, ly, x, y, d) else -1
+    assert tcs == eq
+    # identical lattices: tcs = 2N, and the CBBit path agrees
+    assert O.correlate(a, a, 0, lx, ly)[4] == 2 << (lx + ly)
+    ca, cb = O.local_to_cbbit(a, lx, ly), O.local_to_cbbit(b, lx, ly)
+    assert O.correlate(ca, cb, 1, lx, ly) == (s1, t1, s2, s11, tcs)
+
+
+def test_inconsistent_lattice_detected():
+    lx, ly = 6, 6
+    lat = O.flat(lx, ly, 0)
+    lat[3] ^= 1 << 2
+    assert not O.consistent(lat, 0, lx, ly)
+
+
+@pytest.mark.parametrize("upd", [O.UPD_KPZ, O.UPD_PQ])
+def test_local_update_rule_vs_height_model(upd):
+    lx, ly = 6, 6
+    lat = evolved_local(lx, ly, 3)
+    hm = HeightLattice(O.heights(lat, 0, lx, ly))
+    rng = np.random.default_rng(7)
+    p, q = (1.0, 0.0) if upd == O.UPD_KPZ else (0.6, 0.35)
+    tp, tq = thresh23(p), thresh23(q)
+    for _ in range(3000):
+        x, y = int(rng.integers(64)), int(rng.integers(64))
+        r = int(rng.integers(1 << 23))
+        O.update_local(lat, lx, ly, x, y, upd, r / 8388608.0, p, q)
+        if upd == O.UPD_KPZ:
+            hm.update(x, y, True, False)
+        else:
+            hm.update(x, y, r < tp, r < tq)
+    assert np.array_equal(normalized(O.heights(lat, 0, lx, ly)), hm.normalized())
+    assert O.consistent(lat, 0, lx, ly)
+
+
+def test_cbbit_update_rule_vs_height_model():
+    lx, ly = 7, 6
+    lat = O.local_to_cbbit(evolved_local(lx, ly, 4), lx, ly)
+    hm = HeightLattice(O.heights(lat, 1, lx, ly))
+    rng = np.random.default_rng(8)
+    p, q = 0.8, 0.45
+    tp, tq = thresh23(p), thresh23(q)
+    for _ in range(3000):
+        x, y = int(rng.integers(128)), int(rng.integers(64))
+        r = int(rng.integers(1 << 23))
+        O.update_cbbit(lat, lx, ly, x, y, r / 8388608.0, p, q)
+        hm.update(x, y, r < tp, r < tq)
+    assert np.array_equal(normalized(O.heights(lat, 1, lx, ly)), hm.normalized())
+
+
+def test_threshold_is_exact_float_comparison():
+    # randomFloat() < p evaluated as float-vs-double, exactly as the reference
+    for p in [0.5, 0.95, 0.05, 1.0 / 3.0, 0.1, 1.0, 0.0, 1e-9, 0.999999999]:
+        t = thresh23(p)
+        for r in {max(t - 1, 0), t, min(t + 1, (1 << 23) - 1), 0, (1 << 23) - 1}:
+            assert (np.float64(np.float32(r / 8388608.0)) < p) == (r < t)
+
+
+@pytest.mark.parametrize("lx,ly", [(7, 5), (8, 4), (6, 4)])
+@pytest.mark.parametrize("signum", [0, 1])
+def test_bitparallel_halfsweep_equals_scalar_in_any_order(lx, ly, signum):
+    lat0 = O.local_to_cbbit(evolved_local(lx, ly, 9 + lx), lx, ly)
+    wbits = 64 if lx >= 7 else 32
+    nw = 1 << (lx + ly - (7 if wbits == 64 else 6))
+    rng = np.random.default_rng(lx * 10 + signum)
+    masks = rng.integers(0, 1 << 63, nw, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, nw, dtype=np.uint64)
+    if wbits == 32:
+        masks &= np.uint64(0xFFFFFFFF)
+    a = lat0.copy()
+    O.cbbit_halfsweep_masked(a, lx, ly, signum, masks)
+    X = 1 << lx
+    W = X // (128 if wbits == 64 else 64)
+    sites = []
+    for y in range(1 << ly):
+        par = signum ^ (y & 1)
+        for xw in range(W):
+            mw = int(masks[y * W + xw])
+            for k in range(wbits):
+                if (mw >> k) & 1:
+                    sites.append((xw * 2 * wbits + 2 * k + par, y))
+    for order in range(2):
+        b = lat0.copy()
+        idx = np.arange(len(sites))
+        if order:
+            rng.shuffle(idx)
+        for i in idx:
+            x, y = sites[i]
+            O.update_cbbit(b, lx, ly, x, y, 0.0, 1.0, 0.0)
+        assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------- compositions
+def _dt_block(lx, ly, lbx, lby, ds, i):
+    hx = 1 << (lx - lbx - 1)
+    return ((2 * (i % hx) + (ds & 1)) << lbx, (2 * (i // hx) + (ds >> 1)) << lby)
+
+
+def py_dt(hm, lx, ly, lbx, lby, st, master, n, random_origin, upd, p, q):
+    X, Y = 1 << lx, 1 << ly
+    nb = 1 << (lx - lbx + ly - lby - 2)
+    bs = 1 << (lbx + lby)
+    tp, tq = thresh23(p), thresh23(q)
+    for _ in range(n):
+        seq = master.shuffle4()
+        sx = sy = 0
+        if random_origin:
+            sx = int(master.close_open() * (1 << lbx))
+            sy = int(master.close_open() * (1 << lby))
+        for ds in seq:
+            for t in range(st.T):
+                for i in range(t, nb, st.T):
+                    ox, oy = _dt_block(lx, ly, lbx, lby, ds, i)
+                    for _k in range(bs):
+                        if random_origin:
+                            x = (((st.draw(t) & 0x7FFFFF) >> (23 - lbx)) + ox + sx) % X
+                            y = (((st.draw(t) & 0x7FFFFF) >> (23 - lby)) + oy + sy) % Y
+                        else:
+                            x = (st.draw(t) >> (32 - lbx)) + ox
+                            y = (st.draw(t) >> (32 - lby)) + oy
+                        if upd == O.UPD_KPZ:
+                            hm.update(x, y, True, False)
+                        else:
+                            r = st.draw(t) & 0x7FFFFF
+                            hm.update(x, y, r < tp, r < tq)
+
+
+@pytest.mark.parametrize("random_origin", [False, True])
+@pytest.mark.parametrize("upd,p,q", [(O.UPD_KPZ, 1.0, 0.0), (O.UPD_PQ, 0.5, 0.5), (O.UPD_PQ, 0.9, 0.2)])
+@pytest.mark.parametrize("T", [1, 3])
+def test_dt_composition(random_origin, upd, p, q, T):
+    lx, ly, lbx, lby = 6, 5, 2, 2
+    m1, m2 = O.Dsfmt(42), O.Dsfmt(42)
+    s1, s2 = O.Streams(T, m1), O.Streams(T, m2)
+    lat = O.flat(lx, ly, 0)
+    hm = HeightLattice(O.heights(lat, 0, lx, ly))
+    O.dt_mcs(lat, lx, ly, lbx, lby, s1, m1, 2, random_origin, upd, p, q)
+    py_dt(hm, lx, ly, lbx, lby, s2, m2, 2, random_origin, upd, p, q)
+    assert np.array_equal(normalized(O.heights(lat, 0, lx, ly)), hm.normalized())
+    assert np.array_equal(s1.states, s2.states)
+    assert m1.close_open() == m2.close_open()
+    assert O.consistent(lat, 0, lx, ly)
+
+
+def py_cbbit(hm, lx, ly, st, n, mode, p, q):
+    X, Y = 1 << lx, 1 << ly
+    wbits = 64 if lx >= 7 else 32
+    tp, tq = thresh23(p), thresh23(q)
+    per, rem = Y // st.T, Y % st.T
+    for _ in range(n):
+        for s in range(2):
+            lo = 0
+            for t in range(st.T):
+                hi = lo + per + (1 if t < rem else 0)
+                for y in range(lo, hi):
+                    par = s ^ (y & 1)
+                    if mode == 0:
+                        for xw in range(X // (2 * wbits)):
+                            if wbits == 64:
+                                a = st.draw(t)
+                                b = st.draw(t)
+                                rnd = (a << 32) ^ b
+                            else:
+                                rnd = st.draw(t)
+                            for k in range(wbits):
+                                if (rnd >> k) & 1:
+                                    hm.update(xw * 2 * wbits + 2 * k + par, y, True, False)
+                    else:
+                        for x in range(par, X, 2):
+                            r = st.draw(t) & 0x7FFFFF
+                            hm.update(x, y, r < tp, r < tq)
+                lo = hi
+
+
+@pytest.mark.parametrize("mode,p,q", [(0, 0.5, 0.0), (1, 0.95, 0.05), (1, 0.5, 0.5), (1, 1.0, 0.0)])
+@pytest.mark.parametrize("lx,ly,T", [(7, 4, 1), (8, 3, 3), (6, 4, 5)])
+def test_cbbit_composition(mode, p, q, lx, ly, T):
+    m1, m2 = O.Dsfmt(7), O.Dsfmt(7)
+    s1, s2 = O.Streams(T, m1), O.Streams(T, m2)
+    lat = O.flat(lx, ly, 1)
+    hm = HeightLattice(O.heights(lat, 1, lx, ly))
+    O.cbbit_mcs(lat, lx, ly, s1, 2, mode, p, q)
+    py_cbbit(hm, lx, ly, s2, 2, mode, p, q)
+    assert np.array_equal(normalized(O.heights(lat, 1, lx, ly)), hm.normalized())
+    assert np.array_equal(s1.states, s2.states)
+    assert O.consistent(lat, 1, lx, ly)
+
+
+def py_cb(hm, lx, ly, st, n, p, q):
+    X, Y = 1 << lx, 1 << ly
+    tp, tq = thresh23(p), thresh23(q)
+    dimxw = X // 4
+    per, rem = (dimxw // st.T) * 4, dimxw % st.T
+
+    def loop(t, s, x0, x1):
+        for yw in range(0, Y, 4):
+            for xw in range(x0, x1, 4):
+                for y in range(yw, yw + 4):
+                    x = xw + ((y & 1) ^ s)
+                    for xx in (x, x + 2):
+                        r = st.draw(t) & 0x7FFFFF
+                        hm.update(xx, y, r < tp, r < tq)
+
+    for _ in range(n):
+        for s in range(2):
+            x1 = 0
+            for t in range(st.T):
+                x0 = x1
+                x1 = x0 + per + (4 if t < rem else 0)
+                xc = x0 + (((x1 - x0) // 2) & ~3)
+                loop(t, s, x0, xc)
+                loop(t, s, xc, x1)
+
+
+@pytest.mark.parametrize("T", [1, 3, 8])
+def test_cb_composition(T):
+    lx, ly = 5, 4
+    m1, m2 = O.Dsfmt(3), O.Dsfmt(3)
+    s1, s2 = O.Streams(T, m1), O.Streams(T, m2)
+    lat = O.flat(lx, ly, 0)
+    hm = HeightLattice(O.heights(lat, 0, lx, ly))
+    O.cb_mcs(lat, lx, ly, s1, 2, 0.8, 0.3)
+    py_cb(hm, lx, ly, s2, 2, 0.8, 0.3)
+    assert np.array_equal(normalized(O.heights(lat, 0, lx, ly)), hm.normalized())
+    assert np.array_equal(s1.states, s2.states)
+
+
+def test_dt_invalid_layout_rejected():
+    m = O.Dsfmt(1)
+    st = O.Streams(1, m)
+    lat = O.flat(6, 6, 0)
+    with pytest.raises(ValueError):
+        O.dt_mcs(lat, 6, 6, 6, 2, st, m, 1)  # one block per row: no double tiling
+    with pytest.raises(ValueError):
+        O.dt_mcs(lat, 6, 6, 1, 2, st, m, 1)  # block smaller than the reference minimum
