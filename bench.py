@@ -1,0 +1,193 @@
+"""Benchmark of the MI355X KPZ engine on its headline workload.
+
+Workload (one "step" = one Monte-Carlo step, N = 65536^2 site updates):
+random-sequential double tiling (SchedulerDT semantics, static origin,
+16x16-site blocks, one SLCG stream per block = 2^22 streams), flat initial
+condition, p=1, q=0, with exact w^2 sampled at the log-spaced times
+ceil(exp(log(t+10)+0.1)) inside the timed region.  The lattice (1 GiB,
+ENC_LOCAL) and all RNG state stay resident in HBM.
+
+Prints ONE JSON line (rank 0, N=1).  Timing uses HIP events recorded on the
+engine's own stream.  The dominant kernel's average launch duration gives
+roofline.achieved; cpu_baseline times the oracle (sequential C restatement of
+SchedulerCPUDT) on a bounded sample of the same workload.
+
+    python bench.py [--steps K] [--warmup W] [--lx 16] [--no-cpu] [--secondary]
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "site updates/s (DT, SCABit @65536²); batched replica-MCS/s vs serial"
+HBM_PEAK_GBS = 8000.0
+
+
+class HipEvents:
+    """hipEvent timing on an explicit stream (the engine's, not torch's)."""
+
+    def __init__(self):
+        self.h = C.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.h.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.h.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, C.c_void_p(stream)) == 0
+
+    def ms(self, a, b):
+        assert self.h.hipEventSynchronize(b) == 0
+        t = C.c_float()
+        assert self.h.hipEventElapsedTime(C.byref(t), a, b) == 0
+        return float(t.value)
+
+
+def sample_times(total):
+    """kpz/sampling.h:84-100 default: next = ceil(exp(log(t+10)+0.1))"""
+    out, t = [], 0
+    while True:
+        t = int(math.ceil(math.exp(math.log(t + 10) + 0.1)))
+        if t > total:
+            return out
+        out.append(t)
+
+
+def run_dt(K, args, ev):
+    s = K.SchedulerDT(args.lx, args.lx, layout=(4, 4))
+    s.supplySeed(args.seed)
+    s.initHomogeneous()
+    stream = s.stream
+    s.mcs(args.warmup)
+    s.sync()
+    s.initHomogeneous()  # measure from the flat state
+    samples = set(sample_times(args.steps))
+    e0, e1 = ev.event(), ev.event()
+    a, b = ev.event(), ev.event()
+    mcs_ms = 0.0
+    w2 = []
+    ev.record(e0, stream)
+    done = 0
+    while done < args.steps:
+        nxt = min([t for t in samples if t > done] + [args.steps])
+        ev.record(a, stream)
+        s.mcs(nxt - done)
+        ev.record(b, stream)
+        mcs_ms += ev.ms(a, b)
+        done = nxt
+        if done in samples:
+            w2.append((done, s.roughness().w2))
+    ev.record(e1, stream)
+    total_ms = ev.ms(e0, e1)
+    return s, total_ms, mcs_ms, w2
+
+
+def run_scabit(K, lx, p, q, steps, seed, ev):
+    s = K.SchedulerSCABit(lx, lx)
+    s.supplySeed(seed)
+    s.initHomogeneous()
+    s.setPQ(p, q)
+    s.mcsPQSyncRng(2)
+    a, b = ev.event(), ev.event()
+    ev.record(a, s.stream)
+    s.mcsPQSyncRng(steps)
+    ev.record(b, s.stream)
+    ms = ev.ms(a, b)
+    s.close()
+    return ms
+
+
+def cpu_baseline(seconds=12.0):
+    """Oracle SchedulerCPUDT restatement, one core, 2048^2 sample, same layout
+    (16x16 blocks, one stream per block)."""
+    from oracle import oracle as O
+    lx = 11
+    m = O.Dsfmt(42)
+    nb = 1 << (lx - 4 + lx - 4 - 2)
+    st = O.Streams(nb, m)
+    lat = O.flat(lx, lx, O.ENC_LOCAL)
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        O.dt_mcs(lat, lx, lx, 4, 4, st, m, 1)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": n * (1 << (2 * lx)) / dt, "unit": "site updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle SchedulerCPUDT restatement, 2048x2048, 16x16 blocks, {n} MCS in {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--lx", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--secondary", action="store_true", help="also time SCABit at 65536^2")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
+              "(see DESIGN.md for how the lattice would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import cudakpz_amd as K
+    ev = HipEvents()
+    N = 1 << (2 * args.lx)
+    s, total_ms, mcs_ms, w2 = run_dt(K, args, ev)
+    launches = 4 * args.steps
+    avg_launch_ms = mcs_ms / launches
+    bytes_per_launch = 0.5 * N / 4  # 0.5 B per site update, N/4 updates per phase
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    value = N * args.steps / (total_ms * 1e-3)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "site updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"SchedulerDT {1 << args.lx}x{1 << args.lx} ENC_LOCAL, flat IC, p=1 q=0, "
+                               f"16x16-site DT blocks, one SLCG64 stream per block ({s.threads} streams), "
+                               f"{args.steps} MCS, exact w2 at {len(w2)} log-spaced times",
+                   "lattice": f"{1 << args.lx}x{1 << args.lx}", "mcs": args.steps, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "dt_phase_kernel<4,4,KPZ,static>", "avg_launch_ms": avg_launch_ms,
+                     "bytes_per_launch": bytes_per_launch},
+        "w2_final": w2[-1][1] if w2 else None,
+    }
+    s.close()
+    if args.secondary:
+        sec = {}
+        for name, p, q in [("scabit_p0.5_q0", 0.5, 0.0), ("scabit_p0.95_q0.05", 0.95, 0.05)]:
+            steps = 100
+            ms = run_scabit(K, args.lx, p, q, steps, args.seed, ev)
+            ups = N * steps / (ms * 1e-3)
+            sec[name] = {"site_updates_per_s": ups, "hbm_GBs_algorithmic": ups * 1.0 / 1e9,
+                         "hbm_frac": ups / 1e9 / HBM_PEAK_GBS, "ms_per_mcs": ms / steps}
+        line["secondary"] = sec
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,303 @@
+"""cudakpz_amd -- MI355X-native engine for the 2+1D octahedron KPZ model.
+
+Python mirror of the CudaKpz scheduler interface (kpz/schedulerService.h:
+73-255) over the C ABI of libkpz_amd.so (include/kpz_amd.h).  Every compute
+call runs the hand-written gfx950 kernels; there is no CPU fallback: if the
+library is missing or no GPU is present, the calls raise.
+
+    from cudakpz_amd import SchedulerDT
+    s = SchedulerDT(16, 16)            # 65536 x 65536, ENC_LOCAL
+    s.supplySeed(42); s.initHomogeneous()
+    s.mcs(100)
+    r = s.roughness()                  # exact sums, r.w2
+"""
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkpz_amd.so")
+_LIB = None
+
+KPZ_DT, KPZ_DTR, KPZ_SCABIT = 0, 1, 2
+ENC_LOCAL, ENC_CBBIT = 0, 1
+
+
+class KpzError(RuntimeError):
+    """Raised when a C ABI call fails (the message is kpz_last_error())."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def build(arch="gfx950", jobs=8):
+    """Compile libkpz_amd.so in-tree with hipcc."""
+    subprocess.check_call(["make", "-s", "-j", str(jobs), "-C", os.path.join(_HERE, "csrc"), f"ARCH={arch}"])
+
+
+class _Params(C.Structure):
+    _fields_ = [("threads", C.c_int64), ("layout_x", C.c_int), ("layout_y", C.c_int),
+                ("skip", C.c_uint64), ("device", C.c_int)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise KpzError(-2, f"{LIB_PATH} is not built; run cudakpz_amd.build()")
+        L = C.CDLL(LIB_PATH)
+        h = C.c_void_p
+        i64p = C.POINTER(C.c_int64)
+        u64p = C.POINTER(C.c_uint64)
+        L.kpz_last_error.restype = C.c_char_p
+        L.kpz_create.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(_Params), C.POINTER(h)]
+        L.kpz_destroy.argtypes = [h]
+        L.kpz_set_seed.argtypes = [h, C.c_uint64]
+        L.kpz_set_pq.argtypes = [h, C.c_double, C.c_double]
+        for f in ("kpz_init_flat", "kpz_sync", "kpz_corr_tag"):
+            getattr(L, f).argtypes = [h]
+        L.kpz_upload.argtypes = [h, C.c_void_p]
+        L.kpz_download.argtypes = [h, C.c_void_p]
+        L.kpz_mcs.argtypes = [h, C.c_int]
+        L.kpz_mcs_pq.argtypes = [h, C.c_int]
+        L.kpz_roughness.argtypes = [h, i64p]
+        L.kpz_correlate.argtypes = [h, i64p]
+        L.kpz_threads.argtypes = [h]
+        L.kpz_threads.restype = C.c_int64
+        L.kpz_get_streams.argtypes = [h, u64p]
+        L.kpz_set_streams.argtypes = [h, u64p]
+        L.kpz_get_master.argtypes = [h, C.c_void_p]
+        L.kpz_set_master.argtypes = [h, C.c_void_p]
+        L.kpz_master_draw.argtypes = [h]
+        L.kpz_master_draw.restype = C.c_double
+        L.kpz_mcs_count.argtypes = [h]
+        L.kpz_mcs_count.restype = C.c_uint32
+        L.kpz_mcs_scale.argtypes = [h]
+        L.kpz_mcs_scale.restype = C.c_double
+        L.kpz_encoding.argtypes = [h]
+        L.kpz_size_words.argtypes = [h]
+        L.kpz_size_words.restype = C.c_uint64
+        L.kpz_skip.argtypes = [h]
+        L.kpz_skip.restype = C.c_uint64
+        L.kpz_device_lattice.argtypes = [h]
+        L.kpz_device_lattice.restype = C.c_void_p
+        L.kpz_stream.argtypes = [h]
+        L.kpz_stream.restype = C.c_void_p
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise KpzError(rc, lib().kpz_last_error().decode())
+
+
+def _i128(lo, hi):
+    return (int(hi) << 64) | (int(lo) & 0xFFFFFFFFFFFFFFFF)
+
+
+@dataclass
+class Roughness:
+    """Exact height sums (heights as Correlator::set) and the derived w^2
+    (kpz/roughness.cpp:38-43)."""
+    N: int
+    sum_h: int
+    sum_h2: int
+
+    @property
+    def h(self):
+        return self.sum_h / self.N
+
+    @property
+    def w2(self):
+        from fractions import Fraction
+        return float(Fraction(self.sum_h2, self.N) - Fraction(self.sum_h, self.N) ** 2)
+
+
+@dataclass
+class Correlation:
+    """Autocorrelation against the tagged snapshot (kpz/correlator.cpp:450-455):
+    ch = <h_t h_s> - <h_t><h_s>, cs = slope overlap / N / 2."""
+    N: int
+    sum_h: int
+    sum_hs: int
+    sum_h2: int
+    sum_hhs: int
+    tcs: int
+
+    @property
+    def ch(self):
+        from fractions import Fraction
+        return float(Fraction(self.sum_hhs, self.N) - Fraction(self.sum_h, self.N) * Fraction(self.sum_hs, self.N))
+
+    @property
+    def cs(self):
+        return self.tcs / self.N / 2.0
+
+    @property
+    def w2(self):
+        from fractions import Fraction
+        return float(Fraction(self.sum_h2, self.N) - Fraction(self.sum_h, self.N) ** 2)
+
+
+class SchedulerService:
+    """One device lattice with its worker streams and master generator."""
+
+    KIND = None
+    ENCODING = None
+
+    def __init__(self, lx, ly, threads=0, layout=(4, 4), skip=0, device=-1, kind=None):
+        self.lx, self.ly = int(lx), int(ly)
+        kind = self.KIND if kind is None else kind
+        p = _Params(int(threads), int(layout[0]), int(layout[1]), int(skip), int(device))
+        h = C.c_void_p()
+        _check(lib().kpz_create(kind, self.lx, self.ly, C.byref(p), C.byref(h)))
+        self._h = h
+        self.kind = kind
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kpz_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- geometry
+    @property
+    def N(self):
+        return 1 << (self.lx + self.ly)
+
+    @property
+    def size_words(self):
+        return int(lib().kpz_size_words(self._h))
+
+    @property
+    def threads(self):
+        return int(lib().kpz_threads(self._h))
+
+    @property
+    def skip(self):
+        return int(lib().kpz_skip(self._h))
+
+    def encoding(self):
+        return int(lib().kpz_encoding(self._h))
+
+    # -- reference API
+    def supplySeed(self, seed):
+        _check(lib().kpz_set_seed(self._h, int(seed) & 0xFFFFFFFFFFFFFFFF))
+
+    def setPQ(self, p, q):
+        _check(lib().kpz_set_pq(self._h, float(p), float(q)))
+
+    def initHomogeneous(self):
+        _check(lib().kpz_init_flat(self._h))
+
+    def mcs(self, n):
+        _check(lib().kpz_mcs(self._h, int(n)))
+
+    def mcsSyncRng(self, n):
+        self.mcs(n)
+
+    def mcsPQSyncRng(self, n):
+        _check(lib().kpz_mcs_pq(self._h, int(n)))
+
+    def mcsPQ(self, n):
+        self.mcsPQSyncRng(n)
+
+    def mcsScale(self):
+        return float(lib().kpz_mcs_scale(self._h))
+
+    def mcsCount(self):
+        return int(lib().kpz_mcs_count(self._h))
+
+    def sync(self):
+        _check(lib().kpz_sync(self._h))
+
+    def pushSystem(self, lattice):
+        a = np.ascontiguousarray(lattice, dtype=np.uint32)
+        if a.size != self.size_words:
+            raise ValueError(f"lattice has {a.size} words, expected {self.size_words}")
+        _check(lib().kpz_upload(self._h, a.ctypes.data))
+
+    def popSystem(self):
+        a = np.empty(self.size_words, np.uint32)
+        _check(lib().kpz_download(self._h, a.ctypes.data))
+        return a
+
+    def roughness(self):
+        out = (C.c_int64 * 3)()
+        _check(lib().kpz_roughness(self._h, out))
+        return Roughness(self.N, int(out[0]), _i128(out[1], out[2]))
+
+    def correlateTag(self):
+        _check(lib().kpz_corr_tag(self._h))
+
+    def correlate(self):
+        out = (C.c_int64 * 7)()
+        _check(lib().kpz_correlate(self._h, out))
+        return Correlation(self.N, int(out[0]), int(out[1]), _i128(out[2], out[3]), _i128(out[4], out[5]),
+                           int(out[6]))
+
+    # -- RNG state
+    def streams(self):
+        a = np.empty(self.threads, np.uint64)
+        _check(lib().kpz_get_streams(self._h, a.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return a
+
+    def set_streams(self, states):
+        a = np.ascontiguousarray(states, dtype=np.uint64)
+        if a.size != self.threads:
+            raise ValueError("one state per stream expected")
+        _check(lib().kpz_set_streams(self._h, a.ctypes.data_as(C.POINTER(C.c_uint64))))
+
+    def master_draw(self):
+        return float(lib().kpz_master_draw(self._h))
+
+    @property
+    def device_lattice(self):
+        return int(lib().kpz_device_lattice(self._h))
+
+    @property
+    def stream(self):
+        return int(lib().kpz_stream(self._h))
+
+
+class SchedulerDT(SchedulerService):
+    """Random-sequential double tiling, static origin (SchedulerCPUDT)."""
+    KIND = KPZ_DT
+    ENCODING = ENC_LOCAL
+
+
+class SchedulerDTr(SchedulerService):
+    """Random-sequential double tiling, random origin (SchedulerCPUDTr)."""
+    KIND = KPZ_DTR
+    ENCODING = ENC_LOCAL
+
+
+class SchedulerSCABit(SchedulerService):
+    """Checkerboard SCA, bit-vectorised ENC_CBBIT (SchedulerCPUCBBit)."""
+    KIND = KPZ_SCABIT
+    ENCODING = ENC_CBBIT
+
+
+def make_scheduler(name, lx, ly, **kw):
+    """SchedulerConfig::mkScheduler by scheduler id (kpz/kpzFrontend.cpp:93-250)."""
+    table = {"dtStatic": SchedulerDT, "dt": SchedulerDTr, "scaBit": SchedulerSCABit}
+    if name not in table:
+        raise KpzError(-1, f"No such scheduler: {name}")
+    return table[name](lx, ly, **kw)

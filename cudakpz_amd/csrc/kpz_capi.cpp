@@ -1,0 +1,236 @@
+// extern "C" boundary over the C++ schedulers (include/kpz_amd.h).
+#include "../../include/kpz_amd.h"
+#include "kpz_scheduler.hpp"
+
+#include <cstring>
+#include <string>
+
+struct kpz_scheduler {
+	std::unique_ptr<kpz::SchedulerService> s;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+	g_err = msg;
+	return code;
+}
+
+template<class F>
+int guard(F&& f)
+{
+	try
+	{
+		f();
+		return KPZ_OK;
+	}
+	catch(const kpz::Error& e)
+	{
+		return fail(e.code, e.what());
+	}
+	catch(const std::bad_alloc&)
+	{
+		return fail(KPZ_ENOMEM, "host allocation failed");
+	}
+	catch(const std::exception& e)
+	{
+		return fail(KPZ_EDEVICE, e.what());
+	}
+}
+
+#define HANDLE(h)                                     \
+	if(!(h) || !(h)->s)                               \
+		return fail(KPZ_EINVAL, "null scheduler handle");
+} // namespace
+
+extern "C" {
+
+const char* kpz_last_error(void) { return g_err.c_str(); }
+
+int kpz_create(int kind, int lx, int ly, const kpz_params* params, kpz_scheduler** out)
+{
+	if(!out)
+		return fail(KPZ_EINVAL, "null output pointer");
+	*out = nullptr;
+	kpz::Params p;
+	if(params)
+	{
+		if(params->threads < 0)
+			return fail(KPZ_EINVAL, "threads must be >= 0");
+		p.threads = params->threads;
+		p.layout_x = params->layout_x ? params->layout_x : 4;
+		p.layout_y = params->layout_y ? params->layout_y : 4;
+		p.skip = params->skip;
+		p.device = params->device;
+		if(p.skip && !(p.skip & 1))
+			return fail(KPZ_EINVAL, "SLCG skip must be odd");
+	}
+	return guard([&] {
+		auto h = new kpz_scheduler;
+		try
+		{
+			h->s = kpz::makeScheduler(kind, lx, ly, p);
+		}
+		catch(...)
+		{
+			delete h;
+			throw;
+		}
+		*out = h;
+	});
+}
+
+int kpz_destroy(kpz_scheduler* h)
+{
+	if(!h)
+		return KPZ_OK;
+	return guard([&] { delete h; });
+}
+
+int kpz_set_seed(kpz_scheduler* h, uint64_t seed)
+{
+	HANDLE(h);
+	return guard([&] { h->s->supplySeed(seed); });
+}
+
+int kpz_set_pq(kpz_scheduler* h, double p, double q)
+{
+	HANDLE(h);
+	if(!h->s->setPQ(p, q))
+		return fail(KPZ_EINVAL, "p and q must lie in [0, 1]");
+	return KPZ_OK;
+}
+
+int kpz_init_flat(kpz_scheduler* h)
+{
+	HANDLE(h);
+	return guard([&] { h->s->initHomogeneous(); });
+}
+
+int kpz_upload(kpz_scheduler* h, const uint32_t* lattice)
+{
+	HANDLE(h);
+	if(!lattice)
+		return fail(KPZ_EINVAL, "null lattice");
+	return guard([&] { h->s->pushSystem(lattice); });
+}
+
+int kpz_download(kpz_scheduler* h, uint32_t* lattice)
+{
+	HANDLE(h);
+	if(!lattice)
+		return fail(KPZ_EINVAL, "null lattice");
+	return guard([&] { h->s->popSystem(lattice); });
+}
+
+int kpz_mcs(kpz_scheduler* h, int n)
+{
+	HANDLE(h);
+	if(n < 0)
+		return fail(KPZ_EINVAL, "negative MCS count");
+	return guard([&] { h->s->mcs(n); });
+}
+
+int kpz_mcs_pq(kpz_scheduler* h, int n)
+{
+	HANDLE(h);
+	if(n < 0)
+		return fail(KPZ_EINVAL, "negative MCS count");
+	return guard([&] { h->s->mcsPQSyncRng(n); });
+}
+
+int kpz_sync(kpz_scheduler* h)
+{
+	HANDLE(h);
+	return guard([&] { h->s->sync(); });
+}
+
+int kpz_roughness(kpz_scheduler* h, int64_t out[3])
+{
+	HANDLE(h);
+	if(!out)
+		return fail(KPZ_EINVAL, "null output");
+	return guard([&] {
+		const kpz::Sums s = h->s->roughness();
+		out[0] = s.sumH;
+		out[1] = (int64_t)(uint64_t)s.sumH2;
+		out[2] = (int64_t)(s.sumH2 >> 64);
+	});
+}
+
+int kpz_corr_tag(kpz_scheduler* h)
+{
+	HANDLE(h);
+	return guard([&] { h->s->correlateTag(); });
+}
+
+int kpz_correlate(kpz_scheduler* h, int64_t out[7])
+{
+	HANDLE(h);
+	if(!out)
+		return fail(KPZ_EINVAL, "null output");
+	if(!h->s->hasTag())
+		return fail(KPZ_ESTATE, "correlate() needs a correlateTag() snapshot");
+	return guard([&] {
+		const kpz::Sums s = h->s->correlate();
+		out[0] = s.sumH;
+		out[1] = s.sumHs;
+		out[2] = (int64_t)(uint64_t)s.sumH2;
+		out[3] = (int64_t)(s.sumH2 >> 64);
+		out[4] = (int64_t)(uint64_t)s.sumHHs;
+		out[5] = (int64_t)(s.sumHHs >> 64);
+		out[6] = s.tcs;
+	});
+}
+
+int64_t kpz_threads(const kpz_scheduler* h) { return (h && h->s) ? h->s->threads() : 0; }
+
+int kpz_get_streams(kpz_scheduler* h, uint64_t* states)
+{
+	HANDLE(h);
+	if(!states)
+		return fail(KPZ_EINVAL, "null output");
+	return guard([&] { h->s->getStreams(states); });
+}
+
+int kpz_set_streams(kpz_scheduler* h, const uint64_t* states)
+{
+	HANDLE(h);
+	if(!states)
+		return fail(KPZ_EINVAL, "null input");
+	return guard([&] { h->s->setStreams(states); });
+}
+
+int kpz_master_bytes(void) { return (int)kpz::MasterRng::STATE_BYTES; }
+
+int kpz_get_master(kpz_scheduler* h, uint8_t* state)
+{
+	HANDLE(h);
+	if(!state)
+		return fail(KPZ_EINVAL, "null output");
+	h->s->master().save(state);
+	return KPZ_OK;
+}
+
+int kpz_set_master(kpz_scheduler* h, const uint8_t* state)
+{
+	HANDLE(h);
+	if(!state)
+		return fail(KPZ_EINVAL, "null input");
+	h->s->master().load(state);
+	return KPZ_OK;
+}
+
+double kpz_master_draw(kpz_scheduler* h) { return (h && h->s) ? h->s->master().close_open() : -1.0; }
+
+uint32_t kpz_mcs_count(const kpz_scheduler* h) { return (h && h->s) ? h->s->mcsCount() : 0; }
+double kpz_mcs_scale(const kpz_scheduler* h) { return (h && h->s) ? h->s->mcsScale() : 0.0; }
+int kpz_encoding(const kpz_scheduler* h) { return (h && h->s) ? (int)h->s->encoding() : -1; }
+uint64_t kpz_size_words(const kpz_scheduler* h) { return (h && h->s) ? h->s->size().sizeW() : 0; }
+uint64_t kpz_skip(const kpz_scheduler* h) { return (h && h->s) ? h->s->skip() : 0; }
+void* kpz_device_lattice(const kpz_scheduler* h) { return (h && h->s) ? (void*)h->s->deviceLattice() : nullptr; }
+void* kpz_stream(const kpz_scheduler* h) { return (h && h->s) ? (void*)h->s->stream() : nullptr; }
+
+} // extern "C"

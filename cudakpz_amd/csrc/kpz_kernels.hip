@@ -1,0 +1,661 @@
+// gfx950 kernels of the KPZ octahedron-model engine.
+//
+// Design notes (MI355X / CDNA4, wave64):
+//  * DT: one lane owns one double-tiling block for a whole phase.  The block
+//    plus its +1 halo (right column of x slopes, top row of y slopes) is
+//    staged in LDS as bit-plane rows {X row, Y row} (uint2 per site row),
+//    lane-interleaved ([row][lane]) so every lane's random row access hits
+//    its own bank pair: no LDS bank conflicts whatever the random site.
+//    The SLCG state lives in registers; one update is 2 (3 for pq) SLCG
+//    steps, two LDS reads, a 4-bit configuration test and two LDS writes.
+//  * SCABit: a group of G <= 64 lanes owns one site row of one half-sweep;
+//    lane g handles u64 words g, g+G, ...  HBM traffic is fully coalesced
+//    8-byte-per-lane streams; the +-1 bit carry between neighbouring words
+//    crosses lanes with wave shuffles.  Per-site random numbers come from
+//    each worker's stream via precomputed affine jump-ahead, so any lane
+//    order reproduces the CPU counterpart's sequential draw order.
+//  * Reductions: one wave per site row computes exact int64 partial sums
+//    relative to the row's start height; the host combines rows in int128.
+#include "kpz_kernels.hpp"
+
+namespace kpz {
+
+namespace {
+
+// ---------------------------------------------------------------- bit utils
+// bits 0,2,..,30 -> 0..15
+__device__ __forceinline__ uint32_t even16(uint32_t v)
+{
+	v &= 0x55555555u;
+	v = (v | (v >> 1)) & 0x33333333u;
+	v = (v | (v >> 2)) & 0x0F0F0F0Fu;
+	v = (v | (v >> 4)) & 0x00FF00FFu;
+	v = (v | (v >> 8)) & 0x0000FFFFu;
+	return v;
+}
+
+// 0..15 -> bits 0,2,..,30
+__device__ __forceinline__ uint32_t spread16(uint32_t v)
+{
+	v &= 0xFFFFu;
+	v = (v | (v << 8)) & 0x00FF00FFu;
+	v = (v | (v << 4)) & 0x0F0F0F0Fu;
+	v = (v | (v << 2)) & 0x33333333u;
+	v = (v | (v << 1)) & 0x55555555u;
+	return v;
+}
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src, int width)
+{
+	const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, width);
+	const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, width);
+	return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t shflw(uint32_t v, int src, int width) { return (uint32_t)__shfl((int)v, src, width); }
+__device__ __forceinline__ uint64_t shflw(uint64_t v, int src, int width) { return shfl64(v, src, width); }
+
+// ======================================================================
+// DT phase kernel
+// ======================================================================
+constexpr int DT_WG = 256;
+
+template<int LBX, int LBY, int UPD, bool RO>
+__global__ __launch_bounds__(DT_WG) void dt_phase_kernel(DtPhaseArgs a, int64_t nb)
+{
+	constexpr int BX = 1 << LBX, BY = 1 << LBY, ROWS = BY + 1;
+	constexpr int NWX = BX / 4 + 1; // words covering BX+1 sites (+ shift)
+	constexpr int NWY = BY / 4 + 1; // word rows covering BY+1 site rows
+	constexpr uint32_t MX = (2u << BX) - 1u; // X row: columns 0..BX (BX = halo)
+	constexpr uint32_t MY = (1u << BX) - 1u; // Y row: columns 0..BX-1
+	__shared__ uint2 tile[ROWS * DT_WG];
+
+	const int tid = threadIdx.x;
+	const int64_t i = (int64_t)blockIdx.x * DT_WG + tid;
+	if(i >= nb)
+		return; // no barrier in this kernel: idle lanes may leave
+
+	// block geometry, Kmc::DoubleTilingDD<2>::setBlock (inc/doubleTilingDD.h:71-78)
+	const int lhx = a.lx - LBX - 1;
+	const int64_t bxi = i & ((1ll << lhx) - 1), byi = i >> lhx;
+	const int ox = (int)(((2 * bxi + (a.ds & 1)) << LBX) + a.sx);
+	const int oy = (int)(((2 * byi + (a.ds >> 1)) << LBY) + a.sy);
+	const int XW = 1 << (a.lx - 2);
+	const int mXW = XW - 1, mYW = (1 << (a.ly - 2)) - 1;
+	const int shx = RO ? (ox & 3) : 0;
+	const int shy = RO ? (oy & 3) : 0;
+	const int c0 = ox >> 2, wr0 = oy >> 2;
+	uint32_t* __restrict__ lat = a.lat;
+	uint2* __restrict__ my = tile + tid;
+
+	// ---- stage ENC_LOCAL words -> bit-plane rows
+#pragma unroll
+	for(int wk = 0; wk < NWY; ++wk)
+	{
+		const size_t rowBase = (size_t)((wr0 + wk) & mYW) * XW;
+		uint32_t cx[NWX], cy[NWX];
+#pragma unroll
+		for(int k = 0; k < NWX; ++k)
+		{
+			const uint32_t w = lat[rowBase + ((c0 + k) & mXW)];
+			cx[k] = even16(w);
+			cy[k] = even16(w >> 1);
+		}
+#pragma unroll
+		for(int rr = 0; rr < 4; ++rr)
+		{
+			const int r = 4 * wk + rr - shy;
+			if(r < 0 || r > BY)
+				continue;
+			uint32_t sxv = 0, syv = 0;
+#pragma unroll
+			for(int k = 0; k < NWX; ++k)
+			{
+				sxv |= ((cx[k] >> (4 * rr)) & 0xFu) << (4 * k);
+				syv |= ((cy[k] >> (4 * rr)) & 0xFu) << (4 * k);
+			}
+			const uint32_t X = (r < BY) ? ((sxv >> shx) & MX) : 0u;
+			my[r * DT_WG] = make_uint2(X, (syv >> shx) & MY);
+		}
+	}
+
+	// ---- stream state: block j of stream t starts j blocks of draws in
+	const bool own = a.T >= nb;
+	const int64_t t = own ? i : i % a.T;
+	const int64_t j = own ? 0 : i / a.T;
+	uint64_t s = a.st_in[t];
+	if(!own)
+		s = a.jumps[j](s);
+	const Affine step = a.step;
+	const uint32_t tp = a.tp, tq = a.tq;
+
+	// ---- BX*BY random-sequential updates (Func_MCSBase::exec,
+	// kpz/schedulerCPUDT_nested.h:46-66; pqOneUpdate kpz/schedulerCPU.inl:21-33)
+#pragma unroll 2
+	for(int k = 0; k < BX * BY; ++k)
+	{
+		s = step(s);
+		const uint32_t r1 = slcg_fold(s);
+		s = step(s);
+		const uint32_t r2 = slcg_fold(s);
+		int x, y;
+		if(RO)
+		{
+			x = (int)((r1 & 0x7FFFFFu) >> (23 - LBX));
+			y = (int)((r2 & 0x7FFFFFu) >> (23 - LBY));
+		}
+		else
+		{
+			x = (int)(r1 >> (32 - LBX));
+			y = (int)(r2 >> (32 - LBY));
+		}
+		bool accMin = true, accMax = false;
+		if(UPD == 1)
+		{
+			s = step(s);
+			const uint32_t r23 = slcg_fold(s) & 0x7FFFFFu;
+			accMin = r23 < tp;
+			accMax = r23 < tq;
+		}
+		uint2 row = my[y * DT_WG];
+		const uint32_t up = my[(y + 1) * DT_WG].y;
+		// bit0 own X, bit1 right X, bit2 own Y, bit3 up Y
+		const uint32_t cfg = ((row.x >> x) & 3u) | (((row.y >> x) & 1u) << 2) | (((up >> x) & 1u) << 3);
+		const bool flip = (cfg == 0xAu && accMin) || (cfg == 0x5u && accMax);
+		const uint32_t fm = flip ? (1u << x) : 0u;
+		row.x ^= fm | (fm << 1);
+		row.y ^= fm;
+		my[y * DT_WG] = row;
+		my[(y + 1) * DT_WG].y = up ^ fm;
+	}
+
+	if(own || i + a.T >= nb)
+		a.st_out[t] = s;
+
+	// ---- bit-plane rows -> ENC_LOCAL words (masked read-modify-write at the
+	// block edges; no other lane of this phase touches these words)
+	constexpr uint32_t SEGX = MX, SEGY = MY;
+#pragma unroll
+	for(int wk = 0; wk < NWY; ++wk)
+	{
+		const size_t rowBase = (size_t)((wr0 + wk) & mYW) * XW;
+#pragma unroll
+		for(int k = 0; k < NWX; ++k)
+		{
+			const uint32_t nibMX = ((SEGX << shx) >> (4 * k)) & 0xFu;
+			const uint32_t nibMY = ((SEGY << shx) >> (4 * k)) & 0xFu;
+			uint32_t mx16 = 0, my16 = 0, vx16 = 0, vy16 = 0;
+#pragma unroll
+			for(int rr = 0; rr < 4; ++rr)
+			{
+				const int r = 4 * wk + rr - shy;
+				if(r < 0 || r > BY)
+					continue;
+				const uint2 tv = my[r * DT_WG];
+				const uint32_t mX = (r < BY) ? nibMX : 0u;
+				vx16 |= (((tv.x << shx) >> (4 * k)) & mX) << (4 * rr);
+				mx16 |= mX << (4 * rr);
+				vy16 |= (((tv.y << shx) >> (4 * k)) & nibMY) << (4 * rr);
+				my16 |= nibMY << (4 * rr);
+			}
+			if((mx16 | my16) == 0u)
+				continue;
+			const uint32_t m32 = spread16(mx16) | (spread16(my16) << 1);
+			const uint32_t v32 = spread16(vx16) | (spread16(vy16) << 1);
+			const size_t idx = rowBase + ((c0 + k) & mXW);
+			lat[idx] = (m32 == 0xFFFFFFFFu) ? v32 : ((lat[idx] & ~m32) | v32);
+		}
+	}
+}
+
+// ======================================================================
+// SCABit half-sweep (ENC_CBBIT)
+// ======================================================================
+template<typename W, int MODE>
+__global__ __launch_bounds__(256) void cbbit_half_kernel(CbHalfArgs a)
+{
+	constexpr int WB = sizeof(W) * 8;
+	constexpr int LW = (WB == 64) ? 7 : 6; // log2 sites per word (two sublattices)
+	const int lG = a.lG, G = 1 << lG;
+	const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const int Y = 1 << a.ly;
+	const int y = (int)(gid >> lG);
+	const int g = (int)(gid & (G - 1));
+	W* __restrict__ lat = (W*)a.lat;
+	const int Wn = 1 << (a.lx - LW); // words per row and quadrant
+	const int M = Wn >> lG;
+	const int64_t Q = (int64_t)1 << (a.lx + a.ly - LW);
+
+	// worker t owns rows [yMin_t, ySup_t) (kpz/schedulerCPUCBBit.cpp:46-62)
+	const int big = a.rem * (a.per + 1);
+	int t, r;
+	if(y < big)
+	{
+		t = y / (a.per + 1);
+		r = y - t * (a.per + 1);
+	}
+	else
+	{
+		const int yy = y - big;
+		t = a.rem + yy / a.per;
+		r = yy - (t - a.rem) * a.per;
+	}
+	const uint64_t s0 = a.st_in[t];
+	uint64_t s = a.laneJ[g](a.rowJ[r](s0));
+	if(r == 0 && g == 0)
+		a.st_out[t] = (t < a.rem ? a.fullBig : a.fullSmall)(s0);
+
+	const bool par = (a.signum == (y & 1));
+	const int64_t thisB = a.signum ? 2 * Q : 0, othB = a.signum ? 0 : 2 * Q;
+	const int64_t row = (int64_t)y * Wn, rowUp = (int64_t)((y + 1) & (Y - 1)) * Wn;
+	const Affine step = a.step, stepJ = a.stepJ;
+	const uint32_t tp = a.tp, tq = a.tq;
+	const int gUp = (g + 1) & (G - 1), gDn = (g - 1) & (G - 1);
+
+	W pending = 0, x0old = 0, f0 = 0;
+	for(int m = 0; m < M; ++m)
+	{
+		const int xw = g + (m << lG);
+		const W tX = lat[thisB + row + xw];
+		const W tY = lat[thisB + Q + row + xw];
+		const W xX = lat[othB + row + xw];
+		const W yY = lat[othB + Q + rowUp + xw];
+		// LSB source for the shifted neighbour slope (pSyncRngUpdateVL3,
+		// kpz/schedulerCPUCBBit_nested.inl:226-233): lane g+1, or for the last
+		// lane the first word of the next stride (wrapping to word 0)
+		W nbr = shflw(xX, gUp, G);
+		if(g == G - 1)
+			nbr = lat[othB + row + ((xw + 1) & (Wn - 1))];
+		const W xXs = par ? xX : (W)((xX >> 1) | (nbr << (WB - 1)));
+		const W minm = ~(tX | tY) & xXs & yY;
+		W flip;
+		if(MODE == 0)
+		{
+			W rnd;
+			if(WB == 64)
+			{
+				s = step(s);
+				const uint64_t hi = slcg_fold(s);
+				s = step(s);
+				const uint64_t lo = slcg_fold(s);
+				rnd = (W)((hi << 32) ^ lo);
+			}
+			else
+			{
+				s = step(s);
+				rnd = (W)slcg_fold(s);
+			}
+			flip = minm & rnd;
+		}
+		else
+		{
+			const W maxm = tX & tY & ~xXs & ~yY;
+			W pb = 0, qb = 0;
+#pragma unroll 8
+			for(int k = 0; k < WB; ++k)
+			{
+				s = step(s);
+				const uint32_t r23 = slcg_fold(s) & 0x7FFFFFu;
+				pb |= (W)(r23 < tp) << k;
+				qb |= (W)(r23 < tq) << k;
+			}
+			flip = (minm & pb) | (maxm & qb);
+		}
+		s = stepJ(s);
+		lat[thisB + row + xw] = tX ^ flip;
+		lat[thisB + Q + row + xw] = tY ^ flip;
+		lat[othB + Q + rowUp + xw] = yY ^ flip;
+		// carry of the shifted flip into the next word's LSB
+		const W recv = shflw((W)(flip >> (WB - 1)), gDn, G);
+		W cin;
+		if(g == 0)
+		{
+			cin = pending;
+			pending = recv;
+		}
+		else
+			cin = recv;
+		if(par)
+			lat[othB + row + xw] = xX ^ flip;
+		else if(g == 0 && m == 0)
+		{
+			x0old = xX;
+			f0 = flip;
+		}
+		else
+			lat[othB + row + xw] = xX ^ (W)(flip << 1) ^ cin;
+	}
+	if(!par && g == 0)
+		lat[othB + row] = x0old ^ (W)(f0 << 1) ^ pending;
+}
+
+// ======================================================================
+// row statistics (roughness / correlator)
+// ======================================================================
+// x-slope bits of sites x = 64j .. 64j+63 of row y (bit i <-> x = 64j+i)
+template<int ENC>
+__device__ __forceinline__ uint64_t xbits(const uint32_t* lat, int lx, int ly, int j, int y)
+{
+	if(ENC == ENC_LOCAL)
+	{
+		const int nw = (lx >= 6) ? 16 : (1 << (lx - 2));
+		const size_t base = ((size_t)(y >> 2) << (lx - 2)) + (size_t)j * 16;
+		const int sh = (y & 3) << 3;
+		uint64_t b = 0;
+		for(int k = 0; k < nw; ++k)
+		{
+			const uint32_t byte = (lat[base + k] >> sh) & 0xFFu;
+			b |= (uint64_t)(even16(byte) & 0xFu) << (4 * k);
+		}
+		return b;
+	}
+	else
+	{
+		const size_t sw = (size_t)1 << (lx + ly - 4);
+		const size_t w = ((size_t)y << (lx - 6)) + j;
+		const uint32_t e = lat[w + ((y & 1) ? (sw >> 1) : 0)];       // even x: sub = y&1
+		const uint32_t o = lat[w + ((y & 1) ? 0 : (sw >> 1))];       // odd x
+		const uint64_t se = (uint64_t)spread16(e & 0xFFFFu) | ((uint64_t)spread16(e >> 16) << 32);
+		const uint64_t so = (uint64_t)spread16(o & 0xFFFFu) | ((uint64_t)spread16(o >> 16) << 32);
+		return se | (so << 1);
+	}
+}
+
+// matching slope bits (x and y) of the chunk's sites: count of equal bits
+template<int ENC>
+__device__ __forceinline__ int eqbits(const uint32_t* a, const uint32_t* b, int lx, int ly, int j, int y)
+{
+	if(ENC == ENC_LOCAL)
+	{
+		const int nw = (lx >= 6) ? 16 : (1 << (lx - 2));
+		const size_t base = ((size_t)(y >> 2) << (lx - 2)) + (size_t)j * 16;
+		const uint32_t m = 0xFFu << ((y & 3) << 3);
+		int e = 0;
+		for(int k = 0; k < nw; ++k)
+			e += __popc(~(a[base + k] ^ b[base + k]) & m);
+		return e;
+	}
+	else
+	{
+		const size_t q = (size_t)1 << (lx + ly - 6);
+		const size_t w = ((size_t)y << (lx - 6)) + j;
+		int e = 0;
+		for(int k = 0; k < 4; ++k)
+			e += __popc(~(a[w + k * q] ^ b[w + k * q]));
+		return e;
+	}
+}
+
+template<int ENC>
+__device__ __forceinline__ int ybit0(const uint32_t* lat, int lx, int ly, int y)
+{
+	if(ENC == ENC_LOCAL)
+		return (lat[(size_t)(y >> 2) << (lx - 2)] >> (((y & 3) << 3) | 1)) & 1;
+	const size_t sw = (size_t)1 << (lx + ly - 4);
+	return lat[(sw >> 2) + ((y & 1) ? (sw >> 1) : 0) + ((size_t)y << (lx - 6))] & 1;
+}
+
+// local prefix statistics of one chunk: q = sum d, sq = sum Q_i, sq2 = sum Q_i^2
+// over the n counted sites (site x = 0 is excluded from the row prefix)
+__device__ __forceinline__ void chunk_prefix(uint64_t bits, int nbits, bool skip0, int& q, int64_t& sq, int64_t& sq2)
+{
+	int qq = 0, a1 = 0, a2 = 0;
+	for(int k = skip0 ? 1 : 0; k < nbits; ++k)
+	{
+		qq += ((bits >> k) & 1) ? 1 : -1;
+		a1 += qq;
+		a2 += qq * qq;
+	}
+	q = qq;
+	sq = a1;
+	sq2 = a2;
+}
+
+__device__ __forceinline__ void chunk_prefix2(uint64_t b1, uint64_t b2, int nbits, bool skip0,
+		int& q1, int64_t& s1, int64_t& s11, int& q2, int64_t& s2, int64_t& s22, int64_t& s12)
+{
+	int a = 0, b = 0, sa = 0, saa = 0, sb = 0, sbb = 0, sab = 0;
+	for(int k = skip0 ? 1 : 0; k < nbits; ++k)
+	{
+		a += ((b1 >> k) & 1) ? 1 : -1;
+		b += ((b2 >> k) & 1) ? 1 : -1;
+		sa += a;
+		saa += a * a;
+		sb += b;
+		sbb += b * b;
+		sab += a * b;
+	}
+	q1 = a;
+	s1 = sa;
+	s11 = saa;
+	q2 = b;
+	s2 = sb;
+	s22 = sbb;
+	s12 = sab;
+}
+
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t v, int lane)
+{
+#pragma unroll
+	for(int o = 1; o < 64; o <<= 1)
+	{
+		const int64_t u = (int64_t)shfl64((uint64_t)v, lane >= o ? lane - o : lane, 64);
+		if(lane >= o)
+			v += u;
+	}
+	return v;
+}
+
+__device__ __forceinline__ int64_t wave_sum(int64_t v)
+{
+#pragma unroll
+	for(int o = 32; o > 0; o >>= 1)
+		v += (int64_t)shfl64((uint64_t)v, (threadIdx.x & 63) ^ o, 64);
+	return v;
+}
+
+template<int ENC, bool CORR>
+__global__ __launch_bounds__(256) void row_stats_kernel(const uint32_t* __restrict__ lat, const uint32_t* __restrict__ snap,
+		int lx, int ly, int64_t* __restrict__ out)
+{
+	const int lane = threadIdx.x & 63;
+	const int y = blockIdx.x * 4 + (threadIdx.x >> 6);
+	if(y >= (1 << ly))
+		return; // uniform per wave
+	const int nbits = (lx >= 6) ? 64 : (1 << lx);
+	const int nch = (lx >= 6) ? (1 << (lx - 6)) : 1;
+	int64_t S1 = 0, S2 = 0, T1 = 0, T2 = 0, S12 = 0, tcs = 0;
+	int64_t run = 0, runS = 0;
+	int d0 = 0, d0s = 0;
+	for(int base = 0; base < nch; base += 64)
+	{
+		const int j = base + lane;
+		const bool valid = j < nch;
+		int q = 0, qs = 0;
+		int64_t sq = 0, sq2 = 0, ss = 0, ss2 = 0, s12 = 0;
+		int n = 0;
+		uint64_t b = 0, bs = 0;
+		if(valid)
+		{
+			b = xbits<ENC>(lat, lx, ly, j, y);
+			n = (j == 0) ? nbits - 1 : nbits;
+			if(CORR)
+			{
+				bs = xbits<ENC>(snap, lx, ly, j, y);
+				chunk_prefix2(b, bs, nbits, j == 0, q, sq, sq2, qs, ss, ss2, s12);
+				tcs += 2 * eqbits<ENC>(lat, snap, lx, ly, j, y) - 2 * nbits;
+			}
+			else
+				chunk_prefix(b, nbits, j == 0, q, sq, sq2);
+			if(j == 0)
+			{
+				d0 = (b & 1) ? 1 : -1;
+				d0s = (bs & 1) ? 1 : -1;
+			}
+		}
+		const int64_t incl = wave_incl_scan(q, lane);
+		const int64_t c = run + incl - q;
+		S1 += n * c + sq;
+		S2 += n * c * c + 2 * c * sq + sq2;
+		run += (int64_t)shfl64((uint64_t)incl, 63, 64);
+		if(CORR)
+		{
+			const int64_t incs = wave_incl_scan(qs, lane);
+			const int64_t cs = runS + incs - qs;
+			T1 += n * cs + ss;
+			T2 += n * cs * cs + 2 * cs * ss + ss2;
+			S12 += n * c * cs + c * ss + cs * sq + s12;
+			runS += (int64_t)shfl64((uint64_t)incs, 63, 64);
+		}
+	}
+	S1 = wave_sum(S1);
+	S2 = wave_sum(S2);
+	if(CORR)
+	{
+		T1 = wave_sum(T1);
+		T2 = wave_sum(T2);
+		S12 = wave_sum(S12);
+		tcs = wave_sum(tcs);
+	}
+	d0 = __shfl(d0, 0, 64);
+	d0s = __shfl(d0s, 0, 64);
+	if(lane == 0)
+	{
+		// site x = 0 sits at the end of the row walk: P_0 = D (all x slopes)
+		const int64_t D = run + d0;
+		int64_t* o = out + (size_t)y * ROW_STATS;
+		o[0] = S1 + D;
+		o[1] = S2 + D * D;
+		o[2] = D;
+		o[3] = ybit0<ENC>(lat, lx, ly, y);
+		if(CORR)
+		{
+			const int64_t Ds = runS + d0s;
+			o[4] = T1 + Ds;
+			o[5] = Ds;
+			o[6] = ybit0<ENC>(snap, lx, ly, y);
+			o[7] = S12 + D * Ds;
+			o[8] = tcs;
+			o[9] = T2 + Ds * Ds;
+		}
+		else
+		{
+			o[4] = o[5] = o[6] = o[7] = o[8] = o[9] = 0;
+		}
+	}
+}
+
+// ---------------------------------------------------------------- fill
+__global__ void fill_kernel(uint32_t* lat, uint64_t n, uint64_t split, uint32_t lo, uint32_t hi)
+{
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for(uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+		lat[k] = k < split ? lo : hi;
+}
+
+template<int LBX, int LBY>
+hipError_t dt_dispatch_upd(const DtPhaseArgs& a, int64_t nb, dim3 grid, hipStream_t stream)
+{
+	if(a.upd == 0)
+	{
+		if(a.random_origin)
+			hipLaunchKernelGGL((dt_phase_kernel<LBX, LBY, 0, true>), grid, dim3(DT_WG), 0, stream, a, nb);
+		else
+			hipLaunchKernelGGL((dt_phase_kernel<LBX, LBY, 0, false>), grid, dim3(DT_WG), 0, stream, a, nb);
+	}
+	else
+	{
+		if(a.random_origin)
+			hipLaunchKernelGGL((dt_phase_kernel<LBX, LBY, 1, true>), grid, dim3(DT_WG), 0, stream, a, nb);
+		else
+			hipLaunchKernelGGL((dt_phase_kernel<LBX, LBY, 1, false>), grid, dim3(DT_WG), 0, stream, a, nb);
+	}
+	return hipGetLastError();
+}
+
+template<int LBX>
+hipError_t dt_dispatch_y(const DtPhaseArgs& a, int64_t nb, dim3 grid, hipStream_t stream)
+{
+	switch(a.lby)
+	{
+		case 2: return dt_dispatch_upd<LBX, 2>(a, nb, grid, stream);
+		case 3: return dt_dispatch_upd<LBX, 3>(a, nb, grid, stream);
+		case 4: return dt_dispatch_upd<LBX, 4>(a, nb, grid, stream);
+		default: return hipErrorInvalidValue;
+	}
+}
+
+} // namespace
+
+bool dt_layout_supported(int lbx, int lby) { return lbx >= 2 && lbx <= 4 && lby >= 2 && lby <= 4; }
+
+hipError_t launch_dt_phase(const DtPhaseArgs& a, hipStream_t stream)
+{
+	if(!dt_layout_supported(a.lbx, a.lby))
+		return hipErrorInvalidValue;
+	const int64_t nb = 1ll << (a.lx - a.lbx + a.ly - a.lby - 2);
+	const dim3 grid((unsigned)((nb + DT_WG - 1) / DT_WG));
+	switch(a.lbx)
+	{
+		case 2: return dt_dispatch_y<2>(a, nb, grid, stream);
+		case 3: return dt_dispatch_y<3>(a, nb, grid, stream);
+		case 4: return dt_dispatch_y<4>(a, nb, grid, stream);
+		default: return hipErrorInvalidValue;
+	}
+}
+
+hipError_t launch_cbbit_half(const CbHalfArgs& a, hipStream_t stream)
+{
+	const int64_t threads = (int64_t)1 << (a.ly + a.lG);
+	const int block = (int)(threads < 256 ? threads : 256);
+	const dim3 grid((unsigned)(threads / block));
+	const bool wide = a.lx >= 7;
+	if(a.mode == 0)
+	{
+		if(wide)
+			hipLaunchKernelGGL((cbbit_half_kernel<uint64_t, 0>), grid, dim3(block), 0, stream, a);
+		else
+			hipLaunchKernelGGL((cbbit_half_kernel<uint32_t, 0>), grid, dim3(block), 0, stream, a);
+	}
+	else
+	{
+		if(wide)
+			hipLaunchKernelGGL((cbbit_half_kernel<uint64_t, 1>), grid, dim3(block), 0, stream, a);
+		else
+			hipLaunchKernelGGL((cbbit_half_kernel<uint32_t, 1>), grid, dim3(block), 0, stream, a);
+	}
+	return hipGetLastError();
+}
+
+hipError_t launch_row_stats(const uint32_t* lat, const uint32_t* snap, int enc, int lx, int ly,
+		int64_t* out, hipStream_t stream)
+{
+	const dim3 grid((unsigned)(((1 << ly) + 3) / 4));
+	if(enc == ENC_LOCAL)
+	{
+		if(snap)
+			hipLaunchKernelGGL((row_stats_kernel<ENC_LOCAL, true>), grid, dim3(256), 0, stream, lat, snap, lx, ly, out);
+		else
+			hipLaunchKernelGGL((row_stats_kernel<ENC_LOCAL, false>), grid, dim3(256), 0, stream, lat, snap, lx, ly, out);
+	}
+	else
+	{
+		if(snap)
+			hipLaunchKernelGGL((row_stats_kernel<ENC_CBBIT, true>), grid, dim3(256), 0, stream, lat, snap, lx, ly, out);
+		else
+			hipLaunchKernelGGL((row_stats_kernel<ENC_CBBIT, false>), grid, dim3(256), 0, stream, lat, snap, lx, ly, out);
+	}
+	return hipGetLastError();
+}
+
+hipError_t launch_fill_flat(uint32_t* lat, int enc, int lx, int ly, hipStream_t stream)
+{
+	const uint64_t n = 1ull << (lx + ly - 4);
+	const unsigned blocks = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+	if(enc == ENC_LOCAL)
+		hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, stream, lat, n, n, 0x33CC33CCu, 0x33CC33CCu);
+	else
+		hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, stream, lat, n, n >> 1, 0u, 0xFFFFFFFFu);
+	return hipGetLastError();
+}
+
+} // namespace kpz

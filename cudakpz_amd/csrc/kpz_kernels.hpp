@@ -1,0 +1,60 @@
+// Launchers of the gfx950 kernels (kpz_kernels.hip).  All pointers are
+// device pointers; every launcher enqueues on `stream` and returns the
+// hipError_t of the launch.
+#pragma once
+
+#include "kpz_common.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace kpz {
+
+// ---- DT: one double-tiling domain set (phase) --------------------------
+struct DtPhaseArgs {
+	uint32_t* lat;          // ENC_LOCAL lattice
+	int lx, ly, lbx, lby;   // log2 system and block sizes
+	const uint64_t* st_in;  // stream states (T)
+	uint64_t* st_out;       // == st_in when every stream owns <= 1 block
+	int64_t T;              // streams (CPU worker count of the counterpart)
+	const Affine* jumps;    // jumps[j]: advance to the j-th block of a stream
+	Affine step;            // per-draw SLCG step (skip form)
+	int ds;                 // domain set 0..3
+	int sx, sy;             // DT origin shift (0 for the static DT)
+	int upd;                // 0 = KPZ (pqOneUpdate), 1 = pq (pqSyncRngUpdate)
+	bool random_origin;     // DTr coordinate/mask convention
+	uint32_t tp, tq;        // 23-bit thresholds
+};
+hipError_t launch_dt_phase(const DtPhaseArgs& a, hipStream_t stream);
+bool dt_layout_supported(int lbx, int lby);
+
+// ---- SCABit: one checkerboard half-sweep on ENC_CBBIT ------------------
+struct CbHalfArgs {
+	uint32_t* lat;
+	int lx, ly, signum;
+	int mode;               // 0 = bit-parallel (VL3), 1 = per-site pq
+	const uint64_t* st_in;
+	uint64_t* st_out;
+	int64_t T;
+	int per, rem;           // rows per worker, workers with one extra row
+	const Affine* rowJ;     // rowJ[r]: skip r rows of draws
+	const Affine* laneJ;    // laneJ[g]: skip g words of draws
+	Affine stepJ;           // skip from a lane's word to its next word
+	Affine step;            // per-draw step
+	Affine fullBig, fullSmall; // whole half-sweep of a worker (per+1 / per rows)
+	uint32_t tp, tq;
+	int lG;                 // log2 lanes per row
+};
+hipError_t launch_cbbit_half(const CbHalfArgs& a, hipStream_t stream);
+
+// ---- reductions ----------------------------------------------------------
+// Per site row y, relative to the row's start height (see kpz_kernels.hip):
+// out[y*ROW_STATS + k] = {S1, S2, D, sy0, S1', D', sy0', S11, tcs, 0}, the
+// primed values from `snap` (zero when snap is null).
+constexpr int ROW_STATS = 10;
+hipError_t launch_row_stats(const uint32_t* lat, const uint32_t* snap, int enc, int lx, int ly,
+		int64_t* out, hipStream_t stream);
+
+// ---- utilities -------------------------------------------------------------
+hipError_t launch_fill_flat(uint32_t* lat, int enc, int lx, int ly, hipStream_t stream);
+
+} // namespace kpz

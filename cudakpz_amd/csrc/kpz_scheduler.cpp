@@ -1,0 +1,397 @@
+#include "kpz_scheduler.hpp"
+
+#include <cstring>
+
+namespace kpz {
+
+void check(hipError_t e, const char* what)
+{
+	if(e != hipSuccess)
+		throw Error(-2, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+DevBuf::~DevBuf()
+{
+	if(p_)
+		(void)hipFree(p_);
+}
+
+void DevBuf::alloc(size_t bytes)
+{
+	if(p_)
+	{
+		(void)hipFree(p_);
+		p_ = nullptr;
+		n_ = 0;
+	}
+	if(bytes == 0)
+		return;
+	const hipError_t e = hipMalloc(&p_, bytes);
+	if(e != hipSuccess)
+	{
+		p_ = nullptr;
+		throw Error(-3, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+	}
+	n_ = bytes;
+}
+
+// ======================================================================
+SchedulerService::SchedulerService(int lx, int ly, Encoding enc, const Params& p)
+	: size_{lx, ly}, enc_(enc), T_(p.threads), S_(p.skip), device_(p.device)
+{
+	if(lx < 2 || ly < 2 || lx > 30 || ly > 30)
+		throw Error(-1, "system size out of range (log2 sizes 2..30)");
+	if(device_ >= 0)
+		check(hipSetDevice(device_), "hipSetDevice");
+	else
+		check(hipGetDevice(&device_), "hipGetDevice");
+	check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+	lat_.alloc(size_.sizeW() * sizeof(uint32_t));
+	rows_.alloc((size_t)size_.dimY() * ROW_STATS * sizeof(int64_t));
+	hostRows_.resize((size_t)size_.dimY() * ROW_STATS);
+	master_.reseed(0);
+}
+
+SchedulerService::~SchedulerService()
+{
+	if(stream_)
+	{
+		(void)hipStreamSynchronize(stream_);
+		(void)hipStreamDestroy(stream_);
+	}
+}
+
+void SchedulerService::initHomogeneous()
+{
+	check(launch_fill_flat(deviceLattice(), enc_, size_.lx, size_.ly, stream_), "fill_flat");
+}
+
+void SchedulerService::supplySeed(uint64_t seed)
+{
+	master_.reseed(seed32(seed));
+	randomize();
+}
+
+// SLCG64::randomize, kpz/GPURandom.cu:42-54 (first master draw = low half)
+void SchedulerService::randomize()
+{
+	const double m1 = master_.close_open();
+	const double m2 = master_.close_open();
+	uint64_t s = (uint64_t)(m1 * 4294967295.0) ^ ((uint64_t)(m2 * 4294967295.0) << 32);
+	std::vector<uint64_t> h((size_t)T_);
+	for(int64_t t = 0; t < T_; ++t)
+	{
+		h[(size_t)t] = s;
+		s = s * SLCG_A + SLCG_C;
+	}
+	setStreams(h.data());
+}
+
+bool SchedulerService::setPQ(double p, double q)
+{
+	if(!(p >= 0.0 && p <= 1.0 && q >= 0.0 && q <= 1.0))
+		return false;
+	p_ = p;
+	q_ = q;
+	return true;
+}
+
+void SchedulerService::pushSystem(const uint32_t* host)
+{
+	check(hipMemcpyAsync(deviceLattice(), host, size_.sizeW() * 4, hipMemcpyHostToDevice, stream_), "pushSystem");
+	sync();
+}
+
+void SchedulerService::popSystem(uint32_t* host)
+{
+	check(hipMemcpyAsync(host, deviceLattice(), size_.sizeW() * 4, hipMemcpyDeviceToHost, stream_), "popSystem");
+	sync();
+}
+
+void SchedulerService::sync() { check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+void SchedulerService::getStreams(uint64_t* out)
+{
+	check(hipMemcpyAsync(out, stIn(), (size_t)T_ * 8, hipMemcpyDeviceToHost, stream_), "getStreams");
+	sync();
+}
+
+void SchedulerService::setStreams(const uint64_t* in)
+{
+	check(hipMemcpyAsync(stIn(), in, (size_t)T_ * 8, hipMemcpyHostToDevice, stream_), "setStreams");
+	sync();
+}
+
+const Affine* SchedulerService::upload(std::vector<Affine>& host, DevBuf& buf)
+{
+	buf.alloc(host.size() * sizeof(Affine));
+	check(hipMemcpyAsync(buf.get(), host.data(), host.size() * sizeof(Affine), hipMemcpyHostToDevice, stream_), "upload");
+	sync();
+	return buf.as<Affine>();
+}
+
+// exact combination of per-row partial sums relative to each row's start
+// height; heights follow Correlator::set (kpz/correlator.cpp:37-106)
+static void combine_rows(const std::vector<int64_t>& rows, int ly, int lx, bool corr, Sums& s)
+{
+	const int64_t L = 1ll << lx;
+	int64_t prev = 0, prevS = 0;
+	__int128 s1 = 0, s2 = 0, t1 = 0, s12 = 0;
+	int64_t tcs = 0;
+	for(int y = 0; y < (1 << ly); ++y)
+	{
+		const int64_t* o = &rows[(size_t)y * ROW_STATS];
+		const int64_t hs = prev + (o[3] ? 1 : -1);
+		s1 += (__int128)L * hs + o[0];
+		s2 += (__int128)L * hs * hs + (__int128)2 * hs * o[0] + o[1];
+		prev = hs + o[2];
+		if(corr)
+		{
+			const int64_t hss = prevS + (o[6] ? 1 : -1);
+			t1 += (__int128)L * hss + o[4];
+			s12 += (__int128)L * hs * hss + (__int128)hs * o[4] + (__int128)hss * o[0] + o[7];
+			prevS = hss + o[5];
+			tcs += o[8];
+		}
+	}
+	s.sumH = (int64_t)s1;
+	s.sumH2 = s2;
+	s.sumHs = (int64_t)t1;
+	s.sumHHs = s12;
+	s.tcs = tcs;
+}
+
+Sums SchedulerService::roughness()
+{
+	check(launch_row_stats(deviceLattice(), nullptr, enc_, size_.lx, size_.ly, rows_.as<int64_t>(), stream_), "row_stats");
+	check(hipMemcpyAsync(hostRows_.data(), rows_.get(), hostRows_.size() * 8, hipMemcpyDeviceToHost, stream_), "row_stats D2H");
+	sync();
+	Sums s;
+	combine_rows(hostRows_, size_.ly, size_.lx, false, s);
+	return s;
+}
+
+void SchedulerService::correlateTag()
+{
+	if(!snap_.get())
+		snap_.alloc(size_.sizeW() * 4);
+	check(hipMemcpyAsync(snap_.get(), deviceLattice(), size_.sizeW() * 4, hipMemcpyDeviceToDevice, stream_), "snapshot");
+}
+
+Sums SchedulerService::correlate()
+{
+	if(!snap_.get())
+		throw Error(-4, "correlate() needs a correlateTag() snapshot");
+	check(launch_row_stats(deviceLattice(), snap_.as<uint32_t>(), enc_, size_.lx, size_.ly, rows_.as<int64_t>(), stream_), "row_stats");
+	check(hipMemcpyAsync(hostRows_.data(), rows_.get(), hostRows_.size() * 8, hipMemcpyDeviceToHost, stream_), "row_stats D2H");
+	sync();
+	Sums s;
+	combine_rows(hostRows_, size_.ly, size_.lx, true, s);
+	return s;
+}
+
+// ======================================================================
+SchedulerDT::SchedulerDT(int lx, int ly, const Params& p, bool randomOrigin)
+	: SchedulerService(lx, ly, ENC_LOCAL, p), lbx_(p.layout_x), lby_(p.layout_y), ro_(randomOrigin)
+{
+	if(!dt_layout_supported(lbx_, lby_))
+		throw Error(-1, "DT block layout must be 2..4 x 2..4 (log2 sites)");
+	if(lx - lbx_ < 1 || ly - lby_ < 1)
+		throw Error(-1, "DT needs an even number (>= 2) of blocks per dimension");
+	nb_ = 1ll << (lx - lbx_ + ly - lby_ - 2);
+	if(T_ <= 0)
+		T_ = nb_;
+	if(S_ == 0)
+		S_ = default_skip((uint64_t)T_);
+	step_ = slcg_skip(S_);
+	st_[0].alloc((size_t)T_ * 8);
+	if(T_ < nb_)
+		st_[1].alloc((size_t)T_ * 8);
+	p_ = 1.0;
+	q_ = 0.0;
+}
+
+const Affine* SchedulerDT::jumps(int du)
+{
+	if(T_ >= nb_)
+		return nullptr;
+	auto it = jumpBuf_.find(du);
+	if(it != jumpBuf_.end())
+		return it->second->as<Affine>();
+	const int64_t nj = (nb_ + T_ - 1) / T_;
+	const Affine blk = affine_pow(step_, (uint64_t)du << (lbx_ + lby_));
+	std::vector<Affine> h((size_t)nj);
+	Affine acc{1, 0};
+	for(int64_t j = 0; j < nj; ++j)
+	{
+		h[(size_t)j] = acc;
+		acc = compose(blk, acc);
+	}
+	auto buf = std::make_unique<DevBuf>();
+	const Affine* d = upload(h, *buf);
+	jumpBuf_[du] = std::move(buf);
+	return d;
+}
+
+void SchedulerDT::run(int n, int upd)
+{
+	const int du = upd ? 3 : 2;
+	DtPhaseArgs a{};
+	a.lat = deviceLattice();
+	a.lx = size_.lx;
+	a.ly = size_.ly;
+	a.lbx = lbx_;
+	a.lby = lby_;
+	a.T = T_;
+	a.jumps = jumps(du);
+	a.step = step_;
+	a.upd = upd;
+	a.random_origin = ro_;
+	a.tp = thresh23(upd ? p_ : 1.0);
+	a.tq = thresh23(upd ? q_ : 0.0);
+	for(int m = 0; m < n; ++m)
+	{
+		// SchedulerCPUDT::mcs (kpz/schedulerCPUDT.cpp:65-90) / SchedulerCPUDTr::doMcs
+		// (kpz/schedulerCPUDTr.cpp:66-98): domain order, then the origin shift
+		int seq[4];
+		master_.shuffle4(seq);
+		a.sx = a.sy = 0;
+		if(ro_)
+		{
+			a.sx = (int)(unsigned)(master_.close_open() * (double)(1 << lbx_));
+			a.sy = (int)(unsigned)(master_.close_open() * (double)(1 << lby_));
+		}
+		for(int d = 0; d < 4; ++d)
+		{
+			a.ds = seq[d];
+			a.st_in = stIn();
+			a.st_out = (T_ < nb_) ? stOut() : stIn();
+			check(launch_dt_phase(a, stream_), "dt_phase");
+			if(T_ < nb_)
+				swapStreams();
+		}
+		++mcs_;
+	}
+}
+
+void SchedulerDT::mcs(int n) { run(n, 0); }
+void SchedulerDT::mcsPQSyncRng(int n) { run(n, 1); }
+
+// ======================================================================
+SchedulerSCABit::SchedulerSCABit(int lx, int ly, const Params& p)
+	: SchedulerService(lx, ly, ENC_CBBIT, p)
+{
+	if(lx < 6 || ly < 1)
+		throw Error(-1, "ENC_CBBIT needs log2 width >= 6");
+	const int Y = 1 << ly;
+	if(T_ <= 0)
+		T_ = Y;
+	if(T_ > Y)
+		throw Error(-1, "SCABit: more workers than rows");
+	if(S_ == 0)
+		S_ = default_skip((uint64_t)T_);
+	step_ = slcg_skip(S_);
+	per_ = (int)(Y / T_);
+	rem_ = (int)(Y % T_);
+	const int lw = (lx >= 7) ? 7 : 6;
+	const int lWn = lx - lw;
+	lG_ = lWn < 6 ? lWn : 6;
+	st_[0].alloc((size_t)T_ * 8);
+	st_[1].alloc((size_t)T_ * 8);
+	p_ = 0.5;
+	q_ = 0.0;
+}
+
+SchedulerSCABit::Tables& SchedulerSCABit::tables(int mode)
+{
+	auto it = tab_.find(mode);
+	if(it != tab_.end())
+		return *it->second;
+	const bool wide = size_.lx >= 7;
+	const int wb = wide ? 64 : 32;
+	const uint64_t du = mode == 0 ? (wide ? 2 : 1) : (uint64_t)wb; // draws per word
+	const uint64_t Wn = 1ull << (size_.lx - (wide ? 7 : 6));
+	const int G = 1 << lG_;
+	auto t = std::make_unique<Tables>();
+	std::vector<Affine> row((size_t)per_ + 1), lane((size_t)G);
+	const Affine rowStep = affine_pow(step_, du * Wn);
+	Affine acc{1, 0};
+	for(size_t r = 0; r < row.size(); ++r)
+	{
+		row[r] = acc;
+		acc = compose(rowStep, acc);
+	}
+	const Affine wordStep = affine_pow(step_, du);
+	acc = Affine{1, 0};
+	for(int g = 0; g < G; ++g)
+	{
+		lane[(size_t)g] = acc;
+		acc = compose(wordStep, acc);
+	}
+	upload(row, t->row);
+	upload(lane, t->lane);
+	t->stepJ = affine_pow(step_, du * (uint64_t)(G - 1));
+	t->fullBig = affine_pow(rowStep, (uint64_t)per_ + 1);
+	t->fullSmall = affine_pow(rowStep, (uint64_t)per_);
+	Tables& ref = *t;
+	tab_[mode] = std::move(t);
+	return ref;
+}
+
+void SchedulerSCABit::run(int n, int mode)
+{
+	Tables& tb = tables(mode);
+	CbHalfArgs a{};
+	a.lat = deviceLattice();
+	a.lx = size_.lx;
+	a.ly = size_.ly;
+	a.mode = mode;
+	a.T = T_;
+	a.per = per_;
+	a.rem = rem_;
+	a.rowJ = tb.row.as<Affine>();
+	a.laneJ = tb.lane.as<Affine>();
+	a.stepJ = tb.stepJ;
+	a.step = step_;
+	a.fullBig = tb.fullBig;
+	a.fullSmall = tb.fullSmall;
+	a.tp = thresh23(p_);
+	a.tq = thresh23(q_);
+	a.lG = lG_;
+	for(int m = 0; m < n; ++m)
+	{
+		// SchedulerCPUCBBit::doMcs (kpz/schedulerCPUCBBit.cpp:64-78)
+		for(int s = 0; s < 2; ++s)
+		{
+			a.signum = s;
+			a.st_in = stIn();
+			a.st_out = stOut();
+			check(launch_cbbit_half(a, stream_), "cbbit_half");
+			swapStreams();
+		}
+		++mcs_;
+	}
+}
+
+// SchedulerSCABit::mcs -> mcsSyncRng sets p=.5, q=0 (kpz/schedulerSCABit.h:61-67)
+void SchedulerSCABit::mcs(int n)
+{
+	setPQ(0.5, 0.0);
+	run(n, 0);
+}
+
+// SchedulerCPUCBBit::mcsPQSyncRng (kpz/schedulerCPUCBBit.cpp:95-102)
+void SchedulerSCABit::mcsPQSyncRng(int n) { run(n, (q_ == 0.0 && p_ == 0.5) ? 0 : 1); }
+
+std::unique_ptr<SchedulerService> makeScheduler(int kind, int lx, int ly, const Params& p)
+{
+	switch(kind)
+	{
+		case KIND_DT: return std::make_unique<SchedulerDT>(lx, ly, p, false);
+		case KIND_DTR: return std::make_unique<SchedulerDT>(lx, ly, p, true);
+		case KIND_SCABIT: return std::make_unique<SchedulerSCABit>(lx, ly, p);
+		default: throw Error(-1, "unknown scheduler kind");
+	}
+}
+
+} // namespace kpz

@@ -1,0 +1,175 @@
+// C++ host side of the engine: SchedulerService-style classes that own a
+// device lattice, the per-worker SLCG streams and the master dSFMT, and
+// enqueue the gfx950 kernels.  Names and semantics follow
+// kpz/schedulerService.h:73-255 (mcs, mcsPQSyncRng, setPQ, initHomogeneous,
+// pushSystem/popSystem, roughness, correlateTag/correlate, mcsScale).
+#pragma once
+
+#include "kpz_common.hpp"
+#include "kpz_kernels.hpp"
+#include "kpz_master.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace kpz {
+
+struct Error : std::runtime_error {
+	int code;
+	Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+enum Kind { KIND_DT = 0, KIND_DTR = 1, KIND_SCABIT = 2 };
+
+// device buffer owned by a scheduler
+class DevBuf {
+public:
+	DevBuf() = default;
+	explicit DevBuf(size_t bytes) { alloc(bytes); }
+	~DevBuf();
+	DevBuf(const DevBuf&) = delete;
+	DevBuf& operator=(const DevBuf&) = delete;
+	void alloc(size_t bytes);
+	void* get() const { return p_; }
+	size_t bytes() const { return n_; }
+	template<class T> T* as() const { return (T*)p_; }
+
+private:
+	void* p_ = nullptr;
+	size_t n_ = 0;
+};
+
+void check(hipError_t e, const char* what);
+
+struct Params {
+	int64_t threads = 0;  // streams T = worker count of the CPU counterpart (0: engine default)
+	int layout_x = 4;     // log2 DT block width  (SchedulerCPUDT::setLayout)
+	int layout_y = 4;     // log2 DT block height
+	uint64_t skip = 0;    // SLCG skip S (0: default_skip(T))
+	int device = -1;      // HIP device (-1: current)
+};
+
+// exact reductions: sums over all N sites
+struct Sums {
+	int64_t sumH = 0;
+	__int128 sumH2 = 0;
+	int64_t sumHs = 0;      // snapshot heights (correlate)
+	__int128 sumHHs = 0;    // sum h * h_snapshot
+	int64_t tcs = 0;        // slope overlap count (kpz/correlator.cpp:127-164)
+};
+
+class SchedulerService {
+public:
+	SchedulerService(int lx, int ly, Encoding enc, const Params& p);
+	virtual ~SchedulerService();
+
+	const Size& size() const { return size_; }
+	Encoding encoding() const { return enc_; }
+	int64_t threads() const { return T_; }
+	uint64_t skip() const { return S_; }
+	hipStream_t stream() const { return stream_; }
+
+	// ---- state set-up (kpz/schedulerService.cpp:156-175, 460-495)
+	virtual void initHomogeneous();          // flat initial condition, on device
+	void supplySeed(uint64_t seed);          // reSeed master + randomize()
+	virtual void randomize();                // stream states from two master draws
+	virtual bool setPQ(double p, double q);
+	double rateP() const { return p_; }
+	double rateQ() const { return q_; }
+
+	// ---- dynamics
+	virtual void mcs(int n) = 0;             // default dynamics of the scheduler
+	virtual void mcsSyncRng(int n) { mcs(n); }
+	virtual void mcsPQSyncRng(int n) = 0;    // general p, q
+	void mcsPQ(int n) { mcsPQSyncRng(n); }
+	virtual double mcsScale() const { return 1.0; }
+	unsigned int mcsCount() const { return mcs_; }
+	void setMcs(unsigned int m) { mcs_ = m; }
+
+	// ---- host copies (kpz/schedulerBase.cu:35-63)
+	void pushSystem(const uint32_t* host);   // H2D of a reference-layout lattice
+	void popSystem(uint32_t* host);          // D2H
+	void sync();
+
+	// ---- measurements (exact, on device)
+	Sums roughness();
+	void correlateTag();                     // snapshot for the autocorrelator
+	bool hasTag() const { return snap_.get() != nullptr; }
+	Sums correlate();
+
+	// ---- RNG state access (parity checks, checkpoints)
+	void getStreams(uint64_t* out);
+	void setStreams(const uint64_t* in);
+	MasterRng& master() { return master_; }
+
+	uint32_t* deviceLattice() const { return lat_.as<uint32_t>(); }
+
+protected:
+	uint64_t* stIn() const { return st_[cur_].as<uint64_t>(); }
+	uint64_t* stOut() const { return st_[cur_ ^ 1].as<uint64_t>(); }
+	void swapStreams() { cur_ ^= 1; }
+	const Affine* upload(std::vector<Affine>& host, DevBuf& buf);
+
+	Size size_;
+	Encoding enc_;
+	int64_t T_;
+	uint64_t S_;
+	Affine step_;
+	int device_;
+	hipStream_t stream_ = nullptr;
+	DevBuf lat_, snap_, rows_;
+	DevBuf st_[2];
+	int cur_ = 0;
+	MasterRng master_;
+	double p_ = 1.0, q_ = 0.0;
+	unsigned int mcs_ = 0;
+	std::vector<int64_t> hostRows_;
+};
+
+// Random-sequential double tiling: SchedulerCPUDT (static origin) and
+// SchedulerCPUDTr (random origin) semantics on ENC_LOCAL.
+class SchedulerDT : public SchedulerService {
+public:
+	SchedulerDT(int lx, int ly, const Params& p, bool randomOrigin);
+	void mcs(int n) override;                // p=1, q=0 (pqOneUpdate)
+	void mcsPQSyncRng(int n) override;       // per-update pq rule
+	int layoutX() const { return lbx_; }
+	int layoutY() const { return lby_; }
+	int64_t blocksPerDomainSet() const { return nb_; }
+
+private:
+	void run(int n, int upd);
+	const Affine* jumps(int du);
+	int lbx_, lby_;
+	bool ro_;
+	int64_t nb_;
+	std::map<int, std::unique_ptr<DevBuf>> jumpBuf_;
+};
+
+// Checkerboard SCA on ENC_CBBIT: SchedulerCPUCBBit semantics.
+class SchedulerSCABit : public SchedulerService {
+public:
+	SchedulerSCABit(int lx, int ly, const Params& p);
+	void mcs(int n) override;                // p=.5, q=0 bit-parallel (mcsSyncRng)
+	void mcsPQSyncRng(int n) override;       // bit-parallel if (.5, 0), else per site
+	double mcsScale() const override { return p_ + q_; }
+
+private:
+	struct Tables {
+		DevBuf row, lane;
+		Affine stepJ, fullBig, fullSmall;
+	};
+	void run(int n, int mode);
+	Tables& tables(int mode);
+	int per_, rem_, lG_;
+	std::map<int, std::unique_ptr<Tables>> tab_;
+};
+
+std::unique_ptr<SchedulerService> makeScheduler(int kind, int lx, int ly, const Params& p);
+
+} // namespace kpz

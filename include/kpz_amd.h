@@ -1,0 +1,112 @@
+/*
+ * kpz_amd.h -- C ABI of the MI355X-native KPZ octahedron-model engine.
+ *
+ * Drop-in boundary for the scheduler layer of CudaKpz: one opaque handle is
+ * one Kpz::SchedulerService (kpz/schedulerService.h:73-255) with its device
+ * lattice, per-worker SLCG64 streams and master dSFMT.  Plain pointers and
+ * sizes only; no exceptions cross the ABI: every call returns KPZ_OK (0) or
+ * a negative KPZ_E* code and kpz_last_error() describes the failure
+ * (thread-local).  One handle must be used by one host thread at a time.
+ *
+ * Lattices passed through kpz_upload/kpz_download use the reference layout
+ * bit for bit: ENC_LOCAL (4x4 sites per u32) for DT/DTr, ENC_CBBIT (four
+ * sublattice bit-plane quadrants) for SCABit, 2^(lx+ly-4) u32 words.
+ */
+#ifndef KPZ_AMD_H
+#define KPZ_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kpz_scheduler kpz_scheduler;
+
+/* scheduler kinds; the CPU counterpart each one is bit-exact against */
+enum {
+	KPZ_DT = 0,     /* SchedulerCPUDT<Rng> (static DT)  kpz/schedulerCPUDT.cpp:65-90 */
+	KPZ_DTR = 1,    /* SchedulerCPUDTr<Rng> (random-origin DT) kpz/schedulerCPUDTr.cpp:66-98 */
+	KPZ_SCABIT = 2  /* SchedulerCPUCBBit<Rng> (checkerboard, bit vectorised) kpz/schedulerCPUCBBit.cpp:43-108 */
+};
+
+enum { KPZ_ENC_LOCAL = 0, KPZ_ENC_CBBIT = 1 };
+
+enum {
+	KPZ_OK = 0,
+	KPZ_EINVAL = -1,  /* bad argument or unsupported configuration */
+	KPZ_EDEVICE = -2, /* HIP runtime error */
+	KPZ_ENOMEM = -3,  /* device allocation failed */
+	KPZ_ESTATE = -4   /* call out of order (e.g. correlate before corr_tag) */
+};
+
+typedef struct kpz_params {
+	int64_t threads;   /* RNG streams T = worker threads of the CPU counterpart
+	                      (SchedulerCPUpThreadRng(lx, ly, threads)); 0 = one per
+	                      DT block / one per SCABit row */
+	int layout_x;      /* log2 DT block width  (SchedulerCPUDT::setLayout), 2..4; 0 = 4 */
+	int layout_y;      /* log2 DT block height, 2..4; 0 = 4 */
+	uint64_t skip;     /* SLCG skip length S (kpz/kpzConst.h:85); 0 = 30721 if T < 30721
+	                      else the smallest odd S > T */
+	int device;        /* HIP device ordinal; -1 = current device */
+} kpz_params;
+
+/* thread-local description of the last failure */
+const char* kpz_last_error(void);
+
+/* SchedulerConfig::mkScheduler(x, y) + the scheduler constructor
+ * (kpz/kpzFrontend.cpp:234-250): lx, ly are log2 lattice sizes */
+int kpz_create(int kind, int lx, int ly, const kpz_params* params, kpz_scheduler** out);
+int kpz_destroy(kpz_scheduler* h);
+
+/* SchedulerService::supplySeed + randomize (kpz/schedulerService.cpp:460-482,
+ * kpz/GPURandom.cu:42-54): master dSFMT reSeed(seed ^ seed>>32), then the T
+ * stream states from two master draws */
+int kpz_set_seed(kpz_scheduler* h, uint64_t seed);
+/* SchedulerService::setPQ (kpz/schedulerService.h:190-197) */
+int kpz_set_pq(kpz_scheduler* h, double p, double q);
+/* initHomogeneous (kpz/schedulerService.cpp:156-175), on the device */
+int kpz_init_flat(kpz_scheduler* h);
+/* pushSystem / popSystem with a caller-owned host lattice (kpz/schedulerBase.cu:35-63) */
+int kpz_upload(kpz_scheduler* h, const uint32_t* lattice);
+int kpz_download(kpz_scheduler* h, uint32_t* lattice);
+/* mcs(n): the scheduler's default dynamics (DT: p=1,q=0; SCABit: p=.5,q=0) */
+int kpz_mcs(kpz_scheduler* h, int n);
+/* mcsPQSyncRng(n): dynamics with the current p, q */
+int kpz_mcs_pq(kpz_scheduler* h, int n);
+int kpz_sync(kpz_scheduler* h);
+
+/* exact height sums over all sites, heights as Correlator::set
+ * (kpz/correlator.cpp:37-106): out = {sum h, sum h^2 low 64, sum h^2 high 64} */
+int kpz_roughness(kpz_scheduler* h, int64_t out[3]);
+/* correlateTag: device snapshot of the current lattice (kpz/schedulerService.cpp:266-271) */
+int kpz_corr_tag(kpz_scheduler* h);
+/* correlate against the snapshot (kpz/correlator.cpp:127-164, 450-455):
+ * out = {sum h, sum h_snap, sum h^2 lo, hi, sum h*h_snap lo, hi, slope overlap} */
+int kpz_correlate(kpz_scheduler* h, int64_t out[7]);
+
+/* RNG state: stream states (T u64) and the master dSFMT (kpz_master_bytes()) */
+int64_t kpz_threads(const kpz_scheduler* h);
+int kpz_get_streams(kpz_scheduler* h, uint64_t* states);
+int kpz_set_streams(kpz_scheduler* h, const uint64_t* states);
+int kpz_master_bytes(void);
+int kpz_get_master(kpz_scheduler* h, uint8_t* state);
+int kpz_set_master(kpz_scheduler* h, const uint8_t* state);
+/* next master draw, as SchedulerService::genrand_close_open (schedulerService.h:222) */
+double kpz_master_draw(kpz_scheduler* h);
+
+/* counters and geometry */
+uint32_t kpz_mcs_count(const kpz_scheduler* h);
+double kpz_mcs_scale(const kpz_scheduler* h);     /* SchedulerSCABit::mcsScale = p + q */
+int kpz_encoding(const kpz_scheduler* h);
+uint64_t kpz_size_words(const kpz_scheduler* h);
+uint64_t kpz_skip(const kpz_scheduler* h);
+/* device pointer of the lattice and the HIP stream all work is queued on */
+void* kpz_device_lattice(const kpz_scheduler* h);
+void* kpz_stream(const kpz_scheduler* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

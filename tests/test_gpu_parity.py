@@ -1,0 +1,156 @@
+"""Bit-exact parity of the HIP engine (through the C ABI) with the oracle's
+restatement of the CPU counterparts: same seed, same T worker streams, same
+decomposition -> identical lattice words, stream states and master state."""
+import numpy as np
+import pytest
+
+import cudakpz_amd as K
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def run_oracle_dt(lx, ly, lbx, lby, T, seed, n, ro, upd, p, q):
+    m = O.Dsfmt(O.seed32(seed))
+    st = O.Streams(T, m)
+    lat = O.flat(lx, ly, O.ENC_LOCAL)
+    O.dt_mcs(lat, lx, ly, lbx, lby, st, m, n, ro, upd, p, q)
+    return lat, st.states, m
+
+
+def run_gpu(cls, lx, ly, seed, n, pq=None, **kw):
+    s = cls(lx, ly, **kw)
+    s.supplySeed(seed)
+    s.initHomogeneous()
+    if pq is None:
+        s.mcs(n)
+    else:
+        s.setPQ(*pq)
+        s.mcsPQSyncRng(n)
+    return s
+
+
+def assert_same(s, lat, states, master):
+    got = s.popSystem()
+    if not np.array_equal(got, lat):
+        bad = np.flatnonzero(got != lat)
+        pytest.fail(f"{bad.size} lattice words differ, first at {bad[:8]}")
+    assert np.array_equal(s.streams(), states)
+    assert s.master_draw() == master.close_open()
+
+
+DT_CASES = [
+    # lx, ly, lbx, lby, T
+    (6, 5, 2, 2, 1),
+    (6, 5, 2, 2, 3),
+    (6, 5, 2, 2, 0),
+    (9, 9, 4, 4, 1),
+    (9, 9, 4, 4, 7),
+    (9, 9, 4, 4, 0),
+    (10, 8, 3, 2, 5),
+    (8, 10, 2, 4, 0),
+    (7, 7, 3, 3, 64),
+]
+
+
+@pytest.mark.parametrize("lx,ly,lbx,lby,T", DT_CASES)
+@pytest.mark.parametrize("ro", [False, True])
+def test_dt_kpz_parity(lx, ly, lbx, lby, T, ro):
+    cls = K.SchedulerDTr if ro else K.SchedulerDT
+    s = run_gpu(cls, lx, ly, 42, 3, threads=T, layout=(lbx, lby))
+    lat, st, m = run_oracle_dt(lx, ly, lbx, lby, s.threads, 42, 3, ro, O.UPD_KPZ, 1.0, 0.0)
+    assert_same(s, lat, st, m)
+    assert s.mcsCount() == 3
+
+
+@pytest.mark.parametrize("lx,ly,lbx,lby,T", DT_CASES[::2])
+@pytest.mark.parametrize("ro", [False, True])
+@pytest.mark.parametrize("p,q", [(0.5, 0.5), (0.95, 0.05), (0.7, 0.0)])
+def test_dt_pq_parity(lx, ly, lbx, lby, T, ro, p, q):
+    cls = K.SchedulerDTr if ro else K.SchedulerDT
+    s = run_gpu(cls, lx, ly, 7, 2, pq=(p, q), threads=T, layout=(lbx, lby))
+    lat, st, m = run_oracle_dt(lx, ly, lbx, lby, s.threads, 7, 2, ro, O.UPD_PQ, p, q)
+    assert_same(s, lat, st, m)
+
+
+def run_oracle_cbbit(lx, ly, T, seed, n, mode, p, q):
+    m = O.Dsfmt(O.seed32(seed))
+    st = O.Streams(T, m)
+    lat = O.flat(lx, ly, O.ENC_CBBIT)
+    O.cbbit_mcs(lat, lx, ly, st, n, mode, p, q)
+    return lat, st.states, m
+
+
+CB_CASES = [
+    # lx, ly, T
+    (7, 4, 1),
+    (8, 6, 3),
+    (6, 5, 5),
+    (10, 10, 0),
+    (12, 6, 7),
+    (14, 8, 0),
+    (13, 9, 100),
+]
+
+
+@pytest.mark.parametrize("lx,ly,T", CB_CASES)
+def test_scabit_bitparallel_parity(lx, ly, T):
+    s = run_gpu(K.SchedulerSCABit, lx, ly, 42, 3, threads=T)
+    assert s.mcsScale() == 0.5
+    lat, st, m = run_oracle_cbbit(lx, ly, s.threads, 42, 3, 0, 0.5, 0.0)
+    assert_same(s, lat, st, m)
+
+
+@pytest.mark.parametrize("lx,ly,T", CB_CASES)
+@pytest.mark.parametrize("p,q", [(0.95, 0.05), (0.5, 0.5), (1.0, 0.0), (0.3, 0.6)])
+def test_scabit_pq_parity(lx, ly, T, p, q):
+    s = run_gpu(K.SchedulerSCABit, lx, ly, 11, 2, pq=(p, q), threads=T)
+    lat, st, m = run_oracle_cbbit(lx, ly, s.threads, 11, 2, 1, p, q)
+    assert_same(s, lat, st, m)
+    assert s.mcsScale() == p + q
+
+
+def test_scabit_pq_half_zero_uses_bitparallel_path():
+    # mcsPQSyncRng with (p, q) = (.5, 0) is the VL3 path (kpz/schedulerCPUCBBit.cpp:95-102)
+    s = run_gpu(K.SchedulerSCABit, 9, 7, 3, 2, pq=(0.5, 0.0), threads=4)
+    lat, st, m = run_oracle_cbbit(9, 7, 4, 3, 2, 0, 0.5, 0.0)
+    assert_same(s, lat, st, m)
+
+
+@pytest.mark.parametrize("cls,enc,lx,ly", [(K.SchedulerDT, 0, 9, 8), (K.SchedulerSCABit, 1, 9, 8),
+                                           (K.SchedulerDT, 0, 5, 4), (K.SchedulerSCABit, 1, 6, 3)])
+def test_roughness_and_correlate_parity(cls, enc, lx, ly):
+    kw = {"layout": (2, 2)} if cls is K.SchedulerDT else {}
+    s = cls(lx, ly, **kw)
+    s.supplySeed(5)
+    s.initHomogeneous()
+    N = 1 << (lx + ly)
+    r0 = s.roughness()
+    assert (r0.sum_h, r0.sum_h2) == (-N // 2, N // 2)
+    if cls is K.SchedulerDT:
+        s.mcs(2)
+    else:
+        s.setPQ(0.9, 0.1)
+        s.mcsPQSyncRng(2)
+    s.correlateTag()
+    snap = s.popSystem()
+    if cls is K.SchedulerDT:
+        s.setPQ(0.8, 0.3)
+        s.mcsPQSyncRng(3)
+    else:
+        s.mcsPQSyncRng(3)
+    cur = s.popSystem()
+    r = s.roughness()
+    assert (r.sum_h, r.sum_h2) == O.roughness(cur, enc, lx, ly)
+    c = s.correlate()
+    assert (c.sum_h, c.sum_hs, c.sum_h2, c.sum_hhs, c.tcs) == O.correlate(cur, snap, enc, lx, ly)
+    assert O.consistent(cur, enc, lx, ly)
+
+
+def test_upload_download_roundtrip():
+    s = K.SchedulerSCABit(8, 6)
+    lat = O.flat(8, 6, O.ENC_CBBIT)
+    rng = np.random.default_rng(0)
+    lat ^= rng.integers(0, 1 << 32, lat.size, dtype=np.uint64).astype(np.uint32)
+    s.pushSystem(lat)
+    assert np.array_equal(s.popSystem(), lat)
