@@ -136,6 +136,9 @@ def main():
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--secondary", action="store_true", help="also time SCABit at 65536^2")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="K",
+                    help="after the run, K lattice copies (1 GiB read + 1 GiB write each) whose "
+                         "FETCH_SIZE/WRITE_SIZE calibrate the counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
@@ -151,6 +154,11 @@ def main():
     bytes_per_launch = 0.5 * N / 4  # 0.5 B per site update, N/4 updates per phase
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     value = N * args.steps / (total_ms * 1e-3)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    key = f"dt_phase_kernel<4,4,KPZ,static>@{1 << args.lx}"
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get(key, {}).get("bytes_per_launch")
     line = {
         "metric": METRIC,
         "value": value,
@@ -169,7 +177,7 @@ def main():
                                f"{args.steps} MCS, exact w2 at {len(w2)} log-spaced times",
                    "lattice": f"{1 << args.lx}x{1 << args.lx}", "mcs": args.steps, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "dt_phase_kernel<4,4,KPZ,static>", "avg_launch_ms": avg_launch_ms,
                      "bytes_per_launch": bytes_per_launch},
         "w2_final": w2[-1][1] if w2 else None,
@@ -184,6 +192,17 @@ def main():
             sec[name] = {"site_updates_per_s": ups, "hbm_GBs_algorithmic": ups * 1.0 / 1e9,
                          "hbm_frac": ups / 1e9 / HBM_PEAK_GBS, "ms_per_mcs": ms / steps}
         line["secondary"] = sec
+    if args.copy_calibration:
+        src = K.SchedulerDT(args.lx, args.lx, layout=(4, 4), threads=1)
+        dst = K.SchedulerDT(args.lx, args.lx, layout=(4, 4), threads=1)
+        src.initHomogeneous()
+        for _ in range(args.copy_calibration):
+            dst.copySystem(src)
+        dst.sync()
+        line["copy_calibration"] = {"copies": args.copy_calibration, "bytes_read_per_copy": N // 4,
+                                    "bytes_written_per_copy": N // 4, "kernel": "copy_kernel"}
+        src.close()
+        dst.close()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line))

@@ -62,6 +62,7 @@ def lib():
             getattr(L, f).argtypes = [h]
         L.kpz_upload.argtypes = [h, C.c_void_p]
         L.kpz_download.argtypes = [h, C.c_void_p]
+        L.kpz_copy_system.argtypes = [h, h]
         L.kpz_mcs.argtypes = [h, C.c_int]
         L.kpz_mcs_pq.argtypes = [h, C.c_int]
         L.kpz_roughness.argtypes = [h, i64p]
@@ -238,6 +239,9 @@ class SchedulerService:
         a = np.empty(self.size_words, np.uint32)
         _check(lib().kpz_download(self._h, a.ctypes.data))
         return a
+
+    def copySystem(self, other):
+        _check(lib().kpz_copy_system(self._h, other._h))
 
     def roughness(self):
         out = (C.c_int64 * 3)()

@@ -147,6 +147,13 @@ int kpz_sync(kpz_scheduler* h)
 	return guard([&] { h->s->sync(); });
 }
 
+int kpz_copy_system(kpz_scheduler* dst, const kpz_scheduler* src)
+{
+	HANDLE(dst);
+	HANDLE(src);
+	return guard([&] { dst->s->copySystem(*src->s); });
+}
+
 int kpz_roughness(kpz_scheduler* h, int64_t out[3])
 {
 	HANDLE(h);

@@ -87,18 +87,30 @@ __global__ __launch_bounds__(DT_WG) void dt_phase_kernel(DtPhaseArgs a, int64_t 
 	uint32_t* __restrict__ lat = a.lat;
 	uint2* __restrict__ my = tile + tid;
 
-	// ---- stage ENC_LOCAL words -> bit-plane rows
+	// ---- stage ENC_LOCAL words -> bit-plane rows.  The raw words stay in
+	// registers: the write-back merges into them instead of re-reading HBM
+	// (no other lane of this phase writes them).
+	uint32_t raw[NWY][NWX];
 #pragma unroll
 	for(int wk = 0; wk < NWY; ++wk)
 	{
 		const size_t rowBase = (size_t)((wr0 + wk) & mYW) * XW;
+#pragma unroll
+		for(int k = 0; k < NWX; ++k)
+		{
+			const bool needed = RO || wk < NWY - 1 || k < NWX - 1; // static DT: corner unused
+			raw[wk][k] = needed ? lat[rowBase + ((c0 + k) & mXW)] : 0u;
+		}
+	}
+#pragma unroll
+	for(int wk = 0; wk < NWY; ++wk)
+	{
 		uint32_t cx[NWX], cy[NWX];
 #pragma unroll
 		for(int k = 0; k < NWX; ++k)
 		{
-			const uint32_t w = lat[rowBase + ((c0 + k) & mXW)];
-			cx[k] = even16(w);
-			cy[k] = even16(w >> 1);
+			cx[k] = even16(raw[wk][k]);
+			cy[k] = even16(raw[wk][k] >> 1);
 		}
 #pragma unroll
 		for(int rr = 0; rr < 4; ++rr)
@@ -148,20 +160,25 @@ __global__ __launch_bounds__(DT_WG) void dt_phase_kernel(DtPhaseArgs a, int64_t 
 			x = (int)(r1 >> (32 - LBX));
 			y = (int)(r2 >> (32 - LBY));
 		}
-		bool accMin = true, accMax = false;
-		if(UPD == 1)
+		uint2 row = my[y * DT_WG];
+		const uint32_t up = my[(y + 1) * DT_WG].y;
+		// whole-row bit algebra, then select column x: a local minimum has
+		// own X = 0, right X = 1, own Y = 0, up Y = 1; a maximum the opposite
+		const uint32_t rx = row.x >> 1;
+		uint32_t fm;
+		if(UPD == 0)
+			fm = rx & ~row.x & ~row.y & up & (1u << x);
+		else
 		{
 			s = step(s);
 			const uint32_t r23 = slcg_fold(s) & 0x7FFFFFu;
-			accMin = r23 < tp;
-			accMax = r23 < tq;
+			// all-ones when accepted: sign of r23 - t (both below 2^24)
+			const uint32_t aMin = (uint32_t)((int32_t)(r23 - tp) >> 31);
+			const uint32_t aMax = (uint32_t)((int32_t)(r23 - tq) >> 31);
+			const uint32_t mn = rx & ~row.x & ~row.y & up;
+			const uint32_t mx = ~rx & row.x & row.y & ~up;
+			fm = ((mn & aMin) | (mx & aMax)) & (1u << x);
 		}
-		uint2 row = my[y * DT_WG];
-		const uint32_t up = my[(y + 1) * DT_WG].y;
-		// bit0 own X, bit1 right X, bit2 own Y, bit3 up Y
-		const uint32_t cfg = ((row.x >> x) & 3u) | (((row.y >> x) & 1u) << 2) | (((up >> x) & 1u) << 3);
-		const bool flip = (cfg == 0xAu && accMin) || (cfg == 0x5u && accMax);
-		const uint32_t fm = flip ? (1u << x) : 0u;
 		row.x ^= fm | (fm << 1);
 		row.y ^= fm;
 		my[y * DT_WG] = row;
@@ -201,8 +218,7 @@ __global__ __launch_bounds__(DT_WG) void dt_phase_kernel(DtPhaseArgs a, int64_t 
 				continue;
 			const uint32_t m32 = spread16(mx16) | (spread16(my16) << 1);
 			const uint32_t v32 = spread16(vx16) | (spread16(vy16) << 1);
-			const size_t idx = rowBase + ((c0 + k) & mXW);
-			lat[idx] = (m32 == 0xFFFFFFFFu) ? v32 : ((lat[idx] & ~m32) | v32);
+			lat[rowBase + ((c0 + k) & mXW)] = (raw[wk][k] & ~m32) | v32;
 		}
 	}
 }
@@ -289,14 +305,37 @@ __global__ __launch_bounds__(256) void cbbit_half_kernel(CbHalfArgs a)
 		else
 		{
 			const W maxm = tX & tY & ~xXs & ~yY;
+			// acceptance bits, site k of the word -> bit k: the sign bit of
+			// r23 - t is funnel-shifted into an accumulator (v_alignbit), 32
+			// sites per accumulator, which ends bit-reversed
 			W pb = 0, qb = 0;
-#pragma unroll 8
-			for(int k = 0; k < WB; ++k)
+#pragma unroll
+			for(int h = 0; h < WB / 32; ++h)
 			{
-				s = step(s);
-				const uint32_t r23 = slcg_fold(s) & 0x7FFFFFu;
-				pb |= (W)(r23 < tp) << k;
-				qb |= (W)(r23 < tq) << k;
+				uint32_t pa = 0, qa = 0;
+				if(tq)
+				{
+#pragma unroll
+					for(int k = 0; k < 32; ++k)
+					{
+						s = step(s);
+						const uint32_t r23 = slcg_fold(s) & 0x7FFFFFu;
+						pa = __builtin_amdgcn_alignbit(pa, r23 - tp, 31);
+						qa = __builtin_amdgcn_alignbit(qa, r23 - tq, 31);
+					}
+				}
+				else
+				{
+#pragma unroll
+					for(int k = 0; k < 32; ++k)
+					{
+						s = step(s);
+						const uint32_t r23 = slcg_fold(s) & 0x7FFFFFu;
+						pa = __builtin_amdgcn_alignbit(pa, r23 - tp, 31);
+					}
+				}
+				pb |= (W)__brev(pa) << (32 * h);
+				qb |= (W)__brev(qa) << (32 * h);
 			}
 			flip = (minm & pb) | (maxm & qb);
 		}
@@ -552,6 +591,13 @@ __global__ void fill_kernel(uint32_t* lat, uint64_t n, uint64_t split, uint32_t 
 		lat[k] = k < split ? lo : hi;
 }
 
+__global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n)
+{
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for(uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+		dst[k] = src[k];
+}
+
 template<int LBX, int LBY>
 hipError_t dt_dispatch_upd(const DtPhaseArgs& a, int64_t nb, dim3 grid, hipStream_t stream)
 {
@@ -655,6 +701,14 @@ hipError_t launch_fill_flat(uint32_t* lat, int enc, int lx, int ly, hipStream_t 
 		hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, stream, lat, n, n, 0x33CC33CCu, 0x33CC33CCu);
 	else
 		hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, stream, lat, n, n >> 1, 0u, 0xFFFFFFFFu);
+	return hipGetLastError();
+}
+
+hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t stream)
+{
+	const uint64_t n = bytes / 16;
+	const unsigned blocks = (unsigned)((n + 255) / 256 < 16384 ? (n + 255) / 256 : 16384);
+	hipLaunchKernelGGL(copy_kernel, dim3(blocks ? blocks : 1), dim3(256), 0, stream, (uint4*)dst, (const uint4*)src, n);
 	return hipGetLastError();
 }
 

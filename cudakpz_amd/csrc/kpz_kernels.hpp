@@ -56,5 +56,7 @@ hipError_t launch_row_stats(const uint32_t* lat, const uint32_t* snap, int enc, 
 
 // ---- utilities -------------------------------------------------------------
 hipError_t launch_fill_flat(uint32_t* lat, int enc, int lx, int ly, hipStream_t stream);
+// dst <- src, 16-byte vector stream copy (bytes must be a multiple of 16)
+hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t stream);
 
 } // namespace kpz

@@ -108,6 +108,18 @@ void SchedulerService::popSystem(uint32_t* host)
 	sync();
 }
 
+void SchedulerService::copySystem(const SchedulerService& other)
+{
+	if(other.size_.lx != size_.lx || other.size_.ly != size_.ly || other.enc_ != enc_)
+		throw Error(-1, "copySystem: size or encoding differs");
+	check(hipStreamSynchronize(other.stream_), "hipStreamSynchronize");
+	const uint64_t bytes = size_.sizeW() * 4;
+	if(bytes % 16)
+		check(hipMemcpyAsync(deviceLattice(), other.deviceLattice(), bytes, hipMemcpyDeviceToDevice, stream_), "copySystem");
+	else
+		check(launch_copy(deviceLattice(), other.deviceLattice(), bytes, stream_), "copy");
+}
+
 void SchedulerService::sync() { check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
 
 void SchedulerService::getStreams(uint64_t* out)

@@ -95,6 +95,8 @@ public:
 	void pushSystem(const uint32_t* host);   // H2D of a reference-layout lattice
 	void popSystem(uint32_t* host);          // D2H
 	void sync();
+	// SchedulerService::copySystem (kpz/schedulerService.cpp:497-504): device copy
+	void copySystem(const SchedulerService& other);
 
 	// ---- measurements (exact, on device)
 	Sums roughness();

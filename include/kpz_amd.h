@@ -75,6 +75,9 @@ int kpz_mcs(kpz_scheduler* h, int n);
 /* mcsPQSyncRng(n): dynamics with the current p, q */
 int kpz_mcs_pq(kpz_scheduler* h, int n);
 int kpz_sync(kpz_scheduler* h);
+/* SchedulerService::copySystem (kpz/schedulerService.cpp:497-504): device-side
+ * copy of src's lattice into dst (same size and encoding), queued on dst's stream */
+int kpz_copy_system(kpz_scheduler* dst, const kpz_scheduler* src);
 
 /* exact height sums over all sites, heights as Correlator::set
  * (kpz/correlator.cpp:37-106): out = {sum h, sum h^2 low 64, sum h^2 high 64} */
