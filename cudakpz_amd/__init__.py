@@ -75,6 +75,7 @@ def lib():
         L.kpz_set_master.argtypes = [h, C.c_void_p]
         L.kpz_master_draw.argtypes = [h]
         L.kpz_master_draw.restype = C.c_double
+        L.kpz_randomize.argtypes = [h]
         L.kpz_mcs_count.argtypes = [h]
         L.kpz_mcs_count.restype = C.c_uint32
         L.kpz_mcs_scale.argtypes = [h]
@@ -271,6 +272,23 @@ class SchedulerService:
 
     def master_draw(self):
         return float(lib().kpz_master_draw(self._h))
+
+    def randomize(self):
+        """SLCG64::randomize: re-seed the T streams from the master."""
+        _check(lib().kpz_randomize(self._h))
+
+    def master_state(self):
+        """master dSFMT state, laid out as the reference's dsfmt_t"""
+        b = C.create_string_buffer(lib().kpz_master_bytes())
+        _check(lib().kpz_get_master(self._h, b))
+        return b.raw
+
+    def set_master_state(self, state):
+        n = lib().kpz_master_bytes()
+        if len(state) < n:
+            raise ValueError(f"master state needs {n} bytes")
+        b = C.create_string_buffer(bytes(state[:n]), n)
+        _check(lib().kpz_set_master(self._h, b))
 
     @property
     def device_lattice(self):

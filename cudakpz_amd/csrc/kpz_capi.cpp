@@ -95,6 +95,12 @@ int kpz_set_seed(kpz_scheduler* h, uint64_t seed)
 	return guard([&] { h->s->supplySeed(seed); });
 }
 
+int kpz_randomize(kpz_scheduler* h)
+{
+	HANDLE(h);
+	return guard([&] { h->s->randomize(); });
+}
+
 int kpz_set_pq(kpz_scheduler* h, double p, double q)
 {
 	HANDLE(h);

@@ -63,6 +63,10 @@ int kpz_destroy(kpz_scheduler* h);
  * kpz/GPURandom.cu:42-54): master dSFMT reSeed(seed ^ seed>>32), then the T
  * stream states from two master draws */
 int kpz_set_seed(kpz_scheduler* h, uint64_t seed);
+/* randomize(): re-seed the T streams from two draws of the engine's master
+ * (SLCG64::randomize, kpz/GPURandom.cu:42-54; SchedulerDT::randomize
+ * kpz/schedulerDT.h:98) */
+int kpz_randomize(kpz_scheduler* h);
 /* SchedulerService::setPQ (kpz/schedulerService.h:190-197) */
 int kpz_set_pq(kpz_scheduler* h, double p, double q);
 /* initHomogeneous (kpz/schedulerService.cpp:156-175), on the device */
@@ -88,7 +92,9 @@ int kpz_corr_tag(kpz_scheduler* h);
  * out = {sum h, sum h_snap, sum h^2 lo, hi, sum h*h_snap lo, hi, slope overlap} */
 int kpz_correlate(kpz_scheduler* h, int64_t out[7]);
 
-/* RNG state: stream states (T u64) and the master dSFMT (kpz_master_bytes()) */
+/* RNG state: stream states (T u64) and the master dSFMT.  The master state
+ * is kpz_master_bytes() bytes laid out as the reference's dsfmt_t
+ * (inc/dSFMT/dSFMT.h: 192 x 128-bit status words, then int idx). */
 int64_t kpz_threads(const kpz_scheduler* h);
 int kpz_get_streams(kpz_scheduler* h, uint64_t* states);
 int kpz_set_streams(kpz_scheduler* h, const uint64_t* states);

@@ -154,3 +154,15 @@ def test_upload_download_roundtrip():
     lat ^= rng.integers(0, 1 << 32, lat.size, dtype=np.uint64).astype(np.uint32)
     s.pushSystem(lat)
     assert np.array_equal(s.popSystem(), lat)
+
+
+def test_master_handover_and_randomize():
+    # the reference's dsfmt_t bytes go in, randomize() seeds the streams from them
+    m = O.Dsfmt(1234)
+    st = O.Streams(37, m.copy())
+    s = K.SchedulerSCABit(8, 7, threads=37)
+    s.set_master_state(bytes(m.buf))
+    s.randomize()
+    assert np.array_equal(s.streams(), st.states)
+    m.close_open(), m.close_open()
+    assert s.master_draw() == m.close_open()
