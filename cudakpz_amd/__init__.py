@@ -317,6 +317,140 @@ class SchedulerSCABit(SchedulerService):
     ENCODING = ENC_CBBIT
 
 
+class _Replica(C.Structure):
+    _fields_ = [("kind", C.c_int), ("lx", C.c_int), ("ly", C.c_int), ("p", C.c_double), ("q", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class BatchTuning(C.Structure):
+    """Performance knobs of a Batch; none of them changes any result."""
+    _fields_ = [("max_lanes_log2", C.c_int), ("pq_words64", C.c_int), ("chunk", C.c_int),
+                ("concurrent", C.c_int), ("sort_by_size", C.c_int)]
+
+    @classmethod
+    def default(cls):
+        return cls(6, 1, 50, 1, 1)
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def _declare_batch(L):
+    if getattr(L, "_batch_declared", False):
+        return
+    h = C.c_void_p
+    L.kpz_batch_create.argtypes = [C.POINTER(_Replica), C.c_int, C.POINTER(BatchTuning), C.c_int, C.POINTER(h)]
+    L.kpz_batch_destroy.argtypes = [h]
+    L.kpz_batch_init_flat.argtypes = [h]
+    L.kpz_batch_mcs.argtypes = [h, C.c_int]
+    L.kpz_batch_roughness.argtypes = [h, C.POINTER(C.c_int64)]
+    L.kpz_batch_download.argtypes = [h, C.c_int, C.c_void_p]
+    L.kpz_batch_get_streams.argtypes = [h, C.c_int, C.c_void_p]
+    L.kpz_batch_threads.argtypes = [h, C.c_int]
+    L.kpz_batch_threads.restype = C.c_int64
+    L.kpz_batch_sites.argtypes = [h]
+    L.kpz_batch_sites.restype = C.c_uint64
+    L.kpz_batch_sync.argtypes = [h]
+    L.kpz_batch_stream.argtypes = [h]
+    L.kpz_batch_stream.restype = C.c_void_p
+    L.kpz_batch_autotune.argtypes = [C.POINTER(_Replica), C.c_int, C.c_int, C.c_int, C.POINTER(BatchTuning),
+                                     C.POINTER(C.c_double)]
+    L._batch_declared = True
+
+
+def _replicas(reps):
+    arr = (_Replica * len(reps))()
+    for i, r in enumerate(reps):
+        kind, lx, ly, p, q, seed = r
+        arr[i] = _Replica(int(kind), int(lx), int(ly), float(p), float(q), int(seed) & 0xFFFFFFFFFFFFFFFF)
+    return arr
+
+
+class Batch:
+    """Many independent replicas advanced together (hipGraph-replayed chunks).
+
+    replicas: sequence of (kind, lx, ly, p, q, seed); kind KPZ_DT, KPZ_DTR or
+    KPZ_SCABIT.  Replica i evolves exactly like the stand-alone scheduler of
+    that kind, size, seed and (p, q) with default threads: DT/DTr with
+    (p, q) = (1, 0) as mcs(n), everything else as mcsPQSyncRng(n)."""
+
+    def __init__(self, replicas, tuning=None, device=-1):
+        L = lib()
+        _declare_batch(L)
+        self.replicas = [tuple(r) for r in replicas]
+        arr = _replicas(self.replicas)
+        h = C.c_void_p()
+        t = C.byref(tuning) if tuning is not None else None
+        _check(L.kpz_batch_create(arr, len(arr), t, int(device), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kpz_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.replicas)
+
+    @property
+    def sites(self):
+        return int(lib().kpz_batch_sites(self._h))
+
+    @property
+    def stream(self):
+        return int(lib().kpz_batch_stream(self._h))
+
+    def initHomogeneous(self):
+        _check(lib().kpz_batch_init_flat(self._h))
+
+    def mcs(self, n):
+        _check(lib().kpz_batch_mcs(self._h, int(n)))
+
+    def sync(self):
+        _check(lib().kpz_batch_sync(self._h))
+
+    def roughness(self):
+        n = len(self.replicas)
+        out = (C.c_int64 * (3 * n))()
+        _check(lib().kpz_batch_roughness(self._h, out))
+        res = []
+        for i, r in enumerate(self.replicas):
+            N = 1 << (r[1] + r[2])
+            res.append(Roughness(N, int(out[3 * i]), _i128(out[3 * i + 1], out[3 * i + 2])))
+        return res
+
+    def popSystem(self, i):
+        lx, ly = self.replicas[i][1], self.replicas[i][2]
+        a = np.empty(1 << (lx + ly - 4), np.uint32)
+        _check(lib().kpz_batch_download(self._h, int(i), a.ctypes.data))
+        return a
+
+    def threads(self, i):
+        return int(lib().kpz_batch_threads(self._h, int(i)))
+
+    def streams(self, i):
+        a = np.empty(self.threads(i), np.uint64)
+        _check(lib().kpz_batch_get_streams(self._h, int(i), a.ctypes.data))
+        return a
+
+    @staticmethod
+    def autotune(replicas, mcs=5, tuning=None, device=-1):
+        """Returns (best BatchTuning, ms per MCS)."""
+        L = lib()
+        _declare_batch(L)
+        arr = _replicas(replicas)
+        t = BatchTuning.default() if tuning is None else tuning
+        ms = C.c_double()
+        _check(L.kpz_batch_autotune(arr, len(arr), int(mcs), int(device), C.byref(t), C.byref(ms)))
+        return t, ms.value
+
+
 def make_scheduler(name, lx, ly, **kw):
     """SchedulerConfig::mkScheduler by scheduler id (kpz/kpzFrontend.cpp:93-250)."""
     table = {"dtStatic": SchedulerDT, "dt": SchedulerDTr, "scaBit": SchedulerSCABit}

@@ -1,5 +1,6 @@
 // extern "C" boundary over the C++ schedulers (include/kpz_amd.h).
 #include "../../include/kpz_amd.h"
+#include "kpz_batch.hpp"
 #include "kpz_scheduler.hpp"
 
 #include <cstring>
@@ -7,6 +8,10 @@
 
 struct kpz_scheduler {
 	std::unique_ptr<kpz::SchedulerService> s;
+};
+
+struct kpz_batch {
+	std::unique_ptr<kpz::Batch> b;
 };
 
 namespace {
@@ -245,5 +250,137 @@ uint64_t kpz_size_words(const kpz_scheduler* h) { return (h && h->s) ? h->s->siz
 uint64_t kpz_skip(const kpz_scheduler* h) { return (h && h->s) ? h->s->skip() : 0; }
 void* kpz_device_lattice(const kpz_scheduler* h) { return (h && h->s) ? (void*)h->s->deviceLattice() : nullptr; }
 void* kpz_stream(const kpz_scheduler* h) { return (h && h->s) ? (void*)h->s->stream() : nullptr; }
+
+// ---------------------------------------------------------------- batches
+static bool to_specs(const kpz_replica* reps, int n, std::vector<kpz::ReplicaSpec>& out)
+{
+	if(!reps || n <= 0)
+		return false;
+	out.resize((size_t)n);
+	for(int i = 0; i < n; ++i)
+		out[(size_t)i] = kpz::ReplicaSpec{reps[i].kind, reps[i].lx, reps[i].ly, reps[i].p, reps[i].q, reps[i].seed};
+	return true;
+}
+
+static kpz::BatchTuning to_tuning(const kpz_batch_tuning* t)
+{
+	kpz::BatchTuning k;
+	if(t)
+	{
+		k.maxLanesLog2 = t->max_lanes_log2;
+		k.pqWords64 = t->pq_words64;
+		k.chunk = t->chunk;
+		k.concurrent = t->concurrent;
+		k.sortBySize = t->sort_by_size;
+	}
+	return k;
+}
+
+#define BHANDLE(h)                                    \
+	if(!(h) || !(h)->b)                               \
+		return fail(KPZ_EINVAL, "null batch handle");
+
+int kpz_batch_create(const kpz_replica* reps, int n, const kpz_batch_tuning* tuning, int device, kpz_batch** out)
+{
+	if(!out)
+		return fail(KPZ_EINVAL, "null output pointer");
+	*out = nullptr;
+	std::vector<kpz::ReplicaSpec> specs;
+	if(!to_specs(reps, n, specs))
+		return fail(KPZ_EINVAL, "no replicas");
+	return guard([&] {
+		auto h = new kpz_batch;
+		try
+		{
+			h->b = std::make_unique<kpz::Batch>(specs, to_tuning(tuning), device);
+		}
+		catch(...)
+		{
+			delete h;
+			throw;
+		}
+		*out = h;
+	});
+}
+
+int kpz_batch_destroy(kpz_batch* b)
+{
+	if(!b)
+		return KPZ_OK;
+	return guard([&] { delete b; });
+}
+
+int kpz_batch_init_flat(kpz_batch* b)
+{
+	BHANDLE(b);
+	return guard([&] { b->b->initHomogeneous(); });
+}
+
+int kpz_batch_mcs(kpz_batch* b, int n)
+{
+	BHANDLE(b);
+	if(n < 0)
+		return fail(KPZ_EINVAL, "negative MCS count");
+	return guard([&] { b->b->mcs(n); });
+}
+
+int kpz_batch_roughness(kpz_batch* b, int64_t* out)
+{
+	BHANDLE(b);
+	if(!out)
+		return fail(KPZ_EINVAL, "null output");
+	return guard([&] { b->b->roughness(out); });
+}
+
+int kpz_batch_download(kpz_batch* b, int replica, uint32_t* lattice)
+{
+	BHANDLE(b);
+	if(!lattice || replica < 0 || replica >= b->b->size())
+		return fail(KPZ_EINVAL, "bad replica index or null lattice");
+	return guard([&] { b->b->download(replica, lattice); });
+}
+
+int kpz_batch_get_streams(kpz_batch* b, int replica, uint64_t* states)
+{
+	BHANDLE(b);
+	if(!states || replica < 0 || replica >= b->b->size())
+		return fail(KPZ_EINVAL, "bad replica index or null output");
+	return guard([&] { b->b->getStreams(replica, states); });
+}
+
+int64_t kpz_batch_threads(const kpz_batch* b, int replica)
+{
+	if(!b || !b->b || replica < 0 || replica >= b->b->size())
+		return 0;
+	return b->b->threads(replica);
+}
+
+uint64_t kpz_batch_sites(const kpz_batch* b) { return (b && b->b) ? b->b->sites() : 0; }
+
+int kpz_batch_sync(kpz_batch* b)
+{
+	BHANDLE(b);
+	return guard([&] { b->b->sync(); });
+}
+
+void* kpz_batch_stream(const kpz_batch* b) { return (b && b->b) ? (void*)b->b->stream() : nullptr; }
+
+int kpz_batch_autotune(const kpz_replica* reps, int n, int mcs, int device, kpz_batch_tuning* tuning, double* ms_per_mcs)
+{
+	std::vector<kpz::ReplicaSpec> specs;
+	if(!to_specs(reps, n, specs) || !tuning)
+		return fail(KPZ_EINVAL, "no replicas or null tuning");
+	return guard([&] {
+		kpz::BatchTuning t = to_tuning(tuning);
+		const double ms = kpz::autotuneBatch(specs, mcs, device, t);
+		tuning->max_lanes_log2 = t.maxLanesLog2;
+		tuning->pq_words64 = t.pqWords64;
+		tuning->chunk = t.chunk;
+		tuning->concurrent = t.concurrent;
+		tuning->sort_by_size = t.sortBySize;
+		if(ms_per_mcs)
+			*ms_per_mcs = ms;
+	});
+}
 
 } // extern "C"

@@ -59,20 +59,16 @@ __device__ __forceinline__ uint64_t shflw(uint64_t v, int src, int width) { retu
 // ======================================================================
 constexpr int DT_WG = 256;
 
+// One DT block i of one phase, staged in this lane's LDS column `my`
+// (row r at my[r * DT_WG]).  nb = blocks per domain set.
 template<int LBX, int LBY, int UPD, bool RO>
-__global__ __launch_bounds__(DT_WG) void dt_phase_kernel(DtPhaseArgs a, int64_t nb)
+__device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_t nb, uint2* __restrict__ my)
 {
-	constexpr int BX = 1 << LBX, BY = 1 << LBY, ROWS = BY + 1;
+	constexpr int BX = 1 << LBX, BY = 1 << LBY;
 	constexpr int NWX = BX / 4 + 1; // words covering BX+1 sites (+ shift)
 	constexpr int NWY = BY / 4 + 1; // word rows covering BY+1 site rows
 	constexpr uint32_t MX = (2u << BX) - 1u; // X row: columns 0..BX (BX = halo)
 	constexpr uint32_t MY = (1u << BX) - 1u; // Y row: columns 0..BX-1
-	__shared__ uint2 tile[ROWS * DT_WG];
-
-	const int tid = threadIdx.x;
-	const int64_t i = (int64_t)blockIdx.x * DT_WG + tid;
-	if(i >= nb)
-		return; // no barrier in this kernel: idle lanes may leave
 
 	// block geometry, Kmc::DoubleTilingDD<2>::setBlock (inc/doubleTilingDD.h:71-78)
 	const int lhx = a.lx - LBX - 1;
@@ -85,7 +81,6 @@ __global__ __launch_bounds__(DT_WG) void dt_phase_kernel(DtPhaseArgs a, int64_t 
 	const int shy = RO ? (oy & 3) : 0;
 	const int c0 = ox >> 2, wr0 = oy >> 2;
 	uint32_t* __restrict__ lat = a.lat;
-	uint2* __restrict__ my = tile + tid;
 
 	// ---- stage ENC_LOCAL words -> bit-plane rows.  The raw words stay in
 	// registers: the write-back merges into them instead of re-reading HBM
@@ -223,19 +218,32 @@ __global__ __launch_bounds__(DT_WG) void dt_phase_kernel(DtPhaseArgs a, int64_t 
 	}
 }
 
+template<int LBX, int LBY>
+constexpr int dt_tile_rows() { return (1 << LBY) + 1; }
+
+template<int LBX, int LBY, int UPD, bool RO>
+__global__ __launch_bounds__(DT_WG) void dt_phase_kernel(DtPhaseArgs a, int64_t nb)
+{
+	__shared__ uint2 tile[dt_tile_rows<LBX, LBY>() * DT_WG];
+	const int64_t i = (int64_t)blockIdx.x * DT_WG + threadIdx.x;
+	if(i >= nb)
+		return; // no barrier in this kernel: idle lanes may leave
+	dt_block<LBX, LBY, UPD, RO>(a, i, nb, tile + threadIdx.x);
+}
+
 // ======================================================================
 // SCABit half-sweep (ENC_CBBIT)
 // ======================================================================
+// Site row y of one half-sweep, lane g of the row's group of G = 1 << a.lG
+// lanes (groups are aligned to G within the wave; every lane of the wave
+// must call this, the shuffles are wave-wide).
 template<typename W, int MODE>
-__global__ __launch_bounds__(256) void cbbit_half_kernel(CbHalfArgs a)
+__device__ __forceinline__ void cbbit_row(const CbHalfArgs& a, int y, int g)
 {
 	constexpr int WB = sizeof(W) * 8;
 	constexpr int LW = (WB == 64) ? 7 : 6; // log2 sites per word (two sublattices)
 	const int lG = a.lG, G = 1 << lG;
-	const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	const int Y = 1 << a.ly;
-	const int y = (int)(gid >> lG);
-	const int g = (int)(gid & (G - 1));
 	W* __restrict__ lat = (W*)a.lat;
 	const int Wn = 1 << (a.lx - LW); // words per row and quadrant
 	const int M = Wn >> lG;
@@ -367,6 +375,13 @@ __global__ __launch_bounds__(256) void cbbit_half_kernel(CbHalfArgs a)
 		lat[othB + row] = x0old ^ (W)(f0 << 1) ^ pending;
 }
 
+template<typename W, int MODE>
+__global__ __launch_bounds__(256) void cbbit_half_kernel(CbHalfArgs a)
+{
+	const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	cbbit_row<W, MODE>(a, (int)(gid >> a.lG), (int)(gid & ((1 << a.lG) - 1)));
+}
+
 // ======================================================================
 // row statistics (roughness / correlator)
 // ======================================================================
@@ -492,14 +507,12 @@ __device__ __forceinline__ int64_t wave_sum(int64_t v)
 	return v;
 }
 
+// One wave computes the statistics of site row y; lane 0 writes them to o[]
+// (ROW_STATS values, or the first 4 when !CORR and `short4` is set).
 template<int ENC, bool CORR>
-__global__ __launch_bounds__(256) void row_stats_kernel(const uint32_t* __restrict__ lat, const uint32_t* __restrict__ snap,
-		int lx, int ly, int64_t* __restrict__ out)
+__device__ __forceinline__ void row_stats_row(const uint32_t* __restrict__ lat, const uint32_t* __restrict__ snap,
+		int lx, int ly, int y, int lane, int64_t* __restrict__ o, bool short4)
 {
-	const int lane = threadIdx.x & 63;
-	const int y = blockIdx.x * 4 + (threadIdx.x >> 6);
-	if(y >= (1 << ly))
-		return; // uniform per wave
 	const int nbits = (lx >= 6) ? 64 : (1 << lx);
 	const int nch = (lx >= 6) ? (1 << (lx - 6)) : 1;
 	int64_t S1 = 0, S2 = 0, T1 = 0, T2 = 0, S12 = 0, tcs = 0;
@@ -561,11 +574,12 @@ __global__ __launch_bounds__(256) void row_stats_kernel(const uint32_t* __restri
 	{
 		// site x = 0 sits at the end of the row walk: P_0 = D (all x slopes)
 		const int64_t D = run + d0;
-		int64_t* o = out + (size_t)y * ROW_STATS;
 		o[0] = S1 + D;
 		o[1] = S2 + D * D;
 		o[2] = D;
 		o[3] = ybit0<ENC>(lat, lx, ly, y);
+		if(short4)
+			return;
 		if(CORR)
 		{
 			const int64_t Ds = runS + d0s;
@@ -583,6 +597,16 @@ __global__ __launch_bounds__(256) void row_stats_kernel(const uint32_t* __restri
 	}
 }
 
+template<int ENC, bool CORR>
+__global__ __launch_bounds__(256) void row_stats_kernel(const uint32_t* __restrict__ lat, const uint32_t* __restrict__ snap,
+		int lx, int ly, int64_t* __restrict__ out)
+{
+	const int y = blockIdx.x * 4 + (threadIdx.x >> 6);
+	if(y >= (1 << ly))
+		return; // uniform per wave
+	row_stats_row<ENC, CORR>(lat, snap, lx, ly, y, threadIdx.x & 63, out + (size_t)y * ROW_STATS, false);
+}
+
 // ---------------------------------------------------------------- fill
 __global__ void fill_kernel(uint32_t* lat, uint64_t n, uint64_t split, uint32_t lo, uint32_t hi)
 {
@@ -596,6 +620,144 @@ __global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst, cons
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
 	for(uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
 		dst[k] = src[k];
+}
+
+// ======================================================================
+// batched replicas
+// ======================================================================
+// One DT phase of every DT/DTr replica: lane -> {replica, block}.  The host
+// keeps each wave within one (kind, mode) class, so the dispatch below is
+// wave-uniform.  Block layout 16 x 16 sites.
+__global__ __launch_bounds__(DT_WG) void batch_dt_kernel(BatchDtArgs a)
+{
+	__shared__ uint2 tile[dt_tile_rows<4, 4>() * DT_WG];
+	const int64_t lane = (int64_t)blockIdx.x * DT_WG + threadIdx.x;
+	if(lane >= a.nlanes)
+		return;
+	const uint2 w = a.lanes[lane];
+	if(w.x == 0xFFFFFFFFu)
+		return;
+	const BatchRep& rep = a.reps[w.x];
+	DtPhaseArgs p;
+	p.lat = rep.lat;
+	p.lx = rep.lx;
+	p.ly = rep.ly;
+	p.lbx = p.lby = 4;
+	p.st_in = p.st_out = rep.st[0];
+	p.jumps = nullptr;
+	p.step = rep.step;
+	p.tp = rep.tp;
+	p.tq = rep.tq;
+	const size_t mi = (size_t)a.m * a.nreps + w.x;
+	p.ds = (a.order[mi] >> (2 * a.d)) & 3;
+	const int64_t nb = 1ll << (rep.lx + rep.ly - 10);
+	p.T = nb;
+	uint2* my = tile + threadIdx.x;
+	if(rep.kind == BK_DTR)
+	{
+		const uint16_t sh = a.shift[mi];
+		p.sx = sh & 0xFF;
+		p.sy = sh >> 8;
+		if(rep.mode)
+			dt_block<4, 4, 1, true>(p, w.y, nb, my);
+		else
+			dt_block<4, 4, 0, true>(p, w.y, nb, my);
+	}
+	else
+	{
+		p.sx = p.sy = 0;
+		if(rep.mode)
+			dt_block<4, 4, 1, false>(p, w.y, nb, my);
+		else
+			dt_block<4, 4, 0, false>(p, w.y, nb, my);
+	}
+}
+
+// One half-sweep of every SCABit replica: wave -> {replica, first row}; the
+// wave covers 64 >> lG rows of G = 1 << lG lanes each.
+__global__ __launch_bounds__(256) void batch_cbbit_kernel(BatchCbArgs a)
+{
+	const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	if(gw >= a.nwaves)
+		return; // whole wave
+	const uint2 w = a.waves[gw];
+	const BatchRep& rep = a.reps[w.x];
+	const int lane = threadIdx.x & 63;
+	CbHalfArgs p;
+	p.lat = rep.lat;
+	p.lx = rep.lx;
+	p.ly = rep.ly;
+	p.signum = a.signum;
+	p.mode = rep.mode;
+	p.st_in = rep.st[a.buf];
+	p.st_out = rep.st[a.buf ^ 1];
+	p.T = 1ll << rep.ly;
+	p.per = 1;
+	p.rem = 0;
+	p.rowJ = rep.rowJ;
+	p.laneJ = rep.laneJ;
+	p.stepJ = rep.stepJ;
+	p.step = rep.step;
+	p.fullBig = p.fullSmall = rep.full;
+	p.tp = rep.tp;
+	p.tq = rep.tq;
+	p.lG = rep.lG;
+	const int y = (int)w.y + (lane >> rep.lG);
+	const int g = lane & ((1 << rep.lG) - 1);
+	if(rep.wide)
+	{
+		if(rep.mode)
+			cbbit_row<uint64_t, 1>(p, y, g);
+		else
+			cbbit_row<uint64_t, 0>(p, y, g);
+	}
+	else
+	{
+		if(rep.mode)
+			cbbit_row<uint32_t, 1>(p, y, g);
+		else
+			cbbit_row<uint32_t, 0>(p, y, g);
+	}
+}
+
+// per-row roughness statistics of every replica (one wave per row)
+__global__ __launch_bounds__(256) void batch_rowstats_kernel(const BatchRep* __restrict__ reps, const uint2* __restrict__ rows,
+		int64_t nrows, int64_t* __restrict__ scratch)
+{
+	const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	if(gw >= nrows)
+		return;
+	const uint2 w = rows[gw];
+	const BatchRep& rep = reps[w.x];
+	int64_t* o = scratch + 4 * (rep.rowBase + w.y);
+	if(rep.kind == BK_SCABIT)
+		row_stats_row<ENC_CBBIT, false>(rep.lat, nullptr, rep.lx, rep.ly, (int)w.y, threadIdx.x & 63, o, true);
+	else
+		row_stats_row<ENC_LOCAL, false>(rep.lat, nullptr, rep.lx, rep.ly, (int)w.y, threadIdx.x & 63, o, true);
+}
+
+// exact combination per replica (one thread per replica), int128 sums
+__global__ void batch_combine_kernel(const BatchRep* __restrict__ reps, int nreps, const int64_t* __restrict__ scratch,
+		int64_t* __restrict__ out)
+{
+	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	if(r >= nreps)
+		return;
+	const BatchRep& rep = reps[r];
+	const int64_t L = 1ll << rep.lx;
+	const int64_t* s = scratch + 4 * rep.rowBase;
+	int64_t prev = 0;
+	__int128 s1 = 0, s2 = 0;
+	for(int y = 0; y < (1 << rep.ly); ++y, s += 4)
+	{
+		const int64_t hs = prev + (s[3] ? 1 : -1);
+		s1 += (__int128)L * hs + s[0];
+		s2 += (__int128)(L * hs) * hs + (__int128)(2 * hs) * s[0] + s[1];
+		prev = hs + s[2];
+	}
+	out[3 * r] = (int64_t)s1;
+	out[3 * r + 1] = (int64_t)(uint64_t)s2;
+	out[3 * r + 2] = (int64_t)(s2 >> 64);
 }
 
 template<int LBX, int LBY>
@@ -701,6 +863,32 @@ hipError_t launch_fill_flat(uint32_t* lat, int enc, int lx, int ly, hipStream_t 
 		hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, stream, lat, n, n, 0x33CC33CCu, 0x33CC33CCu);
 	else
 		hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, stream, lat, n, n >> 1, 0u, 0xFFFFFFFFu);
+	return hipGetLastError();
+}
+
+hipError_t launch_batch_dt(const BatchDtArgs& a, hipStream_t stream)
+{
+	if(a.nlanes <= 0)
+		return hipSuccess;
+	hipLaunchKernelGGL(batch_dt_kernel, dim3((unsigned)((a.nlanes + DT_WG - 1) / DT_WG)), dim3(DT_WG), 0, stream, a);
+	return hipGetLastError();
+}
+
+hipError_t launch_batch_cbbit(const BatchCbArgs& a, hipStream_t stream)
+{
+	if(a.nwaves <= 0)
+		return hipSuccess;
+	hipLaunchKernelGGL(batch_cbbit_kernel, dim3((unsigned)((a.nwaves + 3) / 4)), dim3(256), 0, stream, a);
+	return hipGetLastError();
+}
+
+hipError_t launch_batch_roughness(const BatchRep* reps, int nreps, const uint2* rows, int64_t nrows,
+		int64_t* scratch, int64_t* out, hipStream_t stream)
+{
+	if(nrows > 0)
+		hipLaunchKernelGGL(batch_rowstats_kernel, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0, stream, reps, rows, nrows, scratch);
+	if(nreps > 0)
+		hipLaunchKernelGGL(batch_combine_kernel, dim3((unsigned)((nreps + 63) / 64)), dim3(64), 0, stream, reps, nreps, scratch, out);
 	return hipGetLastError();
 }
 

@@ -54,6 +54,52 @@ constexpr int ROW_STATS = 10;
 hipError_t launch_row_stats(const uint32_t* lat, const uint32_t* snap, int enc, int lx, int ly,
 		int64_t* out, hipStream_t stream);
 
+// ---- batched replicas ------------------------------------------------------
+// One device-table entry per replica of a batch.  DT replicas use one stream
+// per block (in place); SCABit replicas one stream per row (ping-pong st[0/1]).
+enum BatchKind { BK_DT = 0, BK_DTR = 1, BK_SCABIT = 2 };
+struct BatchRep {
+	uint32_t* lat;
+	uint64_t* st[2];
+	const Affine* laneJ;    // SCABit: laneJ[g] = J(g * draws per word)
+	const Affine* rowJ;     // SCABit: {identity} (one stream per row)
+	Affine step;            // per-draw step (skip form)
+	Affine stepJ;           // SCABit: J(draws per word * (G-1))
+	Affine full;            // SCABit: J(draws per row)
+	int lx, ly;
+	int kind;               // BatchKind
+	int mode;               // DT: 0 KPZ (pqOneUpdate), 1 pq; SCABit: 0 bit-parallel, 1 per-site
+	int lG;                 // SCABit: log2 lanes per row
+	int wide;               // SCABit: 64-bit words (else 32-bit)
+	uint32_t tp, tq;
+	int64_t rowBase;        // first row of this replica in batch row tables
+};
+
+struct BatchDtArgs {
+	const BatchRep* reps;
+	const uint2* lanes;     // per lane {replica, block}; replica ~0u = idle lane
+	const uint8_t* order;   // [m * nreps + r]: domain order, 2 bits per position
+	const uint16_t* shift;  // [m * nreps + r]: DTr origin shift, sx | sy << 8
+	int nreps;
+	int64_t nlanes;
+	int m, d;               // MCS within the chunk, position in the domain order
+};
+hipError_t launch_batch_dt(const BatchDtArgs& a, hipStream_t stream);
+
+struct BatchCbArgs {
+	const BatchRep* reps;
+	const uint2* waves;     // per wave {replica, first row}
+	int64_t nwaves;
+	int signum;
+	int buf;                // read st[buf], write st[buf ^ 1]
+};
+hipError_t launch_batch_cbbit(const BatchCbArgs& a, hipStream_t stream);
+
+// exact roughness of every replica: rows[] = per-wave {replica, y} over all
+// rows, scratch = 4 int64 per row, out[3 r..] = {sum h, sum h^2 lo, hi}
+hipError_t launch_batch_roughness(const BatchRep* reps, int nreps, const uint2* rows, int64_t nrows,
+		int64_t* scratch, int64_t* out, hipStream_t stream);
+
 // ---- utilities -------------------------------------------------------------
 hipError_t launch_fill_flat(uint32_t* lat, int enc, int lx, int ly, hipStream_t stream);
 // dst <- src, 16-byte vector stream copy (bytes must be a multiple of 16)

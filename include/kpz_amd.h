@@ -114,6 +114,49 @@ uint64_t kpz_skip(const kpz_scheduler* h);
 void* kpz_device_lattice(const kpz_scheduler* h);
 void* kpz_stream(const kpz_scheduler* h);
 
+/* ---------------------------------------------------------------------
+ * Batched replicas (no counterpart in the reference: CudaKpz runs one
+ * lattice per process).  Each replica is bit-identical to a stand-alone
+ * scheduler of the same kind, size, seed and (p, q) with default threads
+ * and 16x16 DT blocks: DT/DTr with (p, q) = (1, 0) run kpz_mcs, every other
+ * replica kpz_mcs_pq.  All replicas advance together; chunks of MCS are
+ * replayed from captured hipGraphs.
+ * ------------------------------------------------------------------- */
+typedef struct kpz_batch kpz_batch;
+
+typedef struct kpz_replica {
+	int kind;          /* KPZ_DT, KPZ_DTR or KPZ_SCABIT */
+	int lx, ly;        /* log2 sizes */
+	double p, q;
+	uint64_t seed;     /* as kpz_set_seed */
+} kpz_replica;
+
+/* performance knobs; no knob changes any result */
+typedef struct kpz_batch_tuning {
+	int max_lanes_log2; /* SCABit lanes per row = min(words per row, 2^this), 0..6 */
+	int pq_words64;     /* SCABit per-site replicas on 64-bit (1) or 32-bit (0) words */
+	int chunk;          /* MCS per captured graph */
+	int concurrent;     /* DT and SCABit work on two streams */
+	int sort_by_size;   /* large replicas first */
+} kpz_batch_tuning;
+
+/* tuning = NULL: defaults {6, 1, 50, 1, 1} */
+int kpz_batch_create(const kpz_replica* reps, int n, const kpz_batch_tuning* tuning, int device, kpz_batch** out);
+int kpz_batch_destroy(kpz_batch* b);
+int kpz_batch_init_flat(kpz_batch* b);
+int kpz_batch_mcs(kpz_batch* b, int n);
+/* out: 3 int64 per replica = {sum h, sum h^2 low 64, high 64} */
+int kpz_batch_roughness(kpz_batch* b, int64_t* out);
+int kpz_batch_download(kpz_batch* b, int replica, uint32_t* lattice);
+int kpz_batch_get_streams(kpz_batch* b, int replica, uint64_t* states);
+int64_t kpz_batch_threads(const kpz_batch* b, int replica);
+uint64_t kpz_batch_sites(const kpz_batch* b);
+int kpz_batch_sync(kpz_batch* b);
+void* kpz_batch_stream(const kpz_batch* b);
+/* autotuner: times `mcs` MCS per candidate on throwaway batches of the same
+ * shapes; *tuning is the start point and receives the best knobs */
+int kpz_batch_autotune(const kpz_replica* reps, int n, int mcs, int device, kpz_batch_tuning* tuning, double* ms_per_mcs);
+
 #ifdef __cplusplus
 }
 #endif
