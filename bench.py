@@ -109,6 +109,85 @@ def run_scabit(K, lx, p, q, steps, seed, ev):
     return ms
 
 
+def batch_mix(n=4096, base_seed=1000, umin=6, umax=11, rng_seed=4096):
+    """4096 replicas, L = 2^u with u uniform in {6..11}, half DT / half SCABit,
+    half KPZ (p=1, q=0) / half EW (p=q=.5), seed_i = base + i."""
+    import random
+    import cudakpz_amd as K
+    r = random.Random(rng_seed)
+    reps = []
+    for i in range(n):
+        u = r.randint(umin, umax)
+        kind = K.KPZ_DT if i % 2 == 0 else K.KPZ_SCABIT
+        p, q = (1.0, 0.0) if (i // 2) % 2 == 0 else (0.5, 0.5)
+        reps.append((kind, u, u, p, q, base_seed + i))
+    return reps
+
+
+def run_batch_bench(K, ev, mcs, serial_mcs):
+    reps = batch_mix()
+    t0 = time.perf_counter()
+    tuning, tune_ms = K.Batch.autotune(reps, mcs=4)
+    tune_s = time.perf_counter() - t0
+    # serial: one stand-alone scheduler per replica, one lattice per launch
+    cls = {K.KPZ_DT: K.SchedulerDT, K.KPZ_SCABIT: K.SchedulerSCABit}
+    alone = []
+    for kind, lx, ly, p, q, seed in reps:
+        s = cls[kind](lx, ly)
+        s.supplySeed(seed)
+        s.initHomogeneous()
+        if not (kind == K.KPZ_DT and (p, q) == (1.0, 0.0)):
+            s.setPQ(p, q)
+        alone.append(s)
+    for s in alone:
+        s.sync()
+    t0 = time.perf_counter()
+    for (kind, lx, ly, p, q, seed), s in zip(reps, alone):
+        if kind == K.KPZ_DT and (p, q) == (1.0, 0.0):
+            s.mcs(serial_mcs)
+        else:
+            s.mcsPQSyncRng(serial_mcs)
+        s.sync()
+    serial_s = time.perf_counter() - t0
+    # batched: same replicas, identical results after serial_mcs
+    b = K.Batch(reps, tuning)
+    b.initHomogeneous()
+    b.mcs(serial_mcs)
+    b.sync()
+    same = sum(int(np.array_equal(b.popSystem(i), s.popSystem())) for i, s in enumerate(alone))
+    for s in alone:
+        s.close()
+    del alone
+    samples = set(sample_times(mcs))
+    e0, e1 = ev.event(), ev.event()
+    ev.record(e0, b.stream)
+    done, w2 = 0, []
+    while done < mcs:
+        nxt = min([t for t in samples if t > done] + [mcs])
+        b.mcs(nxt - done)
+        done = nxt
+        if done in samples:
+            rs = b.roughness()
+            w2.append((done + serial_mcs, float(np.mean([r.w2 for r in rs]))))
+    ev.record(e1, b.stream)
+    batch_ms = ev.ms(e0, e1)
+    sites = b.sites
+    b.close()
+    n = len(reps)
+    batch_rate = n * mcs / (batch_ms * 1e-3)
+    serial_rate = n * serial_mcs / serial_s
+    return {"replicas": n, "sites": sites, "mcs": mcs, "replica_mcs_per_s": batch_rate,
+            "site_updates_per_s": sites * mcs / (batch_ms * 1e-3),
+            "serial_replica_mcs_per_s": serial_rate, "serial_mcs": serial_mcs,
+            "speedup_vs_serial": batch_rate / serial_rate,
+            "identical_to_standalone": f"{same}/{n}", "tuning": tuning.as_dict(),
+            "autotune_s": tune_s, "ensemble_w2_final": w2[-1][1] if w2 else None,
+            "ensemble_w2_samples": len(w2)}
+
+
+import numpy as np  # noqa: E402
+
+
 def cpu_baseline(seconds=12.0):
     """Oracle SchedulerCPUDT restatement, one core, 2048^2 sample, same layout
     (16x16 blocks, one stream per block)."""
@@ -136,6 +215,9 @@ def main():
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--secondary", action="store_true", help="also time SCABit at 65536^2")
+    ap.add_argument("--batch", type=int, default=2000, metavar="MCS",
+                    help="batched 4096-replica mix for MCS steps vs serial per-replica launches (0: skip)")
+    ap.add_argument("--serial-mcs", type=int, default=10, help="MCS of the serial per-replica baseline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="K",
                     help="after the run, K lattice copies (1 GiB read + 1 GiB write each) whose "
                          "FETCH_SIZE/WRITE_SIZE calibrate the counters")
@@ -192,6 +274,8 @@ def main():
             sec[name] = {"site_updates_per_s": ups, "hbm_GBs_algorithmic": ups * 1.0 / 1e9,
                          "hbm_frac": ups / 1e9 / HBM_PEAK_GBS, "ms_per_mcs": ms / steps}
         line["secondary"] = sec
+    if args.batch:
+        line["batched"] = run_batch_bench(K, ev, args.batch, args.serial_mcs)
     if args.copy_calibration:
         src = K.SchedulerDT(args.lx, args.lx, layout=(4, 4), threads=1)
         dst = K.SchedulerDT(args.lx, args.lx, layout=(4, 4), threads=1)
