@@ -185,6 +185,54 @@ def run_batch_bench(K, ev, mcs, serial_mcs):
             "ensemble_w2_samples": len(w2)}
 
 
+def run_serve_bench(K, window_s, rate=200.0, seed=5, kind_name="scaBit", budget=64 << 30):
+    """Poisson stream of jobs (rate/s for window_s seconds): L = 2^u, u in
+    {8..13}; MCS log-uniform 10^2..10^4; p ~ U[.5, 1], q = 1 - p; random u64
+    seeds.  Latency = completion - scheduled arrival."""
+    rng = np.random.default_rng(seed)
+    kind = {"scaBit": K.KPZ_SCABIT, "dt": K.KPZ_DTR, "dtStatic": K.KPZ_DT}[kind_name]
+    gaps = rng.exponential(1.0 / rate, int(rate * window_s * 3) + 10)
+    arr = np.cumsum(gaps)
+    arr = arr[arr < window_s]
+    n = len(arr)
+    us = rng.integers(8, 14, n)
+    mcs = np.round(np.exp(rng.uniform(np.log(100), np.log(10000), n))).astype(int)
+    ps = rng.uniform(0.5, 1.0, n)
+    seeds = rng.integers(0, 2 ** 63, n, dtype=np.int64).astype(np.uint64) * np.uint64(2) + np.uint64(1)
+    srv = K.Server(hbm_budget=budget, max_slots=1024, max_tick_mcs=16)
+    base = srv.now() + 0.05
+    arr = arr + base
+    due = {}
+    lat, work, i, first_done, last_done = [], 0, 0, None, None
+    while True:
+        now = srv.now()
+        while i < n and arr[i] <= now:
+            jid = srv.submit(kind, int(us[i]), int(us[i]), int(mcs[i]), float(ps[i]), 1.0 - float(ps[i]), int(seeds[i]))
+            due[jid] = arr[i]
+            work += int(mcs[i]) << (2 * int(us[i]))
+            i += 1
+        left = srv.step()
+        while True:
+            r = srv.pop()
+            if r is None:
+                break
+            lat.append(r.t_end - due.pop(r.id))
+            last_done = r.t_end
+        if i == n and left == 0:
+            break
+        if left == 0 and i < n:
+            time.sleep(max(0.0, arr[i] - srv.now()))
+    st = srv.stats()
+    srv.close()
+    lat = np.array(lat)
+    span = last_done - base
+    return {"jobs": n, "kind": kind_name, "rate_per_s": rate, "window_s": window_s,
+            "offered_site_updates": work, "site_updates_per_s": st["site_updates"] / span,
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+            "latency_max_s": float(lat.max()), "makespan_s": span,
+            "offered_load_vs_capacity": (work / window_s) / (st["site_updates"] / span)}
+
+
 import numpy as np  # noqa: E402
 
 
@@ -218,6 +266,8 @@ def main():
     ap.add_argument("--batch", type=int, default=2000, metavar="MCS",
                     help="batched 4096-replica mix for MCS steps vs serial per-replica launches (0: skip)")
     ap.add_argument("--serial-mcs", type=int, default=10, help="MCS of the serial per-replica baseline")
+    ap.add_argument("--serve", type=float, default=2.0, metavar="SECONDS",
+                    help="serving loop: Poisson arrivals at 200/s for SECONDS (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="K",
                     help="after the run, K lattice copies (1 GiB read + 1 GiB write each) whose "
                          "FETCH_SIZE/WRITE_SIZE calibrate the counters")
@@ -276,6 +326,8 @@ def main():
         line["secondary"] = sec
     if args.batch:
         line["batched"] = run_batch_bench(K, ev, args.batch, args.serial_mcs)
+    if args.serve > 0:
+        line["serving"] = run_serve_bench(K, args.serve)
     if args.copy_calibration:
         src = K.SchedulerDT(args.lx, args.lx, layout=(4, 4), threads=1)
         dst = K.SchedulerDT(args.lx, args.lx, layout=(4, 4), threads=1)

@@ -451,6 +451,112 @@ class Batch:
         return t, ms.value
 
 
+class _Job(C.Structure):
+    _fields_ = [("kind", C.c_int), ("lx", C.c_int), ("ly", C.c_int), ("mcs", C.c_int), ("p", C.c_double),
+                ("q", C.c_double), ("seed", C.c_uint64), ("keep_lattice", C.c_int)]
+
+
+@dataclass
+class JobResult:
+    """A finished serving job: exact sums at its sample times, timestamps
+    (s on the server clock) and optionally the final lattice."""
+    id: int
+    N: int
+    t_submit: float
+    t_start: float
+    t_end: float
+    series: list  # [(mcs, Roughness)]
+    lattice: object = None
+
+    @property
+    def w2(self):
+        return [(m, r.w2) for m, r in self.series]
+
+
+def _declare_server(L):
+    if getattr(L, "_server_declared", False):
+        return
+    h = C.c_void_p
+    L.kpz_server_create.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.POINTER(h)]
+    L.kpz_server_destroy.argtypes = [h]
+    L.kpz_server_submit.argtypes = [h, C.POINTER(_Job), C.POINTER(C.c_uint64)]
+    L.kpz_server_step.argtypes = [h]
+    L.kpz_server_step.restype = C.c_int64
+    L.kpz_server_peek.argtypes = [h, C.POINTER(C.c_int), C.POINTER(C.c_uint64)]
+    L.kpz_server_pop.argtypes = [h, C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_int64), C.c_void_p]
+    L.kpz_server_stats.argtypes = [h, C.POINTER(C.c_uint64)]
+    L.kpz_server_now.argtypes = [h]
+    L.kpz_server_now.restype = C.c_double
+    L._server_declared = True
+
+
+class Server:
+    """Serving loop: submit jobs at any time, call step() repeatedly, pop()
+    finished results.  Jobs share batched launches (continuous batching) and
+    are admitted as HBM frees up."""
+
+    def __init__(self, hbm_budget=8 << 30, max_slots=256, max_tick_mcs=16, device=-1):
+        L = lib()
+        _declare_server(L)
+        h = C.c_void_p()
+        _check(L.kpz_server_create(int(hbm_budget), int(max_slots), int(max_tick_mcs), int(device), C.byref(h)))
+        self._h = h
+        self._specs = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kpz_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def now(self):
+        return float(lib().kpz_server_now(self._h))
+
+    def submit(self, kind, lx, ly, mcs, p, q, seed, keep_lattice=False):
+        j = _Job(int(kind), int(lx), int(ly), int(mcs), float(p), float(q), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                 int(bool(keep_lattice)))
+        i = C.c_uint64()
+        _check(lib().kpz_server_submit(self._h, C.byref(j), C.byref(i)))
+        self._specs[i.value] = (kind, lx, ly, mcs, p, q, seed)
+        return i.value
+
+    def step(self):
+        r = lib().kpz_server_step(self._h)
+        if r < 0:
+            _check(int(r))
+        return int(r)
+
+    def pop(self):
+        n = C.c_int()
+        words = C.c_uint64()
+        if not lib().kpz_server_peek(self._h, C.byref(n), C.byref(words)):
+            return None
+        i = C.c_uint64()
+        times = (C.c_double * 3)()
+        mcs = (C.c_uint32 * max(n.value, 1))()
+        sums = (C.c_int64 * max(3 * n.value, 1))()
+        lat = np.empty(words.value, np.uint32) if words.value else None
+        _check(lib().kpz_server_pop(self._h, C.byref(i), times, mcs, sums,
+                                    lat.ctypes.data if lat is not None else None))
+        spec = self._specs.pop(i.value)
+        N = 1 << (spec[1] + spec[2])
+        series = [(int(mcs[k]), Roughness(N, int(sums[3 * k]), _i128(sums[3 * k + 1], sums[3 * k + 2])))
+                  for k in range(n.value)]
+        return JobResult(i.value, N, times[0], times[1], times[2], series, lat)
+
+    def stats(self):
+        out = (C.c_uint64 * 5)()
+        _check(lib().kpz_server_stats(self._h, out))
+        return {"queued": out[0], "active": out[1], "hbm_used": out[2], "hbm_capacity": out[3],
+                "site_updates": out[4]}
+
+
 def make_scheduler(name, lx, ly, **kw):
     """SchedulerConfig::mkScheduler by scheduler id (kpz/kpzFrontend.cpp:93-250)."""
     table = {"dtStatic": SchedulerDT, "dt": SchedulerDTr, "scaBit": SchedulerSCABit}

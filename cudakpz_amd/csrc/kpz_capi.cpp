@@ -2,6 +2,7 @@
 #include "../../include/kpz_amd.h"
 #include "kpz_batch.hpp"
 #include "kpz_scheduler.hpp"
+#include "kpz_server.hpp"
 
 #include <cstring>
 #include <string>
@@ -12,6 +13,10 @@ struct kpz_scheduler {
 
 struct kpz_batch {
 	std::unique_ptr<kpz::Batch> b;
+};
+
+struct kpz_server {
+	std::unique_ptr<kpz::Server> s;
 };
 
 namespace {
@@ -364,6 +369,127 @@ int kpz_batch_sync(kpz_batch* b)
 }
 
 void* kpz_batch_stream(const kpz_batch* b) { return (b && b->b) ? (void*)b->b->stream() : nullptr; }
+
+// ---------------------------------------------------------------- server
+#define SHANDLE(h)                                    \
+	if(!(h) || !(h)->s)                               \
+		return fail(KPZ_EINVAL, "null server handle");
+
+int kpz_server_create(uint64_t hbm_budget_bytes, int max_slots, int max_tick_mcs, int device, kpz_server** out)
+{
+	if(!out)
+		return fail(KPZ_EINVAL, "null output pointer");
+	*out = nullptr;
+	return guard([&] {
+		auto h = new kpz_server;
+		try
+		{
+			h->s = std::make_unique<kpz::Server>((size_t)hbm_budget_bytes, max_slots, max_tick_mcs, device);
+		}
+		catch(...)
+		{
+			delete h;
+			throw;
+		}
+		*out = h;
+	});
+}
+
+int kpz_server_destroy(kpz_server* s)
+{
+	if(!s)
+		return KPZ_OK;
+	return guard([&] { delete s; });
+}
+
+int kpz_server_submit(kpz_server* s, const kpz_job* job, uint64_t* id)
+{
+	SHANDLE(s);
+	if(!job)
+		return fail(KPZ_EINVAL, "null job");
+	return guard([&] {
+		const uint64_t i = s->s->submit(kpz::JobSpec{job->kind, job->lx, job->ly, job->mcs, job->p, job->q, job->seed,
+				job->keep_lattice});
+		if(id)
+			*id = i;
+	});
+}
+
+int64_t kpz_server_step(kpz_server* s)
+{
+	SHANDLE(s);
+	int64_t left = 0;
+	const int rc = guard([&] { left = (int64_t)s->s->step(); });
+	return rc ? rc : left;
+}
+
+namespace {
+thread_local kpz::JobResult g_pending;
+thread_local const kpz_server* g_pendingOwner = nullptr;
+}
+
+int kpz_server_peek(kpz_server* s, int* nsamples, uint64_t* lattice_words)
+{
+	SHANDLE(s);
+	if(g_pendingOwner != s)
+	{
+		if(!s->s->pop(g_pending))
+			return 0;
+		g_pendingOwner = s;
+	}
+	if(nsamples)
+		*nsamples = (int)g_pending.series.size();
+	if(lattice_words)
+		*lattice_words = g_pending.lattice.size();
+	return 1;
+}
+
+int kpz_server_pop(kpz_server* s, uint64_t* id, double times[3], uint32_t* mcs, int64_t* sums, uint32_t* lattice)
+{
+	SHANDLE(s);
+	if(g_pendingOwner != s && !kpz_server_peek(s, nullptr, nullptr))
+		return fail(KPZ_ESTATE, "no finished job");
+	const kpz::JobResult& r = g_pending;
+	if(id)
+		*id = r.id;
+	if(times)
+	{
+		times[0] = r.tSubmit;
+		times[1] = r.tStart;
+		times[2] = r.tEnd;
+	}
+	for(size_t i = 0; i < r.series.size(); ++i)
+	{
+		if(mcs)
+			mcs[i] = r.series[i].mcs;
+		if(sums)
+		{
+			sums[3 * i] = r.series[i].sumH;
+			sums[3 * i + 1] = (int64_t)(uint64_t)r.series[i].sumH2;
+			sums[3 * i + 2] = (int64_t)(r.series[i].sumH2 >> 64);
+		}
+	}
+	if(lattice && !r.lattice.empty())
+		std::memcpy(lattice, r.lattice.data(), r.lattice.size() * 4);
+	g_pendingOwner = nullptr;
+	g_pending = kpz::JobResult{};
+	return KPZ_OK;
+}
+
+int kpz_server_stats(kpz_server* s, uint64_t out[5])
+{
+	SHANDLE(s);
+	if(!out)
+		return fail(KPZ_EINVAL, "null output");
+	out[0] = s->s->queued();
+	out[1] = s->s->active();
+	out[2] = s->s->hbmUsed();
+	out[3] = s->s->hbmCapacity();
+	out[4] = s->s->siteUpdates();
+	return KPZ_OK;
+}
+
+double kpz_server_now(const kpz_server* s) { return (s && s->s) ? s->s->now() : -1.0; }
 
 int kpz_batch_autotune(const kpz_replica* reps, int n, int mcs, int device, kpz_batch_tuning* tuning, double* ms_per_mcs)
 {

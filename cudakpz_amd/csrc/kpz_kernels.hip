@@ -637,6 +637,8 @@ __global__ __launch_bounds__(DT_WG) void batch_dt_kernel(BatchDtArgs a)
 	const uint2 w = a.lanes[lane];
 	if(w.x == 0xFFFFFFFFu)
 		return;
+	if(a.limit && a.m >= a.limit[w.x])
+		return;
 	const BatchRep& rep = a.reps[w.x];
 	DtPhaseArgs p;
 	p.lat = rep.lat;
@@ -681,6 +683,8 @@ __global__ __launch_bounds__(256) void batch_cbbit_kernel(BatchCbArgs a)
 	if(gw >= a.nwaves)
 		return; // whole wave
 	const uint2 w = a.waves[gw];
+	if(a.limit && a.m >= a.limit[w.x])
+		return; // whole wave: one replica per wave
 	const BatchRep& rep = a.reps[w.x];
 	const int lane = threadIdx.x & 63;
 	CbHalfArgs p;

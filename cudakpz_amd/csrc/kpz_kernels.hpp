@@ -83,6 +83,7 @@ struct BatchDtArgs {
 	int nreps;
 	int64_t nlanes;
 	int m, d;               // MCS within the chunk, position in the domain order
+	const int* limit;       // optional [r]: replica r only runs MCS m < limit[r]
 };
 hipError_t launch_batch_dt(const BatchDtArgs& a, hipStream_t stream);
 
@@ -92,6 +93,8 @@ struct BatchCbArgs {
 	int64_t nwaves;
 	int signum;
 	int buf;                // read st[buf], write st[buf ^ 1]
+	int m;                  // MCS within the chunk
+	const int* limit;       // optional [r]: replica r only runs MCS m < limit[r]
 };
 hipError_t launch_batch_cbbit(const BatchCbArgs& a, hipStream_t stream);
 

@@ -1,0 +1,510 @@
+#include "kpz_server.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace kpz {
+
+namespace {
+int ceilLog2(size_t v)
+{
+	int l = 0;
+	while((1ull << l) < v)
+		++l;
+	return l;
+}
+int floorLog2(size_t v)
+{
+	int l = 0;
+	while((1ull << (l + 1)) <= v)
+		++l;
+	return l;
+}
+} // namespace
+
+// ---------------------------------------------------------------- buddy arena
+void BuddyArena::init(size_t bytes, size_t minBlock)
+{
+	minOrder_ = ceilLog2(minBlock < 256 ? 256 : minBlock);
+	maxOrder_ = floorLog2(bytes);
+	if(maxOrder_ < minOrder_)
+		throw Error(-1, "HBM budget smaller than one block");
+	cap_ = 1ull << maxOrder_;
+	buf_.alloc(cap_);
+	free_.assign((size_t)maxOrder_ + 1, {});
+	free_[(size_t)maxOrder_].push_back(0);
+	live_.clear();
+	used_ = 0;
+}
+
+bool BuddyArena::alloc(size_t bytes, uint64_t& off)
+{
+	const int o = std::max(minOrder_, ceilLog2(bytes));
+	if(o > maxOrder_)
+		return false;
+	int k = o;
+	while(k <= maxOrder_ && free_[(size_t)k].empty())
+		++k;
+	if(k > maxOrder_)
+		return false;
+	const uint64_t b = free_[(size_t)k].back();
+	free_[(size_t)k].pop_back();
+	while(k > o)
+	{
+		--k;
+		free_[(size_t)k].push_back(b + (1ull << k));
+	}
+	live_[b] = o;
+	used_ += 1ull << o;
+	off = b;
+	return true;
+}
+
+void BuddyArena::release(uint64_t off)
+{
+	auto it = live_.find(off);
+	if(it == live_.end())
+		return;
+	int o = it->second;
+	live_.erase(it);
+	used_ -= 1ull << o;
+	uint64_t b = off;
+	while(o < maxOrder_)
+	{
+		const uint64_t buddy = b ^ (1ull << o);
+		auto& fl = free_[(size_t)o];
+		auto f = std::find(fl.begin(), fl.end(), buddy);
+		if(f == fl.end())
+			break;
+		fl.erase(f);
+		b = std::min(b, buddy);
+		++o;
+	}
+	free_[(size_t)o].push_back(b);
+}
+
+// ---------------------------------------------------------------- server
+Server::Server(size_t hbmBudget, int maxSlots, int maxTickMcs, int device)
+	: maxSlots_(maxSlots), maxTick_(maxTickMcs < 1 ? 1 : maxTickMcs)
+{
+	if(maxSlots_ < 1)
+		throw Error(-1, "need at least one slot");
+	if(device >= 0)
+		check(hipSetDevice(device), "hipSetDevice");
+	check(hipGetDevice(&device_), "hipGetDevice");
+	check(hipStreamCreateWithFlags(&s0_, hipStreamNonBlocking), "hipStreamCreate");
+	arena_.init(hbmBudget, 4096);
+	slots_.assign((size_t)maxSlots_, nullptr);
+	hostReps_.assign((size_t)maxSlots_, BatchRep{});
+	repTable_.alloc(sizeof(BatchRep) * (size_t)maxSlots_);
+	identity_.alloc(sizeof(Affine));
+	const Affine id{1, 0};
+	check(hipMemcpy(identity_.get(), &id, sizeof id, hipMemcpyHostToDevice), "identity");
+	orderHost_.assign((size_t)maxTick_ * (size_t)maxSlots_, 0);
+	shiftHost_.assign((size_t)maxTick_ * (size_t)maxSlots_, 0);
+	orderBuf_.alloc(orderHost_.size());
+	shiftBuf_.alloc(shiftHost_.size() * 2);
+	limitHost_.assign((size_t)maxSlots_, 0);
+	limitBuf_.alloc(limitHost_.size() * sizeof(int));
+	t0_ = std::chrono::steady_clock::now();
+}
+
+Server::~Server()
+{
+	if(s0_)
+		(void)hipStreamSynchronize(s0_);
+	for(Job* j : queue_)
+		delete j;
+	for(Job* j : slots_)
+		delete j;
+	if(s0_)
+		(void)hipStreamDestroy(s0_);
+}
+
+double Server::now() const
+{
+	return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+}
+
+void Server::ensure(DevBuf& b, size_t bytes)
+{
+	if(b.bytes() < bytes)
+		b.alloc(std::max(bytes, 2 * b.bytes()));
+}
+
+size_t Server::footprint(const JobSpec& s, size_t* parts)
+{
+	parts[0] = 1ull << (s.lx + s.ly - 2);
+	if(s.kind == KIND_SCABIT)
+	{
+		parts[1] = 2 * 8 * (1ull << s.ly);
+		parts[2] = 64 * sizeof(Affine);
+	}
+	else
+	{
+		parts[1] = 8 * (1ull << (s.lx + s.ly - 10));
+		parts[2] = 0;
+	}
+	return parts[0] + parts[1] + parts[2];
+}
+
+uint64_t Server::submit(const JobSpec& j)
+{
+	const bool dt = j.kind == KIND_DT || j.kind == KIND_DTR;
+	if(!(dt || j.kind == KIND_SCABIT))
+		throw Error(-1, "unknown job kind");
+	if(dt && (j.lx < 5 || j.ly < 5 || j.lx > 24 || j.ly > 24))
+		throw Error(-1, "DT jobs need 32..2^24 sites per side");
+	if(!dt && (j.lx < 6 || j.ly < 6 || j.lx > 24 || j.ly > 24))
+		throw Error(-1, "SCABit jobs need 64..2^24 sites per side");
+	if(j.mcs < 1 || !(j.p >= 0 && j.p <= 1 && j.q >= 0 && j.q <= 1))
+		throw Error(-1, "bad job parameters");
+	size_t parts[3];
+	footprint(j, parts);
+	if((1ull << ceilLog2(parts[0])) > arena_.capacity())
+		throw Error(-1, "job larger than the HBM budget");
+	Job* job = new Job;
+	job->id = nextId_++;
+	job->spec = j;
+	job->res.id = job->id;
+	job->res.spec = j;
+	job->res.tSubmit = now();
+	queue_.push_back(job);
+	return job->id;
+}
+
+bool Server::admit(Job& j)
+{
+	size_t parts[3];
+	footprint(j.spec, parts);
+	int k = 0;
+	for(; k < 3; ++k)
+		if(parts[k] && !arena_.alloc(parts[k], j.off[k]))
+			break;
+	if(k < 3)
+	{
+		for(int u = 0; u < k; ++u)
+			if(parts[u])
+				arena_.release(j.off[u]);
+		return false;
+	}
+	const JobSpec& s = j.spec;
+	BatchRep& d = j.dev;
+	d = BatchRep{};
+	d.lx = s.lx;
+	d.ly = s.ly;
+	char* base = arena_.base();
+	d.lat = (uint32_t*)(base + j.off[0]);
+	if(s.kind == KIND_SCABIT)
+	{
+		d.kind = BK_SCABIT;
+		d.mode = (s.p == 0.5 && s.q == 0.0) ? 0 : 1;
+		d.wide = s.lx >= 7 ? 1 : 0;
+		d.lG = std::min(s.lx - (d.wide ? 7 : 6), 6);
+		j.T = 1ll << s.ly;
+	}
+	else
+	{
+		d.kind = s.kind == KIND_DT ? BK_DT : BK_DTR;
+		d.mode = (s.p == 1.0 && s.q == 0.0) ? 0 : 1;
+		j.T = 1ll << (s.lx + s.ly - 10);
+	}
+	d.tp = thresh23((d.kind != BK_SCABIT && d.mode == 0) ? 1.0 : s.p);
+	d.tq = thresh23((d.kind != BK_SCABIT && d.mode == 0) ? 0.0 : s.q);
+	d.step = slcg_skip(default_skip((uint64_t)j.T));
+	d.st[0] = (uint64_t*)(base + j.off[1]);
+	d.st[1] = (d.kind == BK_SCABIT) ? d.st[0] + j.T : d.st[0];
+	d.rowJ = identity_.as<Affine>();
+	d.laneJ = identity_.as<Affine>();
+	// seeding: reSeed + SLCG64::randomize, exactly as a stand-alone scheduler
+	j.master.reseed(seed32(s.seed));
+	const double m1 = j.master.close_open();
+	const double m2 = j.master.close_open();
+	uint64_t x = (uint64_t)(m1 * 4294967295.0) ^ ((uint64_t)(m2 * 4294967295.0) << 32);
+	std::vector<uint64_t> st((size_t)j.T);
+	for(auto& v : st)
+	{
+		v = x;
+		x = x * SLCG_A + SLCG_C;
+	}
+	check(hipMemcpyAsync(d.st[0], st.data(), st.size() * 8, hipMemcpyHostToDevice, s0_), "job streams");
+	if(d.kind == BK_SCABIT)
+	{
+		const uint64_t du = d.mode == 0 ? (d.wide ? 2 : 1) : (d.wide ? 64 : 32);
+		const uint64_t Wn = 1ull << (s.lx - (d.wide ? 7 : 6));
+		const int G = 1 << d.lG;
+		std::vector<Affine> lane((size_t)G);
+		const Affine word = affine_pow(d.step, du);
+		Affine acc{1, 0};
+		for(auto& a : lane)
+		{
+			a = acc;
+			acc = compose(word, acc);
+		}
+		d.laneJ = (const Affine*)(base + j.off[2]);
+		check(hipMemcpyAsync((void*)d.laneJ, lane.data(), lane.size() * sizeof(Affine), hipMemcpyHostToDevice, s0_), "lane table");
+		d.stepJ = affine_pow(d.step, du * (uint64_t)(G - 1));
+		d.full = affine_pow(d.step, du * Wn);
+	}
+	check(launch_fill_flat(d.lat, d.kind == BK_SCABIT ? ENC_CBBIT : ENC_LOCAL, s.lx, s.ly, s0_), "fill");
+	// sample times (kpz/sampling.h:84-100) and the final MCS
+	j.samples.clear();
+	for(double t = 0;;)
+	{
+		t = std::ceil(std::exp(std::log(t + 10.0) + 0.1));
+		if(t >= (double)s.mcs)
+			break;
+		j.samples.push_back((uint32_t)t);
+	}
+	j.samples.push_back((uint32_t)s.mcs);
+	j.next = 0;
+	j.done = 0;
+	j.res.tStart = now();
+	// the host vectors above are pageable: make sure the copies are done
+	check(hipStreamSynchronize(s0_), "admit sync");
+	return true;
+}
+
+void Server::rebuild()
+{
+	std::vector<uint2> lanes, waves;
+	for(int kind : {BK_DT, BK_DTR})
+		for(int mode : {0, 1})
+		{
+			for(int sl = 0; sl < maxSlots_; ++sl)
+			{
+				Job* j = slots_[(size_t)sl];
+				if(!j || j->dev.kind != kind || j->dev.mode != mode)
+					continue;
+				for(int64_t b = 0; b < j->T; ++b)
+					lanes.push_back(make_uint2((unsigned)sl, (unsigned)b));
+			}
+			while(lanes.size() % 64)
+				lanes.push_back(make_uint2(0xFFFFFFFFu, 0));
+		}
+	for(int sl = 0; sl < maxSlots_; ++sl)
+	{
+		Job* j = slots_[(size_t)sl];
+		hostReps_[(size_t)sl] = j ? j->dev : BatchRep{};
+		if(!j || j->dev.kind != BK_SCABIT)
+			continue;
+		const int rpw = 64 >> j->dev.lG;
+		for(int y = 0; y < (1 << j->dev.ly); y += rpw)
+			waves.push_back(make_uint2((unsigned)sl, (unsigned)y));
+	}
+	nLanes_ = (int64_t)lanes.size();
+	nWaves_ = (int64_t)waves.size();
+	check(hipMemcpyAsync(repTable_.get(), hostReps_.data(), sizeof(BatchRep) * hostReps_.size(), hipMemcpyHostToDevice, s0_), "reps");
+	if(nLanes_)
+	{
+		ensure(laneTable_, lanes.size() * sizeof(uint2));
+		check(hipMemcpyAsync(laneTable_.get(), lanes.data(), lanes.size() * sizeof(uint2), hipMemcpyHostToDevice, s0_), "lanes");
+	}
+	if(nWaves_)
+	{
+		ensure(waveTable_, waves.size() * sizeof(uint2));
+		check(hipMemcpyAsync(waveTable_.get(), waves.data(), waves.size() * sizeof(uint2), hipMemcpyHostToDevice, s0_), "waves");
+	}
+	check(hipStreamSynchronize(s0_), "rebuild sync");
+	dirty_ = false;
+}
+
+size_t Server::step()
+{
+	// admission: FIFO with bounded backfill by HBM footprint
+	int looked = 0;
+	for(auto it = queue_.begin(); it != queue_.end() && looked < 64; ++looked)
+	{
+		int sl = -1;
+		for(int k = 0; k < maxSlots_; ++k)
+			if(!slots_[(size_t)k])
+			{
+				sl = k;
+				break;
+			}
+		if(sl < 0)
+			break;
+		Job* j = *it;
+		if(admit(*j))
+		{
+			j->slot = sl;
+			slots_[(size_t)sl] = j;
+			++nActive_;
+			dirty_ = true;
+			it = queue_.erase(it);
+		}
+		else
+			++it;
+	}
+	if(dirty_)
+		rebuild();
+	if(nActive_ == 0)
+		return queue_.size();
+
+	// fixed-length tick: job j runs lim_j = min(maxTick, MCS left) MCS; the
+	// kernels skip a job once m >= lim_j.  Samples inside the tick are taken
+	// in stream order right after their MCS; one synchronisation per tick.
+	const size_t R = (size_t)maxSlots_;
+	std::fill(limitHost_.begin(), limitHost_.end(), 0);
+	int k = 0;
+	for(Job* j : slots_)
+		if(j)
+		{
+			const int lim = std::min<int>(maxTick_, j->spec.mcs - (int)j->done);
+			limitHost_[(size_t)j->slot] = lim;
+			k = std::max(k, lim);
+		}
+	for(int m = 0; m < k; ++m)
+		for(Job* j : slots_)
+		{
+			if(!j || j->dev.kind == BK_SCABIT || m >= limitHost_[(size_t)j->slot])
+				continue;
+			int seq[4];
+			j->master.shuffle4(seq);
+			orderHost_[(size_t)m * R + (size_t)j->slot] = (uint8_t)(seq[0] | (seq[1] << 2) | (seq[2] << 4) | (seq[3] << 6));
+			uint16_t sh = 0;
+			if(j->dev.kind == BK_DTR)
+			{
+				const unsigned sx = (unsigned)(j->master.close_open() * 16.0);
+				const unsigned sy = (unsigned)(j->master.close_open() * 16.0);
+				sh = (uint16_t)(sx | (sy << 8));
+			}
+			shiftHost_[(size_t)m * R + (size_t)j->slot] = sh;
+		}
+	check(hipMemcpyAsync(orderBuf_.get(), orderHost_.data(), (size_t)k * R, hipMemcpyHostToDevice, s0_), "order");
+	check(hipMemcpyAsync(shiftBuf_.get(), shiftHost_.data(), (size_t)k * R * 2, hipMemcpyHostToDevice, s0_), "shift");
+	check(hipMemcpyAsync(limitBuf_.get(), limitHost_.data(), R * sizeof(int), hipMemcpyHostToDevice, s0_), "limit");
+
+	// sample plan: (m, job) for every sample time t in (done, done + lim]
+	std::vector<std::pair<int, Job*>> plan;
+	for(Job* j : slots_)
+		if(j)
+			for(size_t s = j->next; s < j->samples.size() && j->samples[s] <= j->done + (uint32_t)limitHost_[(size_t)j->slot]; ++s)
+				plan.push_back({(int)(j->samples[s] - j->done) - 1, j});
+	std::stable_sort(plan.begin(), plan.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+	std::vector<BatchRep> sreps(plan.size());
+	std::vector<uint2> srows;
+	std::vector<int64_t> rowStart(plan.size() + 1);
+	std::vector<size_t> groupStart; // plan index where each m-group starts
+	int64_t rb = 0;
+	for(size_t i = 0; i < plan.size(); ++i)
+	{
+		if(i == 0 || plan[i].first != plan[i - 1].first)
+		{
+			groupStart.push_back(i);
+			rb = 0; // scratch rows restart per group (groups run in stream order)
+		}
+		sreps[i] = plan[i].second->dev;
+		sreps[i].rowBase = rb;
+		rowStart[i] = (int64_t)srows.size();
+		for(int y = 0; y < (1 << sreps[i].ly); ++y)
+			srows.push_back(make_uint2((unsigned)(i - groupStart.back()), (unsigned)y));
+		rb += 1 << sreps[i].ly;
+	}
+	rowStart[plan.size()] = (int64_t)srows.size();
+	groupStart.push_back(plan.size());
+	int64_t maxGroupRows = 0;
+	for(size_t g = 0; g + 1 < groupStart.size(); ++g)
+		maxGroupRows = std::max(maxGroupRows, rowStart[groupStart[g + 1]] - rowStart[groupStart[g]]);
+	if(!plan.empty())
+	{
+		ensure(sampleReps_, sreps.size() * sizeof(BatchRep));
+		ensure(rowTable_, srows.size() * sizeof(uint2));
+		ensure(rowScratch_, (size_t)maxGroupRows * 4 * sizeof(int64_t));
+		ensure(outBuf_, plan.size() * 3 * sizeof(int64_t));
+		check(hipMemcpyAsync(sampleReps_.get(), sreps.data(), sreps.size() * sizeof(BatchRep), hipMemcpyHostToDevice, s0_), "sample reps");
+		check(hipMemcpyAsync(rowTable_.get(), srows.data(), srows.size() * sizeof(uint2), hipMemcpyHostToDevice, s0_), "rows");
+	}
+
+	BatchDtArgs da{};
+	da.reps = repTable_.as<BatchRep>();
+	da.lanes = laneTable_.as<uint2>();
+	da.order = orderBuf_.as<uint8_t>();
+	da.shift = shiftBuf_.as<uint16_t>();
+	da.nreps = maxSlots_;
+	da.nlanes = nLanes_;
+	da.limit = limitBuf_.as<int>();
+	BatchCbArgs ca{};
+	ca.reps = repTable_.as<BatchRep>();
+	ca.waves = waveTable_.as<uint2>();
+	ca.nwaves = nWaves_;
+	ca.limit = limitBuf_.as<int>();
+	size_t g = 0;
+	for(int m = 0; m < k; ++m)
+	{
+		da.m = m;
+		ca.m = m;
+		for(int d = 0; d < 4; ++d)
+		{
+			da.d = d;
+			check(launch_batch_dt(da, s0_), "batch_dt");
+		}
+		for(int s = 0; s < 2; ++s)
+		{
+			ca.signum = s;
+			ca.buf = s;
+			check(launch_batch_cbbit(ca, s0_), "batch_cbbit");
+		}
+		// exact roughness of the jobs whose sample time is this MCS
+		if(g + 1 < groupStart.size() && plan[groupStart[g]].first == m)
+		{
+			const size_t i0 = groupStart[g], i1 = groupStart[g + 1];
+			check(launch_batch_roughness(sampleReps_.as<BatchRep>() + i0, (int)(i1 - i0), rowTable_.as<uint2>() + rowStart[i0],
+					rowStart[i1] - rowStart[i0], rowScratch_.as<int64_t>(), outBuf_.as<int64_t>() + 3 * i0, s0_), "batch_roughness");
+			++g;
+		}
+	}
+	std::vector<int64_t> out(plan.size() * 3);
+	if(!plan.empty())
+		check(hipMemcpyAsync(out.data(), outBuf_.get(), out.size() * 8, hipMemcpyDeviceToHost, s0_), "roughness D2H");
+	check(hipStreamSynchronize(s0_), "tick sync");
+	for(size_t i = 0; i < plan.size(); ++i)
+	{
+		Job* j = plan[i].second;
+		JobSample smp;
+		smp.mcs = j->done + (uint32_t)plan[i].first + 1;
+		smp.sumH = out[3 * i];
+		smp.sumH2 = ((__int128)out[3 * i + 2] << 64) | (__int128)(uint64_t)out[3 * i + 1];
+		j->res.series.push_back(smp);
+		++j->next;
+	}
+	for(Job* j : slots_)
+	{
+		if(!j)
+			continue;
+		const int lim = limitHost_[(size_t)j->slot];
+		j->done += (uint32_t)lim;
+		siteUpdates_ += (uint64_t)lim << (j->spec.lx + j->spec.ly);
+		if(j->done != (uint32_t)j->spec.mcs)
+			continue;
+		// retire: results out, memory back to the arena, slot free
+		if(j->spec.keepLattice)
+		{
+			j->res.lattice.resize((size_t)1 << (j->spec.lx + j->spec.ly - 4));
+			check(hipMemcpy(j->res.lattice.data(), j->dev.lat, j->res.lattice.size() * 4, hipMemcpyDeviceToHost), "final lattice");
+		}
+		size_t parts[3];
+		footprint(j->spec, parts);
+		for(int u = 0; u < 3; ++u)
+			if(parts[u])
+				arena_.release(j->off[u]);
+		j->res.tEnd = now();
+		done_.push_back(std::move(j->res));
+		slots_[(size_t)j->slot] = nullptr;
+		delete j;
+		--nActive_;
+		dirty_ = true;
+	}
+	return queue_.size() + nActive_;
+}
+
+bool Server::pop(JobResult& out)
+{
+	if(done_.empty())
+		return false;
+	out = std::move(done_.front());
+	done_.pop_front();
+	return true;
+}
+
+} // namespace kpz

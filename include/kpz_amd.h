@@ -157,6 +157,42 @@ void* kpz_batch_stream(const kpz_batch* b);
  * shapes; *tuning is the start point and receives the best knobs */
 int kpz_batch_autotune(const kpz_replica* reps, int n, int mcs, int device, kpz_batch_tuning* tuning, double* ms_per_mcs);
 
+/* ---------------------------------------------------------------------
+ * Serving loop (no counterpart in the reference).  Jobs are admitted into
+ * slots as HBM frees up (FIFO with backfill by footprint) and advance
+ * together in batched launches; lattices and RNG state live in one device
+ * arena (buddy allocator).  A job is bit-identical to the stand-alone
+ * scheduler with its kind, size, seed and (p, q) (DT/DTr (1,0): kpz_mcs,
+ * else kpz_mcs_pq) and reports exact sums at the log-spaced times
+ * ceil(exp(log(t+10)+0.1)) (kpz/sampling.h:84-100) and at its last MCS.
+ * ------------------------------------------------------------------- */
+typedef struct kpz_server kpz_server;
+
+typedef struct kpz_job {
+	int kind;          /* KPZ_DT, KPZ_DTR, KPZ_SCABIT */
+	int lx, ly;
+	int mcs;           /* MCS to run */
+	double p, q;
+	uint64_t seed;
+	int keep_lattice;  /* return the final lattice with the result */
+} kpz_job;
+
+int kpz_server_create(uint64_t hbm_budget_bytes, int max_slots, int max_tick_mcs, int device, kpz_server** out);
+int kpz_server_destroy(kpz_server* s);
+int kpz_server_submit(kpz_server* s, const kpz_job* job, uint64_t* id);
+/* one tick; returns queued + active jobs (>= 0) or a negative error code */
+int64_t kpz_server_step(kpz_server* s);
+/* 1 if a finished job is waiting: its sample count and lattice words (0 if not kept) */
+int kpz_server_peek(kpz_server* s, int* nsamples, uint64_t* lattice_words);
+/* take the finished job: times = {submit, start, end} in s since server
+ * start; per sample mcs[i] and sums[3i..3i+2] = {sum h, sum h^2 lo, hi};
+ * lattice may be NULL */
+int kpz_server_pop(kpz_server* s, uint64_t* id, double times[3], uint32_t* mcs, int64_t* sums, uint32_t* lattice);
+/* {queued, active, hbm bytes in use, hbm arena bytes, site updates so far} */
+int kpz_server_stats(kpz_server* s, uint64_t out[5]);
+/* the server's clock (s since creation; monotonic) */
+double kpz_server_now(const kpz_server* s);
+
 #ifdef __cplusplus
 }
 #endif
