@@ -236,22 +236,31 @@ def run_serve_bench(K, window_s, rate=200.0, seed=5, kind_name="scaBit", budget=
 import numpy as np  # noqa: E402
 
 
-def cpu_baseline(seconds=12.0):
-    """Oracle SchedulerCPUDT restatement, one core, 2048^2 sample, same layout
-    (16x16 blocks, one stream per block)."""
+def cpu_baseline(seconds=10.0, lx=12):
+    """The oracle's restatement of SchedulerCPUDT on the host cores: T worker
+    threads = T SLCG streams, spawned and joined per domain set, 16x16-site
+    blocks, p=1 q=0, flat IC, on a 4096^2 sample for a bounded time; plus the
+    same on one core."""
     from oracle import oracle as O
-    lx = 11
-    m = O.Dsfmt(42)
-    nb = 1 << (lx - 4 + lx - 4 - 2)
-    st = O.Streams(nb, m)
-    lat = O.flat(lx, lx, O.ENC_LOCAL)
-    n, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        O.dt_mcs(lat, lx, lx, 4, 4, st, m, 1)
-        n += 1
-    dt = time.perf_counter() - t0
-    return {"value": n * (1 << (2 * lx)) / dt, "unit": "site updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle SchedulerCPUDT restatement, 2048x2048, 16x16 blocks, {n} MCS in {dt:.1f} s"}
+    cores = max(1, min(16, os.cpu_count() or 1))
+
+    def timed(nthreads, budget):
+        m = O.Dsfmt(42)
+        st = O.Streams(nthreads, m)
+        lat = O.flat(lx, lx, O.ENC_LOCAL)
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget:
+            O.dt_mcs_threaded(lat, lx, lx, 4, 4, st, m, 1, nthreads)
+            n += 1
+        return n, time.perf_counter() - t0
+
+    n, dt = timed(cores, seconds)
+    n1, dt1 = timed(1, seconds / 2)
+    N = 1 << (2 * lx)
+    return {"value": n * N / dt, "unit": "site updates/s", "cores": cores, "kind": "port",
+            "value_1core": n1 * N / dt1,
+            "sample": f"oracle SchedulerCPUDT restatement (pthreads per phase), {1 << lx}x{1 << lx}, 16x16 blocks, "
+                      f"T={cores} streams: {n} MCS in {dt:.1f} s; T=1: {n1} MCS in {dt1:.1f} s"}
 
 
 def main():
@@ -262,7 +271,8 @@ def main():
     ap.add_argument("--lx", type=int, default=16)
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--secondary", action="store_true", help="also time SCABit at 65536^2")
+    ap.add_argument("--no-secondary", dest="secondary", action="store_false",
+                    help="skip the SCABit timings at 65536^2")
     ap.add_argument("--batch", type=int, default=2000, metavar="MCS",
                     help="batched 4096-replica mix for MCS steps vs serial per-replica launches (0: skip)")
     ap.add_argument("--serial-mcs", type=int, default=10, help="MCS of the serial per-replica baseline")

@@ -536,6 +536,111 @@ int kor_dt_mcs(uint32_t* lat, int lx, int ly, int lbx, int lby,
 	return 0;
 }
 
+/* Threaded form of the above for timing: worker w runs streams w, w+nthreads,
+ * ...; one pthread per worker per domain set, spawned and joined per phase
+ * as SchedulerCPUpThread::run / joinMCS (kpz/schedulerCPUpThread.h:44-63,
+ * kpz/schedulerCPUDT.cpp:75-86).  Same result as kor_dt_mcs. */
+#include <pthread.h>
+
+typedef struct {
+	uint32_t* lat;
+	int lx, ly, lbx, lby, ds, random_origin, upd;
+	unsigned shift[2];
+	uint64_t* states;
+	int64_t T, w, nw, nb, bs;
+	uint64_t a, c;
+	uint32_t tp, tq;
+} dt_work;
+
+static void* dt_worker(void* arg)
+{
+	dt_work* W = (dt_work*)arg;
+	const int mx = (1 << W->lx) - 1, my = (1 << W->ly) - 1;
+	for(int64_t t = W->w; t < W->T; t += W->nw)
+	{
+		uint64_t s = W->states[t];
+		for(int64_t i = t; i < W->nb; i += W->T)
+		{
+			int off[2];
+			kor_dt_block_offset(W->lx, W->ly, W->lbx, W->lby, W->ds, W->shift, (int)i, off);
+			for(int64_t k = 0; k < W->bs; ++k)
+			{
+				int x, y;
+				if(W->random_origin)
+				{
+					x = ((int)((kor_slcg_draw(&s, W->a, W->c) & 0x7FFFFF) >> (23 - W->lbx)) + off[0]) & mx;
+					y = ((int)((kor_slcg_draw(&s, W->a, W->c) & 0x7FFFFF) >> (23 - W->lby)) + off[1]) & my;
+				}
+				else
+				{
+					x = (int)(kor_slcg_draw(&s, W->a, W->c) >> (32 - W->lbx)) + off[0];
+					y = (int)(kor_slcg_draw(&s, W->a, W->c) >> (32 - W->lby)) + off[1];
+				}
+				uint32_t r23 = 0;
+				if(W->upd == KOR_UPD_PQ)
+					r23 = kor_slcg_draw(&s, W->a, W->c) & 0x7FFFFF;
+				upd_local(W->lat, W->lx, W->ly, x, y, W->upd, r23, W->tp, W->tq);
+			}
+		}
+		W->states[t] = s;
+	}
+	return NULL;
+}
+
+int kor_dt_mcs_threaded(uint32_t* lat, int lx, int ly, int lbx, int lby,
+		uint64_t* states, int64_t T, uint64_t a, uint64_t c,
+		kor_dsfmt* master, int n, int random_origin, int upd, double p, double q, int nthreads)
+{
+	if(lbx < 2 || lby < 2 || lx - lbx < 1 || ly - lby < 1 || T < 1 || lbx > 23 || lby > 23 || nthreads < 1)
+		return -1;
+	pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+	dt_work* wk = (dt_work*)malloc(sizeof(dt_work) * (size_t)nthreads);
+	for(int m = 0; m < n; ++m)
+	{
+		int seq[4];
+		kor_shuffle4(master, seq);
+		unsigned shift[2] = {0, 0};
+		if(random_origin)
+		{
+			shift[0] = (unsigned)(kor_dsfmt_close_open(master) * (double)(1 << lbx));
+			shift[1] = (unsigned)(kor_dsfmt_close_open(master) * (double)(1 << lby));
+		}
+		for(int d = 0; d < 4; ++d)
+		{
+			for(int w = 0; w < nthreads; ++w)
+			{
+				dt_work* W = &wk[w];
+				W->lat = lat;
+				W->lx = lx;
+				W->ly = ly;
+				W->lbx = lbx;
+				W->lby = lby;
+				W->ds = seq[d];
+				W->random_origin = random_origin;
+				W->upd = upd;
+				W->shift[0] = shift[0];
+				W->shift[1] = shift[1];
+				W->states = states;
+				W->T = T;
+				W->w = w;
+				W->nw = nthreads;
+				W->nb = 1ll << (lx - lbx + ly - lby - 2);
+				W->bs = 1ll << (lbx + lby);
+				W->a = a;
+				W->c = c;
+				W->tp = thresh23(p);
+				W->tq = thresh23(q);
+				pthread_create(&th[w], NULL, dt_worker, W);
+			}
+			for(int w = 0; w < nthreads; ++w)
+				pthread_join(th[w], NULL);
+		}
+	}
+	free(th);
+	free(wk);
+	return 0;
+}
+
 /* ======================================================================
  * checkerboard SCA on ENC_CBBIT: SchedulerCPUCBBit<Rng>::doMcs,
  * kpz/schedulerCPUCBBit.cpp:43-79, Func_MCS::exec

@@ -88,6 +88,10 @@ int kor_dt_mcs(uint32_t* lat, int lx, int ly, int lbx, int lby,
 		kor_dsfmt* master, int n, int random_origin, int upd, double p, double q);
 void kor_dt_block_offset(int lx, int ly, int lbx, int lby, int domain_set,
 		const unsigned shift[2], int i, int off[2]);
+/* same, with nthreads pthreads per phase (timing baseline) */
+int kor_dt_mcs_threaded(uint32_t* lat, int lx, int ly, int lbx, int lby,
+		uint64_t* states, int64_t T, uint64_t a, uint64_t c,
+		kor_dsfmt* master, int n, int random_origin, int upd, double p, double q, int nthreads);
 
 /* SchedulerCPUCBBit: mode 0 = pSyncRngUpdateVL3 (p=.5, q=0), 1 = per-site pq */
 int kor_cbbit_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,

@@ -70,6 +70,9 @@ def _declare(L):
     L.kor_consistent.argtypes = [_u32p, C.c_int, C.c_int, C.c_int]
     L.kor_dt_mcs.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, _u64p, C.c_int64,
                              C.c_uint64, C.c_uint64, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
+    L.kor_dt_mcs_threaded.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, _u64p, C.c_int64,
+                                      C.c_uint64, C.c_uint64, vp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                      C.c_double, C.c_int]
     L.kor_dt_block_offset.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.POINTER(C.c_uint), C.c_int, C.POINTER(C.c_int)]
     L.kor_cbbit_mcs.argtypes = [_u32p, C.c_int, C.c_int, _u64p, C.c_int64, C.c_uint64, C.c_uint64,
@@ -216,6 +219,14 @@ def dt_block_offset(lx, ly, lbx, lby, ds, shift, i):
 def dt_mcs(lat, lx, ly, lbx, lby, streams, master, n, random_origin=False, upd=UPD_KPZ, p=1.0, q=0.0):
     rc = lib().kor_dt_mcs(lat, lx, ly, lbx, lby, streams.states, streams.T, streams.a, streams.c,
                           master.ptr, n, int(random_origin), upd, p, q)
+    if rc:
+        raise ValueError("invalid DT layout")
+
+
+def dt_mcs_threaded(lat, lx, ly, lbx, lby, streams, master, n, nthreads, random_origin=False, upd=UPD_KPZ,
+                    p=1.0, q=0.0):
+    rc = lib().kor_dt_mcs_threaded(lat, lx, ly, lbx, lby, streams.states, streams.T, streams.a, streams.c,
+                                   master.ptr, n, int(random_origin), upd, p, q, int(nthreads))
     if rc:
         raise ValueError("invalid DT layout")
 

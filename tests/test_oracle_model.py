@@ -326,3 +326,14 @@ def test_dt_invalid_layout_rejected():
         O.dt_mcs(lat, 6, 6, 6, 2, st, m, 1)  # one block per row: no double tiling
     with pytest.raises(ValueError):
         O.dt_mcs(lat, 6, 6, 1, 2, st, m, 1)  # block smaller than the reference minimum
+
+
+@pytest.mark.parametrize("T,nthreads", [(8, 8), (5, 3), (64, 4)])
+def test_threaded_dt_equals_sequential(T, nthreads):
+    lx, ly = 8, 8
+    m1, m2 = O.Dsfmt(9), O.Dsfmt(9)
+    s1, s2 = O.Streams(T, m1), O.Streams(T, m2)
+    a, b = O.flat(lx, ly, 0), O.flat(lx, ly, 0)
+    O.dt_mcs(a, lx, ly, 3, 3, s1, m1, 3, True, O.UPD_PQ, 0.8, 0.3)
+    O.dt_mcs_threaded(b, lx, ly, 3, 3, s2, m2, 3, nthreads, True, O.UPD_PQ, 0.8, 0.3)
+    assert np.array_equal(a, b) and np.array_equal(s1.states, s2.states)
