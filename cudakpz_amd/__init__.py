@@ -22,7 +22,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libkpz_amd.so")
 _LIB = None
 
-KPZ_DT, KPZ_DTR, KPZ_SCABIT = 0, 1, 2
+KPZ_DT, KPZ_DTR, KPZ_SCABIT, KPZ_SCA = 0, 1, 2, 3
 ENC_LOCAL, ENC_CBBIT = 0, 1
 
 
@@ -317,6 +317,12 @@ class SchedulerSCABit(SchedulerService):
     ENCODING = ENC_CBBIT
 
 
+class SchedulerSCA(SchedulerService):
+    """Checkerboard SCA on ENC_LOCAL (SchedulerCPUCB; mcs() runs p=.95, q=0)."""
+    KIND = KPZ_SCA
+    ENCODING = ENC_LOCAL
+
+
 class _Replica(C.Structure):
     _fields_ = [("kind", C.c_int), ("lx", C.c_int), ("ly", C.c_int), ("p", C.c_double), ("q", C.c_double),
                 ("seed", C.c_uint64)]
@@ -559,7 +565,8 @@ class Server:
 
 def make_scheduler(name, lx, ly, **kw):
     """SchedulerConfig::mkScheduler by scheduler id (kpz/kpzFrontend.cpp:93-250)."""
-    table = {"dtStatic": SchedulerDT, "dt": SchedulerDTr, "scaBit": SchedulerSCABit}
+    table = {"dtStatic": SchedulerDT, "dt": SchedulerDTr, "scaBit": SchedulerSCABit, "sca": SchedulerSCA,
+             "cb": SchedulerSCA}
     if name not in table:
         raise KpzError(-1, f"No such scheduler: {name}")
     return table[name](lx, ly, **kw)

@@ -383,6 +383,104 @@ __global__ __launch_bounds__(256) void cbbit_half_kernel(CbHalfArgs a)
 }
 
 // ======================================================================
+// SCA on ENC_LOCAL (SchedulerCPUCB, kpz/schedulerCPUCB.cpp:43-170)
+// ======================================================================
+// One lane per 4x4-site cell.  In a checkerboard half-sweep every bit a site
+// reads (own slopes, x slope of x+1, y slope of y+1) is written only by that
+// site, so the three words are read once; other lanes flip other bits of the
+// same words, hence the atomic XORs.
+__device__ __forceinline__ int cb_class(const CbLocalArgs& a, int v)
+{
+	int k = 0;
+	for(int i = 1; i < a.nV; ++i)
+		if(a.V[i] == v)
+			k = i;
+	return k;
+}
+
+__global__ __launch_bounds__(256) void cb_local_half_kernel(CbLocalArgs a)
+{
+	const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const int lXC = a.lx - 2;
+	const int XC = 1 << lXC;
+	if(gid >= ((int64_t)XC << (a.ly - 2)))
+		return;
+	const int c = (int)(gid & (XC - 1));
+	const int yb = (int)(gid >> lXC);
+	// worker of this cell column and the cell's place in its strip
+	const int big = a.rem * (a.pc + 1);
+	int t, c0;
+	if(c < big)
+	{
+		t = c / (a.pc + 1);
+		c0 = t * (a.pc + 1);
+	}
+	else
+	{
+		t = a.rem + (c - big) / a.pc;
+		c0 = big + (t - a.rem) * a.pc;
+	}
+	const int w = a.pc + (t < a.rem ? 1 : 0);
+	const int cells0 = w / 2; // xCenter = xMin + ((w*4/2) & ~3)
+	const int cc = c - c0;
+	const bool h = cc >= cells0;
+	const int ci = h ? cc - cells0 : cc;
+	const int cellsH = h ? w - cells0 : cells0;
+	const uint64_t s0 = a.st_in[t];
+	uint64_t s = s0;
+	if(h)
+		s = a.baseJ[cb_class(a, cells0)](s);
+	s = a.rowJ[cb_class(a, cellsH) * a.nyb + yb](s);
+	s = a.cellJ[ci](s);
+	if(yb == 0 && cc == 0)
+		a.st_out[t] = (t < a.rem ? a.fullBig : a.fullSmall)(s0);
+
+	uint32_t* __restrict__ lat = a.lat;
+	const size_t iOwn = ((size_t)yb << lXC) + c;
+	const size_t iRight = ((size_t)yb << lXC) + ((c + 1) & (XC - 1));
+	const size_t iUp = ((size_t)((yb + 1) & (a.nyb - 1)) << lXC) + c;
+	const uint32_t own = lat[iOwn], right = lat[iRight], up = lat[iUp];
+	uint32_t fOwn = 0, fRight = 0, fUp = 0;
+	const Affine step = a.step;
+#pragma unroll
+	for(int yy = 0; yy < 4; ++yy)
+	{
+		const int x0 = ((yy & 1) ^ a.signum); // x & 3 of the row's first active site (x0 = (xW + (y&1)) ^ s)
+#pragma unroll
+		for(int k = 0; k < 2; ++k)
+		{
+			const int xx = x0 + 2 * k;
+			s = step(s);
+			const uint32_t r23 = slcg_fold(s) & 0x7FFFFFu;
+			const int sh = (xx << 1) | (yy << 3);
+			const uint32_t cbits = (own >> sh) & 3u;
+			const uint32_t rb = (xx < 3) ? (own >> (sh + 2)) & 1u : (right >> (yy << 3)) & 1u;
+			const uint32_t lb = (yy < 3) ? (own >> (sh + 9)) & 1u : (up >> ((xx << 1) | 1)) & 1u;
+			const uint32_t cfg = cbits | (rb << 2) | (lb << 3);
+			const bool flip = (cfg == 0xCu && r23 < a.tp) || (cfg == 0x3u && r23 < a.tq);
+			if(flip)
+			{
+				fOwn ^= 3u << sh;
+				if(xx < 3)
+					fOwn ^= 1u << (sh + 2);
+				else
+					fRight ^= 1u << (yy << 3);
+				if(yy < 3)
+					fOwn ^= 1u << (sh + 9);
+				else
+					fUp ^= 1u << ((xx << 1) | 1);
+			}
+		}
+	}
+	if(fOwn)
+		atomicXor(lat + iOwn, fOwn);
+	if(fRight)
+		atomicXor(lat + iRight, fRight);
+	if(fUp)
+		atomicXor(lat + iUp, fUp);
+}
+
+// ======================================================================
 // row statistics (roughness / correlator)
 // ======================================================================
 // x-slope bits of sites x = 64j .. 64j+63 of row y (bit i <-> x = 64j+i)
@@ -867,6 +965,13 @@ hipError_t launch_fill_flat(uint32_t* lat, int enc, int lx, int ly, hipStream_t 
 		hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, stream, lat, n, n, 0x33CC33CCu, 0x33CC33CCu);
 	else
 		hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, stream, lat, n, n >> 1, 0u, 0xFFFFFFFFu);
+	return hipGetLastError();
+}
+
+hipError_t launch_cb_local_half(const CbLocalArgs& a, hipStream_t stream)
+{
+	const int64_t cells = 1ll << (a.lx + a.ly - 4);
+	hipLaunchKernelGGL(cb_local_half_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, stream, a);
 	return hipGetLastError();
 }
 

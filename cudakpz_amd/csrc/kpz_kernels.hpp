@@ -46,6 +46,28 @@ struct CbHalfArgs {
 };
 hipError_t launch_cbbit_half(const CbHalfArgs& a, hipStream_t stream);
 
+// ---- SCA on ENC_LOCAL: one checkerboard half-sweep (SchedulerCPUCB) --------
+// Worker t owns cell columns [c0_t, c0_t + w_t), w_t = pc (+1 for t < rem);
+// its strip is split into halves of w/2 and w - w/2 cells, walked one after
+// the other, each block row (4 site rows) by block row, cell by cell, 8 draws
+// per cell.  V[] lists the distinct half widths (cells); rowJ[k*nyb + yb] =
+// J(yb*V[k]*8), baseJ[k] = J(nyb*V[k]*8), cellJ[i] = J(8 i).
+struct CbLocalArgs {
+	uint32_t* lat;
+	int lx, ly, signum;
+	const uint64_t* st_in;
+	uint64_t* st_out;
+	int pc, rem, nyb;
+	int V[4], nV;
+	const Affine* rowJ;
+	const Affine* baseJ;
+	const Affine* cellJ;
+	Affine fullBig, fullSmall; // J(nyb * w * 8) for w = pc+1 / pc
+	Affine step;
+	uint32_t tp, tq;
+};
+hipError_t launch_cb_local_half(const CbLocalArgs& a, hipStream_t stream);
+
 // ---- reductions ----------------------------------------------------------
 // Per site row y, relative to the row's start height (see kpz_kernels.hip):
 // out[y*ROW_STATS + k] = {S1, S2, D, sy0, S1', D', sy0', S11, tcs, 0}, the

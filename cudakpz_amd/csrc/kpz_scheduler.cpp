@@ -1,5 +1,6 @@
 #include "kpz_scheduler.hpp"
 
+#include <algorithm>
 #include <cstring>
 
 namespace kpz {
@@ -395,6 +396,102 @@ void SchedulerSCABit::mcs(int n)
 // SchedulerCPUCBBit::mcsPQSyncRng (kpz/schedulerCPUCBBit.cpp:95-102)
 void SchedulerSCABit::mcsPQSyncRng(int n) { run(n, (q_ == 0.0 && p_ == 0.5) ? 0 : 1); }
 
+// ======================================================================
+SchedulerSCA::SchedulerSCA(int lx, int ly, const Params& p) : SchedulerService(lx, ly, ENC_LOCAL, p)
+{
+	const int dimXW = 1 << (lx - 2);
+	const int nyb = 1 << (ly - 2);
+	if(T_ <= 0)
+		T_ = dimXW;
+	if(T_ > dimXW)
+		throw Error(-1, "SCA: more workers than 4-site columns");
+	if(S_ == 0)
+		S_ = default_skip((uint64_t)T_);
+	step_ = slcg_skip(S_);
+	st_[0].alloc((size_t)T_ * 8);
+	st_[1].alloc((size_t)T_ * 8);
+	// worker strips (kpz/schedulerCPUCB.cpp:46-62) in cells of 4 sites
+	const int pc = (int)(dimXW / T_), rem = (int)(dimXW % T_);
+	std::vector<int> V;
+	auto addV = [&](int v) { // v = 0: an empty first half (one-cell strips)
+		if(std::find(V.begin(), V.end(), v) == V.end())
+			V.push_back(v);
+	};
+	for(int w : {pc, pc + 1})
+	{
+		if((w == pc && rem == T_) || (w == pc + 1 && rem == 0))
+			continue;
+		addV(w / 2);
+		addV(w - w / 2);
+	}
+	if(V.empty())
+		V.push_back(1);
+	const Affine cell = affine_pow(step_, 8);
+	std::vector<Affine> rowJ(V.size() * (size_t)nyb), baseJ(V.size());
+	int maxV = 0;
+	for(size_t k = 0; k < V.size(); ++k)
+	{
+		const Affine rowStep = affine_pow(cell, (uint64_t)V[k]);
+		Affine acc{1, 0};
+		for(int yb = 0; yb < nyb; ++yb)
+		{
+			rowJ[k * (size_t)nyb + (size_t)yb] = acc;
+			acc = compose(rowStep, acc);
+		}
+		baseJ[k] = acc; // J(nyb * V * 8)
+		maxV = std::max(maxV, V[k]);
+	}
+	std::vector<Affine> cellJ((size_t)maxV + 1);
+	Affine acc{1, 0};
+	for(auto& c : cellJ)
+	{
+		c = acc;
+		acc = compose(cell, acc);
+	}
+	args_.lx = lx;
+	args_.ly = ly;
+	args_.pc = pc;
+	args_.rem = rem;
+	args_.nyb = nyb;
+	args_.nV = (int)V.size();
+	for(size_t k = 0; k < V.size(); ++k)
+		args_.V[k] = V[k];
+	args_.rowJ = upload(rowJ, rowJ_);
+	args_.baseJ = upload(baseJ, baseJ_);
+	args_.cellJ = upload(cellJ, cellJ_);
+	args_.fullBig = affine_pow(cell, (uint64_t)nyb * (uint64_t)(pc + 1));
+	args_.fullSmall = affine_pow(cell, (uint64_t)nyb * (uint64_t)pc);
+	args_.step = step_;
+	p_ = 0.95;
+	q_ = 0.0;
+}
+
+void SchedulerSCA::mcs(int n)
+{
+	setPQ(0.95, 0.0);
+	mcsPQSyncRng(n);
+}
+
+void SchedulerSCA::mcsPQSyncRng(int n)
+{
+	CbLocalArgs a = args_;
+	a.lat = deviceLattice();
+	a.tp = thresh23(p_);
+	a.tq = thresh23(q_);
+	for(int m = 0; m < n; ++m)
+	{
+		for(int s = 0; s < 2; ++s)
+		{
+			a.signum = s;
+			a.st_in = stIn();
+			a.st_out = stOut();
+			check(launch_cb_local_half(a, stream_), "cb_local_half");
+			swapStreams();
+		}
+		++mcs_;
+	}
+}
+
 std::unique_ptr<SchedulerService> makeScheduler(int kind, int lx, int ly, const Params& p)
 {
 	switch(kind)
@@ -402,6 +499,7 @@ std::unique_ptr<SchedulerService> makeScheduler(int kind, int lx, int ly, const 
 		case KIND_DT: return std::make_unique<SchedulerDT>(lx, ly, p, false);
 		case KIND_DTR: return std::make_unique<SchedulerDT>(lx, ly, p, true);
 		case KIND_SCABIT: return std::make_unique<SchedulerSCABit>(lx, ly, p);
+		case KIND_SCA: return std::make_unique<SchedulerSCA>(lx, ly, p);
 		default: throw Error(-1, "unknown scheduler kind");
 	}
 }

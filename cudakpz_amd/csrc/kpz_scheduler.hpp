@@ -24,7 +24,7 @@ struct Error : std::runtime_error {
 	Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
 
-enum Kind { KIND_DT = 0, KIND_DTR = 1, KIND_SCABIT = 2 };
+enum Kind { KIND_DT = 0, KIND_DTR = 1, KIND_SCABIT = 2, KIND_SCA = 3 };
 
 // device buffer owned by a scheduler
 class DevBuf {
@@ -170,6 +170,20 @@ private:
 	Tables& tables(int mode);
 	int per_, rem_, lG_;
 	std::map<int, std::unique_ptr<Tables>> tab_;
+};
+
+// Checkerboard SCA on ENC_LOCAL: SchedulerCPUCB semantics (the CPU
+// counterpart of the GPU SchedulerSCA).
+class SchedulerSCA : public SchedulerService {
+public:
+	SchedulerSCA(int lx, int ly, const Params& p);
+	void mcs(int n) override;                // SchedulerCPUCB::mcs: p=.95, q=0
+	void mcsSyncRng(int n) override { mcsPQSyncRng(n); }
+	void mcsPQSyncRng(int n) override;
+
+private:
+	CbLocalArgs args_{};
+	DevBuf rowJ_, baseJ_, cellJ_;
 };
 
 std::unique_ptr<SchedulerService> makeScheduler(int kind, int lx, int ly, const Params& p);

@@ -27,7 +27,8 @@ typedef struct kpz_scheduler kpz_scheduler;
 enum {
 	KPZ_DT = 0,     /* SchedulerCPUDT<Rng> (static DT)  kpz/schedulerCPUDT.cpp:65-90 */
 	KPZ_DTR = 1,    /* SchedulerCPUDTr<Rng> (random-origin DT) kpz/schedulerCPUDTr.cpp:66-98 */
-	KPZ_SCABIT = 2  /* SchedulerCPUCBBit<Rng> (checkerboard, bit vectorised) kpz/schedulerCPUCBBit.cpp:43-108 */
+	KPZ_SCABIT = 2, /* SchedulerCPUCBBit<Rng> (checkerboard, bit vectorised) kpz/schedulerCPUCBBit.cpp:43-108 */
+	KPZ_SCA = 3     /* SchedulerCPUCB<Rng> (checkerboard on ENC_LOCAL; GPU SchedulerSCA) kpz/schedulerCPUCB.cpp:43-170 */
 };
 
 enum { KPZ_ENC_LOCAL = 0, KPZ_ENC_CBBIT = 1 };
