@@ -249,8 +249,25 @@ class SchedulerService:
         _check(lib().kpz_roughness(self._h, out))
         return Roughness(self.N, int(out[0]), _i128(out[1], out[2]))
 
-    def correlateTag(self):
-        _check(lib().kpz_corr_tag(self._h))
+    def correlateTag(self, stop=None):
+        if stop is None:
+            _check(lib().kpz_corr_tag(self._h))
+        else:
+            lib().kpz_corr_tag_stop.argtypes = [C.c_void_p, C.c_uint32]
+            _check(lib().kpz_corr_tag_stop(self._h, int(stop)))
+
+    def correlateAll(self, max_tags=64):
+        """[(tag MCS, Correlation)] for every live tag, oldest first."""
+        L = lib()
+        L.kpz_correlate_tags.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int]
+        out = (C.c_int64 * (8 * max_tags))()
+        n = L.kpz_correlate_tags(self._h, out, int(max_tags))
+        if n < 0:
+            _check(n)
+        return [(int(out[8 * k]), Correlation(self.N, int(out[8 * k + 1]), int(out[8 * k + 2]),
+                                              _i128(out[8 * k + 3], out[8 * k + 4]),
+                                              _i128(out[8 * k + 5], out[8 * k + 6]), int(out[8 * k + 7])))
+                for k in range(n)]
 
     def correlate(self):
         out = (C.c_int64 * 7)()

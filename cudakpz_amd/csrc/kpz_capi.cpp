@@ -189,6 +189,39 @@ int kpz_corr_tag(kpz_scheduler* h)
 	return guard([&] { h->s->correlateTag(); });
 }
 
+int kpz_corr_tag_stop(kpz_scheduler* h, uint32_t stop)
+{
+	HANDLE(h);
+	return guard([&] { h->s->correlateTag(stop); });
+}
+
+int kpz_correlate_tags(kpz_scheduler* h, int64_t* out, int max_tags)
+{
+	HANDLE(h);
+	if(!out || max_tags < 0)
+		return fail(KPZ_EINVAL, "null output");
+	int n = 0;
+	const int rc = guard([&] {
+		const auto res = h->s->correlateAll();
+		for(const auto& r : res)
+		{
+			if(n >= max_tags)
+				break;
+			int64_t* o = out + 8 * n;
+			o[0] = r.tagMcs;
+			o[1] = r.sums.sumH;
+			o[2] = r.sums.sumHs;
+			o[3] = (int64_t)(uint64_t)r.sums.sumH2;
+			o[4] = (int64_t)(r.sums.sumH2 >> 64);
+			o[5] = (int64_t)(uint64_t)r.sums.sumHHs;
+			o[6] = (int64_t)(r.sums.sumHHs >> 64);
+			o[7] = r.sums.tcs;
+			++n;
+		}
+	});
+	return rc ? rc : n;
+}
+
 int kpz_correlate(kpz_scheduler* h, int64_t out[7])
 {
 	HANDLE(h);

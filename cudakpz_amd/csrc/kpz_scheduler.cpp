@@ -184,18 +184,46 @@ Sums SchedulerService::roughness()
 	return s;
 }
 
-void SchedulerService::correlateTag()
+void SchedulerService::correlateTag(unsigned int stop)
 {
-	if(!snap_.get())
-		snap_.alloc(size_.sizeW() * 4);
-	check(hipMemcpyAsync(snap_.get(), deviceLattice(), size_.sizeW() * 4, hipMemcpyDeviceToDevice, stream_), "snapshot");
+	Tag t;
+	t.snap = std::make_unique<DevBuf>(size_.sizeW() * 4);
+	t.mcs = mcs_;
+	t.stop = stop;
+	const uint64_t bytes = size_.sizeW() * 4;
+	if(bytes % 16)
+		check(hipMemcpyAsync(t.snap->get(), deviceLattice(), bytes, hipMemcpyDeviceToDevice, stream_), "snapshot");
+	else
+		check(launch_copy(t.snap->get(), deviceLattice(), bytes, stream_), "snapshot");
+	tags_.push_back(std::move(t));
 }
 
 Sums SchedulerService::correlate()
 {
-	if(!snap_.get())
+	if(tags_.empty())
 		throw Error(-4, "correlate() needs a correlateTag() snapshot");
-	check(launch_row_stats(deviceLattice(), snap_.as<uint32_t>(), enc_, size_.lx, size_.ly, rows_.as<int64_t>(), stream_), "row_stats");
+	return correlateWith(*tags_.back().snap);
+}
+
+std::vector<SchedulerService::TagResult> SchedulerService::correlateAll()
+{
+	std::vector<TagResult> out;
+	for(auto it = tags_.begin(); it != tags_.end();)
+	{
+		if(mcs_ > it->stop)
+		{
+			it = tags_.erase(it);
+			continue;
+		}
+		out.push_back(TagResult{it->mcs, correlateWith(*it->snap)});
+		++it;
+	}
+	return out;
+}
+
+Sums SchedulerService::correlateWith(const DevBuf& snap)
+{
+	check(launch_row_stats(deviceLattice(), snap.as<uint32_t>(), enc_, size_.lx, size_.ly, rows_.as<int64_t>(), stream_), "row_stats");
 	check(hipMemcpyAsync(hostRows_.data(), rows_.get(), hostRows_.size() * 8, hipMemcpyDeviceToHost, stream_), "row_stats D2H");
 	sync();
 	Sums s;

@@ -100,9 +100,18 @@ public:
 
 	// ---- measurements (exact, on device)
 	Sums roughness();
-	void correlateTag();                     // snapshot for the autocorrelator
-	bool hasTag() const { return snap_.get() != nullptr; }
-	Sums correlate();
+	// correlateTag(stop) (kpz/schedulerService.cpp:266-271): a device snapshot
+	// kept while mcs <= stop; several tags may be live at once
+	void correlateTag(unsigned int stop = 0xFFFFFFFFu);
+	bool hasTag() const { return !tags_.empty(); }
+	Sums correlate();                        // against the most recent tag
+	struct TagResult {
+		unsigned int tagMcs;
+		Sums sums;
+	};
+	// every live tag (threadCorrelate, kpz/schedulerService.cpp:216-251):
+	// tags whose stop time has passed are dropped first
+	std::vector<TagResult> correlateAll();
 
 	// ---- RNG state access (parity checks, checkpoints)
 	void getStreams(uint64_t* out);
@@ -124,7 +133,13 @@ protected:
 	Affine step_;
 	int device_;
 	hipStream_t stream_ = nullptr;
-	DevBuf lat_, snap_, rows_;
+	struct Tag {
+		std::unique_ptr<DevBuf> snap;
+		unsigned int mcs, stop;
+	};
+	Sums correlateWith(const DevBuf& snap);
+	DevBuf lat_, rows_;
+	std::vector<Tag> tags_;
 	DevBuf st_[2];
 	int cur_ = 0;
 	MasterRng master_;

@@ -87,11 +87,19 @@ int kpz_copy_system(kpz_scheduler* dst, const kpz_scheduler* src);
 /* exact height sums over all sites, heights as Correlator::set
  * (kpz/correlator.cpp:37-106): out = {sum h, sum h^2 low 64, sum h^2 high 64} */
 int kpz_roughness(kpz_scheduler* h, int64_t out[3]);
-/* correlateTag: device snapshot of the current lattice (kpz/schedulerService.cpp:266-271) */
+/* correlateTag: device snapshot of the current lattice, kept while the MCS
+ * count is <= stop (kpz/schedulerService.cpp:266-271); kpz_corr_tag keeps it
+ * for ever.  Several tags may be live. */
 int kpz_corr_tag(kpz_scheduler* h);
-/* correlate against the snapshot (kpz/correlator.cpp:127-164, 450-455):
+int kpz_corr_tag_stop(kpz_scheduler* h, uint32_t stop);
+/* correlate against the most recent tag (kpz/correlator.cpp:127-164, 450-455):
  * out = {sum h, sum h_snap, sum h^2 lo, hi, sum h*h_snap lo, hi, slope overlap} */
 int kpz_correlate(kpz_scheduler* h, int64_t out[7]);
+/* every live tag, oldest first, after dropping expired ones
+ * (kpz/schedulerService.cpp:216-251): 8 int64 per tag = {tag MCS, then the 7
+ * values of kpz_correlate}; returns the number of tags written (<= max_tags)
+ * or a negative error */
+int kpz_correlate_tags(kpz_scheduler* h, int64_t* out, int max_tags);
 
 /* RNG state: stream states (T u64) and the master dSFMT.  The master state
  * is kpz_master_bytes() bytes laid out as the reference's dsfmt_t

@@ -156,6 +156,33 @@ def test_upload_download_roundtrip():
     assert np.array_equal(s.popSystem(), lat)
 
 
+@pytest.mark.parametrize("cls,enc", [(K.SchedulerSCABit, 1), (K.SchedulerDT, 0), (K.SchedulerSCA, 0)])
+def test_several_correlator_tags(cls, enc):
+    lx, ly = 8, 8
+    s = cls(lx, ly)
+    s.supplySeed(77)
+    s.initHomogeneous()
+    s.setPQ(0.9, 0.1)
+    snaps = []
+    s.correlateTag(stop=5)            # expires before the evaluation below
+    snaps.append((0, s.popSystem()))
+    s.mcsPQSyncRng(2)
+    s.correlateTag()
+    snaps.append((2, s.popSystem()))
+    s.mcsPQSyncRng(2)
+    s.correlateTag(stop=100)
+    snaps.append((4, s.popSystem()))
+    s.mcsPQSyncRng(3)
+    cur = s.popSystem()
+    res = s.correlateAll()
+    assert [m for m, _ in res] == [2, 4]
+    for (m, c), (m2, snap) in zip(res, snaps[1:]):
+        assert m == m2
+        assert (c.sum_h, c.sum_hs, c.sum_h2, c.sum_hhs, c.tcs) == O.correlate(cur, snap, enc, lx, ly)
+    latest = s.correlate()
+    assert latest.sum_hhs == res[-1][1].sum_hhs
+
+
 def test_master_handover_and_randomize():
     # the reference's dsfmt_t bytes go in, randomize() seeds the streams from them
     m = O.Dsfmt(1234)
