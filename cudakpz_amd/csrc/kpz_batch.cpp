@@ -121,7 +121,7 @@ Batch::Batch(const std::vector<ReplicaSpec>& specs, const BatchTuning& tune, int
 				tab[tabPos++] = acc;
 				acc = compose(word, acc);
 			}
-			d.stepJ = affine_pow(d.step, du * (uint64_t)(G - 1));
+			d.stepJ = affine_pow(d.step, du * (uint64_t)G);
 			d.full = affine_pow(d.step, du * Wn);
 		}
 		else

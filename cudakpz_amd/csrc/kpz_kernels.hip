@@ -51,6 +51,15 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src, int width)
 	const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, width);
 	return ((uint64_t)hi << 32) | lo;
 }
+// a[23:0] * b[23:0] + c, low 32 bits, in one VALU instruction (left to
+// itself the compiler widens a product used mod 2^23 to v_mul_lo_u32 + add)
+__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c)
+{
+	uint32_t r;
+	asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+	return r;
+}
+
 __device__ __forceinline__ uint32_t shflw(uint32_t v, int src, int width) { return (uint32_t)__shfl((int)v, src, width); }
 __device__ __forceinline__ uint64_t shflw(uint64_t v, int src, int width) { return shfl64(v, src, width); }
 
@@ -154,15 +163,25 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 		{
 			x = (int)(r1 >> (32 - LBX));
 			y = (int)(r2 >> (32 - LBY));
+			// hide y's origin from the optimiser: the LDS address is then one
+			// shift-and-add of the row number instead of a shift/mask/scale chain
+			asm("" : "+v"(y));
 		}
 		uint2 row = my[y * DT_WG];
 		const uint32_t up = my[(y + 1) * DT_WG].y;
 		// whole-row bit algebra, then select column x: a local minimum has
 		// own X = 0, right X = 1, own Y = 0, up Y = 1; a maximum the opposite
 		const uint32_t rx = row.x >> 1;
-		uint32_t fm;
 		if(UPD == 0)
-			fm = rx & ~row.x & ~row.y & up & (1u << x);
+		{
+			// at a minimum own X = 0 and right X = 1, own Y = 0 and up Y = 1:
+			// the flip is X - m (bit x set, bit x+1 cleared), Y | m, up - m
+			const uint32_t fm = rx & ~row.x & ~row.y & up & (1u << x);
+			row.x -= fm;
+			row.y |= fm;
+			my[y * DT_WG] = row;
+			my[(y + 1) * DT_WG].y = up - fm;
+		}
 		else
 		{
 			s = step(s);
@@ -172,12 +191,12 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 			const uint32_t aMax = (uint32_t)((int32_t)(r23 - tq) >> 31);
 			const uint32_t mn = rx & ~row.x & ~row.y & up;
 			const uint32_t mx = ~rx & row.x & row.y & ~up;
-			fm = ((mn & aMin) | (mx & aMax)) & (1u << x);
+			const uint32_t fm = ((mn & aMin) | (mx & aMax)) & (1u << x);
+			row.x ^= fm | (fm << 1);
+			row.y ^= fm;
+			my[y * DT_WG] = row;
+			my[(y + 1) * DT_WG].y = up ^ fm;
 		}
-		row.x ^= fm | (fm << 1);
-		row.y ^= fm;
-		my[y * DT_WG] = row;
-		my[(y + 1) * DT_WG].y = up ^ fm;
 	}
 
 	if(own || i + a.T >= nb)
@@ -278,6 +297,7 @@ __device__ __forceinline__ void cbbit_row(const CbHalfArgs& a, int y, int g)
 	W pending = 0, x0old = 0, f0 = 0;
 	for(int m = 0; m < M; ++m)
 	{
+		const uint64_t sW = s; // stream state at this word's first draw
 		const int xw = g + (m << lG);
 		const W tX = lat[thisB + row + xw];
 		const W tY = lat[thisB + Q + row + xw];
@@ -295,24 +315,33 @@ __device__ __forceinline__ void cbbit_row(const CbHalfArgs& a, int y, int g)
 		if(MODE == 0)
 		{
 			W rnd;
+			uint64_t u = sW;
 			if(WB == 64)
 			{
-				s = step(s);
-				const uint64_t hi = slcg_fold(s);
-				s = step(s);
-				const uint64_t lo = slcg_fold(s);
+				u = step(u);
+				const uint64_t hi = slcg_fold(u);
+				u = step(u);
+				const uint64_t lo = slcg_fold(u);
 				rnd = (W)((hi << 32) ^ lo);
 			}
 			else
 			{
-				s = step(s);
-				rnd = (W)slcg_fold(s);
+				u = step(u);
+				rnd = (W)slcg_fold(u);
 			}
 			flip = minm & rnd;
 		}
 		else
 		{
 			const W maxm = tX & tY & ~xXs & ~yY;
+			// Only the low 23 bits of lo ^ hi are used.  The low state word
+			// evolves on its own (lo' = A_lo lo + C mod 2^32) and hi' mod 2^24
+			// needs only hi mod 2^24 and lo, so inside the word the high half
+			// is carried mod 2^24 with two 24-bit multiply-adds; the full
+			// state is rebuilt from the word's start by the stepJ jump below.
+			const uint32_t aLo = (uint32_t)step.a, aHi = (uint32_t)(step.a >> 32);
+			const uint64_t cc = step.c;
+			uint32_t lo = (uint32_t)sW, h24 = (uint32_t)(sW >> 32);
 			// acceptance bits, site k of the word -> bit k: the sign bit of
 			// r23 - t is funnel-shifted into an accumulator (v_alignbit), 32
 			// sites per accumulator, which ends bit-reversed
@@ -326,8 +355,10 @@ __device__ __forceinline__ void cbbit_row(const CbHalfArgs& a, int y, int g)
 #pragma unroll
 					for(int k = 0; k < 32; ++k)
 					{
-						s = step(s);
-						const uint32_t r23 = slcg_fold(s) & 0x7FFFFFu;
+						const uint64_t pr = (uint64_t)lo * aLo + cc;
+						h24 = mad_u24(h24, aLo, mad_u24(lo, aHi, (uint32_t)(pr >> 32)));
+						lo = (uint32_t)pr;
+						const uint32_t r23 = (lo ^ h24) & 0x7FFFFFu;
 						pa = __builtin_amdgcn_alignbit(pa, r23 - tp, 31);
 						qa = __builtin_amdgcn_alignbit(qa, r23 - tq, 31);
 					}
@@ -337,8 +368,10 @@ __device__ __forceinline__ void cbbit_row(const CbHalfArgs& a, int y, int g)
 #pragma unroll
 					for(int k = 0; k < 32; ++k)
 					{
-						s = step(s);
-						const uint32_t r23 = slcg_fold(s) & 0x7FFFFFu;
+						const uint64_t pr = (uint64_t)lo * aLo + cc;
+						h24 = mad_u24(h24, aLo, mad_u24(lo, aHi, (uint32_t)(pr >> 32)));
+						lo = (uint32_t)pr;
+						const uint32_t r23 = (lo ^ h24) & 0x7FFFFFu;
 						pa = __builtin_amdgcn_alignbit(pa, r23 - tp, 31);
 					}
 				}
@@ -347,7 +380,7 @@ __device__ __forceinline__ void cbbit_row(const CbHalfArgs& a, int y, int g)
 			}
 			flip = (minm & pb) | (maxm & qb);
 		}
-		s = stepJ(s);
+		s = stepJ(sW);
 		lat[thisB + row + xw] = tX ^ flip;
 		lat[thisB + Q + row + xw] = tY ^ flip;
 		lat[othB + Q + rowUp + xw] = yY ^ flip;

@@ -38,7 +38,7 @@ struct CbHalfArgs {
 	int per, rem;           // rows per worker, workers with one extra row
 	const Affine* rowJ;     // rowJ[r]: skip r rows of draws
 	const Affine* laneJ;    // laneJ[g]: skip g words of draws
-	Affine stepJ;           // skip from a lane's word to its next word
+	Affine stepJ;           // J(draws per word * G): a lane's word start to its next word start
 	Affine step;            // per-draw step
 	Affine fullBig, fullSmall; // whole half-sweep of a worker (per+1 / per rows)
 	uint32_t tp, tq;
@@ -86,7 +86,7 @@ struct BatchRep {
 	const Affine* laneJ;    // SCABit: laneJ[g] = J(g * draws per word)
 	const Affine* rowJ;     // SCABit: {identity} (one stream per row)
 	Affine step;            // per-draw step (skip form)
-	Affine stepJ;           // SCABit: J(draws per word * (G-1))
+	Affine stepJ;           // SCABit: J(draws per word * G)
 	Affine full;            // SCABit: J(draws per row)
 	int lx, ly;
 	int kind;               // BatchKind

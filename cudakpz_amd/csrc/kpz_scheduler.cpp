@@ -371,7 +371,7 @@ SchedulerSCABit::Tables& SchedulerSCABit::tables(int mode)
 	}
 	upload(row, t->row);
 	upload(lane, t->lane);
-	t->stepJ = affine_pow(step_, du * (uint64_t)(G - 1));
+	t->stepJ = affine_pow(step_, du * (uint64_t)G);
 	t->fullBig = affine_pow(rowStep, (uint64_t)per_ + 1);
 	t->fullSmall = affine_pow(rowStep, (uint64_t)per_);
 	Tables& ref = *t;

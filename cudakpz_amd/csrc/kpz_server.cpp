@@ -243,7 +243,7 @@ bool Server::admit(Job& j)
 		}
 		d.laneJ = (const Affine*)(base + j.off[2]);
 		check(hipMemcpyAsync((void*)d.laneJ, lane.data(), lane.size() * sizeof(Affine), hipMemcpyHostToDevice, s0_), "lane table");
-		d.stepJ = affine_pow(d.step, du * (uint64_t)(G - 1));
+		d.stepJ = affine_pow(d.step, du * (uint64_t)G);
 		d.full = affine_pow(d.step, du * Wn);
 	}
 	check(launch_fill_flat(d.lat, d.kind == BK_SCABIT ? ENC_CBBIT : ENC_LOCAL, s.lx, s.ly, s0_), "fill");
