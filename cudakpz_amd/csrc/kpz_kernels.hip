@@ -68,6 +68,120 @@ __device__ __forceinline__ uint64_t shflw(uint64_t v, int src, int width) { retu
 // ======================================================================
 constexpr int DT_WG = 256;
 
+// First three delta swaps of the outer unshuffle: a 32-bit row of 16
+// interleaved sites (bit 2c = X_c, 2c+1 = Y_c) becomes the bytes
+// {X 0-7, Y 0-7, X 8-15, Y 8-15}; the last (byte) swap is left to the v_perm
+// that splits the planes.  zip_bytes is the inverse.
+__device__ __forceinline__ uint32_t unzip_bytes(uint32_t x)
+{
+	uint32_t t;
+	t = (x ^ (x >> 1)) & 0x22222222u; x ^= t ^ (t << 1);
+	t = (x ^ (x >> 2)) & 0x0C0C0C0Cu; x ^= t ^ (t << 2);
+	t = (x ^ (x >> 4)) & 0x00F000F0u; x ^= t ^ (t << 4);
+	return x;
+}
+__device__ __forceinline__ uint32_t zip_bytes(uint32_t x)
+{
+	uint32_t t;
+	t = (x ^ (x >> 4)) & 0x00F000F0u; x ^= t ^ (t << 4);
+	t = (x ^ (x >> 2)) & 0x0C0C0C0Cu; x ^= t ^ (t << 2);
+	t = (x ^ (x >> 1)) & 0x22222222u; x ^= t ^ (t << 1);
+	return x;
+}
+
+// v_perm_b32 selector: byte i of the result = byte sel_i of {hi:lo}
+// (0-3 lo, 4-7 hi, 12 = zero)
+constexpr uint32_t perm_sel(int b0, int b1, int b2, int b3)
+{
+	return (uint32_t)b0 | ((uint32_t)b1 << 8) | ((uint32_t)b2 << 16) | ((uint32_t)b3 << 24);
+}
+
+// Static DT with 16-site-wide blocks (origin on the 4x4 word grid): every
+// ENC_LOCAL byte is one site row of 4 sites, bits x0 y0 x1 y1 x2 y2 x3 y3.
+// Stage: per word row one 16-byte load of the block's 4 words (+ the halo
+// word); a site row is byte rr of the 4 words gathered by v_perm, then
+// unzipped into its X (even) and Y (odd) bit planes.
+template<int NWY>
+__device__ __forceinline__ void dt_stage_x16(const uint32_t* __restrict__ lat, int XW, int mXW, int mYW, int c0,
+                                             int wr0, uint32_t (&raw)[NWY][5], uint2* __restrict__ my)
+{
+#pragma unroll
+	for(int wk = 0; wk < NWY; ++wk)
+	{
+		const uint32_t* rp = lat + (size_t)((wr0 + wk) & mYW) * XW;
+		const uint4 q = *reinterpret_cast<const uint4*>(rp + c0);
+		raw[wk][0] = q.x;
+		raw[wk][1] = q.y;
+		raw[wk][2] = q.z;
+		raw[wk][3] = q.w;
+		raw[wk][4] = (wk < NWY - 1) ? rp[(c0 + 4) & mXW] : 0u; // top halo row: corner unused
+	}
+#pragma unroll
+	for(int wk = 0; wk < NWY; ++wk)
+	{
+#pragma unroll
+		for(int rr = 0; rr < 4; ++rr)
+		{
+			if(wk == NWY - 1 && rr > 0)
+				break;
+			const uint32_t b = __builtin_amdgcn_perm(raw[wk][1], raw[wk][0], perm_sel(rr, 4 + rr, 12, 12))
+			                 | __builtin_amdgcn_perm(raw[wk][3], raw[wk][2], perm_sel(12, 12, rr, 4 + rr));
+			const uint32_t u = unzip_bytes(b);
+			// X: 16 columns + the halo column (x = 0 of the next word: bit 0 of
+			// its byte rr, moved to bit 16 with that byte)
+			const uint32_t X = (wk < NWY - 1)
+			                 ? __builtin_amdgcn_perm(raw[wk][4], u, perm_sel(0, 2, 4 + rr, 12)) & 0x1FFFFu : 0u;
+			my[(4 * wk + rr) * DT_WG] = make_uint2(X, __builtin_amdgcn_perm(u, u, perm_sel(1, 3, 12, 12)));
+		}
+	}
+}
+
+// Inverse of dt_stage_x16: zip each row back to its interleaved byte form,
+// transpose 4 rows x 4 words of bytes with v_perm, one 16-byte store per
+// word row; halo column (X of x = 16) and top halo row (Y of row BY) are
+// merged into the words read at staging (no other lane of this phase
+// writes them).
+template<int NWY>
+__device__ __forceinline__ void dt_writeback_x16(uint32_t* __restrict__ lat, int XW, int mXW, int mYW, int c0,
+                                                 int wr0, const uint32_t (&raw)[NWY][5],
+                                                 const uint2* __restrict__ my)
+{
+#pragma unroll
+	for(int wk = 0; wk < NWY - 1; ++wk)
+	{
+		uint32_t* rp = lat + (size_t)((wr0 + wk) & mYW) * XW;
+		uint32_t B[4], hb = 0;
+#pragma unroll
+		for(int rr = 0; rr < 4; ++rr)
+		{
+			const uint2 tv = my[(4 * wk + rr) * DT_WG];
+			B[rr] = zip_bytes(__builtin_amdgcn_perm(tv.y, tv.x, perm_sel(0, 4, 1, 5)));
+			hb |= ((tv.x >> 16) & 1u) << (8 * rr);
+		}
+		const uint32_t T0 = __builtin_amdgcn_perm(B[1], B[0], perm_sel(0, 4, 2, 6));
+		const uint32_t T1 = __builtin_amdgcn_perm(B[1], B[0], perm_sel(1, 5, 3, 7));
+		const uint32_t T2 = __builtin_amdgcn_perm(B[3], B[2], perm_sel(0, 4, 2, 6));
+		const uint32_t T3 = __builtin_amdgcn_perm(B[3], B[2], perm_sel(1, 5, 3, 7));
+		uint4 q;
+		q.x = __builtin_amdgcn_perm(T2, T0, perm_sel(0, 1, 4, 5));
+		q.y = __builtin_amdgcn_perm(T3, T1, perm_sel(0, 1, 4, 5));
+		q.z = __builtin_amdgcn_perm(T2, T0, perm_sel(2, 3, 6, 7));
+		q.w = __builtin_amdgcn_perm(T3, T1, perm_sel(2, 3, 6, 7));
+		*reinterpret_cast<uint4*>(rp + c0) = q;
+		rp[(c0 + 4) & mXW] = (raw[wk][4] & ~0x01010101u) | hb;
+	}
+	// top halo row: only the Y bits of its first site row (byte 0, odd bits)
+	constexpr int wk = NWY - 1;
+	uint32_t* rp = lat + (size_t)((wr0 + wk) & mYW) * XW;
+	const uint32_t b = zip_bytes(__builtin_amdgcn_perm(my[(NWY - 1) * 4 * DT_WG].y, 0u, perm_sel(12, 4, 12, 5)));
+	uint4 q;
+	q.x = (raw[wk][0] & ~0xAAu) | (b & 0xAAu);
+	q.y = (raw[wk][1] & ~0xAAu) | ((b >> 8) & 0xAAu);
+	q.z = (raw[wk][2] & ~0xAAu) | ((b >> 16) & 0xAAu);
+	q.w = (raw[wk][3] & ~0xAAu) | ((b >> 24) & 0xAAu);
+	*reinterpret_cast<uint4*>(rp + c0) = q;
+}
+
 // One DT block i of one phase, staged in this lane's LDS column `my`
 // (row r at my[r * DT_WG]).  nb = blocks per domain set.
 template<int LBX, int LBY, int UPD, bool RO>
@@ -94,7 +208,14 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 	// ---- stage ENC_LOCAL words -> bit-plane rows.  The raw words stay in
 	// registers: the write-back merges into them instead of re-reading HBM
 	// (no other lane of this phase writes them).
+	// Static 16-wide blocks take the byte-gather path; shifted origins (DTr)
+	// and narrower blocks the general nibble path below.
+	constexpr bool X16 = !RO && LBX == 4;
 	uint32_t raw[NWY][NWX];
+	if constexpr(X16)
+		dt_stage_x16<NWY>(lat, XW, mXW, mYW, c0, wr0, raw, my);
+	else
+	{
 #pragma unroll
 	for(int wk = 0; wk < NWY; ++wk)
 	{
@@ -132,6 +253,7 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 			const uint32_t X = (r < BY) ? ((sxv >> shx) & MX) : 0u;
 			my[r * DT_WG] = make_uint2(X, (syv >> shx) & MY);
 		}
+	}
 	}
 
 	// ---- stream state: block j of stream t starts j blocks of draws in
@@ -204,6 +326,11 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 
 	// ---- bit-plane rows -> ENC_LOCAL words (masked read-modify-write at the
 	// block edges; no other lane of this phase touches these words)
+	if constexpr(X16)
+	{
+		dt_writeback_x16<NWY>(lat, XW, mXW, mYW, c0, wr0, raw, my);
+		return;
+	}
 	constexpr uint32_t SEGX = MX, SEGY = MY;
 #pragma unroll
 	for(int wk = 0; wk < NWY; ++wk)
