@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "site updates/s (DT, SCABit @65536²); batched replica-MCS/s vs serial"
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
 
 
 class HipEvents:
@@ -94,8 +95,8 @@ def run_dt(K, args, ev):
     return s, total_ms, mcs_ms, w2
 
 
-def run_scabit(K, lx, p, q, steps, seed, ev):
-    s = K.SchedulerSCABit(lx, lx)
+def run_scabit(K, lx, p, q, steps, seed, ev, cls=None):
+    s = (cls or K.SchedulerSCABit)(lx, lx)
     s.supplySeed(seed)
     s.initHomogeneous()
     s.setPQ(p, q)
@@ -296,11 +297,21 @@ def main():
     bytes_per_launch = 0.5 * N / 4  # 0.5 B per site update, N/4 updates per phase
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     value = N * args.steps / (total_ms * 1e-3)
-    traffic = None
+    prof = {}
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     key = f"dt_phase_kernel<4,4,KPZ,static>@{1 << args.lx}"
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get(key, {}).get("bytes_per_launch")
+        prof = json.load(open(tfile)).get(key, {})
+    traffic = prof.get("bytes_per_launch")
+    # VALU issue: wave instructions per launch (rocprof SQ_INSTS_VALU) x 64
+    # lanes over the live launch time, against 256 CU x 4 SIMD x 16 lanes x 2.4 GHz
+    valu = None
+    if prof.get("valu_insts_per_launch"):
+        lane_ops = prof["valu_insts_per_launch"] * 64.0
+        rate = lane_ops / (avg_launch_ms * 1e-3)
+        valu = {"lane_ops_per_launch": lane_ops, "achieved_T_lane_ops": rate / 1e12,
+                "peak_T_lane_ops": VALU_PEAK_LANE_OPS / 1e12, "frac": rate / VALU_PEAK_LANE_OPS,
+                "ops_per_update": lane_ops / (N / 4)}
     line = {
         "metric": METRIC,
         "value": value,
@@ -321,15 +332,17 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "dt_phase_kernel<4,4,KPZ,static>", "avg_launch_ms": avg_launch_ms,
-                     "bytes_per_launch": bytes_per_launch},
+                     "bytes_per_launch": bytes_per_launch, "valu": valu},
         "w2_final": w2[-1][1] if w2 else None,
     }
     s.close()
     if args.secondary:
         sec = {}
-        for name, p, q in [("scabit_p0.5_q0", 0.5, 0.0), ("scabit_p0.95_q0.05", 0.95, 0.05)]:
-            steps = 100
-            ms = run_scabit(K, args.lx, p, q, steps, args.seed, ev)
+        for name, p, q, cls in [("scabit_p0.5_q0", 0.5, 0.0, K.SchedulerSCABit),
+                                ("scabit_p0.95_q0.05", 0.95, 0.05, K.SchedulerSCABit),
+                                ("sca_local_p0.95_q0", 0.95, 0.0, K.SchedulerSCA)]:
+            steps = 100 if cls is K.SchedulerSCABit else 20
+            ms = run_scabit(K, args.lx, p, q, steps, args.seed, ev, cls)
             ups = N * steps / (ms * 1e-3)
             sec[name] = {"site_updates_per_s": ups, "hbm_GBs_algorithmic": ups * 1.0 / 1e9,
                          "hbm_frac": ups / 1e9 / HBM_PEAK_GBS, "ms_per_mcs": ms / steps}
