@@ -66,6 +66,7 @@ def lib():
         L.kpz_mcs.argtypes = [h, C.c_int]
         L.kpz_mcs_pq.argtypes = [h, C.c_int]
         L.kpz_roughness.argtypes = [h, i64p]
+        L.kpz_moments.argtypes = [h, C.POINTER(C.c_double)]
         L.kpz_correlate.argtypes = [h, i64p]
         L.kpz_threads.argtypes = [h]
         L.kpz_threads.restype = C.c_int64
@@ -248,6 +249,12 @@ class SchedulerService:
         out = (C.c_int64 * 3)()
         _check(lib().kpz_roughness(self._h, out))
         return Roughness(self.N, int(out[0]), _i128(out[1], out[2]))
+
+    def moments(self):
+        """<h^k>, k = 1..7 (Roughness::h[], kpz/roughness.cpp:38-43)."""
+        out = (C.c_double * 7)()
+        _check(lib().kpz_moments(self._h, out))
+        return list(out)
 
     def correlateTag(self, stop=None):
         if stop is None:

@@ -183,6 +183,19 @@ int kpz_roughness(kpz_scheduler* h, int64_t out[3])
 	});
 }
 
+int kpz_moments(kpz_scheduler* h, double out[7])
+{
+	HANDLE(h);
+	if(!out)
+		return fail(KPZ_EINVAL, "null output");
+	return guard([&] {
+		long double m[7];
+		h->s->moments(m);
+		for(int k = 0; k < 7; ++k)
+			out[k] = (double)m[k];
+	});
+}
+
 int kpz_corr_tag(kpz_scheduler* h)
 {
 	HANDLE(h);

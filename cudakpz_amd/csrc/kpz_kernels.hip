@@ -545,10 +545,21 @@ __global__ __launch_bounds__(256) void cbbit_half_kernel(CbHalfArgs a)
 // ======================================================================
 // SCA on ENC_LOCAL (SchedulerCPUCB, kpz/schedulerCPUCB.cpp:43-170)
 // ======================================================================
-// One lane per 4x4-site cell.  In a checkerboard half-sweep every bit a site
-// reads (own slopes, x slope of x+1, y slope of y+1) is written only by that
-// site, so the three words are read once; other lanes flip other bits of the
-// same words, hence the atomic XORs.
+// In a checkerboard half-sweep every bit an active site reads (own slopes,
+// x slope of x+1, y slope of y+1) is written only by that site, so the
+// result does not depend on the order of the sites, and a site's flips of
+// its right and up neighbours' bits can be applied to their words at any
+// time before the half-sweep ends.
+//
+// One lane per 4x4-site cell column of a band of 2^lNB cell rows; a group of
+// G = 2^lG lanes (G <= 64, aligned in the wave) covers G adjacent columns.
+// Per cell row the lane loads its own word (kept from the previous row's
+// "up" load), the right word and the up word once, tests all 8 active sites
+// bit-parallel and stores its word once: own flips, the left lane's flips of
+// column 0 (a shuffle) and the flips of row 0 from the cell below (kept in a
+// register) are merged into one plain store.  Flips that leave a group
+// (right edge) or a band (top row) go to small edge buffers, which
+// cb_local_fix_kernel applies afterwards.
 __device__ __forceinline__ int cb_class(const CbLocalArgs& a, int v)
 {
 	int k = 0;
@@ -558,15 +569,85 @@ __device__ __forceinline__ int cb_class(const CbLocalArgs& a, int v)
 	return k;
 }
 
-__global__ __launch_bounds__(256) void cb_local_half_kernel(CbLocalArgs a)
+// active sites of a cell in draw order (yy = 0..3, x = x0, x0 + 2 with
+// x0 = (yy & 1) ^ signum), X-bit positions 2x + 8yy:
+//   signum 0: 0 4 10 14 16 20 26 30      signum 1: 2 6 8 12 18 22 24 28
+// The acceptance bit of draw i is funnel-shifted in as K[i] copies of the
+// sign of r23 - t (bits 31..23 of that difference are all sign), so after a
+// bit reversal draw i covers positions [P[i], P[i+1]) and lands on its site.
+template<int S>
+struct CbCell {
+	static constexpr uint32_t MS = S ? 0x11441144u : 0x44114411u; // active X positions
+	static constexpr int K[8] = {S ? 6 : 4, S ? 2 : 6, 4, S ? 6 : 2, 4, S ? 2 : 6, 4, S ? 4 : 2};
+};
+
+// one cell: 8 draws (stream state at the cell's first draw in sc), flips of
+// the own word, of column 0 of the right word and of row 0 of the up word
+template<int S>
+__device__ __forceinline__ void cb_cell(uint64_t sc, const Affine& step, uint32_t tp, uint32_t tq, uint32_t own,
+                                        uint32_t right, uint32_t up, uint32_t& fOwn, uint32_t& fRight, uint32_t& fUp)
+{
+	// only (lo ^ hi) & 0x7FFFFF is used: high state half carried mod 2^24
+	const uint32_t aLo = (uint32_t)step.a, aHi = (uint32_t)(step.a >> 32);
+	const uint64_t cc = step.c;
+	uint32_t lo = (uint32_t)sc, h24 = (uint32_t)(sc >> 32);
+	uint32_t ap = 0, aq = 0;
+	if(tq)
+	{
+#pragma unroll
+		for(int i = 0; i < 8; ++i)
+		{
+			const uint64_t pr = (uint64_t)lo * aLo + cc;
+			h24 = mad_u24(h24, aLo, mad_u24(lo, aHi, (uint32_t)(pr >> 32)));
+			lo = (uint32_t)pr;
+			const uint32_t r23 = (lo ^ h24) & 0x7FFFFFu;
+			ap = __builtin_amdgcn_alignbit(ap, r23 - tp, 32 - CbCell<S>::K[i]);
+			aq = __builtin_amdgcn_alignbit(aq, r23 - tq, 32 - CbCell<S>::K[i]);
+		}
+	}
+	else
+	{
+#pragma unroll
+		for(int i = 0; i < 8; ++i)
+		{
+			const uint64_t pr = (uint64_t)lo * aLo + cc;
+			h24 = mad_u24(h24, aLo, mad_u24(lo, aHi, (uint32_t)(pr >> 32)));
+			lo = (uint32_t)pr;
+			const uint32_t r23 = (lo ^ h24) & 0x7FFFFFu;
+			ap = __builtin_amdgcn_alignbit(ap, r23 - tp, 32 - CbCell<S>::K[i]);
+		}
+	}
+	ap = __brev(ap);
+	aq = __brev(aq);
+	// slopes at the X positions 2x + 8y: own X, own Y, X of x+1, Y of y+1
+	const uint32_t X = own & 0x55555555u;
+	const uint32_t Yb = (own >> 1) & 0x55555555u;
+	const uint32_t rX = ((X >> 2) & 0x15151515u) | ((right & 0x01010101u) << 6);
+	const uint32_t uY = ((Yb >> 8) & 0x00555555u) | (((up >> 1) & 0x55u) << 24);
+	// pqSyncRngUpdate configurations 0b1100 (minimum) and 0b0011 (maximum)
+	const uint32_t mn = ~X & ~Yb & rX & uY;
+	const uint32_t mx = X & Yb & ~rX & ~uY;
+	const uint32_t f = ((mn & ap) | (mx & aq)) & CbCell<S>::MS;
+	fOwn = f | (f << 1) | ((f & 0x15151515u) << 2) | ((f & 0x00555555u) << 9);
+	fRight = (f >> 6) & 0x01010101u;
+	fUp = ((f >> 24) & 0x55u) << 1;
+}
+
+template<int S>
+__global__ __launch_bounds__(256) void cb_local_band_kernel(CbLocalArgs a)
 {
 	const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	const int lXC = a.lx - 2;
 	const int XC = 1 << lXC;
-	if(gid >= ((int64_t)XC << (a.ly - 2)))
-		return;
-	const int c = (int)(gid & (XC - 1));
-	const int yb = (int)(gid >> lXC);
+	const int G = 1 << a.lG, NB = 1 << a.lNB;
+	const int ngc = XC >> a.lG; // groups per band
+	if(gid >= ((int64_t)XC << (a.ly - 2 - a.lNB)))
+		return; // whole groups only: the shuffles below stay inside live groups
+	const int g = (int)(gid & (G - 1));
+	const int64_t grp = gid >> a.lG;
+	const int cg = (int)(grp % ngc), band = (int)(grp / ngc);
+	const int c = cg * G + g;
+	const int yb0 = band * NB;
 	// worker of this cell column and the cell's place in its strip
 	const int big = a.rem * (a.pc + 1);
 	int t, c0;
@@ -586,58 +667,70 @@ __global__ __launch_bounds__(256) void cb_local_half_kernel(CbLocalArgs a)
 	const bool h = cc >= cells0;
 	const int ci = h ? cc - cells0 : cc;
 	const int cellsH = h ? w - cells0 : cells0;
+	const int clsH = cb_class(a, cellsH);
 	const uint64_t s0 = a.st_in[t];
 	uint64_t s = s0;
 	if(h)
 		s = a.baseJ[cb_class(a, cells0)](s);
-	s = a.rowJ[cb_class(a, cellsH) * a.nyb + yb](s);
+	s = a.rowJ[clsH * a.nyb + yb0](s);
 	s = a.cellJ[ci](s);
-	if(yb == 0 && cc == 0)
+	if(yb0 == 0 && cc == 0)
 		a.st_out[t] = (t < a.rem ? a.fullBig : a.fullSmall)(s0);
+	// the same cell of the next row: 8 draws per cell of the half strip later
+	const Affine rowStep = a.rowJ[clsH * a.nyb + 1];
 
 	uint32_t* __restrict__ lat = a.lat;
-	const size_t iOwn = ((size_t)yb << lXC) + c;
-	const size_t iRight = ((size_t)yb << lXC) + ((c + 1) & (XC - 1));
-	const size_t iUp = ((size_t)((yb + 1) & (a.nyb - 1)) << lXC) + c;
-	const uint32_t own = lat[iOwn], right = lat[iRight], up = lat[iUp];
-	uint32_t fOwn = 0, fRight = 0, fUp = 0;
 	const Affine step = a.step;
-#pragma unroll
-	for(int yy = 0; yy < 4; ++yy)
+	const uint32_t tp = a.tp, tq = a.tq;
+	const bool wrapG = (XC == G); // the group spans the row: the right edge wraps by shuffle
+	uint32_t own = lat[((size_t)yb0 << lXC) + c];
+	uint32_t fromBelow = 0;
+	for(int it = 0; it < NB; ++it)
 	{
-		const int x0 = ((yy & 1) ^ a.signum); // x & 3 of the row's first active site (x0 = (xW + (y&1)) ^ s)
-#pragma unroll
-		for(int k = 0; k < 2; ++k)
-		{
-			const int xx = x0 + 2 * k;
-			s = step(s);
-			const uint32_t r23 = slcg_fold(s) & 0x7FFFFFu;
-			const int sh = (xx << 1) | (yy << 3);
-			const uint32_t cbits = (own >> sh) & 3u;
-			const uint32_t rb = (xx < 3) ? (own >> (sh + 2)) & 1u : (right >> (yy << 3)) & 1u;
-			const uint32_t lb = (yy < 3) ? (own >> (sh + 9)) & 1u : (up >> ((xx << 1) | 1)) & 1u;
-			const uint32_t cfg = cbits | (rb << 2) | (lb << 3);
-			const bool flip = (cfg == 0xCu && r23 < a.tp) || (cfg == 0x3u && r23 < a.tq);
-			if(flip)
-			{
-				fOwn ^= 3u << sh;
-				if(xx < 3)
-					fOwn ^= 1u << (sh + 2);
-				else
-					fRight ^= 1u << (yy << 3);
-				if(yy < 3)
-					fOwn ^= 1u << (sh + 9);
-				else
-					fUp ^= 1u << ((xx << 1) | 1);
-			}
-		}
+		const int yb = yb0 + it;
+		const size_t row = (size_t)yb << lXC;
+		const uint32_t right = lat[row + ((c + 1) & (XC - 1))];
+		const uint32_t up = lat[((size_t)((yb + 1) & (a.nyb - 1)) << lXC) + c];
+		uint32_t fOwn, fRight, fUp;
+		cb_cell<S>(s, step, tp, tq, own, right, up, fOwn, fRight, fUp);
+		uint32_t fromLeft = (uint32_t)__shfl((int)fRight, (g - 1) & (G - 1), G);
+		if(g == 0 && !wrapG)
+			fromLeft = 0;
+		lat[row + c] = own ^ fOwn ^ fromLeft ^ fromBelow;
+		if(g == G - 1 && !wrapG)
+			a.edgeR[(size_t)yb * ngc + cg] = fRight;
+		fromBelow = fUp;
+		own = up; // no other lane writes the bits this lane reads from it
+		s = rowStep(s);
 	}
-	if(fOwn)
-		atomicXor(lat + iOwn, fOwn);
-	if(fRight)
-		atomicXor(lat + iRight, fRight);
-	if(fUp)
-		atomicXor(lat + iUp, fUp);
+	a.edgeU[(size_t)band * XC + c] = fromBelow;
+}
+
+// flips that crossed a group's right edge or a band's top row
+__global__ __launch_bounds__(256) void cb_local_fix_kernel(CbLocalArgs a)
+{
+	const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const int lXC = a.lx - 2;
+	const int XC = 1 << lXC;
+	const int G = 1 << a.lG;
+	const int ngc = XC >> a.lG;
+	const int64_t nR = (XC > G) ? ((int64_t)a.nyb * ngc) : 0;
+	const int64_t nU = (int64_t)(a.nyb >> a.lNB) * XC;
+	if(gid < nR)
+	{
+		const uint32_t v = a.edgeR[gid];
+		const int yb = (int)(gid / ngc), cg = (int)(gid % ngc);
+		if(v)
+			atomicXor(a.lat + ((size_t)yb << lXC) + (((cg + 1) * G) & (XC - 1)), v);
+	}
+	else if(gid < nR + nU)
+	{
+		const int64_t k = gid - nR;
+		const uint32_t v = a.edgeU[k];
+		const int band = (int)(k >> lXC), c = (int)(k & (XC - 1));
+		if(v)
+			atomicXor(a.lat + ((size_t)(((band + 1) << a.lNB) & (a.nyb - 1)) << lXC) + c, v);
+	}
 }
 
 // ======================================================================
@@ -863,6 +956,89 @@ __global__ __launch_bounds__(256) void row_stats_kernel(const uint32_t* __restri
 	if(y >= (1 << ly))
 		return; // uniform per wave
 	row_stats_row<ENC, CORR>(lat, snap, lx, ly, y, threadIdx.x & 63, out + (size_t)y * ROW_STATS, false);
+}
+
+// Height moments of site row y: out[y*7 + k-1] = sum_x h(x, y)^k, k = 1..7, in
+// double (Roughness::addH, kpz/schedulerService.h:56-61), with the row's start
+// height hs[y] from the exact row statistics.  Heights as in row_stats_row:
+// sites x >= 1 at hs + (x slopes 1..x), site 0 at hs + D.
+template<int ENC>
+__global__ __launch_bounds__(256) void row_moments_kernel(const uint32_t* __restrict__ lat, int lx, int ly,
+		const int64_t* __restrict__ hs, double* __restrict__ out)
+{
+	const int y = blockIdx.x * 4 + (threadIdx.x >> 6);
+	if(y >= (1 << ly))
+		return; // uniform per wave
+	const int lane = threadIdx.x & 63;
+	const int nbits = (lx >= 6) ? 64 : (1 << lx);
+	const int nch = (lx >= 6) ? (1 << (lx - 6)) : 1;
+	const int64_t h0 = hs[y];
+	double m[7] = {0, 0, 0, 0, 0, 0, 0};
+	int64_t run = 0;
+	int d0 = 0;
+	for(int base = 0; base < nch; base += 64)
+	{
+		const int j = base + lane;
+		const bool valid = j < nch;
+		uint64_t b = 0;
+		int q = 0;
+		if(valid)
+		{
+			b = xbits<ENC>(lat, lx, ly, j, y);
+			const uint64_t cnt = (j == 0) ? (b & ~1ull) : b;
+			const int n = (j == 0) ? nbits - 1 : nbits;
+			const uint64_t live = (nbits == 64) ? ~0ull : ((1ull << nbits) - 1);
+			q = 2 * __popcll(cnt & live) - n;
+			if(j == 0)
+				d0 = (b & 1) ? 1 : -1;
+		}
+		const int64_t incl = wave_incl_scan(q, lane);
+		int64_t h = h0 + run + incl - q;
+		if(valid)
+		{
+			for(int k = (j == 0) ? 1 : 0; k < nbits; ++k)
+			{
+				h += ((b >> k) & 1) ? 1 : -1;
+				const double v = (double)h;
+				double p = v;
+				m[0] += p;
+#pragma unroll
+				for(int e = 1; e < 7; ++e)
+				{
+					p *= v;
+					m[e] += p;
+				}
+			}
+		}
+		run += (int64_t)shfl64((uint64_t)incl, 63, 64);
+	}
+	d0 = __shfl(d0, 0, 64);
+	if(lane == 0)
+	{
+		// site x = 0 sits at the end of the row walk: hs + D
+		const double v = (double)(h0 + run + d0);
+		double p = v;
+		m[0] += p;
+#pragma unroll
+		for(int e = 1; e < 7; ++e)
+		{
+			p *= v;
+			m[e] += p;
+		}
+	}
+#pragma unroll
+	for(int e = 0; e < 7; ++e)
+	{
+#pragma unroll
+		for(int o = 32; o > 0; o >>= 1)
+			m[e] += __shfl_xor(m[e], o, 64);
+	}
+	if(lane == 0)
+	{
+#pragma unroll
+		for(int e = 0; e < 7; ++e)
+			out[(size_t)y * 7 + e] = m[e];
+	}
 }
 
 // ---------------------------------------------------------------- fill
@@ -1096,6 +1272,17 @@ hipError_t launch_cbbit_half(const CbHalfArgs& a, hipStream_t stream)
 	return hipGetLastError();
 }
 
+hipError_t launch_row_moments(const uint32_t* lat, int enc, int lx, int ly, const int64_t* hs, double* out,
+		hipStream_t stream)
+{
+	const dim3 grid((unsigned)(((1 << ly) + 3) / 4));
+	if(enc == ENC_LOCAL)
+		hipLaunchKernelGGL((row_moments_kernel<ENC_LOCAL>), grid, dim3(256), 0, stream, lat, lx, ly, hs, out);
+	else
+		hipLaunchKernelGGL((row_moments_kernel<ENC_CBBIT>), grid, dim3(256), 0, stream, lat, lx, ly, hs, out);
+	return hipGetLastError();
+}
+
 hipError_t launch_row_stats(const uint32_t* lat, const uint32_t* snap, int enc, int lx, int ly,
 		int64_t* out, hipStream_t stream)
 {
@@ -1130,8 +1317,21 @@ hipError_t launch_fill_flat(uint32_t* lat, int enc, int lx, int ly, hipStream_t 
 
 hipError_t launch_cb_local_half(const CbLocalArgs& a, hipStream_t stream)
 {
-	const int64_t cells = 1ll << (a.lx + a.ly - 4);
-	hipLaunchKernelGGL(cb_local_half_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, stream, a);
+	// the band kernel's shape: groups of 2^lG <= 64 columns inside a row of
+	// 2^(lx-2) cells, bands of 2^lNB cell rows, at least 2 cell rows
+	if(a.lx < 3 || a.ly < 3 || a.lG < 0 || a.lG > 6 || a.lG > a.lx - 2 || a.lNB < 0 || a.lNB > a.ly - 2 ||
+	   !a.edgeU || (a.lG < a.lx - 2 && !a.edgeR))
+		return hipErrorInvalidValue;
+	const int64_t lanes = 1ll << (a.lx - 2 + a.ly - 2 - a.lNB);
+	const unsigned grid = (unsigned)((lanes + 255) / 256);
+	if(a.signum)
+		hipLaunchKernelGGL(cb_local_band_kernel<1>, dim3(grid), dim3(256), 0, stream, a);
+	else
+		hipLaunchKernelGGL(cb_local_band_kernel<0>, dim3(grid), dim3(256), 0, stream, a);
+	const int XC = 1 << (a.lx - 2);
+	const int64_t nR = (a.lG < a.lx - 2) ? ((int64_t)a.nyb * (XC >> a.lG)) : 0;
+	const int64_t nU = (int64_t)(a.nyb >> a.lNB) * XC;
+	hipLaunchKernelGGL(cb_local_fix_kernel, dim3((unsigned)((nR + nU + 255) / 256)), dim3(256), 0, stream, a);
 	return hipGetLastError();
 }
 

@@ -65,7 +65,12 @@ struct CbLocalArgs {
 	Affine fullBig, fullSmall; // J(nyb * w * 8) for w = pc+1 / pc
 	Affine step;
 	uint32_t tp, tq;
+	int lG;                 // log2 cell columns per lane group (<= 6, <= lx - 2)
+	int lNB;                // log2 cell rows per band (<= ly - 2)
+	uint32_t* edgeR;        // [nyb][groups per row]: flips leaving a group's right edge
+	uint32_t* edgeU;        // [bands][cells per row]: flips leaving a band's top row
 };
+// one half-sweep: the band kernel plus the edge fix-up (two launches)
 hipError_t launch_cb_local_half(const CbLocalArgs& a, hipStream_t stream);
 
 // ---- reductions ----------------------------------------------------------
@@ -73,6 +78,10 @@ hipError_t launch_cb_local_half(const CbLocalArgs& a, hipStream_t stream);
 // out[y*ROW_STATS + k] = {S1, S2, D, sy0, S1', D', sy0', S11, tcs, 0}, the
 // primed values from `snap` (zero when snap is null).
 constexpr int ROW_STATS = 10;
+// out[y*7 + k-1] = sum over row y of h^k (double), k = 1..7, given the exact
+// start height hs[y] of every row
+hipError_t launch_row_moments(const uint32_t* lat, int enc, int lx, int ly, const int64_t* hs, double* out,
+		hipStream_t stream);
 hipError_t launch_row_stats(const uint32_t* lat, const uint32_t* snap, int enc, int lx, int ly,
 		int64_t* out, hipStream_t stream);
 

@@ -1,0 +1,712 @@
+// kpz_scaling: the reference's kpzScaling front end (kpz/kpzScaling.cpp:47-237)
+// on the MI355X engine, through the C ABI of include/kpz_amd.h.
+//
+//   kpz_scaling mcs [lx [ly]] [options]
+//
+// Arguments, sampling and event loop follow kpz/kpzFrontend.cpp:400-859 and
+// kpz/sampling.{h,cpp}; the output table is the reference's: one line per
+// measurement, "mcs  w2  <h> <h^2> ... <h^7>" (operator<< of Roughness,
+// kpz/schedulerService.cpp:73-79), with "  tagMcs  ch  cs" per live
+// correlator tag (kpz/schedulerService.cpp:216-251, kpz/correlator.cpp:754),
+// and "mcs  w2  <h>...<h^7>  set corr tag" when a tag is taken, printed with
+// 12 significant digits.  All analysis runs on the device: w2, ch and the
+// slope overlap are exact, the higher moments are double per row summed in
+// long double.
+#include "kpz_amd.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <list>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+const char* VERSION = "kpz_scaling (MI355X engine, gfx950)";
+
+// ---------------------------------------------------------------- arguments
+template<class T>
+bool getNum(T& v, int a, int argc, char** argv)
+{
+	if(a >= argc)
+		return false;
+	std::istringstream s(argv[a]);
+	T t;
+	s >> t;
+	if(s.fail() || !s.eof())
+		return false;
+	v = t;
+	return true;
+}
+
+bool getStr(std::string& v, int a, int argc, char** argv)
+{
+	if(a >= argc)
+		return false;
+	v = argv[a];
+	return true;
+}
+
+// SamplingParam (kpz/sampling.h:55-96)
+struct Sampling {
+	enum Kind { EXP, LINEAR, RANDOM } kind = EXP;
+	double density = .1;
+	unsigned interval = 0;
+	double mean = 0;
+};
+
+// SamplingEvent kinds (kpz/sampling.h:27-112)
+struct Event {
+	enum Kind { BARRIER, END, SAMPLING, TAG } kind;
+	unsigned start;
+	bool countAsMeasurement;
+	Sampling sampling;
+	unsigned stop;
+};
+
+struct Args {
+	unsigned mcs = 0;
+	int lx = 10, ly = 10;
+	std::string scheduler = "default";
+	int64_t mcsThreads = 0;
+	int dd[2] = {0, 0};
+	double p = NAN, q = NAN;
+	int anaThreads = 1;
+	bool countTags = false, save = false;
+	double maxH = 0;
+	std::string file;
+	bool haveSeed = false;
+	uint64_t seed = 0;
+	int device = -1;
+	Sampling current;
+	std::list<Event> events;
+	bool finished = false;
+};
+
+void help()
+{
+	std::cout << "kpz_scaling mcs [x [y]] [options]\n"
+	             "  x, y: log2 lattice sizes (default 10; y defaults to x)\n"
+	             "  --scheduler dt|dtStatic|scaBit|sca|cb|default [--mcsThreads n] [--ddBlocksize bx [by]]\n"
+	             "  --pq p [q]                rates (mcsPQSyncRng)\n"
+	             "  --sampleRate d [start]    measurements at ceil(exp(log(t+10)+d)) (default d = 0.1)\n"
+	             "  --linSample n [start]     every n MCS\n"
+	             "  --randomSample m [start]  uniform in [m/2, 3m/2] MCS apart (master dSFMT)\n"
+	             "  --corrStart t [stop]      correlator tag at t, kept until stop\n"
+	             "  --countTagsAsMeasurement | --noCountTagsAsMeasurement\n"
+	             "  --anaThreads n            accepted; analysis runs on the GPU\n"
+	             "  --maxH hours              wall-clock limit (prints TIMEOUT mcs)\n"
+	             "  --save | --noSave         write KPZheightmap_<mcs>.xyz at the end\n"
+	             "  -f, --file name           output table to a file (-- = stdout)\n"
+	             "  --seed s                  64-bit seed (default: /dev/urandom and time)\n"
+	             "  --device n                HIP device\n";
+}
+
+// parseSysArgs (kpz/kpzFrontend.cpp:470-488): mcs [x [y]] first
+int parseSysArgs(Args& A, int argc, char** argv)
+{
+	int a = 1;
+	if(!getNum(A.mcs, a, argc, argv))
+	{
+		std::cerr << "Number of mcs was not supplied.\n";
+		return -1;
+	}
+	if(!getNum(A.lx, ++a, argc, argv))
+	{
+		A.lx = A.ly = 10;
+		return a;
+	}
+	if(!getNum(A.ly, ++a, argc, argv))
+	{
+		A.ly = A.lx;
+		return a;
+	}
+	return a + 1;
+}
+
+void pushSampling(Args& A, unsigned start, const Sampling& s)
+{
+	if(start == 0)
+		A.current = s;
+	else
+		A.events.push_back(Event{Event::SAMPLING, start, true, s, 0});
+}
+
+// SchedulerConfig::parseArgs (kpz/kpzFrontend.cpp:93-232): the scheduler id
+// and the scheduler options that follow it
+int parseScheduler(Args& A, int argc, char** argv, int a)
+{
+	if(!getStr(A.scheduler, a, argc, argv))
+	{
+		std::cerr << "Expected scheduler id.\n";
+		return -1;
+	}
+	if(A.scheduler == "cb")
+		A.scheduler = "sca";
+	if(A.scheduler != "dt" && A.scheduler != "dtStatic" && A.scheduler != "scaBit" && A.scheduler != "sca" &&
+	   A.scheduler != "default")
+	{
+		std::cerr << "[EE][SchedulerConfig] No such scheduler: " << A.scheduler
+		          << " (this engine runs dt, dtStatic, scaBit, sca/cb)\n";
+		return -1;
+	}
+	for(++a; a < argc; ++a)
+	{
+		if(!strcmp(argv[a], "--yield") || !strcmp(argv[a], "--anaGPU"))
+			std::cerr << "[WW][SchedulerConfig] " << argv[a] << " has no effect on this engine.\n";
+		else if(!strcmp(argv[a], "--byRank"))
+		{
+			int tmp;
+			if(!getNum(tmp, ++a, argc, argv))
+				--a;
+			std::cerr << "[WW][SchedulerConfig] --byRank: this engine runs on one GPU (use --device).\n";
+		}
+		else if(!strcmp(argv[a], "-l") || !strcmp(argv[a], "--localLayout"))
+		{
+			std::string tmp;
+			if(!getStr(tmp, ++a, argc, argv))
+			{
+				std::cerr << "[EE][SchedulerConfig] expected local layout preset id for --localLayout\n";
+				return -1;
+			}
+			std::cerr << "[WW][SchedulerConfig] --localLayout has no effect on this engine.\n";
+		}
+		else if(!strcmp(argv[a], "--1d"))
+		{
+			std::cerr << "[EE][SchedulerConfig] --1d: the 1+1d schedulers are not part of this engine.\n";
+			return -1;
+		}
+		else if(!strcmp(argv[a], "--mcsThreads"))
+		{
+			if(!getNum(A.mcsThreads, ++a, argc, argv))
+			{
+				std::cerr << "Expected number of threads for --mcsThreads.\n";
+				return -1;
+			}
+			if(A.mcsThreads < 1)
+			{
+				std::cerr << "Invalid number of threads given, irgnoring.\n";
+				A.mcsThreads = 1;
+			}
+		}
+		else if(!strcmp(argv[a], "--ddBlocksize"))
+		{
+			if(!getNum(A.dd[0], ++a, argc, argv))
+			{
+				std::cerr << "Expected ddBlocksize for --ddBlocksize.\n";
+				return -1;
+			}
+			if(!getNum(A.dd[1], ++a, argc, argv))
+			{
+				--a;
+				A.dd[1] = A.dd[0];
+			}
+		}
+		else
+		{
+			--a;
+			break;
+		}
+	}
+	if(a >= argc)
+		return --a;
+	return a;
+}
+
+// FrontendArgs::parseArg (kpz/kpzFrontend.cpp:494-733)
+int parseArg(Args& A, int argc, char** argv, int a)
+{
+	if(!strcmp(argv[a], "--scheduler") || !strcmp(argv[a], "--gpuscheduler") || !strcmp(argv[a], "--cpuscheduler"))
+	{
+		int tmp;
+		if(!getNum(tmp, ++a, argc, argv))
+			--a;
+		else if(tmp != 0)
+		{
+			std::cerr << "[EE][FrontendArgs] only scheduler config 0 exists in this engine.\n";
+			return -1;
+		}
+		return parseScheduler(A, argc, argv, ++a);
+	}
+	else if(!strcmp(argv[a], "--disorder2") || !strcmp(argv[a], "--nschedulers") || !strcmp(argv[a], "--mcsFct") ||
+	        !strcmp(argv[a], "--saveH5") || !strcmp(argv[a], "--loadH5") || !strcmp(argv[a], "--retryH5") ||
+	        !strcmp(argv[a], "--prevMeasureOverride"))
+	{
+		std::cerr << "[EE][FrontendArgs] " << argv[a] << " is not supported by this engine.\n";
+		return -1;
+	}
+	else if(!memcmp(argv[a], "--pq", 4))
+	{
+		if(strlen(argv[a]) > 4 && argv[a][4] != '0')
+		{
+			std::cerr << "[EE][FrontendArgs] --pq" << argv[a][4] << ": rate sets beyond 0 need --disorder2.\n";
+			return -1;
+		}
+		if(!getNum(A.p, ++a, argc, argv))
+		{
+			std::cerr << "--pq: Expected a double value for p.\n";
+			return -1;
+		}
+		if(!getNum(A.q, ++a, argc, argv))
+		{
+			A.q = 0.;
+			--a;
+		}
+	}
+	else if(!strcmp(argv[a], "--anaThreads"))
+	{
+		if(!getNum(A.anaThreads, ++a, argc, argv))
+		{
+			std::cerr << "Expected number of threads for --anaThreads.\n";
+			return -1;
+		}
+		if(A.anaThreads < 1)
+		{
+			std::cerr << "Invalid number of threads given, irgnoring. (--anaThreads)\n";
+			A.anaThreads = 1;
+		}
+	}
+	else if(!strcmp(argv[a], "--noSave"))
+		A.save = false;
+	else if(!strcmp(argv[a], "--save"))
+		A.save = true;
+	else if(!strcmp(argv[a], "--countTagsAsMeasurement"))
+		A.countTags = true;
+	else if(!strcmp(argv[a], "--noCountTagsAsMeasurement"))
+		A.countTags = false;
+	else if(!strcmp(argv[a], "--maxH"))
+	{
+		float maxH;
+		if(!getNum(maxH, ++a, argc, argv) || maxH <= 0.)
+		{
+			std::cerr << "Expected a positive floating point value for --maxH.\n";
+			return -1;
+		}
+		A.maxH = maxH;
+	}
+	else if(!strcmp(argv[a], "--version"))
+		std::cout << VERSION << '\n';
+	else if(!strcmp(argv[a], "--sampleRate"))
+	{
+		Sampling s;
+		if(!getNum(s.density, ++a, argc, argv))
+		{
+			std::cerr << "Expected a floating point value for --sampleRate [rate] [startTime=0].\n";
+			return -1;
+		}
+		unsigned start;
+		if(!getNum(start, ++a, argc, argv))
+		{
+			--a;
+			start = 0;
+		}
+		pushSampling(A, start, s);
+	}
+	else if(!strcmp(argv[a], "--linSample"))
+	{
+		Sampling s;
+		s.kind = Sampling::LINEAR;
+		if(!getNum(s.interval, ++a, argc, argv) || s.interval == 0)
+		{
+			std::cerr << "Expected an unsigned int value as first arg for --linSample.\n";
+			return -1;
+		}
+		unsigned start;
+		if(!getNum(start, ++a, argc, argv))
+		{
+			--a;
+			start = 0;
+		}
+		pushSampling(A, start, s);
+	}
+	else if(!strcmp(argv[a], "--randomSample"))
+	{
+		Sampling s;
+		s.kind = Sampling::RANDOM;
+		double mean;
+		if(!getNum(mean, ++a, argc, argv))
+		{
+			std::cerr << "Expected a floating point value for --randomSample.\n";
+			return -1;
+		}
+		s.mean = mean / 2.; // uniform in [mean/2, mean + mean/2]
+		unsigned start;
+		if(!getNum(start, ++a, argc, argv))
+		{
+			--a;
+			start = 0;
+		}
+		pushSampling(A, start, s);
+	}
+	else if(!strcmp(argv[a], "--corrStart"))
+	{
+		unsigned t, s;
+		if(!getNum(t, ++a, argc, argv))
+		{
+			std::cerr << "[EE][FrontendArgs] Expected a unsigned int for --corrStart.\n";
+			return -1;
+		}
+		if(!getNum(s, ++a, argc, argv))
+		{
+			--a;
+			s = 0xFFFFFFFFu;
+		}
+		A.events.push_back(Event{Event::TAG, t, A.countTags, Sampling{}, s});
+	}
+	else if(!strcmp(argv[a], "--seed"))
+	{
+		if(!getNum(A.seed, ++a, argc, argv))
+		{
+			std::cerr << "Expected an unsigned 64-bit integer for --seed.\n";
+			return -1;
+		}
+		A.haveSeed = true;
+	}
+	else if(!strcmp(argv[a], "--device"))
+	{
+		if(!getNum(A.device, ++a, argc, argv))
+		{
+			std::cerr << "Expected a device ordinal for --device.\n";
+			return -1;
+		}
+	}
+	else
+	{
+		std::cerr << "[EE][FrontendArgs] Unknown argument: " << argv[a] << '\n';
+		return -1;
+	}
+	return a;
+}
+
+// ---------------------------------------------------------------- engine
+struct Engine {
+	kpz_scheduler* h = nullptr;
+	int lx = 0, ly = 0;
+	unsigned mcs = 0;
+	bool pq = false;
+	std::ostream* out = &std::cout;
+
+	~Engine()
+	{
+		if(h)
+			kpz_destroy(h);
+	}
+	static void check(int rc, const char* what)
+	{
+		if(rc < 0)
+		{
+			std::cerr << "[EE][kpz_scaling] " << what << ": " << kpz_last_error() << '\n';
+			std::exit(1);
+		}
+	}
+	long double N() const { return std::ldexp(1.0L, lx + ly); }
+
+	// roughness line fields: w2 and <h^k>, k = 1..7 (Roughness, operator<<)
+	void roughness(long double r[8])
+	{
+		int64_t s[3];
+		double m[7];
+		check(kpz_roughness(h, s), "roughness");
+		check(kpz_moments(h, m), "moments");
+		const __int128 s1 = s[0];
+		const __int128 s2 = (__int128)(((unsigned __int128)(uint64_t)s[2] << 64) | (uint64_t)s[1]);
+		const long double n = N();
+		r[0] = (long double)(((__int128)1 << (lx + ly)) * s2 - s1 * s1) / (n * n);
+		r[1] = (long double)s1 / n;
+		r[2] = (long double)s2 / n;
+		for(int k = 2; k < 7; ++k)
+			r[k + 1] = m[k];
+	}
+	void printRoughness(const long double r[8])
+	{
+		for(int k = 0; k < 8; ++k)
+			*out << '\t' << r[k];
+	}
+
+	// SchedulerService::correlate (kpz/schedulerService.cpp:216-251)
+	void correlate()
+	{
+		long double r[8];
+		roughness(r);
+		std::vector<int64_t> c(8 * 64);
+		int n = 0;
+		for(;;)
+		{
+			n = kpz_correlate_tags(h, c.data(), (int)(c.size() / 8));
+			check(n, "correlate");
+			if((size_t)n < c.size() / 8)
+				break;
+			c.resize(c.size() * 2);
+		}
+		std::ostringstream text;
+		text << std::setprecision(12);
+		std::ostream* keep = out;
+		out = &text;
+		for(int i = 0; i < n; ++i)
+		{
+			const int64_t* t = &c[(size_t)i * 8];
+			const __int128 sh = t[1], shs = t[2];
+			const __int128 shhs = (__int128)(((unsigned __int128)(uint64_t)t[6] << 64) | (uint64_t)t[5]);
+			const long double nn = N();
+			const long double ch = (long double)(((__int128)1 << (lx + ly)) * shhs - sh * shs) / (nn * nn);
+			const long double cs = (long double)t[7] / nn / 2.0L;
+			text << mcs;
+			printRoughness(r);
+			text << '\t' << (unsigned)t[0] << '\t' << ch << '\t' << cs << '\n';
+		}
+		if(n == 0)
+		{
+			text << mcs;
+			printRoughness(r);
+			text << '\n';
+		}
+		out = keep;
+		*out << text.str();
+		out->flush();
+	}
+
+	// SchedulerService::correlateTag (kpz/schedulerService.cpp:199-211, 279-284)
+	void correlateTag(unsigned stop)
+	{
+		check(kpz_corr_tag_stop(h, stop), "correlateTag");
+		long double r[8];
+		roughness(r);
+		std::ostringstream text;
+		text << std::setprecision(12) << mcs;
+		std::ostream* keep = out;
+		out = &text;
+		printRoughness(r);
+		out = keep;
+		text << "\tset\tcorr\ttag\n";
+		*out << text.str();
+		out->flush();
+	}
+
+	void run(unsigned n)
+	{
+		if(n)
+			check(pq ? kpz_mcs_pq(h, (int)n) : kpz_mcs(h, (int)n), "mcs");
+		mcs += n;
+	}
+
+	// KPZheightmap_<mcs>.xyz (SchedulerService::saveXYZ, kpz/schedulerService.cpp:403-450)
+	void saveXYZ(const std::string& name)
+	{
+		std::vector<uint32_t> lat(kpz_size_words(h));
+		check(kpz_download(h, lat.data()), "download");
+		const int enc = kpz_encoding(h);
+		const int X = 1 << lx, Y = 1 << ly;
+		auto bit = [&](int x, int y, int d) -> int {
+			if(enc == KPZ_ENC_LOCAL)
+			{
+				const size_t w = ((size_t)(y >> 2) << (lx - 2)) + (size_t)(x >> 2);
+				return (lat[w] >> ((((x & 3) << 1) | ((y & 3) << 3)) + d)) & 1;
+			}
+			const size_t q = (size_t)1 << (lx + ly - 6);
+			const int sub = (x ^ y) & 1;
+			const size_t w = ((size_t)y << (lx - 6)) + (size_t)(x >> 6);
+			return (lat[(size_t)(2 * sub + d) * q + w] >> ((x >> 1) & 31)) & 1;
+		};
+		std::ofstream o(name);
+		o << (4 + (uint64_t)X * Y) << "\n\n";
+		o << "H\t0\t0\t0\t0\n"
+		  << "H\t" << X << "\t0\t0\t0\n"
+		  << "H\t" << X << '\t' << Y << "\t0\t0\n"
+		  << "H\t0\t" << Y << "\t0\t0\n";
+		double hh = 0;
+		for(int y = 0; y < Y; ++y)
+		{
+			hh += bit(0, y, 1) ? 1 : -1;
+			for(int x = 1; x < X; ++x)
+			{
+				hh += bit(x, y, 0) ? 1 : -1;
+				o << "Fe\t" << x << '\t' << y << '\t' << hh << '\t' << hh << '\n';
+			}
+			hh += bit(0, y, 0) ? 1 : -1;
+			o << "Fe\t0\t" << y << '\t' << hh << '\t' << hh << '\n';
+		}
+	}
+};
+
+// SamplingParam::nextEval (kpz/sampling.h:71-88)
+unsigned nextEval(const Sampling& s, unsigned mcs, Engine& e)
+{
+	if(s.kind == Sampling::LINEAR)
+		return mcs + s.interval;
+	if(s.kind == Sampling::EXP)
+		return (unsigned)std::ceil(std::exp(std::log((double)mcs + 10) + s.density));
+	return mcs + (unsigned)(kpz_master_draw(e.h) * s.mean + s.mean);
+}
+
+// FrontendArgs::handleNextEvent (kpz/kpzFrontend.cpp:805-830); returns
+// whether this point counts as a measurement for the sampling schedule
+bool handleNextEvent(Args& A, Engine& e)
+{
+	if(A.events.empty())
+		return true;
+	bool count = A.events.front().start > e.mcs;
+	while(A.events.front().start <= e.mcs)
+	{
+		const Event ev = A.events.front();
+		A.events.pop_front();
+		if(ev.start == e.mcs)
+		{
+			switch(ev.kind)
+			{
+			case Event::END:
+				A.finished = true;
+				break;
+			case Event::SAMPLING:
+				A.current = ev.sampling;
+				break;
+			case Event::TAG:
+				e.correlateTag(ev.stop);
+				break;
+			case Event::BARRIER:
+				break;
+			}
+			count |= ev.countAsMeasurement;
+		}
+		else
+			std::cerr << "[WW][FrontendArgs][handleNextEvent] Event skipped: " << ev.start << '<' << e.mcs << '\n';
+		if(A.events.empty())
+			break;
+	}
+	return count;
+}
+
+} // namespace
+
+int main(int argc, char** argv)
+{
+	if(argc < 2)
+	{
+		help();
+		return 1;
+	}
+	Args A;
+	int a = parseSysArgs(A, argc, argv);
+	if(a < 0)
+		return 1;
+	for(; a < argc; ++a)
+	{
+		if(!strcmp(argv[a], "-f") || !strcmp(argv[a], "--file"))
+		{
+			if(!getStr(A.file, ++a, argc, argv))
+			{
+				std::cerr << "Expected a filename for -f / --file.\n";
+				return 1;
+			}
+		}
+		else if(!strcmp(argv[a], "--help"))
+		{
+			help();
+			std::cout << VERSION << '\n';
+		}
+		else if((a = parseArg(A, argc, argv, a)) < 0)
+			return 1;
+	}
+	// FrontendArgs::init (kpz/kpzFrontend.cpp:735-745)
+	std::cout << "[MM][FrontendArgs] running " << A.anaThreads << " threads for analysis (per process)\n";
+	std::cout << std::setprecision(12);
+	std::cout << "[MM][FrontendArgs] output precision= " << 12 << '\n';
+	A.events.push_back(Event{Event::END, A.mcs, false, Sampling{}, 0});
+	A.events.sort([](const Event& x, const Event& y) { return x.start < y.start; });
+
+	// SchedulerConfig::mkScheduler (kpz/kpzFrontend.cpp:234-395)
+	int kind = KPZ_DTR;
+	if(A.scheduler == "dtStatic")
+		kind = KPZ_DT;
+	else if(A.scheduler == "scaBit")
+		kind = KPZ_SCABIT;
+	else if(A.scheduler == "sca")
+		kind = KPZ_SCA;
+	kpz_params prm{A.mcsThreads, A.dd[0], A.dd[1], 0, A.device};
+	Engine e;
+	e.lx = A.lx;
+	e.ly = A.ly;
+	Engine::check(kpz_create(kind, A.lx, A.ly, &prm, &e.h), "create scheduler");
+	std::cout << "mcsScale " << kpz_mcs_scale(e.h) << '\n';
+
+	std::ofstream fileOut;
+	if(A.file.length() && A.file != "--")
+	{
+		fileOut.open(A.file);
+		if(!fileOut)
+		{
+			std::cerr << "Could not open file: '" << A.file << "' for writing.\n";
+			return 1;
+		}
+		e.out = &fileOut;
+	}
+
+	// seeding (SchedulerService::initURandom / supplySeed,
+	// kpz/schedulerService.cpp:460-495), then initHomogeneous + randomize
+	if(!A.haveSeed)
+	{
+		std::ifstream r("/dev/urandom", std::ios_base::binary);
+		uint64_t u = 0;
+		r.read((char*)&u, sizeof u);
+		A.seed = u ^ (uint64_t)std::time(nullptr);
+		std::cout << "[SchedulerService][initURandom] seed= " << A.seed << '\n';
+	}
+	else
+		std::cout << "[SchedulerService][supplySeed] seed= " << A.seed << '\n';
+	Engine::check(kpz_set_seed(e.h, A.seed), "seed");
+	Engine::check(kpz_init_flat(e.h), "initHomogeneous");
+	if(!std::isnan(A.p))
+	{
+		if(A.p == 1. && A.q == 0.)
+			std::cerr << "[WW] Running dynamic pq implementation with p=1 and "
+			             "q=0, using the fixed implementation would be wore efficient in this case\n";
+		std::cout << "Rates: p = " << A.p << " , q = " << A.q << '\n';
+		Engine::check(kpz_set_pq(e.h, A.p, A.q), "setPQ");
+		e.pq = true;
+	}
+
+	const auto t0 = std::chrono::steady_clock::now();
+	auto timedOut = [&] {
+		return A.maxH > 0 &&
+		       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > A.maxH * 3600.;
+	};
+
+	// main loop (kpz/kpzScaling.cpp:176-215)
+	e.correlate();
+	unsigned prev = e.mcs;
+	bool timeout = false;
+	while(!A.finished)
+	{
+		unsigned next = nextEval(A.current, prev, e);
+		if(!A.events.empty())
+			next = std::min(A.events.front().start, next);
+		e.run(next - e.mcs);
+		if(timedOut())
+		{
+			timeout = true;
+			break;
+		}
+		e.correlate();
+		if(handleNextEvent(A, e))
+			prev = e.mcs;
+	}
+	Engine::check(kpz_sync(e.h), "sync");
+	if(timeout)
+		std::cout << "TIMEOUT " << e.mcs << '\n';
+	if(A.save)
+	{
+		std::ostringstream ss;
+		ss << "KPZheightmap_" << e.mcs << ".xyz";
+		e.saveXYZ(ss.str());
+	}
+	return 0;
+}

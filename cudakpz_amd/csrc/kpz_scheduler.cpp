@@ -144,17 +144,23 @@ const Affine* SchedulerService::upload(std::vector<Affine>& host, DevBuf& buf)
 }
 
 // exact combination of per-row partial sums relative to each row's start
-// height; heights follow Correlator::set (kpz/correlator.cpp:37-106)
-static void combine_rows(const std::vector<int64_t>& rows, int ly, int lx, bool corr, Sums& s)
+// height; heights follow Correlator::set (kpz/correlator.cpp:37-106).
+// rowStart (optional) receives every row's start height.
+static void combine_rows(const std::vector<int64_t>& rows, int ly, int lx, bool corr, Sums& s,
+		std::vector<int64_t>* rowStart = nullptr)
 {
 	const int64_t L = 1ll << lx;
 	int64_t prev = 0, prevS = 0;
 	__int128 s1 = 0, s2 = 0, t1 = 0, s12 = 0;
 	int64_t tcs = 0;
+	if(rowStart)
+		rowStart->resize((size_t)1 << ly);
 	for(int y = 0; y < (1 << ly); ++y)
 	{
 		const int64_t* o = &rows[(size_t)y * ROW_STATS];
 		const int64_t hs = prev + (o[3] ? 1 : -1);
+		if(rowStart)
+			(*rowStart)[(size_t)y] = hs;
 		s1 += (__int128)L * hs + o[0];
 		s2 += (__int128)L * hs * hs + (__int128)2 * hs * o[0] + o[1];
 		prev = hs + o[2];
@@ -182,6 +188,35 @@ Sums SchedulerService::roughness()
 	Sums s;
 	combine_rows(hostRows_, size_.ly, size_.lx, false, s);
 	return s;
+}
+
+void SchedulerService::moments(long double out[7])
+{
+	const size_t ny = (size_t)1 << size_.ly;
+	check(launch_row_stats(deviceLattice(), nullptr, enc_, size_.lx, size_.ly, rows_.as<int64_t>(), stream_), "row_stats");
+	check(hipMemcpyAsync(hostRows_.data(), rows_.get(), hostRows_.size() * 8, hipMemcpyDeviceToHost, stream_), "row_stats D2H");
+	sync();
+	Sums s;
+	std::vector<int64_t> hs;
+	combine_rows(hostRows_, size_.ly, size_.lx, false, s, &hs);
+	if(!rowStart_.get())
+	{
+		rowStart_.alloc(ny * 8);
+		rowMoments_.alloc(ny * 7 * 8);
+	}
+	check(hipMemcpyAsync(rowStart_.get(), hs.data(), ny * 8, hipMemcpyHostToDevice, stream_), "row starts");
+	check(launch_row_moments(deviceLattice(), enc_, size_.lx, size_.ly, rowStart_.as<int64_t>(),
+			rowMoments_.as<double>(), stream_), "row_moments");
+	std::vector<double> m(ny * 7);
+	check(hipMemcpyAsync(m.data(), rowMoments_.get(), m.size() * 8, hipMemcpyDeviceToHost, stream_), "moments D2H");
+	sync();
+	long double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+	for(size_t y = 0; y < ny; ++y)
+		for(int k = 0; k < 7; ++k)
+			acc[k] += m[y * 7 + (size_t)k];
+	const long double N = (long double)size_.sites();
+	for(int k = 0; k < 7; ++k)
+		out[k] = acc[k] / N;
 }
 
 void SchedulerService::correlateTag(unsigned int stop)
@@ -427,6 +462,8 @@ void SchedulerSCABit::mcsPQSyncRng(int n) { run(n, (q_ == 0.0 && p_ == 0.5) ? 0 
 // ======================================================================
 SchedulerSCA::SchedulerSCA(int lx, int ly, const Params& p) : SchedulerService(lx, ly, ENC_LOCAL, p)
 {
+	if(lx < 3 || ly < 3)
+		throw Error(-1, "SCA: lattice sides must be at least 8 sites");
 	const int dimXW = 1 << (lx - 2);
 	const int nyb = 1 << (ly - 2);
 	if(T_ <= 0)
@@ -490,6 +527,14 @@ SchedulerSCA::SchedulerSCA(int lx, int ly, const Params& p) : SchedulerService(l
 	args_.fullBig = affine_pow(cell, (uint64_t)nyb * (uint64_t)(pc + 1));
 	args_.fullSmall = affine_pow(cell, (uint64_t)nyb * (uint64_t)pc);
 	args_.step = step_;
+	// band kernel shape: up to 64 cell columns per lane group, 64 cell rows per band
+	args_.lG = std::min(lx - 2, 6);
+	args_.lNB = std::min(ly - 2, 6);
+	const size_t groups = (size_t)dimXW >> args_.lG;
+	edgeR_.alloc(std::max<size_t>(1, (size_t)nyb * groups) * 4);
+	edgeU_.alloc(((size_t)nyb >> args_.lNB) * (size_t)dimXW * 4);
+	args_.edgeR = edgeR_.as<uint32_t>();
+	args_.edgeU = edgeU_.as<uint32_t>();
 	p_ = 0.95;
 	q_ = 0.0;
 }

@@ -100,6 +100,9 @@ public:
 
 	// ---- measurements (exact, on device)
 	Sums roughness();
+	// height moments <h^k>, k = 1..7, as Roughness::h[] after normAndSetW2
+	// (kpz/roughness.cpp:38-43): per-row double sums, rows added in long double
+	void moments(long double out[7]);
 	// correlateTag(stop) (kpz/schedulerService.cpp:266-271): a device snapshot
 	// kept while mcs <= stop; several tags may be live at once
 	void correlateTag(unsigned int stop = 0xFFFFFFFFu);
@@ -138,7 +141,7 @@ protected:
 		unsigned int mcs, stop;
 	};
 	Sums correlateWith(const DevBuf& snap);
-	DevBuf lat_, rows_;
+	DevBuf lat_, rows_, rowStart_, rowMoments_;
 	std::vector<Tag> tags_;
 	DevBuf st_[2];
 	int cur_ = 0;
@@ -198,7 +201,7 @@ public:
 
 private:
 	CbLocalArgs args_{};
-	DevBuf rowJ_, baseJ_, cellJ_;
+	DevBuf rowJ_, baseJ_, cellJ_, edgeR_, edgeU_;
 };
 
 std::unique_ptr<SchedulerService> makeScheduler(int kind, int lx, int ly, const Params& p);

@@ -87,6 +87,12 @@ int kpz_copy_system(kpz_scheduler* dst, const kpz_scheduler* src);
 /* exact height sums over all sites, heights as Correlator::set
  * (kpz/correlator.cpp:37-106): out = {sum h, sum h^2 low 64, sum h^2 high 64} */
 int kpz_roughness(kpz_scheduler* h, int64_t out[3]);
+/* height moments <h^k>, k = 1..7 (Roughness::h[] after normAndSetW2,
+ * kpz/roughness.cpp:38-43, kpz/schedulerService.h:52-71): per-row double
+ * sums on the device, rows added in long double; the 7 values the
+ * reference prints after w^2 (operator<< of Roughness,
+ * kpz/schedulerService.cpp:73-79) */
+int kpz_moments(kpz_scheduler* h, double out[7]);
 /* correlateTag: device snapshot of the current lattice, kept while the MCS
  * count is <= stop (kpz/schedulerService.cpp:266-271); kpz_corr_tag keeps it
  * for ever.  Several tags may be live. */

@@ -1,0 +1,48 @@
+"""Command-line handling of the kpz_scaling driver (the reference's
+kpzScaling front end, kpz/kpzScaling.cpp:47-100, kpz/kpzFrontend.cpp:93-232,
+400-733).  These cases fail before any device is touched."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "cudakpz_amd", "kpz_scaling")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(EXE), reason="kpz_scaling not built (make -C cudakpz_amd/csrc)")
+
+
+def run(*args):
+    p = subprocess.run([EXE, *map(str, args)], capture_output=True, text=True, timeout=60)
+    return p.returncode, p.stdout, p.stderr
+
+
+def test_usage_without_arguments():
+    rc, out, _ = run()
+    assert rc == 1 and out.startswith("kpz_scaling mcs [x [y]]")
+
+
+def test_missing_mcs():
+    rc, _, err = run("--scheduler", "dt")
+    assert rc == 1 and "Number of mcs was not supplied" in err
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["100", "8", "--scheduler", "db4"], "No such scheduler: db4"),
+    (["100", "8", "--bogus"], "Unknown argument: --bogus"),
+    (["100", "8", "--pq"], "Expected a double value for p"),
+    (["100", "8", "--pq1", "0.5"], "need --disorder2"),
+    (["100", "8", "--disorder2"], "not supported"),
+    (["100", "8", "--saveH5", "x"], "not supported"),
+    (["100", "8", "--corrStart"], "Expected a unsigned int for --corrStart"),
+    (["100", "8", "--linSample", "0"], "--linSample"),
+    (["100", "8", "--maxH", "-1"], "positive floating point value for --maxH"),
+    (["100", "8", "--scheduler", "dt", "--mcsThreads"], "Expected number of threads for --mcsThreads"),
+    (["100", "8", "--scheduler", "dt", "--ddBlocksize"], "Expected ddBlocksize"),
+    (["100", "8", "--scheduler", "sca", "--1d"], "--1d"),
+    (["100", "8", "-f"], "Expected a filename"),
+])
+def test_argument_errors(args, msg):
+    rc, _, err = run(*args)
+    assert rc == 1
+    assert msg in err

@@ -1,0 +1,133 @@
+"""kpz_scaling on the device: its measurement times follow the reference
+front end's sampling and event rules (kpz/kpzScaling.cpp:176-215,
+kpz/kpzFrontend.cpp:805-850, kpz/sampling.h:71-88), and every value of its
+output table equals what the library's own API measures for the same seed
+at the same times."""
+import math
+import os
+import subprocess
+
+import pytest
+
+import cudakpz_amd as K
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "cudakpz_amd", "kpz_scaling")
+
+
+def expected_lines(mcs, tags, lin=None):
+    """Measurement lines of the front end loop: (t, 'r'|'c'|'set') for
+    roughness, correlate and tag lines.  tags: [(start, stop)], exp
+    sampling (density .1) unless lin gives a linear interval."""
+    events = sorted([(t, "tag", stop) for t, stop in tags] + [(mcs, "end", 0)], key=lambda e: e[0])
+    live = []
+    out = []
+
+    def correlate(t):
+        nonlocal live
+        live = [x for x in live if t <= x[1]]
+        out.extend((t, "c", x[0]) for x in live) if live else out.append((t, "r", None))
+
+    t = prev = 0
+    correlate(0)
+    finished = False
+    while not finished:
+        nxt = prev + lin if lin else math.ceil(math.exp(math.log(prev + 10) + 0.1))
+        if events:
+            nxt = min(nxt, events[0][0])
+        t = nxt
+        correlate(t)
+        count = True
+        if events:
+            count = events[0][0] > t
+            while events and events[0][0] <= t:
+                e = events.pop(0)
+                if e[0] == t:
+                    if e[1] == "end":
+                        finished = True
+                    elif e[1] == "tag":
+                        live.append((t, e[2]))
+                        out.append((t, "set", None))
+        if count:
+            prev = t
+    return out
+
+
+def run_driver(tmp_path, args):
+    f = tmp_path / "table.txt"
+    p = subprocess.run([EXE, *map(str, args), "-f", str(f)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    rows = [line.split("\t") for line in f.read_text().splitlines()]
+    return rows, p.stdout
+
+
+def close(a, b, rel=1e-10):
+    return abs(a - b) <= rel * max(1.0, abs(b))
+
+
+@pytest.mark.parametrize("sched,cls,pq", [("dt", K.SchedulerDTr, None), ("dtStatic", K.SchedulerDT, None),
+                                          ("scaBit", K.SchedulerSCABit, (0.95, 0.05)),
+                                          ("sca", K.SchedulerSCA, None)])
+def test_driver_table_matches_api(tmp_path, sched, cls, pq):
+    if not os.path.exists(EXE):
+        pytest.fail("kpz_scaling not built")
+    lx, mcs, seed = 8, 300, 5
+    tags = [(20, 200), (40, 10 ** 9)]
+    args = [mcs, lx, "--scheduler", sched, "--seed", seed, "--corrStart", 20, 200, "--corrStart", 40]
+    if pq:
+        args += ["--pq", *pq]
+    rows, stdout = run_driver(tmp_path, args)
+    want = expected_lines(mcs, tags)
+    assert [(int(r[0])) for r in rows] == [t for t, _, _ in want]
+    assert "mcsScale" in stdout
+
+    s = cls(lx, lx)
+    s.supplySeed(seed)
+    s.initHomogeneous()
+    if pq:
+        s.setPQ(*pq)
+    done = 0
+    for row, (t, kind, tag) in zip(rows, want):
+        if t > done:
+            (s.mcsPQSyncRng if pq else s.mcs)(t - done)
+            done = t
+        vals = [float(v) for v in row[1:9]]
+        r = s.roughness()
+        m = s.moments()
+        assert close(vals[0], r.w2), (t, vals[0], r.w2)
+        assert close(vals[1], r.sum_h / r.N) and close(vals[2], r.sum_h2 / r.N)
+        for k in range(2, 7):
+            assert close(vals[k + 1], m[k]), (t, k, vals[k + 1], m[k])
+        if kind == "set":
+            assert row[9:] == ["set", "corr", "tag"]
+            s.correlateTag(dict(tags)[t])
+        elif kind == "c":
+            res = dict(s.correlateAll())
+            c = res[tag]
+            assert int(row[9]) == tag
+            assert close(float(row[10]), c.ch) and close(float(row[11]), c.cs)
+        else:
+            assert len(row) == 9
+
+
+def test_driver_linear_sampling_and_save(tmp_path):
+    if not os.path.exists(EXE):
+        pytest.fail("kpz_scaling not built")
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        rows, _ = run_driver(tmp_path, [50, 6, "--scheduler", "dtStatic", "--seed", 3, "--linSample", 7, "--save"])
+    finally:
+        os.chdir(cwd)
+    assert [int(r[0]) for r in rows] == [t for t, _, _ in expected_lines(50, [], lin=7)]
+    xyz = (tmp_path / "KPZheightmap_50.xyz").read_text().splitlines()
+    assert int(xyz[0]) == 4 + 64 * 64 and len(xyz) == 2 + 4 + 64 * 64
+    s = K.SchedulerDT(6, 6)
+    s.supplySeed(3)
+    s.initHomogeneous()
+    s.mcs(50)
+    hs = [float(line.split("\t")[3]) for line in xyz[6:]]
+    r = s.roughness()
+    assert sum(hs) == r.sum_h and sum(h * h for h in hs) == r.sum_h2

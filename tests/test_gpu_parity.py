@@ -147,6 +147,27 @@ def test_roughness_and_correlate_parity(cls, enc, lx, ly):
     assert O.consistent(cur, enc, lx, ly)
 
 
+@pytest.mark.parametrize("cls,enc,lx,ly", [(K.SchedulerDT, 0, 9, 8), (K.SchedulerSCABit, 1, 10, 7),
+                                           (K.SchedulerDT, 0, 5, 4), (K.SchedulerSCABit, 1, 6, 3),
+                                           (K.SchedulerSCA, 0, 8, 9)])
+def test_height_moments(cls, enc, lx, ly):
+    # <h^k>, k = 1..7, against exact integer power sums of the oracle's heights
+    s = cls(lx, ly)
+    s.supplySeed(13)
+    s.initHomogeneous()
+    s.setPQ(0.9, 0.1)
+    s.mcsPQSyncRng(25)
+    got = s.moments()
+    h = O.heights(s.popSystem(), enc, lx, ly).astype(np.int64).ravel()
+    N = h.size
+    for k in range(1, 8):
+        exact = sum(int(v) ** k * int(c) for v, c in zip(*np.unique(h, return_counts=True)))
+        want = exact / N
+        assert abs(got[k - 1] - want) <= 1e-12 * max(1.0, abs(want)), (k, got[k - 1], want)
+    r = s.roughness()
+    assert got[0] * N == r.sum_h and got[1] * N == r.sum_h2
+
+
 def test_upload_download_roundtrip():
     s = K.SchedulerSCABit(8, 6)
     lat = O.flat(8, 6, O.ENC_CBBIT)

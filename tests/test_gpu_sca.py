@@ -18,7 +18,8 @@ def oracle_cb(lx, ly, T, seed, n, p, q):
 
 
 @pytest.mark.parametrize("lx,ly,T", [(5, 4, 1), (5, 4, 3), (5, 4, 8), (8, 8, 0), (10, 7, 5), (9, 9, 7),
-                                     (12, 6, 1), (11, 8, 100)])
+                                     (12, 6, 1), (11, 8, 100), (3, 3, 2), (3, 6, 1), (10, 10, 0),
+                                     (12, 10, 37), (7, 11, 3)])
 @pytest.mark.parametrize("p,q", [(0.95, 0.0), (0.5, 0.5), (0.8, 0.3)])
 def test_sca_pq_parity(lx, ly, T, p, q):
     s = K.SchedulerSCA(lx, ly, threads=T)
