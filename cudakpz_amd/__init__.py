@@ -162,6 +162,20 @@ class SchedulerService:
         self._h = h
         self.kind = kind
 
+    @classmethod
+    def _wrap(cls, h, kind, lx, ly):
+        obj = cls.__new__(cls)
+        obj.lx, obj.ly, obj._h, obj.kind = int(lx), int(ly), h, kind
+        return obj
+
+    # -- checkpoint / resume
+    def saveState(self, path, aux=0):
+        """Everything needed to continue bit for bit (kind, sizes, MCS count,
+        p, q, master dSFMT, stream states, lattice, live tags) in one file."""
+        L = lib()
+        L.kpz_save_state.argtypes = [C.c_void_p, C.c_char_p, C.c_uint32]
+        _check(L.kpz_save_state(self._h, os.fsencode(path), int(aux)))
+
     # -- lifetime
     def close(self):
         if getattr(self, "_h", None):
@@ -594,3 +608,20 @@ def make_scheduler(name, lx, ly, **kw):
     if name not in table:
         raise KpzError(-1, f"No such scheduler: {name}")
     return table[name](lx, ly, **kw)
+
+
+def load_state(path, device=-1):
+    """Resume a scheduler written by saveState: returns (scheduler, aux).
+    The file is validated (header, sizes, checksum) before any device work."""
+    L = lib()
+    L.kpz_load_state.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)]
+    L.kpz_kind.argtypes = [C.c_void_p]
+    h = C.c_void_p()
+    aux = C.c_uint32()
+    _check(L.kpz_load_state(os.fsencode(path), int(device), C.byref(h), C.byref(aux)))
+    L.kpz_size.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    kind = L.kpz_kind(h)
+    lx, ly = C.c_int(), C.c_int()
+    _check(L.kpz_size(h, C.byref(lx), C.byref(ly)))
+    cls = {KPZ_DT: SchedulerDT, KPZ_DTR: SchedulerDTr, KPZ_SCABIT: SchedulerSCABit, KPZ_SCA: SchedulerSCA}[kind]
+    return cls._wrap(h, kind, lx.value, ly.value), int(aux.value)

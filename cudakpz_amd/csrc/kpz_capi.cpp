@@ -99,6 +99,50 @@ int kpz_destroy(kpz_scheduler* h)
 	return guard([&] { delete h; });
 }
 
+int kpz_save_state(kpz_scheduler* h, const char* path, uint32_t aux)
+{
+	HANDLE(h);
+	if(!path)
+		return fail(KPZ_EINVAL, "null path");
+	return guard([&] { h->s->saveState(path, aux); });
+}
+
+int kpz_load_state(const char* path, int device, kpz_scheduler** out, uint32_t* aux)
+{
+	if(!out || !path)
+		return fail(KPZ_EINVAL, "null argument");
+	*out = nullptr;
+	return guard([&] {
+		auto h = new kpz_scheduler;
+		try
+		{
+			h->s = kpz::SchedulerService::loadState(path, device, aux);
+		}
+		catch(...)
+		{
+			delete h;
+			throw;
+		}
+		*out = h;
+	});
+}
+
+int kpz_kind(const kpz_scheduler* h)
+{
+	HANDLE(h);
+	return h->s->kind();
+}
+
+int kpz_size(const kpz_scheduler* h, int* lx, int* ly)
+{
+	HANDLE(h);
+	if(!lx || !ly)
+		return fail(KPZ_EINVAL, "null output");
+	*lx = h->s->size().lx;
+	*ly = h->s->size().ly;
+	return KPZ_OK;
+}
+
 int kpz_set_seed(kpz_scheduler* h, uint64_t seed)
 {
 	HANDLE(h);

@@ -84,7 +84,7 @@ struct Args {
 	int anaThreads = 1;
 	bool countTags = false, save = false;
 	double maxH = 0;
-	std::string file;
+	std::string file, saveState, loadState;
 	bool haveSeed = false;
 	uint64_t seed = 0;
 	int device = -1;
@@ -108,6 +108,8 @@ void help()
 	             "  --maxH hours              wall-clock limit (prints TIMEOUT mcs)\n"
 	             "  --save | --noSave         write KPZheightmap_<mcs>.xyz at the end\n"
 	             "  -f, --file name           output table to a file (-- = stdout)\n"
+	             "  --saveState file          checkpoint at the end (lattice, RNG states, tags)\n"
+	             "  --loadState file          resume from a checkpoint (continues bit for bit)\n"
 	             "  --seed s                  64-bit seed (default: /dev/urandom and time)\n"
 	             "  --device n                HIP device\n";
 }
@@ -239,11 +241,25 @@ int parseArg(Args& A, int argc, char** argv, int a)
 		return parseScheduler(A, argc, argv, ++a);
 	}
 	else if(!strcmp(argv[a], "--disorder2") || !strcmp(argv[a], "--nschedulers") || !strcmp(argv[a], "--mcsFct") ||
-	        !strcmp(argv[a], "--saveH5") || !strcmp(argv[a], "--loadH5") || !strcmp(argv[a], "--retryH5") ||
-	        !strcmp(argv[a], "--prevMeasureOverride"))
+	        !strcmp(argv[a], "--retryH5") || !strcmp(argv[a], "--prevMeasureOverride"))
 	{
 		std::cerr << "[EE][FrontendArgs] " << argv[a] << " is not supported by this engine.\n";
 		return -1;
+	}
+	else if(!strcmp(argv[a], "--saveH5") || !strcmp(argv[a], "--loadH5"))
+	{
+		std::cerr << "[EE][FrontendArgs] " << argv[a] << ": libSplash/HDF5 checkpoints are not supported by "
+		          << "this engine; use --saveState / --loadState.\n";
+		return -1;
+	}
+	else if(!strcmp(argv[a], "--saveState") || !strcmp(argv[a], "--loadState"))
+	{
+		std::string& dst = (argv[a][2] == 's') ? A.saveState : A.loadState;
+		if(!getStr(dst, ++a, argc, argv))
+		{
+			std::cerr << "Expected a file name for " << argv[a - 1] << ".\n";
+			return -1;
+		}
 	}
 	else if(!memcmp(argv[a], "--pq", 4))
 	{
@@ -633,9 +649,22 @@ int main(int argc, char** argv)
 		kind = KPZ_SCA;
 	kpz_params prm{A.mcsThreads, A.dd[0], A.dd[1], 0, A.device};
 	Engine e;
-	e.lx = A.lx;
-	e.ly = A.ly;
-	Engine::check(kpz_create(kind, A.lx, A.ly, &prm, &e.h), "create scheduler");
+	uint32_t lastMeasurement = 0;
+	if(A.loadState.empty())
+	{
+		e.lx = A.lx;
+		e.ly = A.ly;
+		Engine::check(kpz_create(kind, A.lx, A.ly, &prm, &e.h), "create scheduler");
+	}
+	else
+	{
+		// --loadH5 path (kpz/kpzScaling.cpp:118-139): size, kind, RNG and tags
+		// come from the file
+		Engine::check(kpz_load_state(A.loadState.c_str(), A.device, &e.h, &lastMeasurement), "load state");
+		Engine::check(kpz_size(e.h, &e.lx, &e.ly), "size");
+		e.mcs = kpz_mcs_count(e.h);
+		std::cout << "[readState] Loading state at t = " << e.mcs << " MCS\n";
+	}
 	std::cout << "mcsScale " << kpz_mcs_scale(e.h) << '\n';
 
 	std::ofstream fileOut;
@@ -652,18 +681,31 @@ int main(int argc, char** argv)
 
 	// seeding (SchedulerService::initURandom / supplySeed,
 	// kpz/schedulerService.cpp:460-495), then initHomogeneous + randomize
-	if(!A.haveSeed)
+	if(A.loadState.empty())
 	{
-		std::ifstream r("/dev/urandom", std::ios_base::binary);
-		uint64_t u = 0;
-		r.read((char*)&u, sizeof u);
-		A.seed = u ^ (uint64_t)std::time(nullptr);
-		std::cout << "[SchedulerService][initURandom] seed= " << A.seed << '\n';
+		if(!A.haveSeed)
+		{
+			std::ifstream r("/dev/urandom", std::ios_base::binary);
+			uint64_t u = 0;
+			r.read((char*)&u, sizeof u);
+			A.seed = u ^ (uint64_t)std::time(nullptr);
+			std::cout << "[SchedulerService][initURandom] seed= " << A.seed << '\n';
+		}
+		else
+			std::cout << "[SchedulerService][supplySeed] seed= " << A.seed << '\n';
+		Engine::check(kpz_set_seed(e.h, A.seed), "seed");
+		Engine::check(kpz_init_flat(e.h), "initHomogeneous");
 	}
 	else
-		std::cout << "[SchedulerService][supplySeed] seed= " << A.seed << '\n';
-	Engine::check(kpz_set_seed(e.h, A.seed), "seed");
-	Engine::check(kpz_init_flat(e.h), "initHomogeneous");
+	{
+		// FrontendArgs::fastForwardEvents (kpz/kpzFrontend.cpp:790-800)
+		while(!A.events.empty() && A.events.front().start < e.mcs)
+		{
+			if(A.events.front().kind == Event::SAMPLING)
+				A.current = A.events.front().sampling;
+			A.events.pop_front();
+		}
+	}
 	if(!std::isnan(A.p))
 	{
 		if(A.p == 1. && A.q == 0.)
@@ -683,6 +725,12 @@ int main(int argc, char** argv)
 	// main loop (kpz/kpzScaling.cpp:176-215)
 	e.correlate();
 	unsigned prev = e.mcs;
+	// a resumed run continues the sampling schedule from the last measurement
+	// stored with the state (overridePrevMeasurementTime, kpz/kpzFrontend.cpp:752-788)
+	// (random sampling draws from the master and is not replayed)
+	if(!A.loadState.empty() && A.current.kind != Sampling::RANDOM && lastMeasurement <= e.mcs &&
+	   nextEval(A.current, lastMeasurement, e) >= e.mcs)
+		prev = lastMeasurement;
 	bool timeout = false;
 	while(!A.finished)
 	{
@@ -702,6 +750,8 @@ int main(int argc, char** argv)
 	Engine::check(kpz_sync(e.h), "sync");
 	if(timeout)
 		std::cout << "TIMEOUT " << e.mcs << '\n';
+	if(!A.saveState.empty())
+		Engine::check(kpz_save_state(e.h, A.saveState.c_str(), prev), "save state");
 	if(A.save)
 	{
 		std::ostringstream ss;

@@ -38,7 +38,7 @@ void DevBuf::alloc(size_t bytes)
 
 // ======================================================================
 SchedulerService::SchedulerService(int lx, int ly, Encoding enc, const Params& p)
-	: size_{lx, ly}, enc_(enc), T_(p.threads), S_(p.skip), device_(p.device)
+	: params_(p), size_{lx, ly}, enc_(enc), T_(p.threads), S_(p.skip), device_(p.device)
 {
 	if(lx < 2 || ly < 2 || lx > 30 || ly > 30)
 		throw Error(-1, "system size out of range (log2 sizes 2..30)");
@@ -567,14 +567,17 @@ void SchedulerSCA::mcsPQSyncRng(int n)
 
 std::unique_ptr<SchedulerService> makeScheduler(int kind, int lx, int ly, const Params& p)
 {
+	std::unique_ptr<SchedulerService> s;
 	switch(kind)
 	{
-		case KIND_DT: return std::make_unique<SchedulerDT>(lx, ly, p, false);
-		case KIND_DTR: return std::make_unique<SchedulerDT>(lx, ly, p, true);
-		case KIND_SCABIT: return std::make_unique<SchedulerSCABit>(lx, ly, p);
-		case KIND_SCA: return std::make_unique<SchedulerSCA>(lx, ly, p);
+		case KIND_DT: s = std::make_unique<SchedulerDT>(lx, ly, p, false); break;
+		case KIND_DTR: s = std::make_unique<SchedulerDT>(lx, ly, p, true); break;
+		case KIND_SCABIT: s = std::make_unique<SchedulerSCABit>(lx, ly, p); break;
+		case KIND_SCA: s = std::make_unique<SchedulerSCA>(lx, ly, p); break;
 		default: throw Error(-1, "unknown scheduler kind");
 	}
+	s->kind_ = kind;
+	return s;
 }
 
 } // namespace kpz

@@ -121,6 +121,12 @@ public:
 	void setStreams(const uint64_t* in);
 	MasterRng& master() { return master_; }
 
+	// ---- checkpoint / resume (kpz_state.cpp): everything needed to continue
+	// bit for bit; aux is 32 bits of caller data stored with it
+	int kind() const { return kind_; }
+	void saveState(const char* path, uint32_t aux);
+	static std::unique_ptr<SchedulerService> loadState(const char* path, int device, uint32_t* aux);
+
 	uint32_t* deviceLattice() const { return lat_.as<uint32_t>(); }
 
 protected:
@@ -129,6 +135,9 @@ protected:
 	void swapStreams() { cur_ ^= 1; }
 	const Affine* upload(std::vector<Affine>& host, DevBuf& buf);
 
+	friend std::unique_ptr<SchedulerService> makeScheduler(int kind, int lx, int ly, const Params& p);
+	int kind_ = -1;
+	Params params_;
 	Size size_;
 	Encoding enc_;
 	int64_t T_;

@@ -59,6 +59,22 @@ const char* kpz_last_error(void);
  * (kpz/kpzFrontend.cpp:234-250): lx, ly are log2 lattice sizes */
 int kpz_create(int kind, int lx, int ly, const kpz_params* params, kpz_scheduler** out);
 int kpz_destroy(kpz_scheduler* h);
+/* the KPZ_* kind the scheduler was created as (>= 0), or a negative error */
+int kpz_kind(const kpz_scheduler* h);
+/* log2 lattice sizes (SchedulerService::size(), kpz/systemSize.h) */
+int kpz_size(const kpz_scheduler* h, int* lx, int* ly);
+
+/* Checkpoint / resume, the role of SchedulerService::collectH5 / retrieveH5
+ * (kpz/schedulerService.cpp:583-676) and of --saveH5 / --loadH5
+ * (kpz/kpzScaling.cpp:118-139, 217-222).  The file holds kind, sizes,
+ * parameters, MCS count, (p, q), the master dSFMT, every SLCG stream state,
+ * the lattice and all live correlator tags, ending with an FNV-1a 64
+ * checksum; a loaded scheduler continues bit for bit.  aux: 32 bits of
+ * caller data kept in the file (kpz_scaling: the last measurement time, the
+ * reference's "prevMeasureOverride" attribute).  kpz_load_state validates
+ * the whole file before it touches a device. */
+int kpz_save_state(kpz_scheduler* h, const char* path, uint32_t aux);
+int kpz_load_state(const char* path, int device, kpz_scheduler** out, uint32_t* aux);
 
 /* SchedulerService::supplySeed + randomize (kpz/schedulerService.cpp:460-482,
  * kpz/GPURandom.cu:42-54): master dSFMT reSeed(seed ^ seed>>32), then the T
