@@ -374,10 +374,53 @@ static uint32_t thresh23(double p)
 	return (uint32_t)ceil(p * 8388608.0);
 }
 
-/* SchedulerCPU::pqOneUpdate, kpz/schedulerCPU.inl:21-33 (upd = KPZ) and
+/* thresholds per disorder class and the disorder map */
+typedef struct {
+	uint32_t tp[4], tq[4];
+	const uint32_t* dis;
+} thr_t;
+
+static thr_t thr_pq(double p, double q)
+{
+	thr_t t;
+	memset(&t, 0, sizeof t);
+	t.tp[0] = thresh23(p);
+	t.tq[0] = thresh23(q);
+	return t;
+}
+
+static thr_t thr_rates(const kor_rates* r)
+{
+	thr_t t;
+	for(int k = 0; k < 4; ++k)
+	{
+		t.tp[k] = thresh23(r->p[k]);
+		t.tq[k] = thresh23(r->q[k]);
+	}
+	t.dis = r->disorder;
+	return t;
+}
+
+/* SchedulerService::generateDisorder2, kpz/schedulerService.cpp:54-69 */
+void kor_generate_disorder2(kor_dsfmt* master, uint32_t* dis, uint64_t size_w, double fill)
+{
+	for(uint64_t a = 0; a < size_w; ++a)
+	{
+		uint32_t t = 0;
+		for(int s = 0; s < 32; s += 2)
+			if(kor_dsfmt_close_open(master) < fill)
+				t |= 1u << s;
+		dis[a] = t;
+	}
+}
+
+/* SchedulerCPU::pqOneUpdate, kpz/schedulerCPU.inl:21-33 (upd = KPZ),
  * Func_MCSBase::pqSyncRngUpdate, kpz/schedulerCPUBaseThreadedRng.inl:25-48
- * (upd = PQ; r23 is the 23-bit randomFloat mantissa) */
-static inline void upd_local(uint32_t* lat, int lx, int ly, int x, int y, int upd, uint32_t r23, uint32_t tp, uint32_t tq)
+ * (upd = PQ; r23 is the 23-bit randomFloat mantissa),
+ * disorder2SyncRngUpdate :50-75 (upd = DIS: rates of the class c(m_disorder),
+ * the 2 bits of the site in the disorder map) and SyncRngEW,
+ * kpz/kpzSimulationPrimitivesTemplates.cu:199-234 (upd = EW) */
+static inline void upd_local(uint32_t* lat, int lx, int ly, int x, int y, int upd, uint32_t r23, const thr_t* th)
 {
 	const uint32_t ic = loc_index(lx, x, y);
 	const int sc = loc_shift(x, y);
@@ -398,14 +441,15 @@ static inline void upd_local(uint32_t* lat, int lx, int ly, int x, int y, int up
 	else
 	{
 		const uint32_t config = c | (r << 2) | (l << 3);
+		const int k = (upd == KOR_UPD_DIS && th->dis) ? (int)((th->dis[ic] >> sc) & 3) : 0;
 		if(config == 0xC)
 		{
-			if(!(r23 < tp))
+			if(upd != KOR_UPD_EW && !(r23 < th->tp[k]))
 				return;
 		}
 		else if(config == 0x3)
 		{
-			if(!(r23 < tq))
+			if(upd != KOR_UPD_EW && !(r23 < th->tq[k]))
 				return;
 		}
 		else
@@ -418,12 +462,21 @@ static inline void upd_local(uint32_t* lat, int lx, int ly, int x, int y, int up
 
 void kor_update_local(uint32_t* lat, int lx, int ly, int x, int y, int upd, float rnd, double p, double q)
 {
-	upd_local(lat, lx, ly, x, y, upd, (uint32_t)(rnd * 8388608.0f), thresh23(p), thresh23(q));
+	const thr_t th = thr_pq(p, q);
+	upd_local(lat, lx, ly, x, y, upd, (uint32_t)(rnd * 8388608.0f), &th);
+}
+
+void kor_update_local_rates(uint32_t* lat, int lx, int ly, int x, int y, int upd, float rnd, const kor_rates* r)
+{
+	const thr_t th = thr_rates(r);
+	upd_local(lat, lx, ly, x, y, upd, (uint32_t)(rnd * 8388608.0f), &th);
 }
 
 /* Func_MCSLocalBase::pqSyncRngUpdate on ENC_CBBIT,
- * kpz/schedulerCPUCBBit_nested.inl:262-295 */
-static inline void upd_cbbit(uint32_t* lat, int lx, int ly, int x, int y, uint32_t r23, uint32_t tp, uint32_t tq)
+ * kpz/schedulerCPUCBBit_nested.inl:262-295, and disorder2SyncRngUpdate
+ * :298-331 (dis: class = cx(m_disorder), the x-slope bit of the site in the
+ * disorder map read with the ENC_CBBIT layout) */
+static inline void upd_cbbit(uint32_t* lat, int lx, int ly, int x, int y, uint32_t r23, const thr_t* th, int dis)
 {
 	const int xr = (x + 1) & ((1 << lx) - 1);
 	const int yu = (y + 1) & ((1 << ly) - 1);
@@ -431,14 +484,15 @@ static inline void upd_cbbit(uint32_t* lat, int lx, int ly, int x, int y, uint32
 		| ((uint32_t)kor_get_slope(lat, 1, lx, ly, x, y, 1) << 1)
 		| ((uint32_t)kor_get_slope(lat, 1, lx, ly, xr, y, 0) << 2)
 		| ((uint32_t)kor_get_slope(lat, 1, lx, ly, x, yu, 1) << 3);
+	const int k = (dis && th->dis) ? kor_get_slope(th->dis, 1, lx, ly, x, y, 0) : 0;
 	if(config == 0xC)
 	{
-		if(!(r23 < tp))
+		if(!(r23 < th->tp[k]))
 			return;
 	}
 	else if(config == 0x3)
 	{
-		if(!(r23 < tq))
+		if(!(r23 < th->tq[k]))
 			return;
 	}
 	else
@@ -451,7 +505,14 @@ static inline void upd_cbbit(uint32_t* lat, int lx, int ly, int x, int y, uint32
 
 void kor_update_cbbit(uint32_t* lat, int lx, int ly, int x, int y, float rnd, double p, double q)
 {
-	upd_cbbit(lat, lx, ly, x, y, (uint32_t)(rnd * 8388608.0f), thresh23(p), thresh23(q));
+	const thr_t th = thr_pq(p, q);
+	upd_cbbit(lat, lx, ly, x, y, (uint32_t)(rnd * 8388608.0f), &th, 0);
+}
+
+void kor_update_cbbit_rates(uint32_t* lat, int lx, int ly, int x, int y, float rnd, const kor_rates* r)
+{
+	const thr_t th = thr_rates(r);
+	upd_cbbit(lat, lx, ly, x, y, (uint32_t)(rnd * 8388608.0f), &th, 1);
 }
 
 /* ======================================================================
@@ -481,16 +542,15 @@ void kor_dt_block_offset(int lx, int ly, int lbx, int lby, int domain_set,
  * (int)(close_open()*2^b) = draw >> (32-b); randomFloat()*2^b = r23 >> (23-b).
  * Blocks of one domain set are disjoint including their +1 halo, so the
  * sequential walk t = 0..T-1 equals any thread interleaving. */
-int kor_dt_mcs(uint32_t* lat, int lx, int ly, int lbx, int lby,
+static int dt_mcs(uint32_t* lat, int lx, int ly, int lbx, int lby,
 		uint64_t* states, int64_t T, uint64_t a, uint64_t c,
-		kor_dsfmt* master, int n, int random_origin, int upd, double p, double q)
+		kor_dsfmt* master, int n, int random_origin, int upd, const thr_t* th)
 {
 	if(lbx < 2 || lby < 2 || lx - lbx < 1 || ly - lby < 1 || T < 1 || lbx > 23 || lby > 23)
 		return -1;
 	const int64_t nb = 1ll << (lx - lbx + ly - lby - 2);
 	const int64_t bs = 1ll << (lbx + lby);
 	const int mx = (1 << lx) - 1, my = (1 << ly) - 1;
-	const uint32_t tp = thresh23(p), tq = thresh23(q);
 	for(int m = 0; m < n; ++m)
 	{
 		int seq[4];
@@ -524,9 +584,9 @@ int kor_dt_mcs(uint32_t* lat, int lx, int ly, int lbx, int lby,
 							y = (int)(kor_slcg_draw(&s, a, c) >> (32 - lby)) + off[1];
 						}
 						uint32_t r23 = 0;
-						if(upd == KOR_UPD_PQ)
+						if(upd == KOR_UPD_PQ || upd == KOR_UPD_DIS)
 							r23 = kor_slcg_draw(&s, a, c) & 0x7FFFFF;
-						upd_local(lat, lx, ly, x, y, upd, r23, tp, tq);
+						upd_local(lat, lx, ly, x, y, upd, r23, th);
 					}
 				}
 				states[t] = s;
@@ -534,6 +594,24 @@ int kor_dt_mcs(uint32_t* lat, int lx, int ly, int lbx, int lby,
 		}
 	}
 	return 0;
+}
+
+int kor_dt_mcs(uint32_t* lat, int lx, int ly, int lbx, int lby,
+		uint64_t* states, int64_t T, uint64_t a, uint64_t c,
+		kor_dsfmt* master, int n, int random_origin, int upd, double p, double q)
+{
+	const thr_t th = thr_pq(p, q);
+	return dt_mcs(lat, lx, ly, lbx, lby, states, T, a, c, master, n, random_origin, upd, &th);
+}
+
+/* SchedulerCPUDTr / SchedulerCPUDT with per-class rates
+ * (mcsDisorder2SyncRng, kpz/schedulerCPUDTr.cpp:115-119) */
+int kor_dt_mcs_rates(uint32_t* lat, int lx, int ly, int lbx, int lby,
+		uint64_t* states, int64_t T, uint64_t a, uint64_t c,
+		kor_dsfmt* master, int n, int random_origin, int upd, const kor_rates* r)
+{
+	const thr_t th = thr_rates(r);
+	return dt_mcs(lat, lx, ly, lbx, lby, states, T, a, c, master, n, random_origin, upd, &th);
 }
 
 /* Threaded form of the above for timing: worker w runs streams w, w+nthreads,
@@ -549,7 +627,7 @@ typedef struct {
 	uint64_t* states;
 	int64_t T, w, nw, nb, bs;
 	uint64_t a, c;
-	uint32_t tp, tq;
+	thr_t th;
 } dt_work;
 
 static void* dt_worker(void* arg)
@@ -577,9 +655,9 @@ static void* dt_worker(void* arg)
 					y = (int)(kor_slcg_draw(&s, W->a, W->c) >> (32 - W->lby)) + off[1];
 				}
 				uint32_t r23 = 0;
-				if(W->upd == KOR_UPD_PQ)
+				if(W->upd == KOR_UPD_PQ || W->upd == KOR_UPD_DIS)
 					r23 = kor_slcg_draw(&s, W->a, W->c) & 0x7FFFFF;
-				upd_local(W->lat, W->lx, W->ly, x, y, W->upd, r23, W->tp, W->tq);
+				upd_local(W->lat, W->lx, W->ly, x, y, W->upd, r23, &W->th);
 			}
 		}
 		W->states[t] = s;
@@ -628,8 +706,7 @@ int kor_dt_mcs_threaded(uint32_t* lat, int lx, int ly, int lbx, int lby,
 				W->bs = 1ll << (lbx + lby);
 				W->a = a;
 				W->c = c;
-				W->tp = thresh23(p);
-				W->tq = thresh23(q);
+				W->th = thr_pq(p, q);
 				pthread_create(&th[w], NULL, dt_worker, W);
 			}
 			for(int w = 0; w < nthreads; ++w)
@@ -731,13 +808,12 @@ static void cbbit_rows(int L, int64_t T, int64_t t, int* yMin, int* ySup)
 	*ySup = (int)(lo + per + (t < rem ? 1 : 0));
 }
 
-int kor_cbbit_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
-		uint64_t a, uint64_t c, int n, int mode, double p, double q)
+static int cbbit_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
+		uint64_t a, uint64_t c, int n, int mode, const thr_t* th)
 {
 	if(lx < 6 || ly < 1 || T < 1)
 		return -1;
 	const int X = 1 << lx, Y = 1 << ly;
-	const uint32_t tp = thresh23(p), tq = thresh23(q);
 	const int wbits = (lx >= 7) ? 64 : 32;
 	for(int m = 0; m < n; ++m)
 		for(int s = 0; s < 2; ++s)
@@ -752,11 +828,27 @@ int kor_cbbit_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
 						cbbit_row_words(lat, lx, ly, s, y, wbits, &st, a, c, NULL);
 					else
 						for(int x = ((y & 1) ^ s); x < X; x += 2)
-							upd_cbbit(lat, lx, ly, x, y, kor_slcg_draw(&st, a, c) & 0x7FFFFF, tp, tq);
+							upd_cbbit(lat, lx, ly, x, y, kor_slcg_draw(&st, a, c) & 0x7FFFFF, th, mode == 2);
 				}
 				states[t] = st;
 			}
 	return 0;
+}
+
+int kor_cbbit_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
+		uint64_t a, uint64_t c, int n, int mode, double p, double q)
+{
+	const thr_t th = thr_pq(p, q);
+	return cbbit_mcs(lat, lx, ly, states, T, a, c, n, mode, &th);
+}
+
+int kor_cbbit_mcs_rates(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
+		uint64_t a, uint64_t c, int n, int mode, const kor_rates* r)
+{
+	if(mode != 1 && mode != 2)
+		return -1;
+	const thr_t th = thr_rates(r);
+	return cbbit_mcs(lat, lx, ly, states, T, a, c, n, mode, &th);
 }
 
 /* ======================================================================
@@ -765,27 +857,26 @@ int kor_cbbit_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
  * :153-168 (strip split at xCenter, first half entirely before second).
  * ====================================================================== */
 static void cb_loop(uint32_t* lat, int lx, int ly, int s, int xMin, int xSup,
-		uint64_t* st, uint64_t a, uint64_t c, uint32_t tp, uint32_t tq)
+		uint64_t* st, uint64_t a, uint64_t c, int upd, const thr_t* th)
 {
 	for(int yW = 0; yW < (1 << ly); yW += 4)
 		for(int xW = xMin; xW < xSup; xW += 4)
 			for(int y = yW; y < yW + 4; ++y)
 			{
 				const int x = (xW + (y & 1)) ^ s;
-				upd_local(lat, lx, ly, x, y, KOR_UPD_PQ, kor_slcg_draw(st, a, c) & 0x7FFFFF, tp, tq);
-				upd_local(lat, lx, ly, x + 2, y, KOR_UPD_PQ, kor_slcg_draw(st, a, c) & 0x7FFFFF, tp, tq);
+				upd_local(lat, lx, ly, x, y, upd, kor_slcg_draw(st, a, c) & 0x7FFFFF, th);
+				upd_local(lat, lx, ly, x + 2, y, upd, kor_slcg_draw(st, a, c) & 0x7FFFFF, th);
 			}
 }
 
-int kor_cb_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
-		uint64_t a, uint64_t c, int n, double p, double q)
+static int cb_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
+		uint64_t a, uint64_t c, int n, int upd, const thr_t* th)
 {
 	if(lx < 2 || ly < 2 || T < 1)
 		return -1;
 	const int dimXW = 1 << (lx - 2);
 	const int per = (int)(dimXW / T) << 2;
 	const int rem = (int)(dimXW % T);
-	const uint32_t tp = thresh23(p), tq = thresh23(q);
 	for(int m = 0; m < n; ++m)
 		for(int s = 0; s < 2; ++s)
 		{
@@ -796,10 +887,27 @@ int kor_cb_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
 				xSup += per + (t < rem ? 4 : 0);
 				const int xCenter = xMin + (((xSup - xMin) / 2) & ~3);
 				uint64_t st = states[t];
-				cb_loop(lat, lx, ly, s, xMin, xCenter, &st, a, c, tp, tq);
-				cb_loop(lat, lx, ly, s, xCenter, xSup, &st, a, c, tp, tq);
+				cb_loop(lat, lx, ly, s, xMin, xCenter, &st, a, c, upd, th);
+				cb_loop(lat, lx, ly, s, xCenter, xSup, &st, a, c, upd, th);
 				states[t] = st;
 			}
 		}
 	return 0;
+}
+
+int kor_cb_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
+		uint64_t a, uint64_t c, int n, double p, double q)
+{
+	const thr_t th = thr_pq(p, q);
+	return cb_mcs(lat, lx, ly, states, T, a, c, n, KOR_UPD_PQ, &th);
+}
+
+/* SchedulerCPUCB::mcsDisorder2SyncRng, kpz/schedulerCPUCB.cpp:103-108 (upd = DIS) */
+int kor_cb_mcs_rates(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
+		uint64_t a, uint64_t c, int n, int upd, const kor_rates* r)
+{
+	if(upd != KOR_UPD_PQ && upd != KOR_UPD_DIS)
+		return -1;
+	const thr_t th = thr_rates(r);
+	return cb_mcs(lat, lx, ly, states, T, a, c, n, upd, &th);
 }

@@ -81,6 +81,20 @@ int kor_consistent(const uint32_t* lat, int enc, int lx, int ly);
 /* update kinds */
 #define KOR_UPD_KPZ 0   /* pqOneUpdate: p=1, q=0, two draws per update */
 #define KOR_UPD_PQ  1   /* pqSyncRngUpdate: one extra randomFloat per update */
+#define KOR_UPD_DIS 2   /* disorder2SyncRngUpdate: as PQ with the rates of the site's class */
+#define KOR_UPD_EW  3   /* SyncRngEW (GPU fast path, p = q = 1): every minimum and maximum
+                           flips, no acceptance draw (contract: no CPU counterpart) */
+
+/* rates per disorder class (m_disorderP/Q, kpz/schedulerService.h:117,
+ * 185-197) and the disorder map (m_disorder, laid out like the lattice it
+ * goes with; NULL = class 0 everywhere) */
+typedef struct kor_rates {
+	double p[4], q[4];
+	const uint32_t* disorder;
+} kor_rates;
+/* SchedulerService::generateDisorder2, kpz/schedulerService.cpp:54-69:
+ * per word 16 master draws, bit 2k set when draw k < fill */
+void kor_generate_disorder2(kor_dsfmt* master, uint32_t* dis, uint64_t size_w, double fill);
 
 /* SchedulerCPUDT (origin_shift=0) and SchedulerCPUDTr (origin_shift=1) */
 int kor_dt_mcs(uint32_t* lat, int lx, int ly, int lbx, int lby,
@@ -106,6 +120,20 @@ int kor_cb_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
 /* single updates (for brute-force checks) */
 void kor_update_local(uint32_t* lat, int lx, int ly, int x, int y, int upd, float rnd, double p, double q);
 void kor_update_cbbit(uint32_t* lat, int lx, int ly, int x, int y, float rnd, double p, double q);
+
+/* the same schedulers with per-class rates and a disorder map
+ * (mcsDisorder2SyncRng: kpz/schedulerCPUDTr.cpp:115-119,
+ * kpz/schedulerCPUCBBit.cpp:105-109, kpz/schedulerCPUCB.cpp:103-108) */
+int kor_dt_mcs_rates(uint32_t* lat, int lx, int ly, int lbx, int lby,
+		uint64_t* states, int64_t T, uint64_t a, uint64_t c,
+		kor_dsfmt* master, int n, int random_origin, int upd, const kor_rates* r);
+/* mode 1 = per-site pq (class 0 rates), 2 = disorder2SyncRngUpdate */
+int kor_cbbit_mcs_rates(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
+		uint64_t a, uint64_t c, int n, int mode, const kor_rates* r);
+int kor_cb_mcs_rates(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
+		uint64_t a, uint64_t c, int n, int upd, const kor_rates* r);
+void kor_update_local_rates(uint32_t* lat, int lx, int ly, int x, int y, int upd, float rnd, const kor_rates* r);
+void kor_update_cbbit_rates(uint32_t* lat, int lx, int ly, int x, int y, float rnd, const kor_rates* r);
 
 #ifdef __cplusplus
 }

@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 ENC_LOCAL, ENC_CBBIT = 0, 1
-UPD_KPZ, UPD_PQ = 0, 1
+UPD_KPZ, UPD_PQ, UPD_DIS, UPD_EW = 0, 1, 2, 3
 
 
 def build(ref=False):
@@ -84,6 +84,33 @@ def _declare(L):
                                    C.c_double, C.c_double]
     L.kor_update_cbbit.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
                                    C.c_double, C.c_double]
+    rp = C.POINTER(Rates)
+    L.kor_generate_disorder2.argtypes = [vp, _u32p, C.c_uint64, C.c_double]
+    L.kor_dt_mcs_rates.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, _u64p, C.c_int64,
+                                   C.c_uint64, C.c_uint64, vp, C.c_int, C.c_int, C.c_int, rp]
+    L.kor_cbbit_mcs_rates.argtypes = [_u32p, C.c_int, C.c_int, _u64p, C.c_int64, C.c_uint64, C.c_uint64,
+                                      C.c_int, C.c_int, rp]
+    L.kor_cb_mcs_rates.argtypes = [_u32p, C.c_int, C.c_int, _u64p, C.c_int64, C.c_uint64, C.c_uint64,
+                                   C.c_int, C.c_int, rp]
+    L.kor_update_local_rates.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, rp]
+    L.kor_update_cbbit_rates.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, rp]
+
+
+class Rates(C.Structure):
+    """Rates per disorder class and the disorder map (kor_rates)."""
+    _fields_ = [("p", C.c_double * 4), ("q", C.c_double * 4), ("disorder", C.c_void_p)]
+
+
+def rates(p, q, disorder=None):
+    """p, q: sequences of up to 4 per-class rates; disorder: uint32 map or None."""
+    r = Rates()
+    for k, v in enumerate(p):
+        r.p[k] = v
+    for k, v in enumerate(q):
+        r.q[k] = v
+    r._keep = disorder
+    r.disorder = None if disorder is None else disorder.ctypes.data
+    return r
 
 
 class Dsfmt:
@@ -253,3 +280,38 @@ def update_local(lat, lx, ly, x, y, upd, rnd, p, q):
 
 def update_cbbit(lat, lx, ly, x, y, rnd, p, q):
     lib().kor_update_cbbit(lat, lx, ly, x, y, rnd, p, q)
+
+
+def generate_disorder2(master, size_words, fill):
+    dis = np.zeros(size_words, np.uint32)
+    lib().kor_generate_disorder2(master.ptr, dis, size_words, fill)
+    return dis
+
+
+def dt_mcs_rates(lat, lx, ly, lbx, lby, streams, master, n, random_origin, upd, r):
+    rc = lib().kor_dt_mcs_rates(lat, lx, ly, lbx, lby, streams.states, streams.T, streams.a, streams.c,
+                                master.ptr, n, int(random_origin), upd, C.byref(r))
+    if rc:
+        raise ValueError("invalid DT layout")
+
+
+def cbbit_mcs_rates(lat, lx, ly, streams, n, mode, r):
+    rc = lib().kor_cbbit_mcs_rates(lat, lx, ly, streams.states, streams.T, streams.a, streams.c, n, mode,
+                                   C.byref(r))
+    if rc:
+        raise ValueError("invalid CBBit size or mode")
+
+
+def cb_mcs_rates(lat, lx, ly, streams, n, upd, r):
+    rc = lib().kor_cb_mcs_rates(lat, lx, ly, streams.states, streams.T, streams.a, streams.c, n, upd,
+                                C.byref(r))
+    if rc:
+        raise ValueError("invalid CB size or update kind")
+
+
+def update_local_rates(lat, lx, ly, x, y, upd, rnd, r):
+    lib().kor_update_local_rates(lat, lx, ly, x, y, upd, rnd, C.byref(r))
+
+
+def update_cbbit_rates(lat, lx, ly, x, y, rnd, r):
+    lib().kor_update_cbbit_rates(lat, lx, ly, x, y, rnd, C.byref(r))
