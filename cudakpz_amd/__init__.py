@@ -218,8 +218,46 @@ class SchedulerService:
     def supplySeed(self, seed):
         _check(lib().kpz_set_seed(self._h, int(seed) & 0xFFFFFFFFFFFFFFFF))
 
-    def setPQ(self, p, q):
-        _check(lib().kpz_set_pq(self._h, float(p), float(q)))
+    def setPQ(self, p, q, n=0):
+        """SchedulerService::setPQ(p, q, n): rates of disorder class n."""
+        if n == 0:
+            _check(lib().kpz_set_pq(self._h, float(p), float(q)))
+        else:
+            L = lib()
+            L.kpz_set_pq_class.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_uint]
+            _check(L.kpz_set_pq_class(self._h, float(p), float(q), int(n)))
+
+    # -- Disorder2 (kpz/schedulerService.cpp:48-69)
+    def generateDisorder2(self, fill):
+        L = lib()
+        L.kpz_generate_disorder2.argtypes = [C.c_void_p, C.c_double]
+        _check(L.kpz_generate_disorder2(self._h, float(fill)))
+
+    def setDisorder(self, m):
+        m = np.ascontiguousarray(m, dtype=np.uint32)
+        if m.size != self.size_words:
+            raise KpzError(-1, "disorder map must have the lattice's size")
+        L = lib()
+        L.kpz_set_disorder.argtypes = [C.c_void_p, C.c_void_p]
+        _check(L.kpz_set_disorder(self._h, m.ctypes.data))
+
+    def disorder(self):
+        out = np.empty(self.size_words, np.uint32)
+        L = lib()
+        L.kpz_get_disorder.argtypes = [C.c_void_p, C.c_void_p]
+        _check(L.kpz_get_disorder(self._h, out.ctypes.data))
+        return out
+
+    def mcsDisorder2SyncRng(self, n):
+        L = lib()
+        L.kpz_mcs_disorder2.argtypes = [C.c_void_p, C.c_int]
+        _check(L.kpz_mcs_disorder2(self._h, int(n)))
+
+    def mcsEW(self, n):
+        """EW fast path (p = q = 1, no acceptance draw); DT schedulers only."""
+        L = lib()
+        L.kpz_mcs_ew.argtypes = [C.c_void_p, C.c_int]
+        _check(L.kpz_mcs_ew(self._h, int(n)))
 
     def initHomogeneous(self):
         _check(lib().kpz_init_flat(self._h))

@@ -227,6 +227,55 @@ int kpz_roughness(kpz_scheduler* h, int64_t out[3])
 	});
 }
 
+int kpz_set_pq_class(kpz_scheduler* h, double p, double q, unsigned n)
+{
+	HANDLE(h);
+	if(n >= 4)
+		return fail(KPZ_EINVAL, "disorder class must be 0..3");
+	return guard([&] {
+		if(!h->s->setPQ(p, q, n))
+			throw kpz::Error(KPZ_EINVAL, "rates must lie in [0, 1]");
+	});
+}
+
+int kpz_generate_disorder2(kpz_scheduler* h, double fill)
+{
+	HANDLE(h);
+	return guard([&] { h->s->generateDisorder2(fill); });
+}
+
+int kpz_set_disorder(kpz_scheduler* h, const uint32_t* map)
+{
+	HANDLE(h);
+	if(!map)
+		return fail(KPZ_EINVAL, "null map");
+	return guard([&] { h->s->setDisorder(map); });
+}
+
+int kpz_get_disorder(kpz_scheduler* h, uint32_t* map)
+{
+	HANDLE(h);
+	if(!map)
+		return fail(KPZ_EINVAL, "null map");
+	return guard([&] { h->s->getDisorder(map); });
+}
+
+int kpz_mcs_disorder2(kpz_scheduler* h, int n)
+{
+	HANDLE(h);
+	if(n < 0)
+		return fail(KPZ_EINVAL, "negative MCS count");
+	return guard([&] { h->s->mcsDisorder2SyncRng(n); });
+}
+
+int kpz_mcs_ew(kpz_scheduler* h, int n)
+{
+	HANDLE(h);
+	if(n < 0)
+		return fail(KPZ_EINVAL, "negative MCS count");
+	return guard([&] { h->s->mcsEW(n); });
+}
+
 int kpz_moments(kpz_scheduler* h, double out[7])
 {
 	HANDLE(h);

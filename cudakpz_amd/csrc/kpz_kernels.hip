@@ -183,9 +183,12 @@ __device__ __forceinline__ void dt_writeback_x16(uint32_t* __restrict__ lat, int
 }
 
 // One DT block i of one phase, staged in this lane's LDS column `my`
-// (row r at my[r * DT_WG]).  nb = blocks per domain set.
+// (row r at my[r * DT_WG]).  nb = blocks per domain set.  UPD: 0 = KPZ
+// (pqOneUpdate), 1 = pqSyncRngUpdate, 2 = disorder2SyncRngUpdate (class rows
+// of the disorder map in myD, 2 bits per site), 3 = EW.
 template<int LBX, int LBY, int UPD, bool RO>
-__device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_t nb, uint2* __restrict__ my)
+__device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_t nb, uint2* __restrict__ my,
+                                         uint32_t* __restrict__ myD)
 {
 	constexpr int BX = 1 << LBX, BY = 1 << LBY;
 	constexpr int NWX = BX / 4 + 1; // words covering BX+1 sites (+ shift)
@@ -255,6 +258,36 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 		}
 	}
 	}
+	if constexpr(UPD == 2)
+	{
+		// class rows of the disorder map (c(m_disorder): the site's 2 bits,
+		// kpz/schedulerCPUBaseThreadedRng.inl:50-75): byte rr of a word is
+		// one site row, 2 bits per site, so a row is 2*BX bits
+		constexpr uint32_t DM = (BX == 16) ? 0xFFFFFFFFu : ((1u << (2 * BX)) - 1u);
+#pragma unroll
+		for(int wk = 0; wk < NWY; ++wk)
+		{
+			if(!RO && wk == NWY - 1)
+				continue; // static DT: the top halo row needs no classes
+			const size_t rowBase = (size_t)((wr0 + wk) & mYW) * XW;
+			uint32_t dw[NWX];
+#pragma unroll
+			for(int k = 0; k < NWX; ++k)
+				dw[k] = (RO || k < NWX - 1) ? a.dis[rowBase + ((c0 + k) & mXW)] : 0u;
+#pragma unroll
+			for(int rr = 0; rr < 4; ++rr)
+			{
+				const int r = 4 * wk + rr - shy;
+				if(r < 0 || r >= BY)
+					continue;
+				uint64_t dv = 0;
+#pragma unroll
+				for(int k = 0; k < NWX; ++k)
+					dv |= (uint64_t)((dw[k] >> (8 * rr)) & 0xFFu) << (8 * k);
+				myD[r * DT_WG] = (uint32_t)(dv >> (2 * shx)) & DM;
+			}
+		}
+	}
 
 	// ---- stream state: block j of stream t starts j blocks of draws in
 	const bool own = a.T >= nb;
@@ -304,13 +337,32 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 			my[y * DT_WG] = row;
 			my[(y + 1) * DT_WG].y = up - fm;
 		}
+		else if(UPD == 3)
+		{
+			// SyncRngEW (kpz/kpzSimulationPrimitivesTemplates.cu:199-234): p = q = 1,
+			// every minimum and maximum flips, no acceptance draw
+			const uint32_t mn = rx & ~row.x & ~row.y & up;
+			const uint32_t mx = ~rx & row.x & row.y & ~up;
+			const uint32_t fm = (mn | mx) & (1u << x);
+			row.x ^= fm | (fm << 1);
+			row.y ^= fm;
+			my[y * DT_WG] = row;
+			my[(y + 1) * DT_WG].y = up ^ fm;
+		}
 		else
 		{
 			s = step(s);
 			const uint32_t r23 = slcg_fold(s) & 0x7FFFFFu;
+			uint32_t tpv = tp, tqv = tq;
+			if(UPD == 2)
+			{
+				const uint32_t cls = (myD[y * DT_WG] >> (2 * x)) & 3u;
+				tpv = (cls & 2) ? ((cls & 1) ? a.tpc[3] : a.tpc[2]) : ((cls & 1) ? a.tpc[1] : a.tpc[0]);
+				tqv = (cls & 2) ? ((cls & 1) ? a.tqc[3] : a.tqc[2]) : ((cls & 1) ? a.tqc[1] : a.tqc[0]);
+			}
 			// all-ones when accepted: sign of r23 - t (both below 2^24)
-			const uint32_t aMin = (uint32_t)((int32_t)(r23 - tp) >> 31);
-			const uint32_t aMax = (uint32_t)((int32_t)(r23 - tq) >> 31);
+			const uint32_t aMin = (uint32_t)((int32_t)(r23 - tpv) >> 31);
+			const uint32_t aMax = (uint32_t)((int32_t)(r23 - tqv) >> 31);
 			const uint32_t mn = rx & ~row.x & ~row.y & up;
 			const uint32_t mx = ~rx & row.x & row.y & ~up;
 			const uint32_t fm = ((mn & aMin) | (mx & aMax)) & (1u << x);
@@ -371,10 +423,11 @@ template<int LBX, int LBY, int UPD, bool RO>
 __global__ __launch_bounds__(DT_WG) void dt_phase_kernel(DtPhaseArgs a, int64_t nb)
 {
 	__shared__ uint2 tile[dt_tile_rows<LBX, LBY>() * DT_WG];
+	__shared__ uint32_t tileD[(UPD == 2) ? (1 << LBY) * DT_WG : 1];
 	const int64_t i = (int64_t)blockIdx.x * DT_WG + threadIdx.x;
 	if(i >= nb)
 		return; // no barrier in this kernel: idle lanes may leave
-	dt_block<LBX, LBY, UPD, RO>(a, i, nb, tile + threadIdx.x);
+	dt_block<LBX, LBY, UPD, RO>(a, i, nb, tile + threadIdx.x, (UPD == 2) ? tileD + threadIdx.x : nullptr);
 }
 
 // ======================================================================
@@ -472,12 +525,32 @@ __device__ __forceinline__ void cbbit_row(const CbHalfArgs& a, int y, int g)
 			// acceptance bits, site k of the word -> bit k: the sign bit of
 			// r23 - t is funnel-shifted into an accumulator (v_alignbit), 32
 			// sites per accumulator, which ends bit-reversed
-			W pb = 0, qb = 0;
+			W pb = 0, qb = 0, pb1 = 0, qb1 = 0;
 #pragma unroll
 			for(int h = 0; h < WB / 32; ++h)
 			{
 				uint32_t pa = 0, qa = 0;
-				if(tq)
+				if(MODE == 2)
+				{
+					// both classes' acceptance bits; the map selects per site below
+					uint32_t pa1 = 0, qa1 = 0;
+					const uint32_t tp1 = a.tp1, tq1 = a.tq1;
+#pragma unroll
+					for(int k = 0; k < 32; ++k)
+					{
+						const uint64_t pr = (uint64_t)lo * aLo + cc;
+						h24 = mad_u24(h24, aLo, mad_u24(lo, aHi, (uint32_t)(pr >> 32)));
+						lo = (uint32_t)pr;
+						const uint32_t r23 = (lo ^ h24) & 0x7FFFFFu;
+						pa = __builtin_amdgcn_alignbit(pa, r23 - tp, 31);
+						qa = __builtin_amdgcn_alignbit(qa, r23 - tq, 31);
+						pa1 = __builtin_amdgcn_alignbit(pa1, r23 - tp1, 31);
+						qa1 = __builtin_amdgcn_alignbit(qa1, r23 - tq1, 31);
+					}
+					pb1 |= (W)__brev(pa1) << (32 * h);
+					qb1 |= (W)__brev(qa1) << (32 * h);
+				}
+				else if(tq)
 				{
 #pragma unroll
 					for(int k = 0; k < 32; ++k)
@@ -504,6 +577,12 @@ __device__ __forceinline__ void cbbit_row(const CbHalfArgs& a, int y, int g)
 				}
 				pb |= (W)__brev(pa) << (32 * h);
 				qb |= (W)__brev(qa) << (32 * h);
+			}
+			if(MODE == 2)
+			{
+				const W D = ((const W*)a.dis)[thisB + row + xw];
+				pb = (pb & ~D) | (pb1 & D);
+				qb = (qb & ~D) | (qb1 & D);
 			}
 			flip = (minm & pb) | (maxm & qb);
 		}
@@ -579,20 +658,41 @@ template<int S>
 struct CbCell {
 	static constexpr uint32_t MS = S ? 0x11441144u : 0x44114411u; // active X positions
 	static constexpr int K[8] = {S ? 6 : 4, S ? 2 : 6, 4, S ? 6 : 2, 4, S ? 2 : 6, 4, S ? 4 : 2};
+	static constexpr int P[8] = {S ? 2 : 0, S ? 6 : 4, S ? 8 : 10, S ? 12 : 14,
+	                             S ? 18 : 16, S ? 22 : 20, S ? 24 : 26, S ? 28 : 30};
 };
 
 // one cell: 8 draws (stream state at the cell's first draw in sc), flips of
-// the own word, of column 0 of the right word and of row 0 of the up word
-template<int S>
+// the own word, of column 0 of the right word and of row 0 of the up word.
+// DIS: disorder2SyncRngUpdate (kpz/schedulerCPUBaseThreadedRng.inl:50-75),
+// the thresholds of each site's class, the site's 2 bits of the map word dw.
+template<int S, bool DIS>
 __device__ __forceinline__ void cb_cell(uint64_t sc, const Affine& step, uint32_t tp, uint32_t tq, uint32_t own,
-                                        uint32_t right, uint32_t up, uint32_t& fOwn, uint32_t& fRight, uint32_t& fUp)
+                                        uint32_t right, uint32_t up, uint32_t& fOwn, uint32_t& fRight, uint32_t& fUp,
+                                        uint32_t dw, const uint32_t* tpc, const uint32_t* tqc)
 {
 	// only (lo ^ hi) & 0x7FFFFF is used: high state half carried mod 2^24
 	const uint32_t aLo = (uint32_t)step.a, aHi = (uint32_t)(step.a >> 32);
 	const uint64_t cc = step.c;
 	uint32_t lo = (uint32_t)sc, h24 = (uint32_t)(sc >> 32);
 	uint32_t ap = 0, aq = 0;
-	if(tq)
+	if(DIS)
+	{
+#pragma unroll
+		for(int i = 0; i < 8; ++i)
+		{
+			const uint64_t pr = (uint64_t)lo * aLo + cc;
+			h24 = mad_u24(h24, aLo, mad_u24(lo, aHi, (uint32_t)(pr >> 32)));
+			lo = (uint32_t)pr;
+			const uint32_t r23 = (lo ^ h24) & 0x7FFFFFu;
+			const uint32_t cls = (dw >> CbCell<S>::P[i]) & 3u;
+			const uint32_t tpv = (cls & 2) ? ((cls & 1) ? tpc[3] : tpc[2]) : ((cls & 1) ? tpc[1] : tpc[0]);
+			const uint32_t tqv = (cls & 2) ? ((cls & 1) ? tqc[3] : tqc[2]) : ((cls & 1) ? tqc[1] : tqc[0]);
+			ap = __builtin_amdgcn_alignbit(ap, r23 - tpv, 32 - CbCell<S>::K[i]);
+			aq = __builtin_amdgcn_alignbit(aq, r23 - tqv, 32 - CbCell<S>::K[i]);
+		}
+	}
+	else if(tq)
 	{
 #pragma unroll
 		for(int i = 0; i < 8; ++i)
@@ -633,7 +733,7 @@ __device__ __forceinline__ void cb_cell(uint64_t sc, const Affine& step, uint32_
 	fUp = ((f >> 24) & 0x55u) << 1;
 }
 
-template<int S>
+template<int S, bool DIS>
 __global__ __launch_bounds__(256) void cb_local_band_kernel(CbLocalArgs a)
 {
 	const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -692,7 +792,8 @@ __global__ __launch_bounds__(256) void cb_local_band_kernel(CbLocalArgs a)
 		const uint32_t right = lat[row + ((c + 1) & (XC - 1))];
 		const uint32_t up = lat[((size_t)((yb + 1) & (a.nyb - 1)) << lXC) + c];
 		uint32_t fOwn, fRight, fUp;
-		cb_cell<S>(s, step, tp, tq, own, right, up, fOwn, fRight, fUp);
+		const uint32_t dw = DIS ? a.dis[row + c] : 0u;
+		cb_cell<S, DIS>(s, step, tp, tq, own, right, up, fOwn, fRight, fUp, dw, a.tpc, a.tqc);
 		uint32_t fromLeft = (uint32_t)__shfl((int)fRight, (g - 1) & (G - 1), G);
 		if(g == 0 && !wrapG)
 			fromLeft = 0;
@@ -1095,17 +1196,17 @@ __global__ __launch_bounds__(DT_WG) void batch_dt_kernel(BatchDtArgs a)
 		p.sx = sh & 0xFF;
 		p.sy = sh >> 8;
 		if(rep.mode)
-			dt_block<4, 4, 1, true>(p, w.y, nb, my);
+			dt_block<4, 4, 1, true>(p, w.y, nb, my, nullptr);
 		else
-			dt_block<4, 4, 0, true>(p, w.y, nb, my);
+			dt_block<4, 4, 0, true>(p, w.y, nb, my, nullptr);
 	}
 	else
 	{
 		p.sx = p.sy = 0;
 		if(rep.mode)
-			dt_block<4, 4, 1, false>(p, w.y, nb, my);
+			dt_block<4, 4, 1, false>(p, w.y, nb, my, nullptr);
 		else
-			dt_block<4, 4, 0, false>(p, w.y, nb, my);
+			dt_block<4, 4, 0, false>(p, w.y, nb, my, nullptr);
 	}
 }
 
@@ -1198,22 +1299,29 @@ __global__ void batch_combine_kernel(const BatchRep* __restrict__ reps, int nrep
 	out[3 * r + 2] = (int64_t)(s2 >> 64);
 }
 
+template<int LBX, int LBY, int UPD>
+void dt_launch(const DtPhaseArgs& a, int64_t nb, dim3 grid, hipStream_t stream)
+{
+	if(a.random_origin)
+		hipLaunchKernelGGL((dt_phase_kernel<LBX, LBY, UPD, true>), grid, dim3(DT_WG), 0, stream, a, nb);
+	else
+		hipLaunchKernelGGL((dt_phase_kernel<LBX, LBY, UPD, false>), grid, dim3(DT_WG), 0, stream, a, nb);
+}
+
 template<int LBX, int LBY>
 hipError_t dt_dispatch_upd(const DtPhaseArgs& a, int64_t nb, dim3 grid, hipStream_t stream)
 {
-	if(a.upd == 0)
+	switch(a.upd)
 	{
-		if(a.random_origin)
-			hipLaunchKernelGGL((dt_phase_kernel<LBX, LBY, 0, true>), grid, dim3(DT_WG), 0, stream, a, nb);
-		else
-			hipLaunchKernelGGL((dt_phase_kernel<LBX, LBY, 0, false>), grid, dim3(DT_WG), 0, stream, a, nb);
-	}
-	else
-	{
-		if(a.random_origin)
-			hipLaunchKernelGGL((dt_phase_kernel<LBX, LBY, 1, true>), grid, dim3(DT_WG), 0, stream, a, nb);
-		else
-			hipLaunchKernelGGL((dt_phase_kernel<LBX, LBY, 1, false>), grid, dim3(DT_WG), 0, stream, a, nb);
+		case 0: dt_launch<LBX, LBY, 0>(a, nb, grid, stream); break;
+		case 1: dt_launch<LBX, LBY, 1>(a, nb, grid, stream); break;
+		case 2:
+			if(!a.dis)
+				return hipErrorInvalidValue;
+			dt_launch<LBX, LBY, 2>(a, nb, grid, stream);
+			break;
+		case 3: dt_launch<LBX, LBY, 3>(a, nb, grid, stream); break;
+		default: return hipErrorInvalidValue;
 	}
 	return hipGetLastError();
 }
@@ -1262,13 +1370,22 @@ hipError_t launch_cbbit_half(const CbHalfArgs& a, hipStream_t stream)
 		else
 			hipLaunchKernelGGL((cbbit_half_kernel<uint32_t, 0>), grid, dim3(block), 0, stream, a);
 	}
-	else
+	else if(a.mode == 1)
 	{
 		if(wide)
 			hipLaunchKernelGGL((cbbit_half_kernel<uint64_t, 1>), grid, dim3(block), 0, stream, a);
 		else
 			hipLaunchKernelGGL((cbbit_half_kernel<uint32_t, 1>), grid, dim3(block), 0, stream, a);
 	}
+	else if(a.mode == 2 && a.dis)
+	{
+		if(wide)
+			hipLaunchKernelGGL((cbbit_half_kernel<uint64_t, 2>), grid, dim3(block), 0, stream, a);
+		else
+			hipLaunchKernelGGL((cbbit_half_kernel<uint32_t, 2>), grid, dim3(block), 0, stream, a);
+	}
+	else
+		return hipErrorInvalidValue;
 	return hipGetLastError();
 }
 
@@ -1324,10 +1441,20 @@ hipError_t launch_cb_local_half(const CbLocalArgs& a, hipStream_t stream)
 		return hipErrorInvalidValue;
 	const int64_t lanes = 1ll << (a.lx - 2 + a.ly - 2 - a.lNB);
 	const unsigned grid = (unsigned)((lanes + 255) / 256);
-	if(a.signum)
-		hipLaunchKernelGGL(cb_local_band_kernel<1>, dim3(grid), dim3(256), 0, stream, a);
+	if(a.dis)
+	{
+		if(a.signum)
+			hipLaunchKernelGGL((cb_local_band_kernel<1, true>), dim3(grid), dim3(256), 0, stream, a);
+		else
+			hipLaunchKernelGGL((cb_local_band_kernel<0, true>), dim3(grid), dim3(256), 0, stream, a);
+	}
 	else
-		hipLaunchKernelGGL(cb_local_band_kernel<0>, dim3(grid), dim3(256), 0, stream, a);
+	{
+		if(a.signum)
+			hipLaunchKernelGGL((cb_local_band_kernel<1, false>), dim3(grid), dim3(256), 0, stream, a);
+		else
+			hipLaunchKernelGGL((cb_local_band_kernel<0, false>), dim3(grid), dim3(256), 0, stream, a);
+	}
 	const int XC = 1 << (a.lx - 2);
 	const int64_t nR = (a.lG < a.lx - 2) ? ((int64_t)a.nyb * (XC >> a.lG)) : 0;
 	const int64_t nU = (int64_t)(a.nyb >> a.lNB) * XC;

@@ -20,9 +20,12 @@ struct DtPhaseArgs {
 	Affine step;            // per-draw SLCG step (skip form)
 	int ds;                 // domain set 0..3
 	int sx, sy;             // DT origin shift (0 for the static DT)
-	int upd;                // 0 = KPZ (pqOneUpdate), 1 = pq (pqSyncRngUpdate)
+	int upd;                // 0 = KPZ (pqOneUpdate), 1 = pq (pqSyncRngUpdate),
+	                        // 2 = disorder2SyncRngUpdate, 3 = EW (p = q = 1, no acceptance draw)
 	bool random_origin;     // DTr coordinate/mask convention
 	uint32_t tp, tq;        // 23-bit thresholds
+	const uint32_t* dis;    // upd 2: disorder map (ENC_LOCAL layout, 2 bits = class per site)
+	uint32_t tpc[4], tqc[4]; // upd 2: thresholds per class
 };
 hipError_t launch_dt_phase(const DtPhaseArgs& a, hipStream_t stream);
 bool dt_layout_supported(int lbx, int lby);
@@ -43,6 +46,10 @@ struct CbHalfArgs {
 	Affine fullBig, fullSmall; // whole half-sweep of a worker (per+1 / per rows)
 	uint32_t tp, tq;
 	int lG;                 // log2 lanes per row
+	// mode 2 (disorder2SyncRngUpdate, kpz/schedulerCPUCBBit_nested.inl:298-331):
+	// class = x-slope bit of the site in the map (ENC_CBBIT layout), class-1 thresholds
+	const uint32_t* dis;
+	uint32_t tp1, tq1;
 };
 hipError_t launch_cbbit_half(const CbHalfArgs& a, hipStream_t stream);
 
@@ -69,6 +76,10 @@ struct CbLocalArgs {
 	int lNB;                // log2 cell rows per band (<= ly - 2)
 	uint32_t* edgeR;        // [nyb][groups per row]: flips leaving a group's right edge
 	uint32_t* edgeU;        // [bands][cells per row]: flips leaving a band's top row
+	// disorder2SyncRngUpdate (kpz/schedulerCPUCB.cpp:103-108): map in ENC_LOCAL
+	// layout (null: plain pq) and thresholds per class
+	const uint32_t* dis;
+	uint32_t tpc[4], tqc[4];
 };
 // one half-sweep: the band kernel plus the edge fix-up (two launches)
 hipError_t launch_cb_local_half(const CbLocalArgs& a, hipStream_t stream);

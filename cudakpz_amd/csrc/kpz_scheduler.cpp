@@ -97,6 +97,72 @@ bool SchedulerService::setPQ(double p, double q)
 	return true;
 }
 
+// SchedulerService::setPQ(p, q, n) (kpz/schedulerService.h:185-191)
+bool SchedulerService::setPQ(double p, double q, unsigned n)
+{
+	if(n >= 4)
+		return false;
+	if(n == 0)
+		return setPQ(p, q);
+	if(!(p >= 0.0 && p <= 1.0 && q >= 0.0 && q <= 1.0))
+		return false;
+	cp_[n] = p;
+	cq_[n] = q;
+	cset_[n] = true;
+	return true;
+}
+
+void SchedulerService::classThresholds(uint32_t tp[4], uint32_t tq[4]) const
+{
+	for(unsigned k = 0; k < 4; ++k)
+	{
+		tp[k] = thresh23(rateP(k));
+		tq[k] = thresh23(rateQ(k));
+	}
+}
+
+// SchedulerService::generateDisorder2 (kpz/schedulerService.cpp:54-69): the
+// map has the lattice's size and layout; every word takes 16 master draws
+void SchedulerService::generateDisorder2(double fill)
+{
+	std::vector<uint32_t> h((size_t)size_.sizeW());
+	for(auto& w : h)
+	{
+		uint32_t t = 0;
+		for(int s = 0; s < 32; s += 2)
+			if(master_.close_open() < fill)
+				t |= 1u << s;
+		w = t;
+	}
+	setDisorder(h.data());
+}
+
+void SchedulerService::setDisorder(const uint32_t* host)
+{
+	if(!dis_.get())
+		dis_.alloc(size_.sizeW() * 4);
+	check(hipMemcpyAsync(dis_.get(), host, size_.sizeW() * 4, hipMemcpyHostToDevice, stream_), "disorder H2D");
+	sync();
+}
+
+void SchedulerService::getDisorder(uint32_t* host)
+{
+	if(!dis_.get())
+		throw Error(-4, "no disorder map (generateDisorder2 / setDisorder first)");
+	check(hipMemcpyAsync(host, dis_.get(), size_.sizeW() * 4, hipMemcpyDeviceToHost, stream_), "disorder D2H");
+	sync();
+}
+
+void SchedulerService::mcsDisorder2SyncRng(int)
+{
+	throw Error(-1, "mcsDisorder2SyncRng is not available for this scheduler");
+}
+
+void SchedulerService::mcsEW(int)
+{
+	throw Error(-1, "the EW fast path (mcsEW) exists for the DT schedulers only");
+}
+
 void SchedulerService::pushSystem(const uint32_t* host)
 {
 	check(hipMemcpyAsync(deviceLattice(), host, size_.sizeW() * 4, hipMemcpyHostToDevice, stream_), "pushSystem");
@@ -311,8 +377,13 @@ const Affine* SchedulerDT::jumps(int du)
 
 void SchedulerDT::run(int n, int upd)
 {
-	const int du = upd ? 3 : 2;
+	if(upd == 2 && !hasDisorder())
+		throw Error(-4, "mcsDisorder2SyncRng needs a disorder map (generateDisorder2 / setDisorder)");
+	// draws per update: x, y, and the acceptance draw of the pq and disorder rules
+	const int du = (upd == 1 || upd == 2) ? 3 : 2;
 	DtPhaseArgs a{};
+	a.dis = dis_.as<uint32_t>();
+	classThresholds(a.tpc, a.tqc);
 	a.lat = deviceLattice();
 	a.lx = size_.lx;
 	a.ly = size_.ly;
@@ -352,6 +423,9 @@ void SchedulerDT::run(int n, int upd)
 
 void SchedulerDT::mcs(int n) { run(n, 0); }
 void SchedulerDT::mcsPQSyncRng(int n) { run(n, 1); }
+// SchedulerCPUDTr::mcsDisorder2SyncRng (kpz/schedulerCPUDTr.cpp:115-119)
+void SchedulerDT::mcsDisorder2SyncRng(int n) { run(n, 2); }
+void SchedulerDT::mcsEW(int n) { run(n, 3); }
 
 // ======================================================================
 SchedulerSCABit::SchedulerSCABit(int lx, int ly, const Params& p)
@@ -416,8 +490,13 @@ SchedulerSCABit::Tables& SchedulerSCABit::tables(int mode)
 
 void SchedulerSCABit::run(int n, int mode)
 {
-	Tables& tb = tables(mode);
+	if(mode == 2 && !hasDisorder())
+		throw Error(-4, "mcsDisorder2SyncRng needs a disorder map (generateDisorder2 / setDisorder)");
+	Tables& tb = tables(mode == 2 ? 1 : mode); // one draw per site in both per-site modes
 	CbHalfArgs a{};
+	a.dis = dis_.as<uint32_t>();
+	a.tp1 = thresh23(rateP(1));
+	a.tq1 = thresh23(rateQ(1));
 	a.lat = deviceLattice();
 	a.lx = size_.lx;
 	a.ly = size_.ly;
@@ -458,6 +537,9 @@ void SchedulerSCABit::mcs(int n)
 
 // SchedulerCPUCBBit::mcsPQSyncRng (kpz/schedulerCPUCBBit.cpp:95-102)
 void SchedulerSCABit::mcsPQSyncRng(int n) { run(n, (q_ == 0.0 && p_ == 0.5) ? 0 : 1); }
+// SchedulerCPUCBBit::mcsDisorder2SyncRng (kpz/schedulerCPUCBBit.cpp:105-109): the
+// class is the site's x-slope bit of the map, so classes 0 and 1 exist
+void SchedulerSCABit::mcsDisorder2SyncRng(int n) { run(n, 2); }
 
 // ======================================================================
 SchedulerSCA::SchedulerSCA(int lx, int ly, const Params& p) : SchedulerService(lx, ly, ENC_LOCAL, p)
@@ -545,12 +627,24 @@ void SchedulerSCA::mcs(int n)
 	mcsPQSyncRng(n);
 }
 
-void SchedulerSCA::mcsPQSyncRng(int n)
+void SchedulerSCA::mcsPQSyncRng(int n) { run(n, false); }
+
+// SchedulerCPUCB::mcsDisorder2SyncRng (kpz/schedulerCPUCB.cpp:103-108)
+void SchedulerSCA::mcsDisorder2SyncRng(int n)
+{
+	if(!hasDisorder())
+		throw Error(-4, "mcsDisorder2SyncRng needs a disorder map (generateDisorder2 / setDisorder)");
+	run(n, true);
+}
+
+void SchedulerSCA::run(int n, bool disorder)
 {
 	CbLocalArgs a = args_;
 	a.lat = deviceLattice();
 	a.tp = thresh23(p_);
 	a.tq = thresh23(q_);
+	a.dis = disorder ? dis_.as<uint32_t>() : nullptr;
+	classThresholds(a.tpc, a.tqc);
 	for(int m = 0; m < n; ++m)
 	{
 		for(int s = 0; s < 2; ++s)

@@ -82,6 +82,24 @@ public:
 	double rateP() const { return p_; }
 	double rateQ() const { return q_; }
 
+	// ---- Disorder2 (kpz/schedulerService.cpp:48-69, kpz/schedulerService.h:185-197)
+	// rates of disorder class n (0..3; class 0 = setPQ(p, q)); a class whose
+	// rates were never set uses class 0's (contract: the reference leaves them
+	// uninitialised)
+	bool setPQ(double p, double q, unsigned n);
+	double rateP(unsigned n) const { return n == 0 ? p_ : (cset_[n & 3] ? cp_[n & 3] : p_); }
+	double rateQ(unsigned n) const { return n == 0 ? q_ : (cset_[n & 3] ? cq_[n & 3] : q_); }
+	// generateDisorder2: per word 16 master draws, bit 2k set when draw k < fill
+	void generateDisorder2(double fill);
+	void setDisorder(const uint32_t* host);  // map in the lattice's layout
+	void getDisorder(uint32_t* host);
+	bool hasDisorder() const { return dis_.get() != nullptr; }
+	// disorder2SyncRngUpdate dynamics (per-site class rates)
+	virtual void mcsDisorder2SyncRng(int n);
+	// SyncRngEW (kpz/kpzSimulationPrimitivesTemplates.cu:199-234): p = q = 1
+	// without an acceptance draw; DT schedulers only
+	virtual void mcsEW(int n);
+
 	// ---- dynamics
 	virtual void mcs(int n) = 0;             // default dynamics of the scheduler
 	virtual void mcsSyncRng(int n) { mcs(n); }
@@ -156,6 +174,10 @@ protected:
 	int cur_ = 0;
 	MasterRng master_;
 	double p_ = 1.0, q_ = 0.0;
+	double cp_[4] = {0, 0, 0, 0}, cq_[4] = {0, 0, 0, 0};
+	bool cset_[4] = {false, false, false, false};
+	DevBuf dis_;
+	void classThresholds(uint32_t tp[4], uint32_t tq[4]) const;
 	unsigned int mcs_ = 0;
 	std::vector<int64_t> hostRows_;
 };
@@ -167,6 +189,8 @@ public:
 	SchedulerDT(int lx, int ly, const Params& p, bool randomOrigin);
 	void mcs(int n) override;                // p=1, q=0 (pqOneUpdate)
 	void mcsPQSyncRng(int n) override;       // per-update pq rule
+	void mcsDisorder2SyncRng(int n) override;
+	void mcsEW(int n) override;
 	int layoutX() const { return lbx_; }
 	int layoutY() const { return lby_; }
 	int64_t blocksPerDomainSet() const { return nb_; }
@@ -186,6 +210,7 @@ public:
 	SchedulerSCABit(int lx, int ly, const Params& p);
 	void mcs(int n) override;                // p=.5, q=0 bit-parallel (mcsSyncRng)
 	void mcsPQSyncRng(int n) override;       // bit-parallel if (.5, 0), else per site
+	void mcsDisorder2SyncRng(int n) override;
 	double mcsScale() const override { return p_ + q_; }
 
 private:
@@ -207,8 +232,10 @@ public:
 	void mcs(int n) override;                // SchedulerCPUCB::mcs: p=.95, q=0
 	void mcsSyncRng(int n) override { mcsPQSyncRng(n); }
 	void mcsPQSyncRng(int n) override;
+	void mcsDisorder2SyncRng(int n) override;
 
 private:
+	void run(int n, bool disorder);
 	CbLocalArgs args_{};
 	DevBuf rowJ_, baseJ_, cellJ_, edgeR_, edgeU_;
 };

@@ -4,15 +4,17 @@
 // One little-endian file holds everything the scheduler needs to continue
 // bit for bit: kind, sizes and construction parameters, MCS count, (p, q),
 // the master dSFMT state (dsfmt_t bytes), every SLCG stream state (which the
-// reference never saved), the lattice in reference layout, and every live
-// correlator tag (MCS, stop time, snapshot).  An FNV-1a 64 checksum over the
-// file's bytes ends it.
+// reference never saved), the lattice in reference layout, the rates of the
+// disorder classes and the disorder map (collectH5 writes .disorderP/Q and
+// .disorder), and every live correlator tag (MCS, stop time, snapshot).  An
+// FNV-1a 64 checksum over the file's bytes ends it.
 //
-//   "KPZSTAT1"  u32 version (1)
+//   "KPZSTAT1"  u32 version (2)
 //   i32 kind, lx, ly | i64 threads | i32 layout_x, layout_y | u64 skip
-//   u32 mcs, aux | f64 p, q | u32 master bytes | u64 streams | u64 words | u32 tags
-//   master bytes | streams x u64 | words x u32 | tags x (u32 mcs, u32 stop, words x u32)
-//   u64 checksum
+//   u32 mcs, aux | f64 p, q | f64 p1..p3, q1..q3 | u32 classes set (bits 1..3)
+//   u32 master bytes | u64 streams | u64 words | u32 tags | u32 has disorder map
+//   master bytes | streams x u64 | words x u32 | [map: words x u32]
+//   tags x (u32 mcs, u32 stop, words x u32) | u64 checksum
 #include "kpz_scheduler.hpp"
 
 #include <cstdio>
@@ -25,7 +27,7 @@ namespace kpz {
 namespace {
 
 constexpr char MAGIC[8] = {'K', 'P', 'Z', 'S', 'T', 'A', 'T', '1'};
-constexpr uint32_t VERSION = 1;
+constexpr uint32_t VERSION = 2;
 
 struct Fnv {
 	uint64_t h = 1469598103934665603ull;
@@ -76,10 +78,11 @@ struct Header {
 	int32_t layoutX, layoutY;
 	uint64_t skip;
 	uint32_t mcs, aux;
-	double p, q;
+	double p, q, cp[3], cq[3];
+	uint32_t classesSet;
 	uint32_t masterBytes;
 	uint64_t streams, words;
-	uint32_t tags;
+	uint32_t tags, hasMap;
 };
 
 } // namespace
@@ -113,13 +116,28 @@ void SchedulerService::saveState(const char* path, uint32_t aux)
 	w.val(aux);
 	w.val(p_);
 	w.val(q_);
+	uint32_t set = 0;
+	for(int k = 1; k < 4; ++k)
+	{
+		w.val(cp_[k]);
+		set |= cset_[k] ? (1u << k) : 0u;
+	}
+	for(int k = 1; k < 4; ++k)
+		w.val(cq_[k]);
+	w.val(set);
 	w.val((uint32_t)master.size());
 	w.val((uint64_t)T_);
 	w.val(words);
 	w.val((uint32_t)tags_.size());
+	w.val((uint32_t)(hasDisorder() ? 1 : 0));
 	w.put(master.data(), master.size());
 	w.put(st.data(), st.size() * 8);
 	w.put(lat.data(), lat.size() * 4);
+	if(hasDisorder())
+	{
+		getDisorder(lat.data());
+		w.put(lat.data(), lat.size() * 4);
+	}
 	for(const Tag& t : tags_)
 	{
 		w.val((uint32_t)t.mcs);
@@ -172,15 +190,23 @@ std::unique_ptr<SchedulerService> SchedulerService::loadState(const char* path, 
 	h.aux = r.val<uint32_t>();
 	h.p = r.val<double>();
 	h.q = r.val<double>();
+	for(int k = 0; k < 3; ++k)
+		h.cp[k] = r.val<double>();
+	for(int k = 0; k < 3; ++k)
+		h.cq[k] = r.val<double>();
+	h.classesSet = r.val<uint32_t>();
 	h.masterBytes = r.val<uint32_t>();
 	h.streams = r.val<uint64_t>();
 	h.words = r.val<uint64_t>();
 	h.tags = r.val<uint32_t>();
+	h.hasMap = r.val<uint32_t>();
 	if(h.kind < KIND_DT || h.kind > KIND_SCA || h.lx < 2 || h.ly < 2 || h.lx > 30 || h.ly > 30 ||
 	   h.words != (1ull << (h.lx + h.ly - 4)) || h.masterBytes != (uint32_t)MasterRng::STATE_BYTES ||
-	   h.streams == 0 || (h.threads > 0 && (uint64_t)h.threads != h.streams))
+	   h.streams == 0 || (h.threads > 0 && (uint64_t)h.threads != h.streams) || h.hasMap > 1 ||
+	   (h.classesSet & ~0xEu))
 		throw Error(-1, "loadState: inconsistent header");
-	const size_t need = r.pos + h.masterBytes + h.streams * 8 + h.words * 4 + (size_t)h.tags * (8 + h.words * 4) + 8;
+	const size_t need = r.pos + h.masterBytes + h.streams * 8 + h.words * 4 * (1 + h.hasMap) +
+	                    (size_t)h.tags * (8 + h.words * 4) + 8;
 	if(need != buf.size())
 		throw Error(-1, "loadState: size does not match the header");
 
@@ -194,12 +220,23 @@ std::unique_ptr<SchedulerService> SchedulerService::loadState(const char* path, 
 	if(s->threads() != (int64_t)h.streams || s->skip() != h.skip)
 		throw Error(-1, "loadState: stream layout differs");
 	s->setPQ(h.p, h.q);
+	for(int k = 1; k < 4; ++k)
+	{
+		s->cp_[k] = h.cp[k - 1];
+		s->cq_[k] = h.cq[k - 1];
+		s->cset_[k] = (h.classesSet >> k) & 1;
+	}
 	s->master_.load(buf.data() + r.pos);
 	r.pos += h.masterBytes;
 	s->setStreams(reinterpret_cast<const uint64_t*>(buf.data() + r.pos));
 	r.pos += h.streams * 8;
 	s->pushSystem(reinterpret_cast<const uint32_t*>(buf.data() + r.pos));
 	r.pos += h.words * 4;
+	if(h.hasMap)
+	{
+		s->setDisorder(reinterpret_cast<const uint32_t*>(buf.data() + r.pos));
+		r.pos += h.words * 4;
+	}
 	for(uint32_t k = 0; k < h.tags; ++k)
 	{
 		Tag t;

@@ -93,6 +93,25 @@ int kpz_upload(kpz_scheduler* h, const uint32_t* lattice);
 int kpz_download(kpz_scheduler* h, uint32_t* lattice);
 /* mcs(n): the scheduler's default dynamics (DT: p=1,q=0; SCABit: p=.5,q=0) */
 int kpz_mcs(kpz_scheduler* h, int n);
+/* Disorder2 (kpz/schedulerService.cpp:48-69, kpz/schedulerService.h:185-197):
+ * kpz_set_pq_class = setPQ(p, q, n) for class n = 0..3 (class 0 is
+ * kpz_set_pq; a class never set uses class 0's rates); the map has the
+ * lattice's size and layout and holds the class of every site (ENC_LOCAL:
+ * the site's 2 bits; ENC_CBBIT: its x-slope bit); kpz_generate_disorder2
+ * draws it from the master dSFMT exactly as generateDisorder2(fill).
+ * kpz_mcs_disorder2 = mcsDisorder2SyncRng (kpz/schedulerCPUDTr.cpp:115-119,
+ * kpz/schedulerCPUCBBit.cpp:105-109, kpz/schedulerCPUCB.cpp:103-108). */
+int kpz_set_pq_class(kpz_scheduler* h, double p, double q, unsigned n);
+int kpz_generate_disorder2(kpz_scheduler* h, double fill);
+int kpz_set_disorder(kpz_scheduler* h, const uint32_t* map);
+int kpz_get_disorder(kpz_scheduler* h, uint32_t* map);
+int kpz_mcs_disorder2(kpz_scheduler* h, int n);
+/* EW fast path of the GPU DT scheduler (SyncRngEW,
+ * kpz/kpzSimulationPrimitivesTemplates.cu:199-234, selected at kpz/kpz.cu:221-228
+ * for p = q = 1): every minimum and maximum flips and no acceptance number is
+ * drawn (two draws per update); KPZ_DT / KPZ_DTR only.  kpz_mcs_pq with
+ * p = q = 1 keeps the CPU counterparts' acceptance draw. */
+int kpz_mcs_ew(kpz_scheduler* h, int n);
 /* mcsPQSyncRng(n): dynamics with the current p, q */
 int kpz_mcs_pq(kpz_scheduler* h, int n);
 int kpz_sync(kpz_scheduler* h);
