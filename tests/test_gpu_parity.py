@@ -152,7 +152,7 @@ def test_roughness_and_correlate_parity(cls, enc, lx, ly):
                                            (K.SchedulerSCA, 0, 8, 9)])
 def test_height_moments(cls, enc, lx, ly):
     # <h^k>, k = 1..7, against exact integer power sums of the oracle's heights
-    s = cls(lx, ly)
+    s = cls(lx, ly, layout=(2, 2)) if cls is K.SchedulerDT else cls(lx, ly)
     s.supplySeed(13)
     s.initHomogeneous()
     s.setPQ(0.9, 0.1)
