@@ -80,7 +80,8 @@ struct Args {
 	std::string scheduler = "default";
 	int64_t mcsThreads = 0;
 	int dd[2] = {0, 0};
-	double p = NAN, q = NAN;
+	double p[4] = {NAN, NAN, NAN, NAN}, q[4] = {NAN, NAN, NAN, NAN}; // --pq[n] per disorder class
+	double disorderFill = NAN;                                         // --disorder2 [fill]
 	int anaThreads = 1;
 	bool countTags = false, save = false;
 	double maxH = 0;
@@ -98,7 +99,8 @@ void help()
 	std::cout << "kpz_scaling mcs [x [y]] [options]\n"
 	             "  x, y: log2 lattice sizes (default 10; y defaults to x)\n"
 	             "  --scheduler dt|dtStatic|scaBit|sca|cb|default [--mcsThreads n] [--ddBlocksize bx [by]]\n"
-	             "  --pq p [q]                rates (mcsPQSyncRng)\n"
+	             "  --pq p [q]                rates (mcsPQSyncRng); --pq1..--pq3: disorder classes 1..3\n"
+	             "  --disorder2 [fill]        disorder map (default fill .5), mcsDisorder2SyncRng with --pq\n"
 	             "  --sampleRate d [start]    measurements at ceil(exp(log(t+10)+d)) (default d = 0.1)\n"
 	             "  --linSample n [start]     every n MCS\n"
 	             "  --randomSample m [start]  uniform in [m/2, 3m/2] MCS apart (master dSFMT)\n"
@@ -240,7 +242,7 @@ int parseArg(Args& A, int argc, char** argv, int a)
 		}
 		return parseScheduler(A, argc, argv, ++a);
 	}
-	else if(!strcmp(argv[a], "--disorder2") || !strcmp(argv[a], "--nschedulers") || !strcmp(argv[a], "--mcsFct") ||
+	else if(!strcmp(argv[a], "--nschedulers") || !strcmp(argv[a], "--mcsFct") ||
 	        !strcmp(argv[a], "--retryH5") || !strcmp(argv[a], "--prevMeasureOverride"))
 	{
 		std::cerr << "[EE][FrontendArgs] " << argv[a] << " is not supported by this engine.\n";
@@ -261,21 +263,34 @@ int parseArg(Args& A, int argc, char** argv, int a)
 			return -1;
 		}
 	}
+	else if(!strcmp(argv[a], "--disorder2"))
+	{
+		if(!getNum(A.disorderFill, ++a, argc, argv))
+		{
+			A.disorderFill = 0.5;
+			--a;
+		}
+	}
 	else if(!memcmp(argv[a], "--pq", 4))
 	{
-		if(strlen(argv[a]) > 4 && argv[a][4] != '0')
+		int n = 0;
+		if(strlen(argv[a]) > 4)
 		{
-			std::cerr << "[EE][FrontendArgs] --pq" << argv[a][4] << ": rate sets beyond 0 need --disorder2.\n";
-			return -1;
+			n = argv[a][4] - '0';
+			if(n < 0 || n > 3 || strlen(argv[a]) > 5)
+			{
+				std::cerr << "[EE][FrontendArgs] " << argv[a] << ": rate sets are --pq, --pq0 .. --pq3.\n";
+				return -1;
+			}
 		}
-		if(!getNum(A.p, ++a, argc, argv))
+		if(!getNum(A.p[n], ++a, argc, argv))
 		{
 			std::cerr << "--pq: Expected a double value for p.\n";
 			return -1;
 		}
-		if(!getNum(A.q, ++a, argc, argv))
+		if(!getNum(A.q[n], ++a, argc, argv))
 		{
-			A.q = 0.;
+			A.q[n] = 0.;
 			--a;
 		}
 	}
@@ -409,7 +424,7 @@ struct Engine {
 	kpz_scheduler* h = nullptr;
 	int lx = 0, ly = 0;
 	unsigned mcs = 0;
-	bool pq = false;
+	enum Mode { MCS, PQ, DISORDER } mode = MCS;
 	std::ostream* out = &std::cout;
 
 	~Engine()
@@ -511,7 +526,8 @@ struct Engine {
 	void run(unsigned n)
 	{
 		if(n)
-			check(pq ? kpz_mcs_pq(h, (int)n) : kpz_mcs(h, (int)n), "mcs");
+			check(mode == PQ ? kpz_mcs_pq(h, (int)n) : mode == DISORDER ? kpz_mcs_disorder2(h, (int)n) : kpz_mcs(h, (int)n),
+			      "mcs");
 		mcs += n;
 	}
 
@@ -695,6 +711,8 @@ int main(int argc, char** argv)
 			std::cout << "[SchedulerService][supplySeed] seed= " << A.seed << '\n';
 		Engine::check(kpz_set_seed(e.h, A.seed), "seed");
 		Engine::check(kpz_init_flat(e.h), "initHomogeneous");
+		if(!std::isnan(A.disorderFill))
+			Engine::check(kpz_generate_disorder2(e.h, A.disorderFill), "generateDisorder2");
 	}
 	else
 	{
@@ -706,14 +724,24 @@ int main(int argc, char** argv)
 			A.events.pop_front();
 		}
 	}
-	if(!std::isnan(A.p))
+	// rates and MCS function (kpz/kpzScaling.cpp:146-175, FrontendArgs::setPQ
+	// kpz/kpzFrontend.cpp:852-859): rate sets up to the first unset one; the
+	// disorder dynamics need --pq
+	if(!std::isnan(A.p[0]))
 	{
-		if(A.p == 1. && A.q == 0.)
+		if(A.p[0] == 1. && A.q[0] == 0.)
 			std::cerr << "[WW] Running dynamic pq implementation with p=1 and "
 			             "q=0, using the fixed implementation would be wore efficient in this case\n";
-		std::cout << "Rates: p = " << A.p << " , q = " << A.q << '\n';
-		Engine::check(kpz_set_pq(e.h, A.p, A.q), "setPQ");
-		e.pq = true;
+		std::cout << "Rates: p = " << A.p[0] << " , q = " << A.q[0] << '\n';
+		for(unsigned n = 0; n < 4 && !std::isnan(A.p[n]); ++n)
+			Engine::check(kpz_set_pq_class(e.h, A.p[n], A.q[n], n), "setPQ");
+		if(std::isnan(A.disorderFill))
+			e.mode = Engine::PQ;
+		else
+		{
+			std::cout << "Disorder-rates: p1 = " << A.p[1] << " , q1 = " << A.q[1] << '\n';
+			e.mode = Engine::DISORDER;
+		}
 	}
 
 	const auto t0 = std::chrono::steady_clock::now();

@@ -31,8 +31,8 @@ def test_missing_mcs():
     (["100", "8", "--scheduler", "db4"], "No such scheduler: db4"),
     (["100", "8", "--bogus"], "Unknown argument: --bogus"),
     (["100", "8", "--pq"], "Expected a double value for p"),
-    (["100", "8", "--pq1", "0.5"], "need --disorder2"),
-    (["100", "8", "--disorder2"], "not supported"),
+    (["100", "8", "--pq5", "0.5"], "rate sets are --pq"),
+    (["100", "8", "--mcsFct", "nop"], "not supported"),
     (["100", "8", "--saveH5", "x"], "not supported"),
     (["100", "8", "--corrStart"], "Expected a unsigned int for --corrStart"),
     (["100", "8", "--linSample", "0"], "--linSample"),

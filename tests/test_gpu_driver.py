@@ -112,6 +112,30 @@ def test_driver_table_matches_api(tmp_path, sched, cls, pq):
             assert len(row) == 9
 
 
+@pytest.mark.parametrize("sched,cls", [("dt", K.SchedulerDTr), ("scaBit", K.SchedulerSCABit)])
+def test_driver_disorder2(tmp_path, sched, cls):
+    # --disorder2 fill with --pq / --pq1 runs mcsDisorder2SyncRng on the map
+    # drawn right after the seed (kpz/kpzScaling.cpp:141-175)
+    if not os.path.exists(EXE):
+        pytest.fail("kpz_scaling not built")
+    lx, mcs, seed = 8, 120, 21
+    rows, stdout = run_driver(tmp_path, [mcs, lx, "--scheduler", sched, "--seed", seed, "--disorder2", 0.3,
+                                         "--pq", 0.9, 0.1, "--pq1", 0.2, 0.7])
+    assert "Disorder-rates: p1 = 0.2 , q1 = 0.7" in stdout
+    s = cls(lx, lx)
+    s.supplySeed(seed)
+    s.initHomogeneous()
+    s.generateDisorder2(0.3)
+    s.setPQ(0.9, 0.1)
+    s.setPQ(0.2, 0.7, 1)
+    done = 0
+    for row, (t, _, _) in zip(rows, expected_lines(mcs, [])):
+        assert int(row[0]) == t
+        s.mcsDisorder2SyncRng(t - done)
+        done = t
+        assert close(float(row[1]), s.roughness().w2)
+
+
 def test_driver_linear_sampling_and_save(tmp_path):
     if not os.path.exists(EXE):
         pytest.fail("kpz_scaling not built")
