@@ -258,8 +258,32 @@ def cpu_baseline(seconds=10.0, lx=12):
     n, dt = timed(cores, seconds)
     n1, dt1 = timed(1, seconds / 2)
     N = 1 << (2 * lx)
+
+    # the reference's CPU-runnable configuration: SchedulerCPUDT on 512^2,
+    # 16x16 blocks, one worker, seed 42, 100 MCS, exact w^2 after every MCS
+    m = O.Dsfmt(42)
+    st = O.Streams(1, m)
+    lat = O.flat(9, 9, O.ENC_LOCAL)
+    t0 = time.perf_counter()
+    for _ in range(100):
+        O.dt_mcs(lat, 9, 9, 4, 4, st, m, 1)
+        O.roughness(lat, O.ENC_LOCAL, 9, 9)
+    c1 = time.perf_counter() - t0
+
+    # plain random-sequential SchedulerCPU (master draws only) on 512^2
+    m = O.Dsfmt(42)
+    lat = O.flat(9, 9, O.ENC_LOCAL)
+    nrs, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 2.0:
+        O.rs_mcs(lat, 9, 9, m, 1, O.RS_MCS)
+        nrs += 1
+    rs = time.perf_counter() - t0
     return {"value": n * N / dt, "unit": "site updates/s", "cores": cores, "kind": "port",
             "value_1core": n1 * N / dt1,
+            "c1_512_dt_1core": {"site_updates_per_s": 100 * (1 << 18) / c1, "seconds": c1,
+                                "workload": "SchedulerCPUDT 512x512, 16x16 blocks, T=1, 100 MCS, w2 every MCS"},
+            "plain_rs_512_1core": {"site_updates_per_s": nrs * (1 << 18) / rs,
+                                   "workload": "SchedulerCPU random-sequential, 512x512, master dSFMT draws"},
             "sample": f"oracle SchedulerCPUDT restatement (pthreads per phase), {1 << lx}x{1 << lx}, 16x16 blocks, "
                       f"T={cores} streams: {n} MCS in {dt:.1f} s; T=1: {n1} MCS in {dt1:.1f} s"}
 

@@ -911,3 +911,69 @@ int kor_cb_mcs_rates(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
 	const thr_t th = thr_rates(r);
 	return cb_mcs(lat, lx, ly, states, T, a, c, n, upd, &th);
 }
+
+/* ======================================================================
+ * plain random-sequential SchedulerCPU: SchedulerCPU::mcs(n, pred),
+ * kpz/schedulerCPU.inl:74-88 with pqOneUpdate :21-33, pqOneOneUpdate :35-48,
+ * pqUpdate :50-71; mcsPQ / mcsDisorder2 / mcsSyncRng / mcsPQSyncRng /
+ * mcsDisorder2SyncRng, kpz/schedulerCPU.cpp:35-179 (RngPackage: three
+ * master draws, kpz/schedulerService.cpp:519-523).
+ * ====================================================================== */
+int kor_rs_mcs(uint32_t* lat, int lx, int ly, kor_dsfmt* master, int n, int mode, const kor_rates* r)
+{
+	if(lx < 2 || ly < 2 || mode < KOR_RS_MCS || mode > KOR_RS_DIS_SYNC)
+		return -1;
+	if((mode == KOR_RS_DIS || mode == KOR_RS_DIS_SYNC) && (!r || !r->disorder))
+		return -1;
+	const int X = 1 << lx, Y = 1 << ly;
+	const uint64_t N = (uint64_t)X * (uint64_t)Y;
+	const int sync = (mode == KOR_RS_SYNC || mode == KOR_RS_PQ_SYNC || mode == KOR_RS_DIS_SYNC);
+	const int ew = (mode == KOR_RS_PQ && r && r->p[0] == 1.0 && r->q[0] == 1.0);
+	for(int m = 0; m < n; ++m)
+		for(uint64_t a = 0; a < N; ++a)
+		{
+			const double rx = kor_dsfmt_close_open(master);
+			const double ry = kor_dsfmt_close_open(master);
+			const double rz = sync ? kor_dsfmt_close_open(master) : 0.0;
+			const int x = (int)(rx * X), y = (int)(ry * Y);
+			const uint32_t ic = loc_index(lx, x, y);
+			const int sc = loc_shift(x, y);
+			const int xr = (x + 1) & (X - 1), yu = (y + 1) & (Y - 1);
+			const uint32_t ir = loc_index(lx, xr, y), il = loc_index(lx, x, yu);
+			const int sr = loc_shift(xr, y), sl = loc_shift(x, yu) | 1;
+			const uint32_t c = (lat[ic] >> sc) & 3, rb = (lat[ir] >> sr) & 1, lb = (lat[il] >> sl) & 1;
+			const uint32_t config = c | (rb << 2) | (lb << 3);
+			if(mode == KOR_RS_MCS || mode == KOR_RS_SYNC)
+			{
+				if(config != 0xC)
+					continue;
+			}
+			else if(ew)
+			{
+				if(config != 0xC && config != 0x3)
+					continue;
+			}
+			else
+			{
+				const int k = (mode == KOR_RS_DIS || mode == KOR_RS_DIS_SYNC) ? (int)((r->disorder[ic] >> sc) & 3) : 0;
+				if(config == 0xC)
+				{
+					const double v = sync ? rz : kor_dsfmt_close_open(master);
+					if(v >= r->p[k])
+						continue;
+				}
+				else if(config == 0x3)
+				{
+					const double v = sync ? rz : kor_dsfmt_close_open(master);
+					if(v >= r->q[k])
+						continue;
+				}
+				else
+					continue;
+			}
+			lat[ic] ^= 3u << sc;
+			lat[ir] ^= 1u << sr;
+			lat[il] ^= 1u << sl;
+		}
+	return 0;
+}

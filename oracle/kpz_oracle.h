@@ -133,6 +133,24 @@ int kor_cbbit_mcs_rates(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t
 int kor_cb_mcs_rates(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
 		uint64_t a, uint64_t c, int n, int upd, const kor_rates* r);
 void kor_update_local_rates(uint32_t* lat, int lx, int ly, int x, int y, int upd, float rnd, const kor_rates* r);
+
+/* plain random-sequential SchedulerCPU on ENC_LOCAL, all draws from the
+ * master dSFMT (kpz/schedulerCPU.cpp:35-179, kpz/schedulerCPU.inl:21-90):
+ *   KOR_RS_MCS        mcs: x, y per update, pqOneUpdate
+ *   KOR_RS_PQ         mcsPQ: x, y; a third draw only at a minimum / maximum
+ *                     (pqUpdate; p = q = 1: pqOneOneUpdate, no third draw)
+ *   KOR_RS_SYNC       mcsSyncRng: RngPackage of 3 draws per update, pqOneUpdate
+ *   KOR_RS_PQ_SYNC    mcsPQSyncRng: RngPackage, rnd[2] against p / q
+ *   KOR_RS_DIS        mcsDisorder2: as KOR_RS_PQ with the rates of the site's class
+ *   KOR_RS_DIS_SYNC   mcsDisorder2SyncRng: as KOR_RS_PQ_SYNC with class rates
+ * rates are compared in double (no 23-bit mantissa here). */
+#define KOR_RS_MCS 0
+#define KOR_RS_PQ 1
+#define KOR_RS_SYNC 2
+#define KOR_RS_PQ_SYNC 3
+#define KOR_RS_DIS 4
+#define KOR_RS_DIS_SYNC 5
+int kor_rs_mcs(uint32_t* lat, int lx, int ly, kor_dsfmt* master, int n, int mode, const kor_rates* r);
 void kor_update_cbbit_rates(uint32_t* lat, int lx, int ly, int x, int y, float rnd, const kor_rates* r);
 
 #ifdef __cplusplus

@@ -94,6 +94,7 @@ def _declare(L):
                                    C.c_int, C.c_int, rp]
     L.kor_update_local_rates.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, rp]
     L.kor_update_cbbit_rates.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, rp]
+    L.kor_rs_mcs.argtypes = [_u32p, C.c_int, C.c_int, vp, C.c_int, C.c_int, rp]
 
 
 class Rates(C.Structure):
@@ -307,6 +308,16 @@ def cb_mcs_rates(lat, lx, ly, streams, n, upd, r):
                                 C.byref(r))
     if rc:
         raise ValueError("invalid CB size or update kind")
+
+
+RS_MCS, RS_PQ, RS_SYNC, RS_PQ_SYNC, RS_DIS, RS_DIS_SYNC = range(6)
+
+
+def rs_mcs(lat, lx, ly, master, n, mode, r=None):
+    """plain random-sequential SchedulerCPU (master dSFMT draws only)"""
+    rc = lib().kor_rs_mcs(lat, lx, ly, master.ptr, n, mode, None if r is None else C.byref(r))
+    if rc:
+        raise ValueError("invalid size, mode or missing disorder map")
 
 
 def update_local_rates(lat, lx, ly, x, y, upd, rnd, r):

@@ -234,3 +234,48 @@ def test_cbbit_and_cb_disorder_compositions(T):
                 loop(t, s, xc, x1)
     assert np.array_equal(normalized(O.heights(lat, 0, lx, ly)), hm.normalized())
     assert np.array_equal(s1.states, s2.states)
+
+
+# ---------------------------------------------------------------- plain RS
+def py_rs(hm, lx, ly, master, n, mode, p, q, dis=None):
+    """SchedulerCPU (kpz/schedulerCPU.cpp:35-179): x, y from the master per
+    update; the pq variants draw their acceptance number at a minimum or
+    maximum only (mcsPQ, mcsDisorder2) or always (the SyncRng variants)"""
+    X, Y = 1 << lx, 1 << ly
+    sync = mode in (O.RS_SYNC, O.RS_PQ_SYNC, O.RS_DIS_SYNC)
+    for _ in range(n):
+        for _a in range(X * Y):
+            x = int(master.close_open() * X)
+            y = int(master.close_open() * Y)
+            z = master.close_open() if sync else None
+            k = hm.kind(x, y)
+            if mode in (O.RS_MCS, O.RS_SYNC):
+                hm.update(x, y, True, False)
+                continue
+            c = class_local(dis, lx, x, y) if dis is not None else 0
+            if k == 0:
+                continue
+            if mode == O.RS_PQ and p[0] == 1.0 and q[0] == 1.0:
+                hm.update(x, y, True, True)
+                continue
+            v = z if sync else master.close_open()
+            hm.update(x, y, k == 1 and v < p[c], k == -1 and v < q[c])
+
+
+@pytest.mark.parametrize("mode,p,q", [(O.RS_MCS, [1.0], [0.0]), (O.RS_PQ, [0.7], [0.4]), (O.RS_PQ, [1.0], [1.0]),
+                                      (O.RS_SYNC, [1.0], [0.0]), (O.RS_PQ_SYNC, [0.6], [0.3]),
+                                      (O.RS_DIS, P4, Q4), (O.RS_DIS_SYNC, P4, Q4)])
+def test_plain_rs_scheduler(mode, p, q):
+    lx, ly = 5, 4
+    dis = None
+    if mode in (O.RS_DIS, O.RS_DIS_SYNC):
+        dis = np.random.default_rng(5).integers(0, 1 << 32, 1 << (lx + ly - 4), dtype=np.uint64).astype(np.uint32)
+    r = O.rates(p, q, dis)
+    m1, m2 = O.Dsfmt(23), O.Dsfmt(23)
+    lat = O.flat(lx, ly, O.ENC_LOCAL)
+    hm = HeightLattice(O.heights(lat, 0, lx, ly))
+    O.rs_mcs(lat, lx, ly, m1, 3, mode, r)
+    py_rs(hm, lx, ly, m2, 3, mode, p, q, dis)
+    assert np.array_equal(normalized(O.heights(lat, 0, lx, ly)), hm.normalized())
+    assert m1.close_open() == m2.close_open()
+    assert O.consistent(lat, 0, lx, ly)
