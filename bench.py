@@ -237,6 +237,26 @@ def run_serve_bench(K, window_s, rate=200.0, seed=5, kind_name="scaBit", budget=
 import numpy as np  # noqa: E402
 
 
+def l1_layout(lx, ly, lmin=2):
+    """Kmc::optimizeForL1 (inc/cacheLayout.h:62-94) for SchedulerCPUDB: shrink
+    the larger block side until the block, at 2 bits per site, fits half of
+    cpu0's L1 data cache (32 KiB assumed where sysfs does not say)."""
+    kib = 32
+    try:
+        txt = open("/sys/devices/system/cpu/cpu0/cache/index0/size").read().strip()
+        kib = int("".join(ch for ch in txt if ch.isdigit()) or 32)
+    except OSError:
+        pass
+    use = int(math.log2(kib)) + 12
+    lm = [lx, ly]
+    while lm[0] + lm[1] + 1 > use:
+        i = 0 if (lm[0] >= lm[1] and lm[0] > lmin) else 1
+        if lm[i] <= lmin:
+            break
+        lm[i] -= 1
+    return lm[0], lm[1]
+
+
 def cpu_baseline(seconds=10.0, lx=12):
     """The oracle's restatement of SchedulerCPUDT on the host cores: T worker
     threads = T SLCG streams, spawned and joined per domain set, 16x16-site
@@ -278,12 +298,30 @@ def cpu_baseline(seconds=10.0, lx=12):
         O.rs_mcs(lat, 9, 9, m, 1, O.RS_MCS)
         nrs += 1
     rs = time.perf_counter() - t0
+
+    # SchedulerCPUDB (dead border, dynamic origin) on 512^2 with the block
+    # layout its layoutFromL1 picks on this host (inc/cacheLayout.h:84-94),
+    # as many workers as it has blocks (at most `cores`)
+    lbx, lby = l1_layout(9, 9)
+    tdb = max(1, min(cores, 1 << (18 - lbx - lby)))
+    m = O.Dsfmt(42)
+    st = O.Streams(tdb, m)
+    lat = O.flat(9, 9, O.ENC_LOCAL)
+    ndb, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 2.0:
+        O.db_mcs(lat, 9, 9, lbx, lby, st, m, 1, 0, tdb)
+        ndb += 1
+    tdb_s = time.perf_counter() - t0
+    db_upd = ((1 << lbx) - 1) * ((1 << lby) - 1) << (18 - lbx - lby)
     return {"value": n * N / dt, "unit": "site updates/s", "cores": cores, "kind": "port",
             "value_1core": n1 * N / dt1,
             "c1_512_dt_1core": {"site_updates_per_s": 100 * (1 << 18) / c1, "seconds": c1,
                                 "workload": "SchedulerCPUDT 512x512, 16x16 blocks, T=1, 100 MCS, w2 every MCS"},
             "plain_rs_512_1core": {"site_updates_per_s": nrs * (1 << 18) / rs,
                                    "workload": "SchedulerCPU random-sequential, 512x512, master dSFMT draws"},
+            "db_512": {"site_updates_per_s": ndb * db_upd / tdb_s, "threads": tdb,
+                       "workload": f"SchedulerCPUDB 512x512, {1 << lbx}x{1 << lby} blocks (layoutFromL1 on this "
+                                   f"host), dead border, dynamic origin, T={tdb}"},
             "sample": f"oracle SchedulerCPUDT restatement (pthreads per phase), {1 << lx}x{1 << lx}, 16x16 blocks, "
                       f"T={cores} streams: {n} MCS in {dt:.1f} s; T=1: {n1} MCS in {dt1:.1f} s"}
 

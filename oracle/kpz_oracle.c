@@ -719,6 +719,112 @@ int kor_dt_mcs_threaded(uint32_t* lat, int lx, int ly, int lbx, int lby,
 }
 
 /* ======================================================================
+ * SchedulerCPUDB<Rng>::mcs, kpz/schedulerCPUDB.cpp:65-88, with the
+ * dead-border worker Func_MCSBase::exec, kpz/schedulerCPUDB_nested.h:56-80,
+ * the dynamic origin Kmc::DynamicOriginDD<2>, inc/dynamicOriginDD.h:51-75,
+ * and the block layout of Kmc::BlockDD::setLayout, inc/blockDD.h:137-160.
+ * Per MCS the origin is drawn from the master (x first, then y:
+ * o = (int)(close_open * 2^(lbd - lORest)), origin = o << lORest); every
+ * block i (x block = low bits) then gets (2^lbx - 1)(2^lby - 1) updates at
+ * ((int)(close_open*(2^lb - 1)) + offset) & (L - 1), x drawn before y,
+ * pqOneUpdate (DEAD_BORDER_DIM = 1, kpz/kpzConst.h:59).  Worker t walks
+ * blocks t, t+T, ... on its own stream (DDIterator::next(T),
+ * inc/ddIterators.h:47-69).  The layout is explicit here (contract): the
+ * reference derives it from the host's L1 size (layoutFromL1,
+ * inc/cacheLayout.h) unless setLayout is called.  Blocks are disjoint
+ * including the one-site border every update reaches, so the order of the
+ * workers does not matter.  close_open() of an SLCG stream is draw * 2^-32
+ * (contract), so (int)(close_open()*m) = (draw * m) >> 32 exactly.
+ * ====================================================================== */
+typedef struct {
+	uint32_t* lat;
+	int lx, ly, lbx, lby;
+	int org[2];
+	uint64_t* states;
+	int64_t T, w, nw, nb;
+	uint64_t a, c;
+	thr_t th;
+} db_work;
+
+static void* db_worker(void* arg)
+{
+	db_work* W = (db_work*)arg;
+	const int mx = (1 << W->lx) - 1, my = (1 << W->ly) - 1;
+	const uint64_t ax = (1u << W->lbx) - 1, ay = (1u << W->lby) - 1;
+	const int mbx = (1 << (W->lx - W->lbx)) - 1;
+	for(int64_t t = W->w; t < W->T; t += W->nw)
+	{
+		uint64_t s = W->states[t];
+		for(int64_t i = t; i < W->nb; i += W->T)
+		{
+			const int offx = (int)((i & mbx) << W->lbx) + W->org[0];
+			const int offy = (int)((i >> (W->lx - W->lbx)) << W->lby) + W->org[1];
+			for(uint64_t k = 0; k < ax * ay; ++k)
+			{
+				const int x = ((int)(((uint64_t)kor_slcg_draw(&s, W->a, W->c) * ax) >> 32) + offx) & mx;
+				const int y = ((int)(((uint64_t)kor_slcg_draw(&s, W->a, W->c) * ay) >> 32) + offy) & my;
+				upd_local(W->lat, W->lx, W->ly, x, y, KOR_UPD_KPZ, 0, &W->th);
+			}
+		}
+		W->states[t] = s;
+	}
+	return NULL;
+}
+
+int kor_db_mcs(uint32_t* lat, int lx, int ly, int lbx, int lby, int lorest,
+		uint64_t* states, int64_t T, uint64_t a, uint64_t c,
+		kor_dsfmt* master, int n, int nthreads)
+{
+	if(lbx < 2 || lby < 2 || lbx > lx || lby > ly || lbx > 16 || lby > 16 || lorest < 0 || lorest > lbx ||
+	   lorest > lby || T < 1 || nthreads < 1)
+		return -1;
+	const int64_t nb = 1ll << (lx - lbx + ly - lby);
+	if(T > nb)
+		return -1; /* a worker without a block would run on an unset iterator */
+	pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+	db_work* wk = (db_work*)malloc(sizeof(db_work) * (size_t)nthreads);
+	const int lbd[2] = {lbx, lby};
+	for(int m = 0; m < n; ++m)
+	{
+		int org[2];
+		for(int d = 0; d < 2; ++d)
+		{
+			const int eff = 1 << (lbd[d] - lorest);
+			org[d] = (int)(kor_dsfmt_close_open(master) * (double)eff) << lorest;
+		}
+		for(int w = 0; w < nthreads; ++w)
+		{
+			db_work* W = &wk[w];
+			W->lat = lat;
+			W->lx = lx;
+			W->ly = ly;
+			W->lbx = lbx;
+			W->lby = lby;
+			W->org[0] = org[0];
+			W->org[1] = org[1];
+			W->states = states;
+			W->T = T;
+			W->w = w;
+			W->nw = nthreads;
+			W->nb = nb;
+			W->a = a;
+			W->c = c;
+			W->th = thr_pq(1.0, 0.0);
+			if(nthreads > 1)
+				pthread_create(&th[w], NULL, db_worker, W);
+			else
+				db_worker(W);
+		}
+		if(nthreads > 1)
+			for(int w = 0; w < nthreads; ++w)
+				pthread_join(th[w], NULL);
+	}
+	free(th);
+	free(wk);
+	return 0;
+}
+
+/* ======================================================================
  * checkerboard SCA on ENC_CBBIT: SchedulerCPUCBBit<Rng>::doMcs,
  * kpz/schedulerCPUCBBit.cpp:43-79, Func_MCS::exec
  * kpz/schedulerCPUCBBit_nested.h:80-95 (rows [yMin, yCenter) then

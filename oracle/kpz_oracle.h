@@ -21,7 +21,7 @@
  *   update rules         brute-force explicit-height octahedron simulator
  *   half-sweeps          per-site scalar equivalence, order independence
  *   heights/roughness    closed forms (flat IC), brute force on tiny lattices
- *   DT/DTr/CBBit/CB MCS  composition of the pinned pieces above plus
+ *   DT/DTr/CBBit/CB/DB   composition of the pinned pieces above plus
  *                        invariants (consistency, height-change accounting)
  */
 #ifndef KPZ_ORACLE_H
@@ -106,6 +106,13 @@ void kor_dt_block_offset(int lx, int ly, int lbx, int lby, int domain_set,
 int kor_dt_mcs_threaded(uint32_t* lat, int lx, int ly, int lbx, int lby,
 		uint64_t* states, int64_t T, uint64_t a, uint64_t c,
 		kor_dsfmt* master, int n, int random_origin, int upd, double p, double q, int nthreads);
+
+/* SchedulerCPUDB (dead border, dynamic origin; pqOneUpdate only) with an
+ * explicit block layout (lbx, lby >= 2) and origin restriction lorest;
+ * T streams <= number of blocks; nthreads pthreads per MCS (1 = inline) */
+int kor_db_mcs(uint32_t* lat, int lx, int ly, int lbx, int lby, int lorest,
+		uint64_t* states, int64_t T, uint64_t a, uint64_t c,
+		kor_dsfmt* master, int n, int nthreads);
 
 /* SchedulerCPUCBBit: mode 0 = pSyncRngUpdateVL3 (p=.5, q=0), 1 = per-site pq */
 int kor_cbbit_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,

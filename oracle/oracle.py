@@ -73,6 +73,8 @@ def _declare(L):
     L.kor_dt_mcs_threaded.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, _u64p, C.c_int64,
                                       C.c_uint64, C.c_uint64, vp, C.c_int, C.c_int, C.c_int, C.c_double,
                                       C.c_double, C.c_int]
+    L.kor_db_mcs.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _u64p, C.c_int64,
+                             C.c_uint64, C.c_uint64, vp, C.c_int, C.c_int]
     L.kor_dt_block_offset.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.POINTER(C.c_uint), C.c_int, C.POINTER(C.c_int)]
     L.kor_cbbit_mcs.argtypes = [_u32p, C.c_int, C.c_int, _u64p, C.c_int64, C.c_uint64, C.c_uint64,
@@ -257,6 +259,14 @@ def dt_mcs_threaded(lat, lx, ly, lbx, lby, streams, master, n, nthreads, random_
                                    master.ptr, n, int(random_origin), upd, p, q, int(nthreads))
     if rc:
         raise ValueError("invalid DT layout")
+
+
+def db_mcs(lat, lx, ly, lbx, lby, streams, master, n, lorest=0, nthreads=1):
+    """SchedulerCPUDB::mcs (dead border, dynamic origin) with an explicit layout."""
+    rc = lib().kor_db_mcs(lat, lx, ly, lbx, lby, lorest, streams.states, streams.T, streams.a, streams.c,
+                          master.ptr, n, int(nthreads))
+    if rc:
+        raise ValueError("invalid DB layout or stream count")
 
 
 def cbbit_mcs(lat, lx, ly, streams, n, mode, p=0.5, q=0.0):

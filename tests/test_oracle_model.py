@@ -1,6 +1,7 @@
 """The oracle's lattice, update rules and MCS compositions against an
 independently written explicit-height model (tests/heightmodel.py) and
 closed forms.  Small sizes only: these run on the CPU in seconds."""
+import math
 import numpy as np
 import pytest
 
@@ -233,6 +234,55 @@ def test_dt_composition(random_origin, upd, p, q, T):
     assert np.array_equal(s1.states, s2.states)
     assert m1.close_open() == m2.close_open()
     assert O.consistent(lat, 0, lx, ly)
+
+
+def py_db(hm, lx, ly, lbx, lby, lorest, st, master, n):
+    """SchedulerCPUDB::mcs written from kpz/schedulerCPUDB.cpp:65-88,
+    kpz/schedulerCPUDB_nested.h:56-80 and inc/dynamicOriginDD.h:51-75 on
+    explicit heights: dead border of one site, origin drawn per MCS."""
+    X, Y = 1 << lx, 1 << ly
+    bx, by = 1 << lbx, 1 << lby
+    nbx = X // bx
+    nb = nbx * (Y // by)
+    for _ in range(n):
+        ox = int(master.close_open() * (1 << (lbx - lorest))) << lorest
+        oy = int(master.close_open() * (1 << (lby - lorest))) << lorest
+        for t in range(st.T):
+            for i in range(t, nb, st.T):
+                offx = (i % nbx) * bx + ox
+                offy = (i // nbx) * by + oy
+                for _k in range((bx - 1) * (by - 1)):
+                    # (int)(close_open() * m) with close_open() = draw / 2^32
+                    x = (math.floor(st.draw(t) / 4294967296.0 * (bx - 1)) + offx) % X
+                    y = (math.floor(st.draw(t) / 4294967296.0 * (by - 1)) + offy) % Y
+                    hm.update(x, y, True, False)
+
+
+@pytest.mark.parametrize("lx,ly,lbx,lby,lorest,T,nthreads", [
+    (6, 5, 2, 2, 0, 1, 1), (6, 5, 3, 2, 0, 3, 1), (6, 6, 3, 3, 1, 4, 2), (5, 4, 5, 4, 0, 1, 1),
+    (6, 5, 2, 3, 2, 16, 3)])
+def test_db_composition(lx, ly, lbx, lby, lorest, T, nthreads):
+    m1, m2 = O.Dsfmt(7 + T), O.Dsfmt(7 + T)
+    s1, s2 = O.Streams(T, m1), O.Streams(T, m2)
+    lat = O.flat(lx, ly, 0)
+    hm = HeightLattice(O.heights(lat, 0, lx, ly))
+    O.db_mcs(lat, lx, ly, lbx, lby, s1, m1, 3, lorest, nthreads)
+    py_db(hm, lx, ly, lbx, lby, lorest, s2, m2, 3)
+    assert np.array_equal(normalized(O.heights(lat, 0, lx, ly)), hm.normalized())
+    assert np.array_equal(s1.states, s2.states)
+    assert m1.close_open() == m2.close_open()
+    assert O.consistent(lat, 0, lx, ly)
+    # the heights moved: (2^lb - 1)^2 updates per block per MCS
+    assert np.any(normalized(O.heights(lat, 0, lx, ly)) != normalized(O.heights(O.flat(lx, ly, 0), 0, lx, ly)))
+
+
+def test_db_rejects_bad_layouts():
+    m = O.Dsfmt(1)
+    lat = O.flat(5, 4, 0)
+    for args in [(1, 2, 0, 1), (2, 2, 3, 1), (6, 2, 0, 1), (3, 3, 0, 9)]:  # lbx, lby, lorest, T (8 blocks)
+        st = O.Streams(args[3], m)
+        with pytest.raises(ValueError):
+            O.db_mcs(lat, 5, 4, args[0], args[1], st, m, 1, args[2])
 
 
 def py_cbbit(hm, lx, ly, st, n, mode, p, q):
