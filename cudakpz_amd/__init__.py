@@ -176,6 +176,14 @@ class SchedulerService:
         L.kpz_save_state.argtypes = [C.c_void_p, C.c_char_p, C.c_uint32]
         _check(L.kpz_save_state(self._h, os.fsencode(path), int(aux)))
 
+    def saveH5(self, file, prefix="", prevMeasureOverride=0):
+        """SchedulerService::writeH5: the reference's HDF5 datasets
+        (<prefix>.system, .disorder, .dsfmt, tags with .corrTag / avgH,
+        /custom attributes) plus the SLCG stream states, in
+        <file>_0_0_0.h5 (or <file> if it ends in .h5)."""
+        L = lib_h5()
+        _check(L.kpz_write_h5(self._h, os.fsencode(file), prefix.encode(), int(prevMeasureOverride)))
+
     # -- lifetime
     def close(self):
         if getattr(self, "_h", None):
@@ -226,6 +234,14 @@ class SchedulerService:
             L = lib()
             L.kpz_set_pq_class.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_uint]
             _check(L.kpz_set_pq_class(self._h, float(p), float(q), int(n)))
+
+    def rates(self, n=0):
+        """(rateP(n), rateQ(n)): the rates disorder class n uses."""
+        L = lib()
+        L.kpz_get_pq_class.argtypes = [C.c_void_p, C.c_uint, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        p, q = C.c_double(), C.c_double()
+        _check(L.kpz_get_pq_class(self._h, int(n), C.byref(p), C.byref(q)))
+        return p.value, q.value
 
     # -- Disorder2 (kpz/schedulerService.cpp:48-69)
     def generateDisorder2(self, fill):
@@ -663,3 +679,42 @@ def load_state(path, device=-1):
     _check(L.kpz_size(h, C.byref(lx), C.byref(ly)))
     cls = {KPZ_DT: SchedulerDT, KPZ_DTR: SchedulerDTr, KPZ_SCABIT: SchedulerSCABit, KPZ_SCA: SchedulerSCA}[kind]
     return cls._wrap(h, kind, lx.value, ly.value), int(aux.value)
+
+
+H5_LIB_PATH = os.path.join(_HERE, "libkpz_amd_h5.so")
+_LIB_H5 = None
+H5_MASTER, H5_STREAMS, H5_DISORDER, H5_RATES, H5_PMO = 1, 2, 4, 8, 16
+
+
+def lib_h5():
+    """libkpz_amd_h5.so (HDF5 checkpoints, include/kpz_amd_h5.h)."""
+    global _LIB_H5
+    if _LIB_H5 is None:
+        lib()  # the engine library first: the HDF5 library links against it
+        if not os.path.exists(H5_LIB_PATH):
+            raise KpzError(-2, f"{H5_LIB_PATH} is not built (needs the HDF5 C library); run cudakpz_amd.build()")
+        L = C.CDLL(H5_LIB_PATH)
+        L.kpz_write_h5.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_uint32]
+        L.kpz_read_h5.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(_Params), C.POINTER(C.c_void_p),
+                                  C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        _LIB_H5 = L
+    return _LIB_H5
+
+
+def load_h5(file, kind, prefix="", threads=0, layout=(4, 4), skip=0, device=-1):
+    """SchedulerService::readH5 on a new scheduler of `kind` (KPZ_*): the size
+    and MCS count come from the file.  Returns (scheduler, prevMeasureOverride,
+    restored), restored being H5_* bits (H5_STREAMS clear: the streams were
+    re-seeded from the restored master, as the reference does)."""
+    L = lib_h5()
+    p = _Params(int(threads), int(layout[0]), int(layout[1]), int(skip), int(device))
+    h = C.c_void_p()
+    pmo, restored = C.c_uint32(), C.c_uint32()
+    _check(L.kpz_read_h5(os.fsencode(file), prefix.encode(), int(kind), C.byref(p), C.byref(h), C.byref(pmo),
+                         C.byref(restored)))
+    Lm = lib()
+    Lm.kpz_size.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lx, ly = C.c_int(), C.c_int()
+    _check(Lm.kpz_size(h, C.byref(lx), C.byref(ly)))
+    cls = {KPZ_DT: SchedulerDT, KPZ_DTR: SchedulerDTr, KPZ_SCABIT: SchedulerSCABit, KPZ_SCA: SchedulerSCA}[kind]
+    return cls._wrap(h, kind, lx.value, ly.value), int(pmo.value), int(restored.value)

@@ -1,15 +1,12 @@
 // extern "C" boundary over the C++ schedulers (include/kpz_amd.h).
 #include "../../include/kpz_amd.h"
 #include "kpz_batch.hpp"
+#include "kpz_capi_internal.hpp"
 #include "kpz_scheduler.hpp"
 #include "kpz_server.hpp"
 
 #include <cstring>
 #include <string>
-
-struct kpz_scheduler {
-	std::unique_ptr<kpz::SchedulerService> s;
-};
 
 struct kpz_batch {
 	std::unique_ptr<kpz::Batch> b;
@@ -21,33 +18,21 @@ struct kpz_server {
 
 namespace {
 thread_local std::string g_err;
+}
 
-int fail(int code, const std::string& msg)
+int kpz::capiFail(int code, const std::string& msg)
 {
 	g_err = msg;
 	return code;
 }
 
+namespace {
+int fail(int code, const std::string& msg) { return kpz::capiFail(code, msg); }
+
 template<class F>
 int guard(F&& f)
 {
-	try
-	{
-		f();
-		return KPZ_OK;
-	}
-	catch(const kpz::Error& e)
-	{
-		return fail(e.code, e.what());
-	}
-	catch(const std::bad_alloc&)
-	{
-		return fail(KPZ_ENOMEM, "host allocation failed");
-	}
-	catch(const std::exception& e)
-	{
-		return fail(KPZ_EDEVICE, e.what());
-	}
+	return kpz::capiGuard(std::forward<F>(f));
 }
 
 #define HANDLE(h)                                     \
@@ -236,6 +221,18 @@ int kpz_set_pq_class(kpz_scheduler* h, double p, double q, unsigned n)
 		if(!h->s->setPQ(p, q, n))
 			throw kpz::Error(KPZ_EINVAL, "rates must lie in [0, 1]");
 	});
+}
+
+int kpz_get_pq_class(const kpz_scheduler* h, unsigned n, double* p, double* q)
+{
+	HANDLE(h);
+	if(n >= 4)
+		return fail(KPZ_EINVAL, "disorder class must be 0..3");
+	if(p)
+		*p = h->s->rateP(n);
+	if(q)
+		*q = h->s->rateQ(n);
+	return KPZ_OK;
 }
 
 int kpz_generate_disorder2(kpz_scheduler* h, double fill)

@@ -13,6 +13,9 @@
 // slope overlap are exact, the higher moments are double per row summed in
 // long double.
 #include "kpz_amd.h"
+#ifdef KPZ_WITH_H5
+#include "kpz_amd_h5.h"
+#endif
 
 #include <algorithm>
 #include <chrono>
@@ -85,7 +88,7 @@ struct Args {
 	int anaThreads = 1;
 	bool countTags = false, save = false;
 	double maxH = 0;
-	std::string file, saveState, loadState;
+	std::string file, saveState, loadState, saveH5, loadH5;
 	bool haveSeed = false;
 	uint64_t seed = 0;
 	int device = -1;
@@ -112,6 +115,9 @@ void help()
 	             "  -f, --file name           output table to a file (-- = stdout)\n"
 	             "  --saveState file          checkpoint at the end (lattice, RNG states, tags)\n"
 	             "  --loadState file          resume from a checkpoint (continues bit for bit)\n"
+	             "  --saveH5 prefix           HDF5 checkpoint <prefix>_0_0_0.h5 (reference dataset names\n"
+	             "                            + SLCG stream states)\n"
+	             "  --loadH5 prefix           resume from an HDF5 checkpoint (size from the file)\n"
 	             "  --seed s                  64-bit seed (default: /dev/urandom and time)\n"
 	             "  --device n                HIP device\n";
 }
@@ -250,9 +256,18 @@ int parseArg(Args& A, int argc, char** argv, int a)
 	}
 	else if(!strcmp(argv[a], "--saveH5") || !strcmp(argv[a], "--loadH5"))
 	{
-		std::cerr << "[EE][FrontendArgs] " << argv[a] << ": libSplash/HDF5 checkpoints are not supported by "
-		          << "this engine; use --saveState / --loadState.\n";
+#ifdef KPZ_WITH_H5
+		std::string& dst = (argv[a][2] == 's') ? A.saveH5 : A.loadH5;
+		if(!getStr(dst, ++a, argc, argv))
+		{
+			std::cerr << "Expected a HDF5 filname-prefix for " << argv[a - 1] << ".\n";
+			return -1;
+		}
+#else
+		std::cerr << "[EE][FrontendArgs] " << argv[a] << ": this build has no HDF5 library; use --saveState / "
+		          << "--loadState.\n";
 		return -1;
+#endif
 	}
 	else if(!strcmp(argv[a], "--saveState") || !strcmp(argv[a], "--loadState"))
 	{
@@ -666,13 +681,19 @@ int main(int argc, char** argv)
 	kpz_params prm{A.mcsThreads, A.dd[0], A.dd[1], 0, A.device};
 	Engine e;
 	uint32_t lastMeasurement = 0;
-	if(A.loadState.empty())
+	if(!A.loadState.empty() && !A.loadH5.empty())
+	{
+		std::cerr << "[EE][FrontendArgs] --loadState and --loadH5 exclude each other.\n";
+		return 1;
+	}
+	const bool resumed = !A.loadState.empty() || !A.loadH5.empty();
+	if(!resumed)
 	{
 		e.lx = A.lx;
 		e.ly = A.ly;
 		Engine::check(kpz_create(kind, A.lx, A.ly, &prm, &e.h), "create scheduler");
 	}
-	else
+	else if(!A.loadState.empty())
 	{
 		// --loadH5 path (kpz/kpzScaling.cpp:118-139): size, kind, RNG and tags
 		// come from the file
@@ -681,6 +702,23 @@ int main(int argc, char** argv)
 		e.mcs = kpz_mcs_count(e.h);
 		std::cout << "[readState] Loading state at t = " << e.mcs << " MCS\n";
 	}
+#ifdef KPZ_WITH_H5
+	else
+	{
+		// SchedulerService::readH5 (kpz/kpzScaling.cpp:121-136): the size comes
+		// from the file, the kind and worker layout from the command line
+		uint32_t restored = 0;
+		Engine::check(kpz_read_h5(A.loadH5.c_str(), "", kind, &prm, &e.h, &lastMeasurement, &restored), "read HDF5");
+		Engine::check(kpz_size(e.h, &e.lx, &e.ly), "size");
+		e.mcs = kpz_mcs_count(e.h);
+		std::cout << "[readH5] Loading Hdf5 file at t = " << e.mcs << " MCS\n";
+		if(restored & KPZ_H5_MASTER)
+			std::cout << "[readH5] restored dSFMT state from file.\n";
+		if(!(restored & KPZ_H5_STREAMS))
+			std::cout << "[retrieveH5][WW] Failed to restore GPU RNG state from file, randomizing. Check if dSFMT "
+			             "was restored.\n";
+	}
+#endif
 	std::cout << "mcsScale " << kpz_mcs_scale(e.h) << '\n';
 
 	std::ofstream fileOut;
@@ -697,7 +735,7 @@ int main(int argc, char** argv)
 
 	// seeding (SchedulerService::initURandom / supplySeed,
 	// kpz/schedulerService.cpp:460-495), then initHomogeneous + randomize
-	if(A.loadState.empty())
+	if(!resumed)
 	{
 		if(!A.haveSeed)
 		{
@@ -756,7 +794,7 @@ int main(int argc, char** argv)
 	// a resumed run continues the sampling schedule from the last measurement
 	// stored with the state (overridePrevMeasurementTime, kpz/kpzFrontend.cpp:752-788)
 	// (random sampling draws from the master and is not replayed)
-	if(!A.loadState.empty() && A.current.kind != Sampling::RANDOM && lastMeasurement <= e.mcs &&
+	if(resumed && A.current.kind != Sampling::RANDOM && lastMeasurement <= e.mcs &&
 	   nextEval(A.current, lastMeasurement, e) >= e.mcs)
 		prev = lastMeasurement;
 	bool timeout = false;
@@ -780,6 +818,11 @@ int main(int argc, char** argv)
 		std::cout << "TIMEOUT " << e.mcs << '\n';
 	if(!A.saveState.empty())
 		Engine::check(kpz_save_state(e.h, A.saveState.c_str(), prev), "save state");
+#ifdef KPZ_WITH_H5
+	// writeH5 + the prevMeasureOverride attribute (kpz/kpzScaling.cpp:225-229)
+	if(!A.saveH5.empty())
+		Engine::check(kpz_write_h5(e.h, A.saveH5.c_str(), "", prev), "write HDF5");
+#endif
 	if(A.save)
 	{
 		std::ostringstream ss;

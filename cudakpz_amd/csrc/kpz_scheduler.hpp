@@ -142,6 +142,14 @@ public:
 	// ---- checkpoint / resume (kpz_state.cpp): everything needed to continue
 	// bit for bit; aux is 32 bits of caller data stored with it
 	int kind() const { return kind_; }
+	// live correlator tags as host data, oldest first, and the way back
+	struct TagImage {
+		unsigned int mcs, stop;
+		std::vector<uint32_t> snap; // reference-layout lattice at the tag's MCS
+		int64_t sumH;               // sum of the snapshot's heights (exact)
+	};
+	std::vector<TagImage> exportTags();
+	void importTag(unsigned int mcs, unsigned int stop, const uint32_t* snap);
 	void saveState(const char* path, uint32_t aux);
 	static std::unique_ptr<SchedulerService> loadState(const char* path, int device, uint32_t* aux);
 

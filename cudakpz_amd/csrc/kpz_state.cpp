@@ -87,6 +87,36 @@ struct Header {
 
 } // namespace
 
+std::vector<SchedulerService::TagImage> SchedulerService::exportTags()
+{
+	const uint64_t words = size_.sizeW();
+	std::vector<TagImage> out;
+	for(const Tag& t : tags_)
+	{
+		TagImage im;
+		im.mcs = t.mcs;
+		im.stop = t.stop;
+		im.snap.resize((size_t)words);
+		check(hipMemcpyAsync(im.snap.data(), t.snap->get(), words * 4, hipMemcpyDeviceToHost, stream_), "tag D2H");
+		sync();
+		im.sumH = correlateWith(*t.snap).sumHs;
+		out.push_back(std::move(im));
+	}
+	return out;
+}
+
+void SchedulerService::importTag(unsigned int mcs, unsigned int stop, const uint32_t* snap)
+{
+	const uint64_t words = size_.sizeW();
+	Tag t;
+	t.mcs = mcs;
+	t.stop = stop;
+	t.snap = std::make_unique<DevBuf>(words * 4);
+	check(hipMemcpyAsync(t.snap->get(), snap, words * 4, hipMemcpyHostToDevice, stream_), "tag H2D");
+	sync();
+	tags_.push_back(std::move(t));
+}
+
 void SchedulerService::saveState(const char* path, uint32_t aux)
 {
 	if(kind_ < 0)

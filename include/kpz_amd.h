@@ -102,6 +102,9 @@ int kpz_mcs(kpz_scheduler* h, int n);
  * kpz_mcs_disorder2 = mcsDisorder2SyncRng (kpz/schedulerCPUDTr.cpp:115-119,
  * kpz/schedulerCPUCBBit.cpp:105-109, kpz/schedulerCPUCB.cpp:103-108). */
 int kpz_set_pq_class(kpz_scheduler* h, double p, double q, unsigned n);
+/* rateP(n) / rateQ(n) (kpz/schedulerService.h:193-194): the rates class n
+ * uses (p, q may each be NULL) */
+int kpz_get_pq_class(const kpz_scheduler* h, unsigned n, double* p, double* q);
 int kpz_generate_disorder2(kpz_scheduler* h, double fill);
 int kpz_set_disorder(kpz_scheduler* h, const uint32_t* map);
 int kpz_get_disorder(kpz_scheduler* h, uint32_t* map);

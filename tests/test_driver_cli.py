@@ -33,7 +33,10 @@ def test_missing_mcs():
     (["100", "8", "--pq"], "Expected a double value for p"),
     (["100", "8", "--pq5", "0.5"], "rate sets are --pq"),
     (["100", "8", "--mcsFct", "nop"], "not supported"),
-    (["100", "8", "--saveH5", "x"], "not supported"),
+    (["100", "8", "--saveH5"], "Expected a HDF5 filname-prefix for --saveH5"),
+    (["100", "8", "--loadH5", "/nonexistent/ckpt"], "cannot open /nonexistent/ckpt_0_0_0.h5"),
+    (["100", "8", "--loadH5", "/nonexistent/ckpt.h5"], "cannot open /nonexistent/ckpt.h5"),
+    (["100", "8", "--loadH5", "a", "--loadState", "b"], "exclude each other"),
     (["100", "8", "--corrStart"], "Expected a unsigned int for --corrStart"),
     (["100", "8", "--linSample", "0"], "--linSample"),
     (["100", "8", "--maxH", "-1"], "positive floating point value for --maxH"),
