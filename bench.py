@@ -366,14 +366,29 @@ def main():
         prof = json.load(open(tfile)).get(key, {})
     traffic = prof.get("bytes_per_launch")
     # VALU issue: wave instructions per launch (rocprof SQ_INSTS_VALU) x 64
-    # lanes over the live launch time, against 256 CU x 4 SIMD x 16 lanes x 2.4 GHz
+    # lanes over the live launch time, against the integer VALU rate measured
+    # here (independent v_xad_u32 chains) and the nominal 256 CU x 4 SIMD x 16
+    # lanes x 2.4 GHz
     valu = None
     if prof.get("valu_insts_per_launch"):
         lane_ops = prof["valu_insts_per_launch"] * 64.0
         rate = lane_ops / (avg_launch_ms * 1e-3)
-        valu = {"lane_ops_per_launch": lane_ops, "achieved_T_lane_ops": rate / 1e12,
-                "peak_T_lane_ops": VALU_PEAK_LANE_OPS / 1e12, "frac": rate / VALU_PEAK_LANE_OPS,
-                "ops_per_update": lane_ops / (N / 4)}
+        slow, fast = K.valu_peak()
+        # issue-time bound of one launch from the instruction mix: the update
+        # loop of dt_phase_kernel<4,4,0,false> (compiled gfx950 code, per
+        # update) issues 11 slow-class instructions (v_mul_lo_u32, v_add3_u32,
+        # shifts, v_lshl_or_b32), 2 v_mad_u64_u32 (1.4 slow-class slots each,
+        # measured) and 7 fast-class ones (xor, sub, or, bitop3); the rest of
+        # the kernel (staging, write-back, stream set-up) is counted as slow
+        # class: lane_ops / update - 20
+        upd = N / 4
+        rest = lane_ops / upd - 20.0
+        bound_s = upd * ((11.0 + 2 * 1.4 + rest) / slow + 7.0 / fast)
+        valu = {"lane_ops_per_launch": lane_ops, "ops_per_update": lane_ops / upd,
+                "achieved_T_lane_ops": rate / 1e12,
+                "peak_T_lane_ops": slow / 1e12, "peak_fast_class_T_lane_ops": fast / 1e12,
+                "nominal_T_lane_ops": VALU_PEAK_LANE_OPS / 1e12,
+                "issue_bound_ms": bound_s * 1e3, "frac": bound_s / (avg_launch_ms * 1e-3)}
     line = {
         "metric": METRIC,
         "value": value,

@@ -718,3 +718,15 @@ def load_h5(file, kind, prefix="", threads=0, layout=(4, 4), skip=0, device=-1):
     _check(Lm.kpz_size(h, C.byref(lx), C.byref(ly)))
     cls = {KPZ_DT: SchedulerDT, KPZ_DTR: SchedulerDTr, KPZ_SCABIT: SchedulerSCABit, KPZ_SCA: SchedulerSCA}[kind]
     return cls._wrap(h, kind, lx.value, ly.value), int(pmo.value), int(restored.value)
+
+
+def valu_peak(device=-1, reps=3):
+    """Measured integer VALU issue rates of the device in lane operations per
+    second: (slow, fast) = (v_xad_u32 class: shifts, multiplies, add3, bfe,
+    perm, alignbit; v_xor_b32 class: xor, add, bitop3), from 8 independent
+    chains per lane on 32 blocks of 256 lanes per CU."""
+    L = lib()
+    L.kpz_valu_peak.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    slow, fast = C.c_double(), C.c_double()
+    _check(L.kpz_valu_peak(int(device), int(reps), C.byref(slow), C.byref(fast)))
+    return slow.value, fast.value

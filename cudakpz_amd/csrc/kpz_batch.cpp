@@ -125,7 +125,10 @@ Batch::Batch(const std::vector<ReplicaSpec>& specs, const BatchTuning& tune, int
 			d.full = affine_pow(d.step, du * Wn);
 		}
 		else
+		{
 			d.laneJ = tabBase;
+			d.full = affine_pow(d.step, (d.mode ? 3u : 2u) * 256u); // one 16 x 16 block
+		}
 	}
 	check(hipMemcpy(stArena_.get(), st.data(), st.size() * 8, hipMemcpyHostToDevice), "upload streams");
 	check(hipMemcpy(tabArena_.get(), tab.data(), tab.size() * sizeof(Affine), hipMemcpyHostToDevice), "upload tables");

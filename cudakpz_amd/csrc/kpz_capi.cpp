@@ -5,6 +5,7 @@
 #include "kpz_scheduler.hpp"
 #include "kpz_server.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -391,6 +392,46 @@ uint64_t kpz_size_words(const kpz_scheduler* h) { return (h && h->s) ? h->s->siz
 uint64_t kpz_skip(const kpz_scheduler* h) { return (h && h->s) ? h->s->skip() : 0; }
 void* kpz_device_lattice(const kpz_scheduler* h) { return (h && h->s) ? (void*)h->s->deviceLattice() : nullptr; }
 void* kpz_stream(const kpz_scheduler* h) { return (h && h->s) ? (void*)h->s->stream() : nullptr; }
+
+int kpz_valu_peak(int device, int reps, double* slow_class, double* fast_class)
+{
+	if(!slow_class || reps < 1)
+		return fail(KPZ_EINVAL, "null output or reps < 1");
+	return guard([&] {
+		if(device >= 0)
+			kpz::check(hipSetDevice(device), "hipSetDevice");
+		int dev = 0, cus = 0;
+		kpz::check(hipGetDevice(&dev), "hipGetDevice");
+		kpz::check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+		// 32 blocks of 256 lanes per CU: enough resident waves to keep every
+		// SIMD issuing (fewer leave the measured rate below the peak)
+		const unsigned blocks = (unsigned)(cus > 0 ? cus : 1) * 32u;
+		const int iters = 1024;
+		kpz::DevBuf out((size_t)blocks * 256 * 4);
+		hipEvent_t e0, e1;
+		kpz::check(hipEventCreate(&e0), "event");
+		kpz::check(hipEventCreate(&e1), "event");
+		for(int fast = 0; fast < (fast_class ? 2 : 1); ++fast)
+		{
+			double best = 0;
+			for(int r = 0; r <= reps; ++r) // the first launch warms up
+			{
+				kpz::check(hipEventRecord(e0, nullptr), "event");
+				kpz::check(kpz::launch_valu_probe(out.as<uint32_t>(), blocks, iters, fast != 0, nullptr), "valu probe");
+				kpz::check(hipEventRecord(e1, nullptr), "event");
+				kpz::check(hipEventSynchronize(e1), "event sync");
+				float ms = 0;
+				kpz::check(hipEventElapsedTime(&ms, e0, e1), "event time");
+				const double ops = (double)blocks * 256.0 * iters * 8.0;
+				if(r > 0 && ms > 0)
+					best = std::max(best, ops / (ms * 1e-3));
+			}
+			*(fast ? fast_class : slow_class) = best;
+		}
+		hipEventDestroy(e0);
+		hipEventDestroy(e1);
+	});
+}
 
 // ---------------------------------------------------------------- batches
 static bool to_specs(const kpz_replica* reps, int n, std::vector<kpz::ReplicaSpec>& out)

@@ -298,16 +298,36 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 		s = a.jumps[j](s);
 	const Affine step = a.step;
 	const uint32_t tp = a.tp, tq = a.tq;
+	// DTr uses every draw as randomFloat() = (lo ^ hi) & 0x7FFFFF: the low
+	// state word evolves on its own and the high word is needed mod 2^24 only
+	// (one v_mad_u64_u32 and two v_mad_u32_u24 per draw); the full state after
+	// the block is one jump from its start.  The static DT's coordinates take
+	// the top bits of lo ^ hi and need the whole state.
+	const uint64_t sBlock = s;
+	uint32_t lo = (uint32_t)s, h24 = (uint32_t)(s >> 32);
+	const uint32_t aLo = (uint32_t)step.a, aHi = (uint32_t)(step.a >> 32);
+	auto draw = [&]() -> uint32_t {
+		if constexpr(RO)
+		{
+			const uint64_t pr = (uint64_t)lo * aLo + step.c;
+			h24 = mad_u24(h24, aLo, mad_u24(lo, aHi, (uint32_t)(pr >> 32)));
+			lo = (uint32_t)pr;
+			return lo ^ h24; // bits 0..22 are those of the full fold
+		}
+		else
+		{
+			s = step(s);
+			return slcg_fold(s);
+		}
+	};
 
 	// ---- BX*BY random-sequential updates (Func_MCSBase::exec,
 	// kpz/schedulerCPUDT_nested.h:46-66; pqOneUpdate kpz/schedulerCPU.inl:21-33)
 #pragma unroll 2
 	for(int k = 0; k < BX * BY; ++k)
 	{
-		s = step(s);
-		const uint32_t r1 = slcg_fold(s);
-		s = step(s);
-		const uint32_t r2 = slcg_fold(s);
+		const uint32_t r1 = draw();
+		const uint32_t r2 = draw();
 		int x, y;
 		if(RO)
 		{
@@ -318,10 +338,10 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 		{
 			x = (int)(r1 >> (32 - LBX));
 			y = (int)(r2 >> (32 - LBY));
-			// hide y's origin from the optimiser: the LDS address is then one
-			// shift-and-add of the row number instead of a shift/mask/scale chain
-			asm("" : "+v"(y));
 		}
+		// hide y's origin from the optimiser: the LDS address is then one
+		// shift-and-add of the row number instead of a shift/mask/scale chain
+		asm("" : "+v"(y));
 		uint2 row = my[y * DT_WG];
 		const uint32_t up = my[(y + 1) * DT_WG].y;
 		// whole-row bit algebra, then select column x: a local minimum has
@@ -351,8 +371,7 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 		}
 		else
 		{
-			s = step(s);
-			const uint32_t r23 = slcg_fold(s) & 0x7FFFFFu;
+			const uint32_t r23 = draw() & 0x7FFFFFu;
 			uint32_t tpv = tp, tqv = tq;
 			if(UPD == 2)
 			{
@@ -374,7 +393,7 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 	}
 
 	if(own || i + a.T >= nb)
-		a.st_out[t] = s;
+		a.st_out[t] = RO ? a.blockJ(sBlock) : s;
 
 	// ---- bit-plane rows -> ENC_LOCAL words (masked read-modify-write at the
 	// block edges; no other lane of this phase touches these words)
@@ -637,8 +656,10 @@ __global__ __launch_bounds__(256) void cbbit_half_kernel(CbHalfArgs a)
 // bit-parallel and stores its word once: own flips, the left lane's flips of
 // column 0 (a shuffle) and the flips of row 0 from the cell below (kept in a
 // register) are merged into one plain store.  Flips that leave a group
-// (right edge) or a band (top row) go to small edge buffers, which
-// cb_local_fix_kernel applies afterwards.
+// (right edge) into a group of the same workgroup are handed over in LDS
+// before that group stores the row (one barrier per row); the others, and
+// those that leave a band (top row), go to small edge buffers, which
+// cb_local_fix_kernel applies afterwards with one thread per target word.
 __device__ __forceinline__ int cb_class(const CbLocalArgs& a, int v)
 {
 	int k = 0;
@@ -666,14 +687,13 @@ struct CbCell {
 // the own word, of column 0 of the right word and of row 0 of the up word.
 // DIS: disorder2SyncRngUpdate (kpz/schedulerCPUBaseThreadedRng.inl:50-75),
 // the thresholds of each site's class, the site's 2 bits of the map word dw.
+// The per-draw step is (aLo, aHi, cc); only (lo ^ hi) & 0x7FFFFF is used, so
+// the high state half is carried mod 2^24.
 template<int S, bool DIS>
-__device__ __forceinline__ void cb_cell(uint64_t sc, const Affine& step, uint32_t tp, uint32_t tq, uint32_t own,
-                                        uint32_t right, uint32_t up, uint32_t& fOwn, uint32_t& fRight, uint32_t& fUp,
-                                        uint32_t dw, const uint32_t* tpc, const uint32_t* tqc)
+__device__ __forceinline__ void cb_cell(uint64_t sc, uint32_t aLo, uint32_t aHi, uint64_t cc, uint32_t tp, uint32_t tq,
+                                        uint32_t own, uint32_t right, uint32_t up, uint32_t& fOwn, uint32_t& fRight,
+                                        uint32_t& fUp, uint32_t dw, const uint32_t* tpc, const uint32_t* tqc)
 {
-	// only (lo ^ hi) & 0x7FFFFF is used: high state half carried mod 2^24
-	const uint32_t aLo = (uint32_t)step.a, aHi = (uint32_t)(step.a >> 32);
-	const uint64_t cc = step.c;
 	uint32_t lo = (uint32_t)sc, h24 = (uint32_t)(sc >> 32);
 	uint32_t ap = 0, aq = 0;
 	if(DIS)
@@ -733,21 +753,35 @@ __device__ __forceinline__ void cb_cell(uint64_t sc, const Affine& step, uint32_
 	fUp = ((f >> 24) & 0x55u) << 1;
 }
 
+// With 64-lane groups and at least four groups per row, a 256-lane
+// workgroup holds four consecutive groups of one band (grp = band * ngc + cg)
+// and every workgroup is full: a group whose right neighbour group is one of
+// them hands its right-edge flips over in LDS, row by row.
+__device__ __forceinline__ bool cb_mail(int lG, int ngc) { return lG == 6 && ngc >= 4; }
+__device__ __forceinline__ bool cb_right_local(int lG, int64_t grp, int cg, int ngc)
+{
+	return cb_mail(lG, ngc) && cg + 1 < ngc && (grp & 3) != 3;
+}
+
 template<int S, bool DIS>
 __global__ __launch_bounds__(256) void cb_local_band_kernel(CbLocalArgs a)
 {
+	__shared__ uint32_t mail[4][2]; // [group in the workgroup][row parity]
 	const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	const int lXC = a.lx - 2;
 	const int XC = 1 << lXC;
 	const int G = 1 << a.lG, NB = 1 << a.lNB;
 	const int ngc = XC >> a.lG; // groups per band
+	// whole groups are live or not (the shuffles below stay inside live
+	// groups); with the LDS hand-over every workgroup is full
 	if(gid >= ((int64_t)XC << (a.ly - 2 - a.lNB)))
-		return; // whole groups only: the shuffles below stay inside live groups
+		return;
 	const int g = (int)(gid & (G - 1));
 	const int64_t grp = gid >> a.lG;
 	const int cg = (int)(grp % ngc), band = (int)(grp / ngc);
 	const int c = cg * G + g;
 	const int yb0 = band * NB;
+	uint32_t* __restrict__ lat = a.lat;
 	// worker of this cell column and the cell's place in its strip
 	const int big = a.rem * (a.pc + 1);
 	int t, c0;
@@ -778,31 +812,61 @@ __global__ __launch_bounds__(256) void cb_local_band_kernel(CbLocalArgs a)
 		a.st_out[t] = (t < a.rem ? a.fullBig : a.fullSmall)(s0);
 	// the same cell of the next row: 8 draws per cell of the half strip later
 	const Affine rowStep = a.rowJ[clsH * a.nyb + 1];
+	const uint32_t rLo = (uint32_t)rowStep.a, rHi = (uint32_t)(rowStep.a >> 32);
 
-	uint32_t* __restrict__ lat = a.lat;
-	const Affine step = a.step;
 	const uint32_t tp = a.tp, tq = a.tq;
+	// step constants stay in VGPRs for the whole loop (a VOP3 multiply-add
+	// takes one scalar operand at most)
+	uint32_t aLo = (uint32_t)a.step.a, aHi = (uint32_t)(a.step.a >> 32);
+	uint64_t stepC = a.step.c, rowC = rowStep.c;
+	uint32_t rLoV = rLo, rHiV = rHi;
+	asm volatile("" : "+v"(aLo), "+v"(aHi), "+v"(rLoV), "+v"(rHiV), "+v"(stepC), "+v"(rowC));
 	const bool wrapG = (XC == G); // the group spans the row: the right edge wraps by shuffle
-	uint32_t own = lat[((size_t)yb0 << lXC) + c];
+	const bool useMail = cb_mail(a.lG, ngc); // launch-uniform
+	const bool mailR = cb_right_local(a.lG, grp, cg, ngc);
+	const bool mailL = cg > 0 && cb_right_local(a.lG, grp - 1, cg - 1, ngc);
+	// 32-bit byte offsets into the lattice and the map (below 4 GiB): the
+	// loads and stores take the scalar base plus one VGPR offset
+	const char* const base = (const char*)lat;
+	const uint32_t rowBytes = (uint32_t)XC * 4u;
+	const uint32_t latMask = (uint32_t)(((uint64_t)a.nyb << lXC) * 4u - 1u);
+	const uint32_t c4 = (uint32_t)c * 4u, cr4 = (uint32_t)((c + 1) & (XC - 1)) * 4u;
+	uint32_t rowOff = ((uint32_t)yb0 << lXC) * 4u;
+	uint32_t own = *(const uint32_t*)(base + (rowOff + c4));
 	uint32_t fromBelow = 0;
 	for(int it = 0; it < NB; ++it)
 	{
-		const int yb = yb0 + it;
-		const size_t row = (size_t)yb << lXC;
-		const uint32_t right = lat[row + ((c + 1) & (XC - 1))];
-		const uint32_t up = lat[((size_t)((yb + 1) & (a.nyb - 1)) << lXC) + c];
+		const uint32_t upOff = (rowOff + rowBytes) & latMask;
+		const uint32_t right = *(const uint32_t*)(base + (rowOff + cr4));
+		const uint32_t up = *(const uint32_t*)(base + (upOff + c4));
 		uint32_t fOwn, fRight, fUp;
-		const uint32_t dw = DIS ? a.dis[row + c] : 0u;
-		cb_cell<S, DIS>(s, step, tp, tq, own, right, up, fOwn, fRight, fUp, dw, a.tpc, a.tqc);
+		const uint32_t dw = DIS ? *(const uint32_t*)((const char*)a.dis + (rowOff + c4)) : 0u;
+		cb_cell<S, DIS>(s, aLo, aHi, stepC, tp, tq, own, right, up, fOwn, fRight, fUp, dw, a.tpc, a.tqc);
 		uint32_t fromLeft = (uint32_t)__shfl((int)fRight, (g - 1) & (G - 1), G);
 		if(g == 0 && !wrapG)
 			fromLeft = 0;
-		lat[row + c] = own ^ fOwn ^ fromLeft ^ fromBelow;
-		if(g == G - 1 && !wrapG)
-			a.edgeR[(size_t)yb * ngc + cg] = fRight;
+		if(useMail)
+		{
+			// the left group's right-edge flips of this row, through LDS (two
+			// slots: a slot is rewritten two barriers after it was read)
+			if(g == G - 1 && mailR)
+				mail[grp & 3][it & 1] = fRight;
+			__syncthreads();
+			if(g == 0 && mailL)
+				fromLeft = mail[(grp - 1) & 3][it & 1];
+		}
+		*(uint32_t*)((char*)lat + (rowOff + c4)) = own ^ fOwn ^ fromLeft ^ fromBelow;
+		if(g == G - 1 && !wrapG && !mailR)
+			a.edgeR[(size_t)(yb0 + it) * ngc + cg] = fRight;
 		fromBelow = fUp;
 		own = up; // no other lane writes the bits this lane reads from it
-		s = rowStep(s);
+		rowOff = upOff;
+		// the jump to the next row's cell, kept exact in the low state word
+		// and mod 2^24 in the high one: all cb_cell uses (the stream's final
+		// state comes from s0 above)
+		const uint64_t pr = (uint64_t)(uint32_t)s * rLoV + rowC;
+		const uint32_t h24 = mad_u24((uint32_t)(s >> 32), rLoV, mad_u24((uint32_t)s, rHiV, (uint32_t)(pr >> 32)));
+		s = ((uint64_t)h24 << 32) | (uint32_t)pr;
 	}
 	a.edgeU[(size_t)band * XC + c] = fromBelow;
 }
@@ -815,22 +879,37 @@ __global__ __launch_bounds__(256) void cb_local_fix_kernel(CbLocalArgs a)
 	const int XC = 1 << lXC;
 	const int G = 1 << a.lG;
 	const int ngc = XC >> a.lG;
-	const int64_t nR = (XC > G) ? ((int64_t)a.nyb * ngc) : 0;
+	const bool edges = XC > G; // groups do not span the row
 	const int64_t nU = (int64_t)(a.nyb >> a.lNB) * XC;
-	if(gid < nR)
+	const int64_t nR = edges ? ((int64_t)a.nyb * ngc) : 0;
+	// every target word has exactly one thread here: plain read-modify-write
+	if(gid < nU)
 	{
-		const uint32_t v = a.edgeR[gid];
-		const int yb = (int)(gid / ngc), cg = (int)(gid % ngc);
+		// first row of band b + 1, column c: the top-row flips of band b and,
+		// at a group start, the right-edge flips of the group to its left
+		// when they did not go through LDS
+		const int b = (int)(gid >> lXC), c = (int)(gid & (XC - 1));
+		const int r = ((b + 1) << a.lNB) & (a.nyb - 1);
+		uint32_t v = a.edgeU[gid];
+		if(edges && (c & (G - 1)) == 0)
+		{
+			const int cgL = ((c >> a.lG) + ngc - 1) & (ngc - 1);
+			if(!cb_right_local(a.lG, (int64_t)(r >> a.lNB) * ngc + cgL, cgL, ngc))
+				v ^= a.edgeR[(size_t)r * ngc + cgL];
+		}
 		if(v)
-			atomicXor(a.lat + ((size_t)yb << lXC) + (((cg + 1) * G) & (XC - 1)), v);
+			a.lat[((size_t)r << lXC) + c] ^= v;
 	}
-	else if(gid < nR + nU)
+	else if(gid < nU + nR)
 	{
-		const int64_t k = gid - nR;
-		const uint32_t v = a.edgeU[k];
-		const int band = (int)(k >> lXC), c = (int)(k & (XC - 1));
+		// row r (not a band's first row), the start of the group right of cg
+		const int64_t k = gid - nU;
+		const int r = (int)(k >> (lXC - a.lG)), cg = (int)(k & (ngc - 1));
+		if((r & ((1 << a.lNB) - 1)) == 0 || cb_right_local(a.lG, (int64_t)(r >> a.lNB) * ngc + cg, cg, ngc))
+			return;
+		const uint32_t v = a.edgeR[k];
 		if(v)
-			atomicXor(a.lat + ((size_t)(((band + 1) << a.lNB) & (a.nyb - 1)) << lXC) + c, v);
+			a.lat[((size_t)r << lXC) + (((cg + 1) * G) & (XC - 1))] ^= v;
 	}
 }
 
@@ -1150,6 +1229,47 @@ __global__ void fill_kernel(uint32_t* lat, uint64_t n, uint64_t split, uint32_t 
 		lat[k] = k < split ? lo : hi;
 }
 
+// Integer VALU issue-rate probe: 8 independent chains per lane, 8
+// instructions per iteration in one asm block (no hazard waits between
+// them), so every SIMD issues back to back from its resident waves.
+// FAST = 0: v_xad_u32, the rate of shifts, multiplies, add3, bfe, perm and
+// alignbit on gfx950; FAST = 1: v_xor_b32, the rate of plain xor / add /
+// bitop3, about twice as high.
+template<int FAST>
+__global__ __launch_bounds__(256) void valu_probe_kernel(uint32_t* __restrict__ out, int iters, uint32_t k0, uint32_t k1)
+{
+	uint32_t a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, a4 = a0 + 4, a5 = a0 + 5, a6 = a0 + 6, a7 = a0 + 7;
+	const uint32_t kv = k1 ^ threadIdx.x;
+	for(int it = 0; it < iters; ++it)
+	{
+		if(FAST)
+			asm volatile(
+				"v_xor_b32 %0, %8, %0\n\t"
+				"v_xor_b32 %1, %8, %1\n\t"
+				"v_xor_b32 %2, %8, %2\n\t"
+				"v_xor_b32 %3, %8, %3\n\t"
+				"v_xor_b32 %4, %8, %4\n\t"
+				"v_xor_b32 %5, %8, %5\n\t"
+				"v_xor_b32 %6, %8, %6\n\t"
+				"v_xor_b32 %7, %8, %7"
+				: "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5), "+v"(a6), "+v"(a7)
+				: "v"(kv));
+		else
+			asm volatile(
+				"v_xad_u32 %0, %0, %8, %9\n\t"
+				"v_xad_u32 %1, %1, %8, %9\n\t"
+				"v_xad_u32 %2, %2, %8, %9\n\t"
+				"v_xad_u32 %3, %3, %8, %9\n\t"
+				"v_xad_u32 %4, %4, %8, %9\n\t"
+				"v_xad_u32 %5, %5, %8, %9\n\t"
+				"v_xad_u32 %6, %6, %8, %9\n\t"
+				"v_xad_u32 %7, %7, %8, %9"
+				: "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5), "+v"(a6), "+v"(a7)
+				: "s"(k0), "v"(kv));
+	}
+	out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3 ^ a4 ^ a5 ^ a6 ^ a7;
+}
+
 __global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n)
 {
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -1183,6 +1303,7 @@ __global__ __launch_bounds__(DT_WG) void batch_dt_kernel(BatchDtArgs a)
 	p.st_in = p.st_out = rep.st[0];
 	p.jumps = nullptr;
 	p.step = rep.step;
+	p.blockJ = rep.full;
 	p.tp = rep.tp;
 	p.tq = rep.tq;
 	const size_t mi = (size_t)a.m * a.nreps + w.x;
@@ -1436,8 +1557,9 @@ hipError_t launch_cb_local_half(const CbLocalArgs& a, hipStream_t stream)
 {
 	// the band kernel's shape: groups of 2^lG <= 64 columns inside a row of
 	// 2^(lx-2) cells, bands of 2^lNB cell rows, at least 2 cell rows
-	if(a.lx < 3 || a.ly < 3 || a.lG < 0 || a.lG > 6 || a.lG > a.lx - 2 || a.lNB < 0 || a.lNB > a.ly - 2 ||
-	   !a.edgeU || (a.lG < a.lx - 2 && !a.edgeR))
+	// (and a lattice below 4 GiB: the band kernel addresses it with 32-bit byte offsets)
+	if(a.lx < 3 || a.ly < 3 || a.lx + a.ly > 33 || a.lG < 0 || a.lG > 6 || a.lG > a.lx - 2 || a.lNB < 0 ||
+	   a.lNB > a.ly - 2 || !a.edgeU || (a.lG < a.lx - 2 && !a.edgeR))
 		return hipErrorInvalidValue;
 	const int64_t lanes = 1ll << (a.lx - 2 + a.ly - 2 - a.lNB);
 	const unsigned grid = (unsigned)((lanes + 255) / 256);
@@ -1485,6 +1607,15 @@ hipError_t launch_batch_roughness(const BatchRep* reps, int nreps, const uint2* 
 		hipLaunchKernelGGL(batch_rowstats_kernel, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0, stream, reps, rows, nrows, scratch);
 	if(nreps > 0)
 		hipLaunchKernelGGL(batch_combine_kernel, dim3((unsigned)((nreps + 63) / 64)), dim3(64), 0, stream, reps, nreps, scratch, out);
+	return hipGetLastError();
+}
+
+hipError_t launch_valu_probe(uint32_t* out, unsigned blocks, int iters, bool fast, hipStream_t stream)
+{
+	if(fast)
+		hipLaunchKernelGGL(valu_probe_kernel<1>, dim3(blocks), dim3(256), 0, stream, out, iters, 0x9E3779B9u, 0x7F4A7C15u);
+	else
+		hipLaunchKernelGGL(valu_probe_kernel<0>, dim3(blocks), dim3(256), 0, stream, out, iters, 0x9E3779B9u, 0x7F4A7C15u);
 	return hipGetLastError();
 }
 

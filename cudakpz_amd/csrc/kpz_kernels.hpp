@@ -18,6 +18,7 @@ struct DtPhaseArgs {
 	int64_t T;              // streams (CPU worker count of the counterpart)
 	const Affine* jumps;    // jumps[j]: advance to the j-th block of a stream
 	Affine step;            // per-draw SLCG step (skip form)
+	Affine blockJ;          // J(draws per block): a stream's state after one block
 	int ds;                 // domain set 0..3
 	int sx, sy;             // DT origin shift (0 for the static DT)
 	int upd;                // 0 = KPZ (pqOneUpdate), 1 = pq (pqSyncRngUpdate),
@@ -107,7 +108,7 @@ struct BatchRep {
 	const Affine* rowJ;     // SCABit: {identity} (one stream per row)
 	Affine step;            // per-draw step (skip form)
 	Affine stepJ;           // SCABit: J(draws per word * G)
-	Affine full;            // SCABit: J(draws per row)
+	Affine full;            // SCABit: J(draws per row); DT: J(draws per block)
 	int lx, ly;
 	int kind;               // BatchKind
 	int mode;               // DT: 0 KPZ (pqOneUpdate), 1 pq; SCABit: 0 bit-parallel, 1 per-site
@@ -149,5 +150,8 @@ hipError_t launch_batch_roughness(const BatchRep* reps, int nreps, const uint2* 
 hipError_t launch_fill_flat(uint32_t* lat, int enc, int lx, int ly, hipStream_t stream);
 // dst <- src, 16-byte vector stream copy (bytes must be a multiple of 16)
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t stream);
+// integer VALU issue-rate probe: blocks x 256 lanes x iters x 8 v_xad_u32
+// (fast: v_xor_b32); out needs blocks * 256 u32
+hipError_t launch_valu_probe(uint32_t* out, unsigned blocks, int iters, bool fast, hipStream_t stream);
 
 } // namespace kpz

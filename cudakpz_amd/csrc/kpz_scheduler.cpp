@@ -392,6 +392,7 @@ void SchedulerDT::run(int n, int upd)
 	a.T = T_;
 	a.jumps = jumps(du);
 	a.step = step_;
+	a.blockJ = affine_pow(step_, (uint64_t)du << (lbx_ + lby_));
 	a.upd = upd;
 	a.random_origin = ro_;
 	a.tp = thresh23(upd ? p_ : 1.0);

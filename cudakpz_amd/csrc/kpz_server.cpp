@@ -246,6 +246,8 @@ bool Server::admit(Job& j)
 		d.stepJ = affine_pow(d.step, du * (uint64_t)G);
 		d.full = affine_pow(d.step, du * Wn);
 	}
+	else
+		d.full = affine_pow(d.step, (d.mode ? 3u : 2u) * 256u); // one 16 x 16 block
 	check(launch_fill_flat(d.lat, d.kind == BK_SCABIT ? ENC_CBBIT : ENC_LOCAL, s.lx, s.ly, s0_), "fill");
 	// sample times (kpz/sampling.h:84-100) and the final MCS
 	j.samples.clear();

@@ -167,6 +167,15 @@ uint64_t kpz_skip(const kpz_scheduler* h);
 void* kpz_device_lattice(const kpz_scheduler* h);
 void* kpz_stream(const kpz_scheduler* h);
 
+/* Measurement utility, no counterpart in the reference: the integer VALU
+ * issue rates of the device, the roofline that bounds the DT and per-site
+ * SCA kernels, in lane operations per second (best of `reps` launches of 8
+ * independent chains per lane on 32 x 256-lane blocks per CU).  slow_class:
+ * v_xad_u32, the rate of shifts, multiplies, add3, bfe, perm and alignbit on
+ * gfx950; fast_class (may be NULL): v_xor_b32, the rate of plain xor, add
+ * and bitop3, about twice as high. */
+int kpz_valu_peak(int device, int reps, double* slow_class, double* fast_class);
+
 /* ---------------------------------------------------------------------
  * Batched replicas (no counterpart in the reference: CudaKpz runs one
  * lattice per process).  Each replica is bit-identical to a stand-alone
