@@ -374,16 +374,17 @@ def main():
         lane_ops = prof["valu_insts_per_launch"] * 64.0
         rate = lane_ops / (avg_launch_ms * 1e-3)
         slow, fast = K.valu_peak()
-        # issue-time bound of one launch from the instruction mix: the update
-        # loop of dt_phase_kernel<4,4,0,false> (compiled gfx950 code, per
-        # update) issues 11 slow-class instructions (v_mul_lo_u32, v_add3_u32,
-        # shifts, v_lshl_or_b32), 2 v_mad_u64_u32 (1.4 slow-class slots each,
-        # measured) and 7 fast-class ones (xor, sub, or, bitop3); the rest of
-        # the kernel (staging, write-back, stream set-up) is counted as slow
-        # class: lane_ops / update - 20
+        # issue-time lower bound of one launch from the instruction mix: the
+        # update loop of dt_phase_kernel<4,4,0,false> (compiled gfx950 code,
+        # per update) issues 8 slow-class instructions (4 v_mul_lo_u32, 2
+        # v_add3_u32, v_lshl_or_b32, v_lshlrev_b32), 2 v_mad_u64_u32 (1.4
+        # slow-class slots each, measured) and 10 fast-class ones (2 xor, 3
+        # right shifts, 2 bitop3, sub, or, sub); the rest of the kernel
+        # (staging, write-back, stream set-up: lane_ops / update - 20) is
+        # counted as fast class, so the bound is a lower bound
         upd = N / 4
         rest = lane_ops / upd - 20.0
-        bound_s = upd * ((11.0 + 2 * 1.4 + rest) / slow + 7.0 / fast)
+        bound_s = upd * ((8.0 + 2 * 1.4) / slow + (10.0 + rest) / fast)
         valu = {"lane_ops_per_launch": lane_ops, "ops_per_update": lane_ops / upd,
                 "achieved_T_lane_ops": rate / 1e12,
                 "peak_T_lane_ops": slow / 1e12, "peak_fast_class_T_lane_ops": fast / 1e12,

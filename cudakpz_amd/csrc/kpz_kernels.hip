@@ -68,6 +68,19 @@ __device__ __forceinline__ uint64_t shflw(uint64_t v, int src, int width) { retu
 // ======================================================================
 constexpr int DT_WG = 256;
 
+// A lane's column of the DT LDS tile: the bit-plane rows in two planes,
+// X rows 0..BY-1 and Y rows 0..BY (BY = the halo row above the block), each
+// [row][lane] with a 4-byte entry, so that every access of the update loop
+// (a row's X, its Y, the Y of the row above) is a conflict-free 4-byte access.
+struct DtTile {
+	uint32_t* x;
+	uint32_t* y;
+	__device__ __forceinline__ uint32_t X(int r) const { return x[r * DT_WG]; }
+	__device__ __forceinline__ uint32_t Y(int r) const { return y[r * DT_WG]; }
+	__device__ __forceinline__ void setX(int r, uint32_t v) const { x[r * DT_WG] = v; }
+	__device__ __forceinline__ void setY(int r, uint32_t v) const { y[r * DT_WG] = v; }
+};
+
 // First three delta swaps of the outer unshuffle: a 32-bit row of 16
 // interleaved sites (bit 2c = X_c, 2c+1 = Y_c) becomes the bytes
 // {X 0-7, Y 0-7, X 8-15, Y 8-15}; the last (byte) swap is left to the v_perm
@@ -103,7 +116,7 @@ constexpr uint32_t perm_sel(int b0, int b1, int b2, int b3)
 // unzipped into its X (even) and Y (odd) bit planes.
 template<int NWY>
 __device__ __forceinline__ void dt_stage_x16(const uint32_t* __restrict__ lat, int XW, int mXW, int mYW, int c0,
-                                             int wr0, uint32_t (&raw)[NWY][5], uint2* __restrict__ my)
+                                             int wr0, uint32_t (&raw)[NWY][5], const DtTile& my)
 {
 #pragma unroll
 	for(int wk = 0; wk < NWY; ++wk)
@@ -128,10 +141,11 @@ __device__ __forceinline__ void dt_stage_x16(const uint32_t* __restrict__ lat, i
 			                 | __builtin_amdgcn_perm(raw[wk][3], raw[wk][2], perm_sel(12, 12, rr, 4 + rr));
 			const uint32_t u = unzip_bytes(b);
 			// X: 16 columns + the halo column (x = 0 of the next word: bit 0 of
-			// its byte rr, moved to bit 16 with that byte)
-			const uint32_t X = (wk < NWY - 1)
-			                 ? __builtin_amdgcn_perm(raw[wk][4], u, perm_sel(0, 2, 4 + rr, 12)) & 0x1FFFFu : 0u;
-			my[(4 * wk + rr) * DT_WG] = make_uint2(X, __builtin_amdgcn_perm(u, u, perm_sel(1, 3, 12, 12)));
+			// its byte rr, moved to bit 16 with that byte); the top halo row
+			// has Y only
+			if(wk < NWY - 1)
+				my.setX(4 * wk + rr, __builtin_amdgcn_perm(raw[wk][4], u, perm_sel(0, 2, 4 + rr, 12)) & 0x1FFFFu);
+			my.setY(4 * wk + rr, __builtin_amdgcn_perm(u, u, perm_sel(1, 3, 12, 12)));
 		}
 	}
 }
@@ -144,7 +158,7 @@ __device__ __forceinline__ void dt_stage_x16(const uint32_t* __restrict__ lat, i
 template<int NWY>
 __device__ __forceinline__ void dt_writeback_x16(uint32_t* __restrict__ lat, int XW, int mXW, int mYW, int c0,
                                                  int wr0, const uint32_t (&raw)[NWY][5],
-                                                 const uint2* __restrict__ my)
+                                                 const DtTile& my)
 {
 #pragma unroll
 	for(int wk = 0; wk < NWY - 1; ++wk)
@@ -154,9 +168,9 @@ __device__ __forceinline__ void dt_writeback_x16(uint32_t* __restrict__ lat, int
 #pragma unroll
 		for(int rr = 0; rr < 4; ++rr)
 		{
-			const uint2 tv = my[(4 * wk + rr) * DT_WG];
-			B[rr] = zip_bytes(__builtin_amdgcn_perm(tv.y, tv.x, perm_sel(0, 4, 1, 5)));
-			hb |= ((tv.x >> 16) & 1u) << (8 * rr);
+			const uint32_t tx = my.X(4 * wk + rr), ty = my.Y(4 * wk + rr);
+			B[rr] = zip_bytes(__builtin_amdgcn_perm(ty, tx, perm_sel(0, 4, 1, 5)));
+			hb |= ((tx >> 16) & 1u) << (8 * rr);
 		}
 		const uint32_t T0 = __builtin_amdgcn_perm(B[1], B[0], perm_sel(0, 4, 2, 6));
 		const uint32_t T1 = __builtin_amdgcn_perm(B[1], B[0], perm_sel(1, 5, 3, 7));
@@ -173,7 +187,7 @@ __device__ __forceinline__ void dt_writeback_x16(uint32_t* __restrict__ lat, int
 	// top halo row: only the Y bits of its first site row (byte 0, odd bits)
 	constexpr int wk = NWY - 1;
 	uint32_t* rp = lat + (size_t)((wr0 + wk) & mYW) * XW;
-	const uint32_t b = zip_bytes(__builtin_amdgcn_perm(my[(NWY - 1) * 4 * DT_WG].y, 0u, perm_sel(12, 4, 12, 5)));
+	const uint32_t b = zip_bytes(__builtin_amdgcn_perm(my.Y((NWY - 1) * 4), 0u, perm_sel(12, 4, 12, 5)));
 	uint4 q;
 	q.x = (raw[wk][0] & ~0xAAu) | (b & 0xAAu);
 	q.y = (raw[wk][1] & ~0xAAu) | ((b >> 8) & 0xAAu);
@@ -182,12 +196,12 @@ __device__ __forceinline__ void dt_writeback_x16(uint32_t* __restrict__ lat, int
 	*reinterpret_cast<uint4*>(rp + c0) = q;
 }
 
-// One DT block i of one phase, staged in this lane's LDS column `my`
-// (row r at my[r * DT_WG]).  nb = blocks per domain set.  UPD: 0 = KPZ
+// One DT block i of one phase, staged in this lane's LDS column `my`.
+// nb = blocks per domain set.  UPD: 0 = KPZ
 // (pqOneUpdate), 1 = pqSyncRngUpdate, 2 = disorder2SyncRngUpdate (class rows
 // of the disorder map in myD, 2 bits per site), 3 = EW.
 template<int LBX, int LBY, int UPD, bool RO>
-__device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_t nb, uint2* __restrict__ my,
+__device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_t nb, const DtTile& my,
                                          uint32_t* __restrict__ myD)
 {
 	constexpr int BX = 1 << LBX, BY = 1 << LBY;
@@ -253,8 +267,9 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 				sxv |= ((cx[k] >> (4 * rr)) & 0xFu) << (4 * k);
 				syv |= ((cy[k] >> (4 * rr)) & 0xFu) << (4 * k);
 			}
-			const uint32_t X = (r < BY) ? ((sxv >> shx) & MX) : 0u;
-			my[r * DT_WG] = make_uint2(X, (syv >> shx) & MY);
+			if(r < BY)
+				my.setX(r, (sxv >> shx) & MX);
+			my.setY(r, (syv >> shx) & MY);
 		}
 	}
 	}
@@ -342,8 +357,8 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 		// hide y's origin from the optimiser: the LDS address is then one
 		// shift-and-add of the row number instead of a shift/mask/scale chain
 		asm("" : "+v"(y));
-		uint2 row = my[y * DT_WG];
-		const uint32_t up = my[(y + 1) * DT_WG].y;
+		uint2 row = make_uint2(my.X(y), my.Y(y));
+		const uint32_t up = my.Y(y + 1);
 		// whole-row bit algebra, then select column x: a local minimum has
 		// own X = 0, right X = 1, own Y = 0, up Y = 1; a maximum the opposite
 		const uint32_t rx = row.x >> 1;
@@ -352,10 +367,9 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 			// at a minimum own X = 0 and right X = 1, own Y = 0 and up Y = 1:
 			// the flip is X - m (bit x set, bit x+1 cleared), Y | m, up - m
 			const uint32_t fm = rx & ~row.x & ~row.y & up & (1u << x);
-			row.x -= fm;
-			row.y |= fm;
-			my[y * DT_WG] = row;
-			my[(y + 1) * DT_WG].y = up - fm;
+			my.setX(y, row.x - fm);
+			my.setY(y, row.y | fm);
+			my.setY(y + 1, up - fm);
 		}
 		else if(UPD == 3)
 		{
@@ -364,10 +378,9 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 			const uint32_t mn = rx & ~row.x & ~row.y & up;
 			const uint32_t mx = ~rx & row.x & row.y & ~up;
 			const uint32_t fm = (mn | mx) & (1u << x);
-			row.x ^= fm | (fm << 1);
-			row.y ^= fm;
-			my[y * DT_WG] = row;
-			my[(y + 1) * DT_WG].y = up ^ fm;
+			my.setX(y, row.x ^ (fm | (fm << 1)));
+			my.setY(y, row.y ^ fm);
+			my.setY(y + 1, up ^ fm);
 		}
 		else
 		{
@@ -385,10 +398,9 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 			const uint32_t mn = rx & ~row.x & ~row.y & up;
 			const uint32_t mx = ~rx & row.x & row.y & ~up;
 			const uint32_t fm = ((mn & aMin) | (mx & aMax)) & (1u << x);
-			row.x ^= fm | (fm << 1);
-			row.y ^= fm;
-			my[y * DT_WG] = row;
-			my[(y + 1) * DT_WG].y = up ^ fm;
+			my.setX(y, row.x ^ (fm | (fm << 1)));
+			my.setY(y, row.y ^ fm);
+			my.setY(y + 1, up ^ fm);
 		}
 	}
 
@@ -419,11 +431,11 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 				const int r = 4 * wk + rr - shy;
 				if(r < 0 || r > BY)
 					continue;
-				const uint2 tv = my[r * DT_WG];
+				const uint32_t tx = (r < BY) ? my.X(r) : 0u, ty = my.Y(r);
 				const uint32_t mX = (r < BY) ? nibMX : 0u;
-				vx16 |= (((tv.x << shx) >> (4 * k)) & mX) << (4 * rr);
+				vx16 |= (((tx << shx) >> (4 * k)) & mX) << (4 * rr);
 				mx16 |= mX << (4 * rr);
-				vy16 |= (((tv.y << shx) >> (4 * k)) & nibMY) << (4 * rr);
+				vy16 |= (((ty << shx) >> (4 * k)) & nibMY) << (4 * rr);
 				my16 |= nibMY << (4 * rr);
 			}
 			if((mx16 | my16) == 0u)
@@ -435,18 +447,18 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 	}
 }
 
-template<int LBX, int LBY>
-constexpr int dt_tile_rows() { return (1 << LBY) + 1; }
-
 template<int LBX, int LBY, int UPD, bool RO>
 __global__ __launch_bounds__(DT_WG) void dt_phase_kernel(DtPhaseArgs a, int64_t nb)
 {
-	__shared__ uint2 tile[dt_tile_rows<LBX, LBY>() * DT_WG];
+	// X plane then Y plane in one array: both planes share one address
+	// register, the plane offset goes into the LDS instructions' offset
+	__shared__ uint32_t tile[(2 * (1 << LBY) + 1) * DT_WG];
 	__shared__ uint32_t tileD[(UPD == 2) ? (1 << LBY) * DT_WG : 1];
 	const int64_t i = (int64_t)blockIdx.x * DT_WG + threadIdx.x;
 	if(i >= nb)
 		return; // no barrier in this kernel: idle lanes may leave
-	dt_block<LBX, LBY, UPD, RO>(a, i, nb, tile + threadIdx.x, (UPD == 2) ? tileD + threadIdx.x : nullptr);
+	dt_block<LBX, LBY, UPD, RO>(a, i, nb, DtTile{tile + threadIdx.x, tile + (1 << LBY) * DT_WG + threadIdx.x},
+	                            (UPD == 2) ? tileD + threadIdx.x : nullptr);
 }
 
 // ======================================================================
@@ -1285,7 +1297,7 @@ __global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst, cons
 // wave-uniform.  Block layout 16 x 16 sites.
 __global__ __launch_bounds__(DT_WG) void batch_dt_kernel(BatchDtArgs a)
 {
-	__shared__ uint2 tile[dt_tile_rows<4, 4>() * DT_WG];
+	__shared__ uint32_t tile[33 * DT_WG]; // X plane (16 rows), Y plane (17 rows)
 	const int64_t lane = (int64_t)blockIdx.x * DT_WG + threadIdx.x;
 	if(lane >= a.nlanes)
 		return;
@@ -1310,7 +1322,7 @@ __global__ __launch_bounds__(DT_WG) void batch_dt_kernel(BatchDtArgs a)
 	p.ds = (a.order[mi] >> (2 * a.d)) & 3;
 	const int64_t nb = 1ll << (rep.lx + rep.ly - 10);
 	p.T = nb;
-	uint2* my = tile + threadIdx.x;
+	const DtTile my{tile + threadIdx.x, tile + 16 * DT_WG + threadIdx.x};
 	if(rep.kind == BK_DTR)
 	{
 		const uint16_t sh = a.shift[mi];
