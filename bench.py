@@ -95,15 +95,17 @@ def run_dt(K, args, ev):
     return s, total_ms, mcs_ms, w2
 
 
-def run_scabit(K, lx, p, q, steps, seed, ev, cls=None):
+def run_scabit(K, lx, p, q, steps, seed, ev, cls=None, kpz=False):
+    """steps MCS of mcsPQSyncRng (kpz: mcs) after 2 warm-up MCS, hipEvent time"""
     s = (cls or K.SchedulerSCABit)(lx, lx)
     s.supplySeed(seed)
     s.initHomogeneous()
     s.setPQ(p, q)
-    s.mcsPQSyncRng(2)
+    run = s.mcs if kpz else s.mcsPQSyncRng
+    run(2)
     a, b = ev.event(), ev.event()
     ev.record(a, s.stream)
-    s.mcsPQSyncRng(steps)
+    run(steps)
     ev.record(b, s.stream)
     ms = ev.ms(a, b)
     s.close()
@@ -424,6 +426,12 @@ def main():
             ups = N * steps / (ms * 1e-3)
             sec[name] = {"site_updates_per_s": ups, "hbm_GBs_algorithmic": ups * 1.0 / 1e9,
                          "hbm_frac": ups / 1e9 / HBM_PEAK_GBS, "ms_per_mcs": ms / steps}
+        # the reference's default "dt" scheduler: DT with a random origin per
+        # MCS (SchedulerCPUDTr semantics), p = 1, q = 0, 0.5 B per update
+        ms = run_scabit(K, args.lx, 1.0, 0.0, 100, args.seed, ev, K.SchedulerDTr, kpz=True)
+        ups = N * 100 / (ms * 1e-3)
+        sec["dtr_p1_q0"] = {"site_updates_per_s": ups, "hbm_GBs_algorithmic": ups * 0.5 / 1e9,
+                            "hbm_frac": ups * 0.5 / 1e9 / HBM_PEAK_GBS, "ms_per_mcs": ms / 100}
         line["secondary"] = sec
     if args.batch:
         line["batched"] = run_batch_bench(K, ev, args.batch, args.serial_mcs)
