@@ -1140,6 +1140,251 @@ __device__ __forceinline__ void row_stats_row(const uint32_t* __restrict__ lat, 
 	}
 }
 
+// Row statistics of ENC_LOCAL lattices without a snapshot, table-driven:
+// one wave per word row (4 site rows, one byte of every word each).  Two
+// consecutive words give 8 consecutive sites of a row; their x-slope bits
+// (even bits of the row's byte) interleaved form an 8-bit index into an LDS
+// table of the walk over those 8 sites from 0: its end q, sum A and sum of
+// squares B of the inclusive prefixes.  A lane walks K words per round with
+// a running offset c relative to its segment start, a wave scan of the
+// segment ends gives each segment's offset C, and the row sums in int64 are
+// sum P = sum_lanes (n C + s1), sum P^2 = sum_lanes (n C^2 + 2 C s1 + s2),
+// with P the inclusive prefix from site 0.  The heights of Correlator::set
+// start the walk at site 1 and put site 0 last (h_0 = D, the row's slope
+// sum), so with s0 = site 0's x slope: S1 = sum P - L s0 + D and
+// S2 = sum P^2 - 2 s0 sum P + L + D^2 (the values row_stats_row writes).
+// Needs 2^(lx-2) >= 2 words per row.
+constexpr int RS_K = 16; // words per lane and round
+
+__global__ __launch_bounds__(256) void row_stats_local_kernel(const uint32_t* __restrict__ lat, int lx, int ly,
+		int64_t* __restrict__ out)
+{
+	__shared__ uint32_t tab[256];
+	{
+		// entry of index i: bit 2k = site k, bit 2k+1 = site 4+k (k < 4);
+		// (q + 8) | (A + 36) << 5 | B << 12
+		const int i = threadIdx.x;
+		int p = 0, A = 0, B = 0;
+#pragma unroll
+		for(int j = 0; j < 8; ++j)
+		{
+			p += ((i >> (j < 4 ? 2 * j : 2 * (j - 4) + 1)) & 1) ? 1 : -1;
+			A += p;
+			B += p * p;
+		}
+		tab[i] = (uint32_t)(p + 8) | ((uint32_t)(A + 36) << 5) | ((uint32_t)B << 12);
+	}
+	__syncthreads();
+	const int yw = blockIdx.x * 4 + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
+	if(yw >= (1 << (ly - 2)))
+		return; // uniform per wave; no barrier below
+	const int XW = 1 << (lx - 2);
+	const int K = XW < RS_K ? XW : RS_K;     // even: XW >= 2
+	const int span = 64 * K;                 // words per round
+	const uint32_t* __restrict__ row = lat + ((size_t)yw << (lx - 2));
+	int64_t S1[4] = {0, 0, 0, 0}, S2[4] = {0, 0, 0, 0};
+	int base[4] = {0, 0, 0, 0};
+	for(int w0 = 0; w0 < XW; w0 += span)
+	{
+		const int wl = w0 + lane * K;
+		const bool live = wl < XW;
+		int c[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
+		// 8 sites of each of the 4 rows from the word pair (a, b)
+		auto pair = [&](uint32_t a, uint32_t b) {
+#pragma unroll
+			for(int r = 0; r < 4; ++r)
+			{
+				const uint32_t t0 = a >> (8 * r), t1 = (b << 1) >> (8 * r);
+				const uint32_t e = tab[(t0 & 0x55u) | (t1 & 0xAAu)];
+				const int q = (int)(e & 31u) - 8, A = (int)((e >> 5) & 127u) - 36, B = (int)(e >> 12);
+				s1[r] += 8 * c[r] + A;
+				s2[r] += 8 * c[r] * c[r] + 2 * c[r] * A + B;
+				c[r] += q;
+			}
+		};
+		if(live && K == RS_K)
+		{
+			// 16-byte loads of the lane's 16 consecutive words
+#pragma unroll
+			for(int k = 0; k < RS_K; k += 4)
+			{
+				const uint4 v = *reinterpret_cast<const uint4*>(row + wl + k);
+				pair(v.x, v.y);
+				pair(v.z, v.w);
+			}
+		}
+		else if(live)
+		{
+			for(int k = 0; k < K; k += 2)
+				pair(row[wl + k], row[wl + k + 1]);
+		}
+		const int n = live ? 4 * K : 0;
+#pragma unroll
+		for(int r = 0; r < 4; ++r)
+		{
+			int incl = c[r];
+#pragma unroll
+			for(int o = 1; o < 64; o <<= 1)
+			{
+				const int u = __shfl_up(incl, o, 64);
+				if(lane >= o)
+					incl += u;
+			}
+			const int64_t C = (int64_t)base[r] + (incl - c[r]);
+			S1[r] += (int64_t)n * C + s1[r];
+			S2[r] += (int64_t)n * C * C + 2 * C * s1[r] + s2[r];
+			base[r] += __shfl(incl, 63, 64);
+		}
+	}
+	const int64_t L = (int64_t)1 << lx;
+#pragma unroll
+	for(int r = 0; r < 4; ++r)
+	{
+		const int64_t sp = wave_sum(S1[r]), spp = wave_sum(S2[r]);
+		if(lane == 0)
+		{
+			const int y = 4 * yw + r;
+			const int64_t D = base[r];
+			const int64_t s0 = ((row[0] >> (8 * r)) & 1u) ? 1 : -1;
+			int64_t* o = out + (size_t)y * ROW_STATS;
+			o[0] = sp - L * s0 + D;
+			o[1] = spp - 2 * s0 * sp + L + D * D;
+			o[2] = D;
+			o[3] = ybit0<ENC_LOCAL>(lat, lx, ly, y);
+			o[4] = o[5] = o[6] = o[7] = o[8] = o[9] = 0;
+		}
+	}
+}
+
+// The same against a snapshot (correlateTag), 4 sites per lookup: a raw
+// byte indexes the walk table of one lattice ((q + 4) | (A + 10) << 4 |
+// B << 9 over its 4 x slopes, y bits ignored), and the two bytes' x slopes
+// interleaved index the cross table (sum of P P' over the 4 sites, + 16).
+// Cross sums: sum (C + c)(C' + c') = n C C' + C s1' + C' s1 + s11, and the
+// heights' convention gives S11 = sum P P' - s0' sum P - s0 sum P' + L s0 s0'
+// + D D'.  tcs (+1 per equal slope bit, -1 per different one) is summed per
+// word into the word row's first row (the host adds the rows).
+constexpr int RSC_K = 8; // words per lane and round
+
+__global__ __launch_bounds__(256) void row_corr_local_kernel(const uint32_t* __restrict__ lat,
+		const uint32_t* __restrict__ snap, int lx, int ly, int64_t* __restrict__ out)
+{
+	__shared__ uint32_t own[256], cross[256];
+	{
+		const int i = threadIdx.x;
+		int p = 0, A = 0, B = 0, p1 = 0, p2 = 0, X = 0;
+#pragma unroll
+		for(int k = 0; k < 4; ++k)
+		{
+			p += ((i >> (2 * k)) & 1) ? 1 : -1;
+			A += p;
+			B += p * p;
+			p1 += ((i >> (2 * k)) & 1) ? 1 : -1;
+			p2 += ((i >> (2 * k + 1)) & 1) ? 1 : -1;
+			X += p1 * p2;
+		}
+		own[i] = (uint32_t)(p + 4) | ((uint32_t)(A + 10) << 4) | ((uint32_t)B << 9);
+		cross[i] = (uint32_t)(X + 16);
+	}
+	__syncthreads();
+	const int yw = blockIdx.x * 4 + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
+	if(yw >= (1 << (ly - 2)))
+		return; // uniform per wave; no barrier below
+	const int XW = 1 << (lx - 2);
+	const int K = XW < RSC_K ? XW : RSC_K;
+	const int span = 64 * K;
+	const size_t rowOff = (size_t)yw << (lx - 2);
+	const uint32_t* __restrict__ ra = lat + rowOff;
+	const uint32_t* __restrict__ rb = snap + rowOff;
+	int64_t S1[4] = {0, 0, 0, 0}, S2[4] = {0, 0, 0, 0}, T1[4] = {0, 0, 0, 0}, T2[4] = {0, 0, 0, 0};
+	int64_t S11[4] = {0, 0, 0, 0};
+	int base[4] = {0, 0, 0, 0}, baseS[4] = {0, 0, 0, 0};
+	int tcs = 0;
+	for(int w0 = 0; w0 < XW; w0 += span)
+	{
+		const int wl = w0 + lane * K;
+		const bool live = wl < XW;
+		int c[4] = {0, 0, 0, 0}, cs[4] = {0, 0, 0, 0};
+		int s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0}, t1[4] = {0, 0, 0, 0}, t2[4] = {0, 0, 0, 0};
+		int x11[4] = {0, 0, 0, 0};
+		if(live)
+		{
+			for(int k = 0; k < K; ++k)
+			{
+				const uint32_t a = ra[wl + k], b = rb[wl + k];
+				tcs += 2 * __popc(~(a ^ b)) - 32;
+#pragma unroll
+				for(int r = 0; r < 4; ++r)
+				{
+					const uint32_t v = (a >> (8 * r)) & 0xFFu, w = (b >> (8 * r)) & 0xFFu;
+					const uint32_t e = own[v], f = own[w];
+					const int X = (int)cross[(v & 0x55u) | ((w << 1) & 0xAAu)] - 16;
+					const int q = (int)(e & 15u) - 4, A = (int)((e >> 4) & 31u) - 10, B = (int)(e >> 9);
+					const int qs = (int)(f & 15u) - 4, As = (int)((f >> 4) & 31u) - 10, Bs = (int)(f >> 9);
+					s1[r] += 4 * c[r] + A;
+					s2[r] += 4 * c[r] * c[r] + 2 * c[r] * A + B;
+					t1[r] += 4 * cs[r] + As;
+					t2[r] += 4 * cs[r] * cs[r] + 2 * cs[r] * As + Bs;
+					x11[r] += 4 * c[r] * cs[r] + c[r] * As + cs[r] * A + X;
+					c[r] += q;
+					cs[r] += qs;
+				}
+			}
+		}
+		const int n = live ? 4 * K : 0;
+#pragma unroll
+		for(int r = 0; r < 4; ++r)
+		{
+			int incl = c[r], incs = cs[r];
+#pragma unroll
+			for(int o = 1; o < 64; o <<= 1)
+			{
+				const int u = __shfl_up(incl, o, 64), us = __shfl_up(incs, o, 64);
+				if(lane >= o)
+				{
+					incl += u;
+					incs += us;
+				}
+			}
+			const int64_t C = (int64_t)base[r] + (incl - c[r]), Cs = (int64_t)baseS[r] + (incs - cs[r]);
+			S1[r] += (int64_t)n * C + s1[r];
+			S2[r] += (int64_t)n * C * C + 2 * C * s1[r] + s2[r];
+			T1[r] += (int64_t)n * Cs + t1[r];
+			T2[r] += (int64_t)n * Cs * Cs + 2 * Cs * t1[r] + t2[r];
+			S11[r] += (int64_t)n * C * Cs + C * t1[r] + Cs * s1[r] + x11[r];
+			base[r] += __shfl(incl, 63, 64);
+			baseS[r] += __shfl(incs, 63, 64);
+		}
+	}
+	const int64_t L = (int64_t)1 << lx;
+	const int64_t tcsRow = wave_sum(tcs);
+#pragma unroll
+	for(int r = 0; r < 4; ++r)
+	{
+		const int64_t sp = wave_sum(S1[r]), spp = wave_sum(S2[r]);
+		const int64_t tp = wave_sum(T1[r]), tpp = wave_sum(T2[r]), x = wave_sum(S11[r]);
+		if(lane == 0)
+		{
+			const int y = 4 * yw + r;
+			const int64_t D = base[r], Ds = baseS[r];
+			const int64_t s0 = ((ra[0] >> (8 * r)) & 1u) ? 1 : -1, s0s = ((rb[0] >> (8 * r)) & 1u) ? 1 : -1;
+			int64_t* o = out + (size_t)y * ROW_STATS;
+			o[0] = sp - L * s0 + D;
+			o[1] = spp - 2 * s0 * sp + L + D * D;
+			o[2] = D;
+			o[3] = ybit0<ENC_LOCAL>(lat, lx, ly, y);
+			o[4] = tp - L * s0s + Ds;
+			o[5] = Ds;
+			o[6] = ybit0<ENC_LOCAL>(snap, lx, ly, y);
+			o[7] = x - s0s * sp - s0 * tp + L * s0 * s0s + D * Ds;
+			o[8] = (r == 0) ? tcsRow : 0;
+			o[9] = tpp - 2 * s0s * tp + L + Ds * Ds;
+		}
+	}
+}
+
 template<int ENC, bool CORR>
 __global__ __launch_bounds__(256) void row_stats_kernel(const uint32_t* __restrict__ lat, const uint32_t* __restrict__ snap,
 		int lx, int ly, int64_t* __restrict__ out)
@@ -1148,6 +1393,106 @@ __global__ __launch_bounds__(256) void row_stats_kernel(const uint32_t* __restri
 	if(y >= (1 << ly))
 		return; // uniform per wave
 	row_stats_row<ENC, CORR>(lat, snap, lx, ly, y, threadIdx.x & 63, out + (size_t)y * ROW_STATS, false);
+}
+
+// Exact lattice totals from the row statistics, on the device: one 1024-lane
+// workgroup; lane i takes rows [i R, (i+1) R).  Row y starts at height
+// hs_y = hs_{y-1} + D_{y-1} +- 1 (its y slope at x = 0, Correlator::set,
+// kpz/correlator.cpp:50-64), so the start of a lane's range is a workgroup
+// scan of the ranges' height changes.  res = {sum h, sum h^2 lo, hi, sum h_s,
+// sum h h_s lo, hi, tcs}; hs (may be null) receives every row's start height.
+__global__ __launch_bounds__(1024) void row_combine_kernel(const int64_t* __restrict__ rows, int lx, int ly, int corr,
+		int64_t* __restrict__ hs, int64_t* __restrict__ res)
+{
+	__shared__ int64_t sa[1024], sb[1024];
+	__shared__ __int128 r2[1024];
+	const int i = threadIdx.x;
+	const int ny = 1 << ly;
+	const int R = ny > 1024 ? ny / 1024 : 1;
+	const int y0 = i * R, y1 = (y0 + R < ny) ? y0 + R : ny;
+	int64_t da = 0, db = 0;
+	for(int y = y0; y < y1; ++y)
+	{
+		const int64_t* o = rows + (size_t)y * ROW_STATS;
+		da += (o[3] ? 1 : -1) + o[2];
+		if(corr)
+			db += (o[6] ? 1 : -1) + o[5];
+	}
+	// inclusive scan of the range changes, then exclusive starts
+	sa[i] = da;
+	sb[i] = db;
+	__syncthreads();
+	for(int off = 1; off < 1024; off <<= 1)
+	{
+		const int64_t ua = (i >= off) ? sa[i - off] : 0, ub = (i >= off) ? sb[i - off] : 0;
+		__syncthreads();
+		sa[i] += ua;
+		sb[i] += ub;
+		__syncthreads();
+	}
+	int64_t prev = sa[i] - da, prevS = sb[i] - db;
+	const int64_t L = (int64_t)1 << lx;
+	int64_t s1 = 0, t1 = 0, tcs = 0;
+	__int128 s2 = 0, s12 = 0;
+	for(int y = y0; y < y1; ++y)
+	{
+		const int64_t* o = rows + (size_t)y * ROW_STATS;
+		const int64_t h = prev + (o[3] ? 1 : -1);
+		if(hs)
+			hs[y] = h;
+		s1 += L * h + o[0];
+		s2 += (__int128)L * h * h + (__int128)2 * h * o[0] + o[1];
+		prev = h + o[2];
+		if(corr)
+		{
+			const int64_t g = prevS + (o[6] ? 1 : -1);
+			t1 += L * g + o[4];
+			s12 += (__int128)L * h * g + (__int128)h * o[4] + (__int128)g * o[0] + o[7];
+			prevS = g + o[5];
+			tcs += o[8];
+		}
+	}
+	__syncthreads();
+	sa[i] = s1;
+	sb[i] = t1;
+	r2[i] = s2;
+	__syncthreads();
+	for(int off = 512; off > 0; off >>= 1)
+	{
+		if(i < off)
+		{
+			sa[i] += sa[i + off];
+			sb[i] += sb[i + off];
+			r2[i] += r2[i + off];
+		}
+		__syncthreads();
+	}
+	if(i == 0)
+	{
+		res[0] = sa[0];
+		res[1] = (int64_t)(uint64_t)r2[0];
+		res[2] = (int64_t)(r2[0] >> 64);
+		res[3] = sb[0];
+	}
+	__syncthreads();
+	sa[i] = tcs;
+	r2[i] = s12;
+	__syncthreads();
+	for(int off = 512; off > 0; off >>= 1)
+	{
+		if(i < off)
+		{
+			sa[i] += sa[i + off];
+			r2[i] += r2[i + off];
+		}
+		__syncthreads();
+	}
+	if(i == 0)
+	{
+		res[4] = (int64_t)(uint64_t)r2[0];
+		res[5] = (int64_t)(r2[0] >> 64);
+		res[6] = sa[0];
+	}
 }
 
 // Height moments of site row y: out[y*7 + k-1] = sum_x h(x, y)^k, k = 1..7, in
@@ -1539,8 +1884,15 @@ hipError_t launch_row_stats(const uint32_t* lat, const uint32_t* snap, int enc, 
 	const dim3 grid((unsigned)(((1 << ly) + 3) / 4));
 	if(enc == ENC_LOCAL)
 	{
-		if(snap)
+		if(snap && ly >= 2)
+			hipLaunchKernelGGL(row_corr_local_kernel, dim3((unsigned)(((1 << (ly - 2)) + 3) / 4)), dim3(256), 0,
+			                   stream, lat, snap, lx, ly, out);
+		else if(snap)
 			hipLaunchKernelGGL((row_stats_kernel<ENC_LOCAL, true>), grid, dim3(256), 0, stream, lat, snap, lx, ly, out);
+		else if(lx >= 3 && ly >= 2)
+			// one wave per word row, four word rows per block
+			hipLaunchKernelGGL(row_stats_local_kernel, dim3((unsigned)(((1 << (ly - 2)) + 3) / 4)), dim3(256), 0,
+			                   stream, lat, lx, ly, out);
 		else
 			hipLaunchKernelGGL((row_stats_kernel<ENC_LOCAL, false>), grid, dim3(256), 0, stream, lat, snap, lx, ly, out);
 	}
@@ -1551,6 +1903,13 @@ hipError_t launch_row_stats(const uint32_t* lat, const uint32_t* snap, int enc, 
 		else
 			hipLaunchKernelGGL((row_stats_kernel<ENC_CBBIT, false>), grid, dim3(256), 0, stream, lat, snap, lx, ly, out);
 	}
+	return hipGetLastError();
+}
+
+hipError_t launch_row_combine(const int64_t* rows, int lx, int ly, bool corr, int64_t* hs, int64_t* res,
+		hipStream_t stream)
+{
+	hipLaunchKernelGGL(row_combine_kernel, dim3(1), dim3(1024), 0, stream, rows, lx, ly, corr ? 1 : 0, hs, res);
 	return hipGetLastError();
 }
 

@@ -49,7 +49,7 @@ SchedulerService::SchedulerService(int lx, int ly, Encoding enc, const Params& p
 	check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
 	lat_.alloc(size_.sizeW() * sizeof(uint32_t));
 	rows_.alloc((size_t)size_.dimY() * ROW_STATS * sizeof(int64_t));
-	hostRows_.resize((size_t)size_.dimY() * ROW_STATS);
+	totals_.alloc(8 * sizeof(int64_t));
 	master_.reseed(0);
 }
 
@@ -209,68 +209,41 @@ const Affine* SchedulerService::upload(std::vector<Affine>& host, DevBuf& buf)
 	return buf.as<Affine>();
 }
 
-// exact combination of per-row partial sums relative to each row's start
-// height; heights follow Correlator::set (kpz/correlator.cpp:37-106).
-// rowStart (optional) receives every row's start height.
-static void combine_rows(const std::vector<int64_t>& rows, int ly, int lx, bool corr, Sums& s,
-		std::vector<int64_t>* rowStart = nullptr)
+// exact totals: per-row partial sums relative to each row's start height
+// (row_stats), combined on the device in int128 (row_combine); heights follow
+// Correlator::set (kpz/correlator.cpp:37-106).  hs (optional) receives every
+// row's start height.
+Sums SchedulerService::totals(const uint32_t* snap, int64_t* hs)
 {
-	const int64_t L = 1ll << lx;
-	int64_t prev = 0, prevS = 0;
-	__int128 s1 = 0, s2 = 0, t1 = 0, s12 = 0;
-	int64_t tcs = 0;
-	if(rowStart)
-		rowStart->resize((size_t)1 << ly);
-	for(int y = 0; y < (1 << ly); ++y)
-	{
-		const int64_t* o = &rows[(size_t)y * ROW_STATS];
-		const int64_t hs = prev + (o[3] ? 1 : -1);
-		if(rowStart)
-			(*rowStart)[(size_t)y] = hs;
-		s1 += (__int128)L * hs + o[0];
-		s2 += (__int128)L * hs * hs + (__int128)2 * hs * o[0] + o[1];
-		prev = hs + o[2];
-		if(corr)
-		{
-			const int64_t hss = prevS + (o[6] ? 1 : -1);
-			t1 += (__int128)L * hss + o[4];
-			s12 += (__int128)L * hs * hss + (__int128)hs * o[4] + (__int128)hss * o[0] + o[7];
-			prevS = hss + o[5];
-			tcs += o[8];
-		}
-	}
-	s.sumH = (int64_t)s1;
-	s.sumH2 = s2;
-	s.sumHs = (int64_t)t1;
-	s.sumHHs = s12;
-	s.tcs = tcs;
-}
-
-Sums SchedulerService::roughness()
-{
-	check(launch_row_stats(deviceLattice(), nullptr, enc_, size_.lx, size_.ly, rows_.as<int64_t>(), stream_), "row_stats");
-	check(hipMemcpyAsync(hostRows_.data(), rows_.get(), hostRows_.size() * 8, hipMemcpyDeviceToHost, stream_), "row_stats D2H");
+	check(launch_row_stats(deviceLattice(), snap, enc_, size_.lx, size_.ly, rows_.as<int64_t>(), stream_), "row_stats");
+	check(launch_row_combine(rows_.as<int64_t>(), size_.lx, size_.ly, snap != nullptr, hs, totals_.as<int64_t>(), stream_),
+			"row_combine");
+	int64_t r[7];
+	check(hipMemcpyAsync(r, totals_.get(), sizeof r, hipMemcpyDeviceToHost, stream_), "totals D2H");
 	sync();
 	Sums s;
-	combine_rows(hostRows_, size_.ly, size_.lx, false, s);
+	s.sumH = r[0];
+	s.sumH2 = ((__int128)r[2] << 64) | (uint64_t)r[1];
+	if(snap)
+	{
+		s.sumHs = r[3];
+		s.sumHHs = ((__int128)r[5] << 64) | (uint64_t)r[4];
+		s.tcs = r[6];
+	}
 	return s;
 }
+
+Sums SchedulerService::roughness() { return totals(nullptr, nullptr); }
 
 void SchedulerService::moments(long double out[7])
 {
 	const size_t ny = (size_t)1 << size_.ly;
-	check(launch_row_stats(deviceLattice(), nullptr, enc_, size_.lx, size_.ly, rows_.as<int64_t>(), stream_), "row_stats");
-	check(hipMemcpyAsync(hostRows_.data(), rows_.get(), hostRows_.size() * 8, hipMemcpyDeviceToHost, stream_), "row_stats D2H");
-	sync();
-	Sums s;
-	std::vector<int64_t> hs;
-	combine_rows(hostRows_, size_.ly, size_.lx, false, s, &hs);
 	if(!rowStart_.get())
 	{
 		rowStart_.alloc(ny * 8);
 		rowMoments_.alloc(ny * 7 * 8);
 	}
-	check(hipMemcpyAsync(rowStart_.get(), hs.data(), ny * 8, hipMemcpyHostToDevice, stream_), "row starts");
+	totals(nullptr, rowStart_.as<int64_t>()); // every row's exact start height, on the device
 	check(launch_row_moments(deviceLattice(), enc_, size_.lx, size_.ly, rowStart_.as<int64_t>(),
 			rowMoments_.as<double>(), stream_), "row_moments");
 	std::vector<double> m(ny * 7);
@@ -322,15 +295,7 @@ std::vector<SchedulerService::TagResult> SchedulerService::correlateAll()
 	return out;
 }
 
-Sums SchedulerService::correlateWith(const DevBuf& snap)
-{
-	check(launch_row_stats(deviceLattice(), snap.as<uint32_t>(), enc_, size_.lx, size_.ly, rows_.as<int64_t>(), stream_), "row_stats");
-	check(hipMemcpyAsync(hostRows_.data(), rows_.get(), hostRows_.size() * 8, hipMemcpyDeviceToHost, stream_), "row_stats D2H");
-	sync();
-	Sums s;
-	combine_rows(hostRows_, size_.ly, size_.lx, true, s);
-	return s;
-}
+Sums SchedulerService::correlateWith(const DevBuf& snap) { return totals(snap.as<uint32_t>(), nullptr); }
 
 // ======================================================================
 SchedulerDT::SchedulerDT(int lx, int ly, const Params& p, bool randomOrigin)

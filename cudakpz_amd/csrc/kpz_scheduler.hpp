@@ -176,7 +176,8 @@ protected:
 		unsigned int mcs, stop;
 	};
 	Sums correlateWith(const DevBuf& snap);
-	DevBuf lat_, rows_, rowStart_, rowMoments_;
+	Sums totals(const uint32_t* snap, int64_t* hs);
+	DevBuf lat_, rows_, totals_, rowStart_, rowMoments_;
 	std::vector<Tag> tags_;
 	DevBuf st_[2];
 	int cur_ = 0;
@@ -187,7 +188,6 @@ protected:
 	DevBuf dis_;
 	void classThresholds(uint32_t tp[4], uint32_t tq[4]) const;
 	unsigned int mcs_ = 0;
-	std::vector<int64_t> hostRows_;
 };
 
 // Random-sequential double tiling: SchedulerCPUDT (static origin) and
