@@ -41,7 +41,11 @@ def build(arch="gfx950", jobs=8):
 
 class _Params(C.Structure):
     _fields_ = [("threads", C.c_int64), ("layout_x", C.c_int), ("layout_y", C.c_int),
-                ("skip", C.c_uint64), ("device", C.c_int)]
+                ("skip", C.c_uint64), ("device", C.c_int), ("rng", C.c_int)]
+
+
+# worker stream generators (kpz_params.rng)
+RNG_SLCG64, RNG_TINYMT32 = 0, 1
 
 
 def lib():
@@ -90,6 +94,11 @@ def lib():
         L.kpz_device_lattice.restype = C.c_void_p
         L.kpz_stream.argtypes = [h]
         L.kpz_stream.restype = C.c_void_p
+        u32p = C.POINTER(C.c_uint32)
+        L.kpz_rng.argtypes = [h]
+        L.kpz_set_tinymt_params.argtypes = [h, u32p, u32p, u32p]
+        L.kpz_get_tinymt.argtypes = [h, u32p]
+        L.kpz_set_tinymt.argtypes = [h, u32p]
         _LIB = L
     return _LIB
 
@@ -153,10 +162,10 @@ class SchedulerService:
     KIND = None
     ENCODING = None
 
-    def __init__(self, lx, ly, threads=0, layout=(4, 4), skip=0, device=-1, kind=None):
+    def __init__(self, lx, ly, threads=0, layout=(4, 4), skip=0, device=-1, kind=None, rng=RNG_SLCG64):
         self.lx, self.ly = int(lx), int(ly)
         kind = self.KIND if kind is None else kind
-        p = _Params(int(threads), int(layout[0]), int(layout[1]), int(skip), int(device))
+        p = _Params(int(threads), int(layout[0]), int(layout[1]), int(skip), int(device), int(rng))
         h = C.c_void_p()
         _check(lib().kpz_create(kind, self.lx, self.ly, C.byref(p), C.byref(h)))
         self._h = h
@@ -362,11 +371,37 @@ class SchedulerService:
             raise ValueError("one state per stream expected")
         _check(lib().kpz_set_streams(self._h, a.ctypes.data_as(C.POINTER(C.c_uint64))))
 
+    @property
+    def rng(self):
+        return int(lib().kpz_rng(self._h))
+
+    def setTinyMTParams(self, mat1, mat2, tmat):
+        """TinyMT32 streams: parameter set t of stream t (tinyMTmat1/2/tmat,
+        T words each); needed before supplySeed / randomize."""
+        arrs = [np.ascontiguousarray(v, dtype=np.uint32) for v in (mat1, mat2, tmat)]
+        if any(a.size != self.threads for a in arrs):
+            raise ValueError("one parameter set per stream expected")
+        u32p = C.POINTER(C.c_uint32)
+        _check(lib().kpz_set_tinymt_params(self._h, *(a.ctypes.data_as(u32p) for a in arrs)))
+
+    def tinymt(self):
+        """TinyMT32 streams as a (7, T) array: status[0..3], mat1, mat2, tmat."""
+        a = np.empty((7, self.threads), np.uint32)
+        _check(lib().kpz_get_tinymt(self._h, a.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return a
+
+    def set_tinymt(self, planes):
+        a = np.ascontiguousarray(planes, dtype=np.uint32)
+        if a.size != 7 * self.threads:
+            raise ValueError("7 planes of T words expected")
+        _check(lib().kpz_set_tinymt(self._h, a.ctypes.data_as(C.POINTER(C.c_uint32))))
+
     def master_draw(self):
         return float(lib().kpz_master_draw(self._h))
 
     def randomize(self):
-        """SLCG64::randomize: re-seed the T streams from the master."""
+        """Re-seed the T streams from the master (SLCG64::randomize, or
+        RngIface::TinyMT32::randomize: one master draw per stream)."""
         _check(lib().kpz_randomize(self._h))
 
     def master_state(self):
@@ -701,13 +736,13 @@ def lib_h5():
     return _LIB_H5
 
 
-def load_h5(file, kind, prefix="", threads=0, layout=(4, 4), skip=0, device=-1):
+def load_h5(file, kind, prefix="", threads=0, layout=(4, 4), skip=0, device=-1, rng=RNG_SLCG64):
     """SchedulerService::readH5 on a new scheduler of `kind` (KPZ_*): the size
     and MCS count come from the file.  Returns (scheduler, prevMeasureOverride,
     restored), restored being H5_* bits (H5_STREAMS clear: the streams were
     re-seeded from the restored master, as the reference does)."""
     L = lib_h5()
-    p = _Params(int(threads), int(layout[0]), int(layout[1]), int(skip), int(device))
+    p = _Params(int(threads), int(layout[0]), int(layout[1]), int(skip), int(device), int(rng))
     h = C.c_void_p()
     pmo, restored = C.c_uint32(), C.c_uint32()
     _check(L.kpz_read_h5(os.fsencode(file), prefix.encode(), int(kind), C.byref(p), C.byref(h), C.byref(pmo),

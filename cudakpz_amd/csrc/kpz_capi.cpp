@@ -60,6 +60,7 @@ int kpz_create(int kind, int lx, int ly, const kpz_params* params, kpz_scheduler
 		p.layout_y = params->layout_y ? params->layout_y : 4;
 		p.skip = params->skip;
 		p.device = params->device;
+		p.rng = params->rng;
 		if(p.skip && !(p.skip & 1))
 			return fail(KPZ_EINVAL, "SLCG skip must be odd");
 	}
@@ -361,6 +362,32 @@ int kpz_set_streams(kpz_scheduler* h, const uint64_t* states)
 	if(!states)
 		return fail(KPZ_EINVAL, "null input");
 	return guard([&] { h->s->setStreams(states); });
+}
+
+int kpz_rng(const kpz_scheduler* h) { return (h && h->s) ? h->s->rng() : KPZ_EINVAL; }
+
+int kpz_set_tinymt_params(kpz_scheduler* h, const uint32_t* mat1, const uint32_t* mat2, const uint32_t* tmat)
+{
+	HANDLE(h);
+	if(!mat1 || !mat2 || !tmat)
+		return fail(KPZ_EINVAL, "null input");
+	return guard([&] { h->s->setTinyMTParams(mat1, mat2, tmat); });
+}
+
+int kpz_get_tinymt(kpz_scheduler* h, uint32_t* planes)
+{
+	HANDLE(h);
+	if(!planes)
+		return fail(KPZ_EINVAL, "null output");
+	return guard([&] { h->s->getTinyMT(planes); });
+}
+
+int kpz_set_tinymt(kpz_scheduler* h, const uint32_t* planes)
+{
+	HANDLE(h);
+	if(!planes)
+		return fail(KPZ_EINVAL, "null input");
+	return guard([&] { h->s->setTinyMT(planes); });
 }
 
 int kpz_master_bytes(void) { return (int)kpz::MasterRng::STATE_BYTES; }

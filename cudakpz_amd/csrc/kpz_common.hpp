@@ -73,9 +73,10 @@ struct Size {
 	KPZ_HD int dimY() const { return 1 << ly; }
 };
 
-// 23-bit threshold: randomFloat() < p  <=>  r23 < thresh23(p), exactly
-// (randomFloat = r23 * 2^-23, compared in double,
-// kpz/schedulerCPUBaseThreadedRng.inl:34-44)
+// randomFloat() < p  <=>  rf < thresh(p), exactly: randomFloat = rf * 2^-bits
+// compared in double (kpz/schedulerCPUBaseThreadedRng.inl:34-44); 23 bits for
+// SLCG64 ((lo ^ hi) & 0x7FFFFF), 24 for TinyMT32 (tempered >> 8)
 uint32_t thresh23(double p);
+uint32_t thresh24(double p);
 
 } // namespace kpz

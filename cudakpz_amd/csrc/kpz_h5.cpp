@@ -177,8 +177,18 @@ void writeH5(kpz::SchedulerService& s, const char* file, const std::string& pre,
 	s.popSystem(lat.data());
 	std::vector<uint32_t> dsfmt(DSFMT_WORDS, 0u);
 	s.master().save(reinterpret_cast<uint8_t*>(dsfmt.data()));
-	std::vector<uint64_t> streams((size_t)s.threads());
-	s.getStreams(streams.data());
+	std::vector<uint64_t> streams;
+	std::vector<uint32_t> tmt;
+	if(s.tinyMT())
+	{
+		tmt.resize((size_t)s.threads() * 7);
+		s.getTinyMT(tmt.data());
+	}
+	else
+	{
+		streams.resize((size_t)s.threads());
+		s.getStreams(streams.data());
+	}
 	std::vector<kpz::SchedulerService::TagImage> tags = s.exportTags();
 
 	Hid f(H5Fcreate(fname.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose, "cannot create " + fname);
@@ -194,13 +204,23 @@ void writeH5(kpz::SchedulerService& s, const char* file, const std::string& pre,
 		}
 		const hsize_t nd = DSFMT_WORDS;
 		writeDataset(g, pre + ".dsfmt", H5T_NATIVE_UINT32, 1, &nd, dsfmt.data());
-		const hsize_t nt = streams.size();
-		writeDataset(g, pre + ".slcg64", H5T_NATIVE_UINT64, 1, &nt, streams.data());
-		Hid ds(H5Dopen2(g, (pre + ".slcg64").c_str(), H5P_DEFAULT), H5Dclose, "open .slcg64");
-		const uint64_t skip = s.skip();
-		const int64_t threads = s.threads();
-		writeAttr(ds, "skip", H5T_NATIVE_UINT64, &skip);
-		writeAttr(ds, "threads", H5T_NATIVE_INT64, &threads);
+		if(s.tinyMT())
+		{
+			// TinyMT32::writeH5 (kpz/GPUTinyMT.cu:113-120): the 7 planes of T
+			// words, libSplash Dimensions(T, 7) = HDF5 extents {7, T}
+			const hsize_t td[2] = {7, (hsize_t)s.threads()};
+			writeDataset(g, pre + ".tinyMT", H5T_NATIVE_UINT32, 2, td, tmt.data());
+		}
+		else
+		{
+			const hsize_t nt = streams.size();
+			writeDataset(g, pre + ".slcg64", H5T_NATIVE_UINT64, 1, &nt, streams.data());
+			Hid ds(H5Dopen2(g, (pre + ".slcg64").c_str(), H5P_DEFAULT), H5Dclose, "open .slcg64");
+			const uint64_t skip = s.skip();
+			const int64_t threads = s.threads();
+			writeAttr(ds, "skip", H5T_NATIVE_UINT64, &skip);
+			writeAttr(ds, "threads", H5T_NATIVE_INT64, &threads);
+		}
 	}
 	const double nSites = std::ldexp(1.0, sz.lx + sz.ly);
 	for(const auto& t : tags)
@@ -292,6 +312,18 @@ std::unique_ptr<kpz::SchedulerService> readH5(const char* file, const std::strin
 		   readAttr(ds, "threads", H5T_NATIVE_INT64, &fileThreads) && (hsize_t)fileThreads == sd[0])
 			streams = readDataset<uint64_t>(g, pre + ".slcg64", H5T_NATIVE_UINT64, (size_t)sd[0]);
 	}
+	// TinyMT32::readH5 (kpz/GPUTinyMT.cu:122-144): {7, T} planes
+	std::vector<uint32_t> tmt;
+	hsize_t tmtStreams = 0;
+	if(linkExists(g, pre + ".tinyMT"))
+	{
+		const std::vector<hsize_t> td = extents(g, pre + ".tinyMT");
+		if(td.size() == 2 && td[0] == 7)
+		{
+			tmtStreams = td[1];
+			tmt = readDataset<uint32_t>(g, pre + ".tinyMT", H5T_NATIVE_UINT32, (size_t)(7 * td[1]));
+		}
+	}
 	struct TagIn {
 		unsigned int mcs, stop;
 		std::vector<uint32_t> snap;
@@ -348,7 +380,17 @@ std::unique_ptr<kpz::SchedulerService> readH5(const char* file, const std::strin
 		s->master().load(master.data());
 		got |= KPZ_H5_MASTER;
 	}
-	if(!streams.empty() && fileThreads == s->threads() && fileSkip == s->skip())
+	if(s->tinyMT())
+	{
+		// the parameter sets travel with the state; without them the streams
+		// cannot be re-seeded here
+		if(tmt.empty() || tmtStreams != (hsize_t)s->threads())
+			throw Error(KPZ_EINVAL, fname + ": no " + pre + ".tinyMT state for " +
+			                        std::to_string(s->threads()) + " TinyMT32 streams");
+		s->setTinyMT(tmt.data());
+		got |= KPZ_H5_STREAMS;
+	}
+	else if(!streams.empty() && fileThreads == s->threads() && fileSkip == s->skip())
 	{
 		s->setStreams(streams.data());
 		got |= KPZ_H5_STREAMS;
@@ -394,6 +436,7 @@ int kpz_read_h5(const char* file, const char* prefix, int kind, const kpz_params
 		p.layout_y = params->layout_y ? params->layout_y : 4;
 		p.skip = params->skip;
 		p.device = params->device;
+		p.rng = params->rng;
 		if(p.skip && !(p.skip & 1))
 			return kpz::capiFail(KPZ_EINVAL, "SLCG skip must be odd");
 	}

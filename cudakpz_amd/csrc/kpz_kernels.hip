@@ -196,13 +196,168 @@ __device__ __forceinline__ void dt_writeback_x16(uint32_t* __restrict__ lat, int
 	*reinterpret_cast<uint4*>(rp + c0) = q;
 }
 
-// One DT block i of one phase, staged in this lane's LDS column `my`.
-// nb = blocks per domain set.  UPD: 0 = KPZ
+// ======================================================================
+// TinyMT32 worker streams: the 127-bit generator of
+// inc/tinyMT/tinymt32.h:70-107 (transition, tempering) and
+// inc/tinyMT/tinymt32.c:59-75 (seeding).  On the device a stream set is 7
+// planes of T words: status[0..3], mat1, mat2, tmat (the layout of
+// TinyMT32::d_Random, kpz/GPUTinyMT.cu:38-61, and of its `.tinyMT` dataset).
+// ======================================================================
+struct TinyMT {
+	uint32_t s0, s1, s2, s3, m1, m2, tm;
+
+	__device__ __forceinline__ void next()
+	{
+		uint32_t y = s3;
+		uint32_t x = (s0 & 0x7FFFFFFFu) ^ s1 ^ s2;
+		x ^= x << 1;
+		y ^= (y >> 1) ^ x;
+		const uint32_t m = 0u - (y & 1u);
+		s0 = s1;
+		s1 = s2 ^ (m & m1);
+		s2 = x ^ (y << 10) ^ (m & m2);
+		s3 = y;
+	}
+	__device__ __forceinline__ uint32_t temper() const
+	{
+		const uint32_t t1 = s0 + (s2 >> 8);
+		return (s3 ^ t1) ^ ((0u - (t1 & 1u)) & tm);
+	}
+	__device__ __forceinline__ uint32_t u32()
+	{
+		next();
+		return temper();
+	}
+	__device__ __forceinline__ static TinyMT load(const uint32_t* __restrict__ g, int64_t T, int64_t t)
+	{
+		TinyMT r;
+		r.s0 = g[t];
+		r.s1 = g[T + t];
+		r.s2 = g[2 * T + t];
+		r.s3 = g[3 * T + t];
+		r.m1 = g[4 * T + t];
+		r.m2 = g[5 * T + t];
+		r.tm = g[6 * T + t];
+		return r;
+	}
+	__device__ __forceinline__ void store(uint32_t* __restrict__ g, int64_t T, int64_t t) const
+	{
+		g[t] = s0;
+		g[T + t] = s1;
+		g[2 * T + t] = s2;
+		g[3 * T + t] = s3;
+	}
+};
+
+// tinymt32_init with the seed in plane 0 (TinyMT32::randomize + seedGPU,
+// kpz/GPUTinyMT.cu:63-98; the CPU streams are seeded alike,
+// kpz/CPURngIface.h:99-103)
+__global__ __launch_bounds__(256) void tmt_seed_kernel(uint32_t* __restrict__ g, int64_t T)
+{
+	const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if(t >= T)
+		return;
+	TinyMT r = TinyMT::load(g, T, t);
+	uint32_t st[4] = {r.s0, r.m1, r.m2, r.tm};
+#pragma unroll
+	for(int i = 1; i < 8; ++i)
+	{
+		const uint32_t p = st[(i - 1) & 3];
+		st[i & 3] ^= (uint32_t)i + 1812433253u * (p ^ (p >> 30));
+	}
+	if((st[0] & 0x7FFFFFFFu) == 0 && st[1] == 0 && st[2] == 0 && st[3] == 0)
+	{
+		st[0] = 'T';
+		st[1] = 'I';
+		st[2] = 'N';
+		st[3] = 'Y';
+	}
+	r.s0 = st[0];
+	r.s1 = st[1];
+	r.s2 = st[2];
+	r.s3 = st[3];
+#pragma unroll
+	for(int i = 0; i < 8; ++i)
+		r.next();
+	r.store(g, T, t);
+}
+
+// Stream cursors of a DT block.  TOP: the coordinates are the draw's top
+// bits; rf(): the randomFloat() numerator compared with the thresholds.
+//
+// SLCG64: block j of stream t starts j blocks of draws in (jump table).
+// DTr uses every draw as randomFloat() = (lo ^ hi) & 0x7FFFFF: the low state
+// word evolves on its own and the high word is needed mod 2^24 only (one
+// v_mad_u64_u32 and two v_mad_u32_u24 per draw); the full state after the
+// block is one jump from its start.  The static DT's coordinates take the top
+// bits of lo ^ hi and need the whole state.
+template<bool RO>
+struct DtSlcg {
+	static constexpr bool TOP = !RO;
+	__device__ __forceinline__ static uint32_t rf(uint32_t u) { return u & 0x7FFFFFu; }
+	uint64_t s, sBlock;
+	uint32_t lo, h24, aLo, aHi;
+	Affine step;
+	int64_t t;
+	bool last;
+
+	__device__ __forceinline__ void begin(const DtPhaseArgs& a, int64_t i, int64_t nb)
+	{
+		const bool own = a.T >= nb;
+		t = own ? i : i % a.T;
+		const int64_t j = own ? 0 : i / a.T;
+		last = own || i + a.T >= nb;
+		s = a.st_in[t];
+		if(!own)
+			s = a.jumps[j](s);
+		step = a.step;
+		sBlock = s;
+		lo = (uint32_t)s;
+		h24 = (uint32_t)(s >> 32);
+		aLo = (uint32_t)step.a;
+		aHi = (uint32_t)(step.a >> 32);
+	}
+	__device__ __forceinline__ uint32_t draw()
+	{
+		if constexpr(RO)
+		{
+			const uint64_t pr = (uint64_t)lo * aLo + step.c;
+			h24 = mad_u24(h24, aLo, mad_u24(lo, aHi, (uint32_t)(pr >> 32)));
+			lo = (uint32_t)pr;
+			return lo ^ h24; // bits 0..22 are those of the full fold
+		}
+		else
+		{
+			s = step(s);
+			return slcg_fold(s);
+		}
+	}
+	__device__ __forceinline__ void end(const DtPhaseArgs& a)
+	{
+		if(last)
+			a.st_out[t] = RO ? a.blockJ(sBlock) : s;
+	}
+};
+
+// TinyMT32 (SchedulerCPUDTr<TinyMT32>, kpz/kpzFrontend.cpp:308-310):
+// randomFloat() = (u >> 8) * 2^-24, so (int)(randomFloat() * 2^b) = u >> (32-b);
+// the lane holds its stream across its blocks
+struct DtTmt {
+	static constexpr bool TOP = true;
+	__device__ __forceinline__ static uint32_t rf(uint32_t u) { return u >> 8; }
+	TinyMT g;
+	__device__ __forceinline__ void begin(const DtPhaseArgs&, int64_t, int64_t) {}
+	__device__ __forceinline__ uint32_t draw() { return g.u32(); }
+	__device__ __forceinline__ void end(const DtPhaseArgs&) {}
+};
+
+// One DT block i of one phase, staged in this lane's LDS column `my`, with
+// the stream cursor `cur`.  nb = blocks per domain set.  UPD: 0 = KPZ
 // (pqOneUpdate), 1 = pqSyncRngUpdate, 2 = disorder2SyncRngUpdate (class rows
 // of the disorder map in myD, 2 bits per site), 3 = EW.
-template<int LBX, int LBY, int UPD, bool RO>
+template<int LBX, int LBY, int UPD, bool RO, class Cur>
 __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_t nb, const DtTile& my,
-                                         uint32_t* __restrict__ myD)
+                                         uint32_t* __restrict__ myD, Cur& cur)
 {
 	constexpr int BX = 1 << LBX, BY = 1 << LBY;
 	constexpr int NWX = BX / 4 + 1; // words covering BX+1 sites (+ shift)
@@ -304,47 +459,19 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 		}
 	}
 
-	// ---- stream state: block j of stream t starts j blocks of draws in
-	const bool own = a.T >= nb;
-	const int64_t t = own ? i : i % a.T;
-	const int64_t j = own ? 0 : i / a.T;
-	uint64_t s = a.st_in[t];
-	if(!own)
-		s = a.jumps[j](s);
-	const Affine step = a.step;
+	// ---- the block's stream
+	cur.begin(a, i, nb);
 	const uint32_t tp = a.tp, tq = a.tq;
-	// DTr uses every draw as randomFloat() = (lo ^ hi) & 0x7FFFFF: the low
-	// state word evolves on its own and the high word is needed mod 2^24 only
-	// (one v_mad_u64_u32 and two v_mad_u32_u24 per draw); the full state after
-	// the block is one jump from its start.  The static DT's coordinates take
-	// the top bits of lo ^ hi and need the whole state.
-	const uint64_t sBlock = s;
-	uint32_t lo = (uint32_t)s, h24 = (uint32_t)(s >> 32);
-	const uint32_t aLo = (uint32_t)step.a, aHi = (uint32_t)(step.a >> 32);
-	auto draw = [&]() -> uint32_t {
-		if constexpr(RO)
-		{
-			const uint64_t pr = (uint64_t)lo * aLo + step.c;
-			h24 = mad_u24(h24, aLo, mad_u24(lo, aHi, (uint32_t)(pr >> 32)));
-			lo = (uint32_t)pr;
-			return lo ^ h24; // bits 0..22 are those of the full fold
-		}
-		else
-		{
-			s = step(s);
-			return slcg_fold(s);
-		}
-	};
 
 	// ---- BX*BY random-sequential updates (Func_MCSBase::exec,
 	// kpz/schedulerCPUDT_nested.h:46-66; pqOneUpdate kpz/schedulerCPU.inl:21-33)
 #pragma unroll 2
 	for(int k = 0; k < BX * BY; ++k)
 	{
-		const uint32_t r1 = draw();
-		const uint32_t r2 = draw();
+		const uint32_t r1 = cur.draw();
+		const uint32_t r2 = cur.draw();
 		int x, y;
-		if(RO)
+		if constexpr(!Cur::TOP)
 		{
 			x = (int)((r1 & 0x7FFFFFu) >> (23 - LBX));
 			y = (int)((r2 & 0x7FFFFFu) >> (23 - LBY));
@@ -384,7 +511,7 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 		}
 		else
 		{
-			const uint32_t r23 = draw() & 0x7FFFFFu;
+			const uint32_t r23 = Cur::rf(cur.draw());
 			uint32_t tpv = tp, tqv = tq;
 			if(UPD == 2)
 			{
@@ -392,7 +519,7 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 				tpv = (cls & 2) ? ((cls & 1) ? a.tpc[3] : a.tpc[2]) : ((cls & 1) ? a.tpc[1] : a.tpc[0]);
 				tqv = (cls & 2) ? ((cls & 1) ? a.tqc[3] : a.tqc[2]) : ((cls & 1) ? a.tqc[1] : a.tqc[0]);
 			}
-			// all-ones when accepted: sign of r23 - t (both below 2^24)
+			// all-ones when accepted: sign of r23 - t (both at most 2^24)
 			const uint32_t aMin = (uint32_t)((int32_t)(r23 - tpv) >> 31);
 			const uint32_t aMax = (uint32_t)((int32_t)(r23 - tqv) >> 31);
 			const uint32_t mn = rx & ~row.x & ~row.y & up;
@@ -404,8 +531,7 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 		}
 	}
 
-	if(own || i + a.T >= nb)
-		a.st_out[t] = RO ? a.blockJ(sBlock) : s;
+	cur.end(a);
 
 	// ---- bit-plane rows -> ENC_LOCAL words (masked read-modify-write at the
 	// block edges; no other lane of this phase touches these words)
@@ -457,8 +583,28 @@ __global__ __launch_bounds__(DT_WG) void dt_phase_kernel(DtPhaseArgs a, int64_t 
 	const int64_t i = (int64_t)blockIdx.x * DT_WG + threadIdx.x;
 	if(i >= nb)
 		return; // no barrier in this kernel: idle lanes may leave
+	DtSlcg<RO> cur;
 	dt_block<LBX, LBY, UPD, RO>(a, i, nb, DtTile{tile + threadIdx.x, tile + (1 << LBY) * DT_WG + threadIdx.x},
-	                            (UPD == 2) ? tileD + threadIdx.x : nullptr);
+	                            (UPD == 2) ? tileD + threadIdx.x : nullptr, cur);
+}
+
+// SchedulerCPUDTr<TinyMT32>: lane t is worker t and updates its blocks
+// t, t+T, ... of the domain set in order with its own stream
+// (SchedulerCPUDTr::Func_MCS::exec, kpz/schedulerCPUDTr.cpp:147-175)
+template<int LBX, int LBY, int UPD>
+__global__ __launch_bounds__(DT_WG) void dt_tmt_kernel(DtPhaseArgs a, int64_t nb, uint32_t* __restrict__ g)
+{
+	__shared__ uint32_t tile[(2 * (1 << LBY) + 1) * DT_WG];
+	__shared__ uint32_t tileD[(UPD == 2) ? (1 << LBY) * DT_WG : 1];
+	const int64_t t = (int64_t)blockIdx.x * DT_WG + threadIdx.x;
+	if(t >= a.T || t >= nb)
+		return; // no barrier in this kernel; workers without a block keep their stream
+	DtTmt cur;
+	cur.g = TinyMT::load(g, a.T, t);
+	const DtTile my{tile + threadIdx.x, tile + (1 << LBY) * DT_WG + threadIdx.x};
+	for(int64_t i = t; i < nb; i += a.T)
+		dt_block<LBX, LBY, UPD, true>(a, i, nb, my, (UPD == 2) ? tileD + threadIdx.x : nullptr, cur);
+	cur.g.store(g, a.T, t);
 }
 
 // ======================================================================
@@ -923,6 +1069,195 @@ __global__ __launch_bounds__(256) void cb_local_fix_kernel(CbLocalArgs a)
 		if(v)
 			a.lat[((size_t)r << lXC) + (((cg + 1) * G) & (XC - 1))] ^= v;
 	}
+}
+
+// ======================================================================
+// TinyMT32 workers of the checkerboard schedulers (kpz/kpzFrontend.cpp:
+// 324-331).  A TinyMT32 stream has no cheap jump ahead, so lane t is worker t
+// and replays the worker's own walk with its own stream.
+// ======================================================================
+// SchedulerCPUCBBit<TinyMT32>, one half-sweep: worker t walks its rows
+// [yMin, ySup) ascending and each row's words ascending
+// (kpz/schedulerCPUCBBit.cpp:43-79); MODE 0 = pSyncRngUpdateVL3 (two draws
+// per u64 word, the first the high half; one per u32 word when L = 64),
+// 1 = per-site pq, 2 = disorder2SyncRngUpdate (one randomFloat per active
+// site, x ascending, kpz/schedulerCPUCBBit_nested.inl:200-331).  A row writes
+// its own words and the other sublattice's Y word of the row above, which no
+// other row of the half-sweep reads or writes: workers never share a word.
+template<class W, int MODE>
+__global__ __launch_bounds__(256) void cbbit_tmt_kernel(CbHalfArgs a, uint32_t* __restrict__ g)
+{
+	const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if(t >= a.T)
+		return;
+	constexpr int WB = (int)sizeof(W) * 8;
+	constexpr W MSB = (W)1 << (WB - 1);
+	TinyMT r = TinyMT::load(g, a.T, t);
+	const int64_t y0 = t * a.per + (t < a.rem ? t : a.rem);
+	const int64_t y1 = y0 + a.per + (t < a.rem ? 1 : 0);
+	W* __restrict__ lat = reinterpret_cast<W*>(a.lat);
+	const W* __restrict__ dis = reinterpret_cast<const W*>(a.dis);
+	const uint64_t Wn = 1ull << (a.lx - (WB == 64 ? 7 : 6)); // words per row and quadrant
+	const uint64_t Q = Wn << a.ly;                            // words per quadrant
+	const uint64_t thisBase = a.signum ? 2 * Q : 0, othBase = a.signum ? 0 : 2 * Q;
+	const int64_t mY = (1ll << a.ly) - 1;
+	for(int64_t y = y0; y < y1; ++y)
+	{
+		const bool parity = a.signum == (int)(y & 1);
+		const uint64_t rowOff = (uint64_t)y * Wn, upOff = (uint64_t)((y + 1) & mY) * Wn;
+		for(uint64_t xw = 0; xw < Wn; ++xw)
+		{
+			const uint64_t off = rowOff + xw;
+			const uint64_t tXi = thisBase + off, tYi = thisBase + Q + off;
+			const uint64_t yIdx = othBase + Q + upOff + xw, xIdx = othBase + off;
+			const W tX = lat[tXi], tY = lat[tYi], yY = lat[yIdx];
+			W xX = lat[xIdx];
+			uint64_t nIdx = xIdx;
+			if(!parity)
+			{
+				nIdx = (xw + 1 == Wn) ? xIdx - xw : xIdx + 1;
+				xX = ((xX >> 1) & ~MSB) | ((W)(lat[nIdx] << (WB - 1)) & MSB);
+			}
+			W m;
+			if constexpr(MODE == 0)
+			{
+				W rnd;
+				if constexpr(WB == 64)
+				{
+					const uint64_t hi = r.u32();
+					const uint64_t lo = r.u32();
+					rnd = (W)((hi << 32) ^ lo);
+				}
+				else
+					rnd = (W)r.u32();
+				m = ~(tX | tY) & xX & yY & rnd;
+			}
+			else
+			{
+				// minima (own X = Y = 0, right X = up Y = 1) and maxima, per site
+				const W mn = ~(tX | tY) & xX & yY;
+				const W mx = tX & tY & ~xX & ~yY;
+				const W cl = (MODE == 2) ? dis[tXi] : (W)0; // class: the site's x-slope bit of the map
+				m = 0;
+				for(int k = 0; k < WB; ++k)
+				{
+					const uint32_t rf = r.u32() >> 8;
+					const bool c1 = (cl >> k) & 1;
+					const uint32_t tp = c1 ? a.tp1 : a.tp, tq = c1 ? a.tq1 : a.tq;
+					const W acc = (W)(rf < tp ? mn : 0) | (W)(rf < tq ? mx : 0);
+					m |= acc & ((W)1 << k);
+				}
+			}
+			W xm = m;
+			if(!parity)
+			{
+				lat[nIdx] = lat[nIdx] ^ (W)((m >> (WB - 1)) & 1);
+				xm = (W)(m << 1);
+			}
+			lat[xIdx] = lat[xIdx] ^ xm;
+			lat[tXi] = tX ^ m;
+			lat[tYi] = tY ^ m;
+			lat[yIdx] = yY ^ m;
+		}
+	}
+	r.store(g, a.T, t);
+}
+
+// SchedulerCPUCB<TinyMT32>, one half-sweep: worker t walks the first half of
+// its strip of cell columns, then the second, cell row by cell row and cell by
+// cell, two active sites per site row, one randomFloat() each
+// (kpz/schedulerCPUCB.cpp:128-168).  Every update of a half-sweep reads only
+// slope bits that no other update of the half-sweep flips, so a cell's flips
+// follow from its words as loaded, and the result does not depend on the
+// order across workers.  A strip's first cell column also takes the right-edge
+// flips of the strip to its left: it is updated with atomics, every other
+// word by its worker alone.
+template<bool DIS>
+__global__ __launch_bounds__(256) void cb_local_tmt_kernel(CbLocalArgs a, int64_t T, uint32_t* __restrict__ g)
+{
+	const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if(t >= T)
+		return;
+	TinyMT r = TinyMT::load(g, T, t);
+	const int64_t XC = 1ll << (a.lx - 2), YC = 1ll << (a.ly - 2);
+	const int64_t w = a.pc + (t < a.rem ? 1 : 0);
+	const int64_t c0 = t * a.pc + (t < a.rem ? t : a.rem);
+	const int64_t wl = w / 2;
+	const int s = a.signum;
+	uint32_t* __restrict__ lat = a.lat;
+	for(int half = 0; half < 2; ++half)
+	{
+		const int64_t cb = half ? c0 + wl : c0, ce = half ? c0 + w : c0 + wl;
+		for(int64_t cy = 0; cy < YC; ++cy)
+		{
+			const size_t rowBase = (size_t)(cy * XC), upBase = (size_t)(((cy + 1) & (YC - 1)) * XC);
+			for(int64_t cx = cb; cx < ce; ++cx)
+			{
+				const size_t io = rowBase + (size_t)cx, iu = upBase + (size_t)cx;
+				const size_t ir = rowBase + (size_t)((cx + 1) & (XC - 1));
+				const uint32_t wo = lat[io], wr = lat[ir], wu = lat[iu];
+				const uint32_t wd = DIS ? a.dis[io] : 0u;
+				uint32_t fo = 0, fr = 0, fu = 0;
+#pragma unroll
+				for(int rr = 0; rr < 4; ++rr)
+				{
+#pragma unroll
+					for(int k = 0; k < 2; ++k)
+					{
+						// x & 3 of the active sites of site row rr: ((y & 1) ^ s) and + 2
+						const int c = ((rr & 1) ^ s) + 2 * k;
+						const int sh = 8 * rr + 2 * c;
+						const uint32_t rf = r.u32() >> 8;
+						const uint32_t rx = (c < 3) ? ((wo >> (sh + 2)) & 1u) : ((wr >> (8 * rr)) & 1u);
+						const uint32_t uy = (rr < 3) ? ((wo >> (sh + 9)) & 1u) : ((wu >> (2 * c + 1)) & 1u);
+						const uint32_t cfg = ((wo >> sh) & 3u) | (rx << 2) | (uy << 3);
+						uint32_t tp = a.tp, tq = a.tq;
+						if(DIS)
+						{
+							const uint32_t cls = (wd >> sh) & 3u;
+							tp = a.tpc[cls];
+							tq = a.tqc[cls];
+						}
+						if((cfg == 0xCu && rf < tp) || (cfg == 0x3u && rf < tq))
+						{
+							fo ^= 3u << sh;
+							if(c < 3)
+								fo ^= 1u << (sh + 2);
+							else
+								fr ^= 1u << (8 * rr);
+							if(rr < 3)
+								fo ^= 1u << (sh + 9);
+							else
+								fu ^= 1u << (2 * c + 1);
+						}
+					}
+				}
+				const bool first = cx == c0;
+				if(fo)
+				{
+					if(first)
+						atomicXor(lat + io, fo);
+					else
+						lat[io] = wo ^ fo;
+				}
+				if(fr)
+				{
+					if(cx + 1 == c0 + w)
+						atomicXor(lat + ir, fr); // the next strip's first column
+					else
+						lat[ir] = wr ^ fr;
+				}
+				if(fu)
+				{
+					if(first)
+						atomicXor(lat + iu, fu);
+					else
+						lat[iu] = wu ^ fu;
+				}
+			}
+		}
+	}
+	r.store(g, T, t);
 }
 
 // ======================================================================
@@ -1673,18 +2008,20 @@ __global__ __launch_bounds__(DT_WG) void batch_dt_kernel(BatchDtArgs a)
 		const uint16_t sh = a.shift[mi];
 		p.sx = sh & 0xFF;
 		p.sy = sh >> 8;
+		DtSlcg<true> cur;
 		if(rep.mode)
-			dt_block<4, 4, 1, true>(p, w.y, nb, my, nullptr);
+			dt_block<4, 4, 1, true>(p, w.y, nb, my, nullptr, cur);
 		else
-			dt_block<4, 4, 0, true>(p, w.y, nb, my, nullptr);
+			dt_block<4, 4, 0, true>(p, w.y, nb, my, nullptr, cur);
 	}
 	else
 	{
 		p.sx = p.sy = 0;
+		DtSlcg<false> cur;
 		if(rep.mode)
-			dt_block<4, 4, 1, false>(p, w.y, nb, my, nullptr);
+			dt_block<4, 4, 1, false>(p, w.y, nb, my, nullptr, cur);
 		else
-			dt_block<4, 4, 0, false>(p, w.y, nb, my, nullptr);
+			dt_block<4, 4, 0, false>(p, w.y, nb, my, nullptr, cur);
 	}
 }
 
@@ -1780,7 +2117,13 @@ __global__ void batch_combine_kernel(const BatchRep* __restrict__ reps, int nrep
 template<int LBX, int LBY, int UPD>
 void dt_launch(const DtPhaseArgs& a, int64_t nb, dim3 grid, hipStream_t stream)
 {
-	if(a.random_origin)
+	if(a.tmt)
+	{
+		const int64_t lanes = a.T < nb ? a.T : nb;
+		hipLaunchKernelGGL((dt_tmt_kernel<LBX, LBY, UPD>), dim3((unsigned)((lanes + DT_WG - 1) / DT_WG)), dim3(DT_WG), 0,
+		                   stream, a, nb, a.tmt);
+	}
+	else if(a.random_origin)
 		hipLaunchKernelGGL((dt_phase_kernel<LBX, LBY, UPD, true>), grid, dim3(DT_WG), 0, stream, a, nb);
 	else
 		hipLaunchKernelGGL((dt_phase_kernel<LBX, LBY, UPD, false>), grid, dim3(DT_WG), 0, stream, a, nb);
@@ -1820,9 +2163,17 @@ hipError_t dt_dispatch_y(const DtPhaseArgs& a, int64_t nb, dim3 grid, hipStream_
 
 bool dt_layout_supported(int lbx, int lby) { return lbx >= 2 && lbx <= 4 && lby >= 2 && lby <= 4; }
 
+hipError_t launch_tmt_seed(uint32_t* g, int64_t T, hipStream_t stream)
+{
+	if(!g || T < 1)
+		return hipErrorInvalidValue;
+	hipLaunchKernelGGL(tmt_seed_kernel, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, stream, g, T);
+	return hipGetLastError();
+}
+
 hipError_t launch_dt_phase(const DtPhaseArgs& a, hipStream_t stream)
 {
-	if(!dt_layout_supported(a.lbx, a.lby))
+	if(!dt_layout_supported(a.lbx, a.lby) || a.T < 1 || (a.tmt && !a.random_origin))
 		return hipErrorInvalidValue;
 	const int64_t nb = 1ll << (a.lx - a.lbx + a.ly - a.lby - 2);
 	const dim3 grid((unsigned)((nb + DT_WG - 1) / DT_WG));
@@ -1837,6 +2188,23 @@ hipError_t launch_dt_phase(const DtPhaseArgs& a, hipStream_t stream)
 
 hipError_t launch_cbbit_half(const CbHalfArgs& a, hipStream_t stream)
 {
+	if(a.tmt)
+	{
+		if(a.lx < 6 || a.ly < 1 || a.T < 1 || a.mode < 0 || a.mode > 2 || (a.mode == 2 && !a.dis))
+			return hipErrorInvalidValue;
+		const dim3 grid((unsigned)((a.T + 255) / 256));
+		const bool wide = a.lx >= 7;
+		switch(a.mode * 2 + (wide ? 1 : 0))
+		{
+			case 0: hipLaunchKernelGGL((cbbit_tmt_kernel<uint32_t, 0>), grid, dim3(256), 0, stream, a, a.tmt); break;
+			case 1: hipLaunchKernelGGL((cbbit_tmt_kernel<uint64_t, 0>), grid, dim3(256), 0, stream, a, a.tmt); break;
+			case 2: hipLaunchKernelGGL((cbbit_tmt_kernel<uint32_t, 1>), grid, dim3(256), 0, stream, a, a.tmt); break;
+			case 3: hipLaunchKernelGGL((cbbit_tmt_kernel<uint64_t, 1>), grid, dim3(256), 0, stream, a, a.tmt); break;
+			case 4: hipLaunchKernelGGL((cbbit_tmt_kernel<uint32_t, 2>), grid, dim3(256), 0, stream, a, a.tmt); break;
+			default: hipLaunchKernelGGL((cbbit_tmt_kernel<uint64_t, 2>), grid, dim3(256), 0, stream, a, a.tmt); break;
+		}
+		return hipGetLastError();
+	}
 	const int64_t threads = (int64_t)1 << (a.ly + a.lG);
 	const int block = (int)(threads < 256 ? threads : 256);
 	const dim3 grid((unsigned)(threads / block));
@@ -1926,6 +2294,19 @@ hipError_t launch_fill_flat(uint32_t* lat, int enc, int lx, int ly, hipStream_t 
 
 hipError_t launch_cb_local_half(const CbLocalArgs& a, hipStream_t stream)
 {
+	if(a.tmt)
+	{
+		// one lane per worker; strips of at least one cell column
+		if(a.lx < 3 || a.ly < 3 || a.T < 1 || a.T > (1ll << (a.lx - 2)) || a.pc != (int)((1ll << (a.lx - 2)) / a.T) ||
+		   a.rem != (int)((1ll << (a.lx - 2)) % a.T))
+			return hipErrorInvalidValue;
+		const dim3 grid((unsigned)((a.T + 255) / 256));
+		if(a.dis)
+			hipLaunchKernelGGL((cb_local_tmt_kernel<true>), grid, dim3(256), 0, stream, a, a.T, a.tmt);
+		else
+			hipLaunchKernelGGL((cb_local_tmt_kernel<false>), grid, dim3(256), 0, stream, a, a.T, a.tmt);
+		return hipGetLastError();
+	}
 	// the band kernel's shape: groups of 2^lG <= 64 columns inside a row of
 	// 2^(lx-2) cells, bands of 2^lNB cell rows, at least 2 cell rows
 	// (and a lattice below 4 GiB: the band kernel addresses it with 32-bit byte offsets)

@@ -27,8 +27,14 @@ struct DtPhaseArgs {
 	uint32_t tp, tq;        // 23-bit thresholds
 	const uint32_t* dis;    // upd 2: disorder map (ENC_LOCAL layout, 2 bits = class per site)
 	uint32_t tpc[4], tqc[4]; // upd 2: thresholds per class
+	uint32_t* tmt;          // TinyMT32 streams (7 planes of T words; random origin only;
+	                        // thresholds then on the 2^24 scale), or null for SLCG64
 };
 hipError_t launch_dt_phase(const DtPhaseArgs& a, hipStream_t stream);
+
+// ---- TinyMT32 stream sets: tinymt32_init of every stream from the seed in
+// plane 0 and the parameters in planes 4..6 (7 planes of T words)
+hipError_t launch_tmt_seed(uint32_t* g, int64_t T, hipStream_t stream);
 bool dt_layout_supported(int lbx, int lby);
 
 // ---- SCABit: one checkerboard half-sweep on ENC_CBBIT ------------------
@@ -51,6 +57,9 @@ struct CbHalfArgs {
 	// class = x-slope bit of the site in the map (ENC_CBBIT layout), class-1 thresholds
 	const uint32_t* dis;
 	uint32_t tp1, tq1;
+	// TinyMT32 streams (7 planes of T words), or null for SLCG64: then one lane
+	// per worker, thresholds on the 2^24 scale, the jump tables unused
+	uint32_t* tmt;
 };
 hipError_t launch_cbbit_half(const CbHalfArgs& a, hipStream_t stream);
 
@@ -81,8 +90,13 @@ struct CbLocalArgs {
 	// layout (null: plain pq) and thresholds per class
 	const uint32_t* dis;
 	uint32_t tpc[4], tqc[4];
+	// TinyMT32 streams (7 planes of T words), or null for SLCG64: then one
+	// launch, one lane per worker, thresholds on the 2^24 scale
+	uint32_t* tmt;
+	int64_t T;
 };
-// one half-sweep: the band kernel plus the edge fix-up (two launches)
+// one half-sweep: the band kernel plus the edge fix-up (two launches), or
+// the TinyMT32 worker kernel (one launch)
 hipError_t launch_cb_local_half(const CbLocalArgs& a, hipStream_t stream);
 
 // ---- reductions ----------------------------------------------------------

@@ -81,13 +81,16 @@ void MasterRng::load(const uint8_t* in)
 	std::memcpy(&idx_, in + sizeof w_, sizeof idx_);
 }
 
-uint32_t thresh23(double p)
+static uint32_t thresh_bits(double p, int bits)
 {
 	if(!(p > 0.0))
 		return 0;
 	if(p >= 1.0)
-		return 1u << 23;
-	return (uint32_t)std::ceil(p * 8388608.0);
+		return 1u << bits;
+	return (uint32_t)std::ceil(p * (double)(1u << bits));
 }
+
+uint32_t thresh23(double p) { return thresh_bits(p, 23); }
+uint32_t thresh24(double p) { return thresh_bits(p, 24); }
 
 } // namespace kpz

@@ -42,6 +42,8 @@ SchedulerService::SchedulerService(int lx, int ly, Encoding enc, const Params& p
 {
 	if(lx < 2 || ly < 2 || lx > 30 || ly > 30)
 		throw Error(-1, "system size out of range (log2 sizes 2..30)");
+	if(p.rng != RNG_SLCG64 && p.rng != RNG_TINYMT32)
+		throw Error(-1, "unknown stream generator (0 = SLCG64, 1 = TinyMT32)");
 	if(device_ >= 0)
 		check(hipSetDevice(device_), "hipSetDevice");
 	else
@@ -73,9 +75,35 @@ void SchedulerService::supplySeed(uint64_t seed)
 	randomize();
 }
 
-// SLCG64::randomize, kpz/GPURandom.cu:42-54 (first master draw = low half)
+void SchedulerService::allocStreams(bool twoBuffers)
+{
+	if(tinyMT())
+	{
+		tmt_.alloc((size_t)T_ * 7 * 4);
+		return;
+	}
+	st_[0].alloc((size_t)T_ * 8);
+	if(twoBuffers)
+		st_[1].alloc((size_t)T_ * 8);
+}
+
+// SLCG64::randomize, kpz/GPURandom.cu:42-54 (first master draw = low half);
+// TinyMT32: RngIface::TinyMT32::randomize (kpz/CPURngIface.h:99-103), one
+// master draw per stream as its tinymt32_init seed
 void SchedulerService::randomize()
 {
+	if(tinyMT())
+	{
+		if(!tmtParams_)
+			throw Error(-4, "TinyMT32 streams need their parameter sets first (setTinyMTParams)");
+		std::vector<uint32_t> seed((size_t)T_);
+		for(auto& v : seed)
+			v = (uint32_t)(master_.close_open() * 4294967295.0);
+		check(hipMemcpyAsync(tmt_.get(), seed.data(), seed.size() * 4, hipMemcpyHostToDevice, stream_), "seeds H2D");
+		check(launch_tmt_seed(tmt_.as<uint32_t>(), T_, stream_), "tmt_seed");
+		sync();
+		return;
+	}
 	const double m1 = master_.close_open();
 	const double m2 = master_.close_open();
 	uint64_t s = (uint64_t)(m1 * 4294967295.0) ^ ((uint64_t)(m2 * 4294967295.0) << 32);
@@ -116,8 +144,8 @@ void SchedulerService::classThresholds(uint32_t tp[4], uint32_t tq[4]) const
 {
 	for(unsigned k = 0; k < 4; ++k)
 	{
-		tp[k] = thresh23(rateP(k));
-		tq[k] = thresh23(rateQ(k));
+		tp[k] = thresh(rateP(k));
+		tq[k] = thresh(rateQ(k));
 	}
 }
 
@@ -191,14 +219,48 @@ void SchedulerService::sync() { check(hipStreamSynchronize(stream_), "hipStreamS
 
 void SchedulerService::getStreams(uint64_t* out)
 {
+	if(tinyMT())
+		throw Error(-1, "TinyMT32 streams: use getTinyMT");
 	check(hipMemcpyAsync(out, stIn(), (size_t)T_ * 8, hipMemcpyDeviceToHost, stream_), "getStreams");
 	sync();
 }
 
 void SchedulerService::setStreams(const uint64_t* in)
 {
+	if(tinyMT())
+		throw Error(-1, "TinyMT32 streams: use setTinyMT");
 	check(hipMemcpyAsync(stIn(), in, (size_t)T_ * 8, hipMemcpyHostToDevice, stream_), "setStreams");
 	sync();
+}
+
+void SchedulerService::setTinyMTParams(const uint32_t* mat1, const uint32_t* mat2, const uint32_t* tmat)
+{
+	if(!tinyMT())
+		throw Error(-1, "setTinyMTParams: this scheduler uses SLCG64 streams");
+	const size_t T = (size_t)T_;
+	const uint32_t* src[3] = {mat1, mat2, tmat};
+	for(int k = 0; k < 3; ++k)
+		check(hipMemcpyAsync(tmt_.as<uint32_t>() + (4 + k) * T, src[k], T * 4, hipMemcpyHostToDevice, stream_),
+				"TinyMT parameters H2D");
+	sync();
+	tmtParams_ = true;
+}
+
+void SchedulerService::getTinyMT(uint32_t* out)
+{
+	if(!tinyMT())
+		throw Error(-1, "getTinyMT: this scheduler uses SLCG64 streams");
+	check(hipMemcpyAsync(out, tmt_.get(), (size_t)T_ * 28, hipMemcpyDeviceToHost, stream_), "getTinyMT");
+	sync();
+}
+
+void SchedulerService::setTinyMT(const uint32_t* in)
+{
+	if(!tinyMT())
+		throw Error(-1, "setTinyMT: this scheduler uses SLCG64 streams");
+	check(hipMemcpyAsync(tmt_.get(), in, (size_t)T_ * 28, hipMemcpyHostToDevice, stream_), "setTinyMT");
+	sync();
+	tmtParams_ = true;
 }
 
 const Affine* SchedulerService::upload(std::vector<Affine>& host, DevBuf& buf)
@@ -308,12 +370,13 @@ SchedulerDT::SchedulerDT(int lx, int ly, const Params& p, bool randomOrigin)
 	nb_ = 1ll << (lx - lbx_ + ly - lby_ - 2);
 	if(T_ <= 0)
 		T_ = nb_;
+	if(tinyMT() && !ro_)
+		throw Error(-1, "TinyMT32 workers exist for the random-origin DT (SchedulerCPUDTr); "
+		                "SchedulerCPUDT draws close_open(), which RngIface::TinyMT32 lacks");
 	if(S_ == 0)
 		S_ = default_skip((uint64_t)T_);
 	step_ = slcg_skip(S_);
-	st_[0].alloc((size_t)T_ * 8);
-	if(T_ < nb_)
-		st_[1].alloc((size_t)T_ * 8);
+	allocStreams(T_ < nb_);
 	p_ = 1.0;
 	q_ = 0.0;
 }
@@ -355,13 +418,14 @@ void SchedulerDT::run(int n, int upd)
 	a.lbx = lbx_;
 	a.lby = lby_;
 	a.T = T_;
-	a.jumps = jumps(du);
+	a.tmt = tinyMT() ? tmt_.as<uint32_t>() : nullptr;
+	a.jumps = tinyMT() ? nullptr : jumps(du);
 	a.step = step_;
 	a.blockJ = affine_pow(step_, (uint64_t)du << (lbx_ + lby_));
 	a.upd = upd;
 	a.random_origin = ro_;
-	a.tp = thresh23(upd ? p_ : 1.0);
-	a.tq = thresh23(upd ? q_ : 0.0);
+	a.tp = thresh(upd ? p_ : 1.0);
+	a.tq = thresh(upd ? q_ : 0.0);
 	for(int m = 0; m < n; ++m)
 	{
 		// SchedulerCPUDT::mcs (kpz/schedulerCPUDT.cpp:65-90) / SchedulerCPUDTr::doMcs
@@ -377,6 +441,11 @@ void SchedulerDT::run(int n, int upd)
 		for(int d = 0; d < 4; ++d)
 		{
 			a.ds = seq[d];
+			if(tinyMT())
+			{
+				check(launch_dt_phase(a, stream_), "dt_phase");
+				continue;
+			}
 			a.st_in = stIn();
 			a.st_out = (T_ < nb_) ? stOut() : stIn();
 			check(launch_dt_phase(a, stream_), "dt_phase");
@@ -412,8 +481,7 @@ SchedulerSCABit::SchedulerSCABit(int lx, int ly, const Params& p)
 	const int lw = (lx >= 7) ? 7 : 6;
 	const int lWn = lx - lw;
 	lG_ = lWn < 6 ? lWn : 6;
-	st_[0].alloc((size_t)T_ * 8);
-	st_[1].alloc((size_t)T_ * 8);
+	allocStreams(true);
 	p_ = 0.5;
 	q_ = 0.0;
 }
@@ -458,6 +526,34 @@ void SchedulerSCABit::run(int n, int mode)
 {
 	if(mode == 2 && !hasDisorder())
 		throw Error(-4, "mcsDisorder2SyncRng needs a disorder map (generateDisorder2 / setDisorder)");
+	if(tinyMT())
+	{
+		// one lane per worker with its own TinyMT32 stream
+		CbHalfArgs a{};
+		a.lat = deviceLattice();
+		a.lx = size_.lx;
+		a.ly = size_.ly;
+		a.mode = mode;
+		a.T = T_;
+		a.per = per_;
+		a.rem = rem_;
+		a.tp = thresh(p_);
+		a.tq = thresh(q_);
+		a.dis = dis_.as<uint32_t>();
+		a.tp1 = thresh(rateP(1));
+		a.tq1 = thresh(rateQ(1));
+		a.tmt = tmt_.as<uint32_t>();
+		for(int m = 0; m < n; ++m)
+		{
+			for(int s = 0; s < 2; ++s)
+			{
+				a.signum = s;
+				check(launch_cbbit_half(a, stream_), "cbbit_half");
+			}
+			++mcs_;
+		}
+		return;
+	}
 	Tables& tb = tables(mode == 2 ? 1 : mode); // one draw per site in both per-site modes
 	CbHalfArgs a{};
 	a.dis = dis_.as<uint32_t>();
@@ -521,10 +617,19 @@ SchedulerSCA::SchedulerSCA(int lx, int ly, const Params& p) : SchedulerService(l
 	if(S_ == 0)
 		S_ = default_skip((uint64_t)T_);
 	step_ = slcg_skip(S_);
-	st_[0].alloc((size_t)T_ * 8);
-	st_[1].alloc((size_t)T_ * 8);
+	allocStreams(true);
 	// worker strips (kpz/schedulerCPUCB.cpp:46-62) in cells of 4 sites
 	const int pc = (int)(dimXW / T_), rem = (int)(dimXW % T_);
+	args_.lx = lx;
+	args_.ly = ly;
+	args_.pc = pc;
+	args_.rem = rem;
+	args_.nyb = nyb;
+	args_.T = T_;
+	p_ = 0.95;
+	q_ = 0.0;
+	if(tinyMT())
+		return; // one lane per worker: no jump tables, no edge buffers
 	std::vector<int> V;
 	auto addV = [&](int v) { // v = 0: an empty first half (one-cell strips)
 		if(std::find(V.begin(), V.end(), v) == V.end())
@@ -561,11 +666,6 @@ SchedulerSCA::SchedulerSCA(int lx, int ly, const Params& p) : SchedulerService(l
 		c = acc;
 		acc = compose(cell, acc);
 	}
-	args_.lx = lx;
-	args_.ly = ly;
-	args_.pc = pc;
-	args_.rem = rem;
-	args_.nyb = nyb;
 	args_.nV = (int)V.size();
 	for(size_t k = 0; k < V.size(); ++k)
 		args_.V[k] = V[k];
@@ -583,8 +683,6 @@ SchedulerSCA::SchedulerSCA(int lx, int ly, const Params& p) : SchedulerService(l
 	edgeU_.alloc(((size_t)nyb >> args_.lNB) * (size_t)dimXW * 4);
 	args_.edgeR = edgeR_.as<uint32_t>();
 	args_.edgeU = edgeU_.as<uint32_t>();
-	p_ = 0.95;
-	q_ = 0.0;
 }
 
 void SchedulerSCA::mcs(int n)
@@ -607,9 +705,10 @@ void SchedulerSCA::run(int n, bool disorder)
 {
 	CbLocalArgs a = args_;
 	a.lat = deviceLattice();
-	a.tp = thresh23(p_);
-	a.tq = thresh23(q_);
+	a.tp = thresh(p_);
+	a.tq = thresh(q_);
 	a.dis = disorder ? dis_.as<uint32_t>() : nullptr;
+	a.tmt = tinyMT() ? tmt_.as<uint32_t>() : nullptr;
 	classThresholds(a.tpc, a.tqc);
 	for(int m = 0; m < n; ++m)
 	{
@@ -619,7 +718,8 @@ void SchedulerSCA::run(int n, bool disorder)
 			a.st_in = stIn();
 			a.st_out = stOut();
 			check(launch_cb_local_half(a, stream_), "cb_local_half");
-			swapStreams();
+			if(!tinyMT())
+				swapStreams();
 		}
 		++mcs_;
 	}

@@ -25,6 +25,10 @@ struct Error : std::runtime_error {
 };
 
 enum Kind { KIND_DT = 0, KIND_DTR = 1, KIND_SCABIT = 2, KIND_SCA = 3 };
+// worker stream generators: SLCG64 (skip form) or TinyMT32, the generator the
+// reference's frontend gives SchedulerCPUDTr / CB / CBBit
+// (kpz/kpzFrontend.cpp:308-331)
+enum Rng { RNG_SLCG64 = 0, RNG_TINYMT32 = 1 };
 
 // device buffer owned by a scheduler
 class DevBuf {
@@ -52,6 +56,7 @@ struct Params {
 	int layout_y = 4;     // log2 DT block height
 	uint64_t skip = 0;    // SLCG skip S (0: default_skip(T))
 	int device = -1;      // HIP device (-1: current)
+	int rng = RNG_SLCG64; // worker stream generator
 };
 
 // exact reductions: sums over all N sites
@@ -135,8 +140,18 @@ public:
 	std::vector<TagResult> correlateAll();
 
 	// ---- RNG state access (parity checks, checkpoints)
-	void getStreams(uint64_t* out);
+	int rng() const { return params_.rng; }
+	bool tinyMT() const { return params_.rng == RNG_TINYMT32; }
+	void getStreams(uint64_t* out);          // SLCG64 states (T)
 	void setStreams(const uint64_t* in);
+	// TinyMT32: parameter set t of stream t (the tinyMTmat1/2/tmat tables of
+	// tinymt32_param.h, absent from the reference tree, so the caller's);
+	// needed before randomize()
+	void setTinyMTParams(const uint32_t* mat1, const uint32_t* mat2, const uint32_t* tmat);
+	bool hasTinyMTParams() const { return tmtParams_; }
+	// the TinyMT32 streams as 7 planes of T words: status[0..3], mat1, mat2, tmat
+	void getTinyMT(uint32_t* out);
+	void setTinyMT(const uint32_t* in);
 	MasterRng& master() { return master_; }
 
 	// ---- checkpoint / resume (kpz_state.cpp): everything needed to continue
@@ -156,6 +171,8 @@ public:
 	uint32_t* deviceLattice() const { return lat_.as<uint32_t>(); }
 
 protected:
+	void allocStreams(bool twoBuffers);
+	uint32_t thresh(double p) const { return tinyMT() ? thresh24(p) : thresh23(p); }
 	uint64_t* stIn() const { return st_[cur_].as<uint64_t>(); }
 	uint64_t* stOut() const { return st_[cur_ ^ 1].as<uint64_t>(); }
 	void swapStreams() { cur_ ^= 1; }
@@ -181,6 +198,8 @@ protected:
 	std::vector<Tag> tags_;
 	DevBuf st_[2];
 	int cur_ = 0;
+	DevBuf tmt_;
+	bool tmtParams_ = false;
 	MasterRng master_;
 	double p_ = 1.0, q_ = 0.0;
 	double cp_[4] = {0, 0, 0, 0}, cq_[4] = {0, 0, 0, 0};

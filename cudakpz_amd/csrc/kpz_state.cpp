@@ -3,17 +3,20 @@
 //
 // One little-endian file holds everything the scheduler needs to continue
 // bit for bit: kind, sizes and construction parameters, MCS count, (p, q),
-// the master dSFMT state (dsfmt_t bytes), every SLCG stream state (which the
-// reference never saved), the lattice in reference layout, the rates of the
+// the master dSFMT state (dsfmt_t bytes), every stream state (SLCG64, which the
+// reference never saved, or TinyMT32 as its `.tinyMT` dataset,
+// kpz/GPUTinyMT.cu:113-144), the lattice in reference layout, the rates of the
 // disorder classes and the disorder map (collectH5 writes .disorderP/Q and
 // .disorder), and every live correlator tag (MCS, stop time, snapshot).  An
 // FNV-1a 64 checksum over the file's bytes ends it.
 //
-//   "KPZSTAT1"  u32 version (2)
+//   "KPZSTAT1"  u32 version (3; 2 is read as well: SLCG64 streams)
 //   i32 kind, lx, ly | i64 threads | i32 layout_x, layout_y | u64 skip
 //   u32 mcs, aux | f64 p, q | f64 p1..p3, q1..q3 | u32 classes set (bits 1..3)
 //   u32 master bytes | u64 streams | u64 words | u32 tags | u32 has disorder map
-//   master bytes | streams x u64 | words x u32 | [map: words x u32]
+//   [version 3: u32 generator, 0 = SLCG64, 1 = TinyMT32]
+//   master bytes | streams x u64 (SLCG64) or 7 x streams x u32 (TinyMT32:
+//   status[0..3], mat1, mat2, tmat planes) | words x u32 | [map: words x u32]
 //   tags x (u32 mcs, u32 stop, words x u32) | u64 checksum
 #include "kpz_scheduler.hpp"
 
@@ -27,7 +30,7 @@ namespace kpz {
 namespace {
 
 constexpr char MAGIC[8] = {'K', 'P', 'Z', 'S', 'T', 'A', 'T', '1'};
-constexpr uint32_t VERSION = 2;
+constexpr uint32_t VERSION = 3;
 
 struct Fnv {
 	uint64_t h = 1469598103934665603ull;
@@ -83,6 +86,7 @@ struct Header {
 	uint32_t masterBytes;
 	uint64_t streams, words;
 	uint32_t tags, hasMap;
+	uint32_t rng;
 };
 
 } // namespace
@@ -124,8 +128,18 @@ void SchedulerService::saveState(const char* path, uint32_t aux)
 	const uint64_t words = size_.sizeW();
 	std::vector<uint8_t> master(MasterRng::STATE_BYTES);
 	master_.save(master.data());
-	std::vector<uint64_t> st((size_t)T_);
-	getStreams(st.data());
+	std::vector<uint64_t> st;
+	std::vector<uint32_t> tm;
+	if(tinyMT())
+	{
+		tm.resize((size_t)T_ * 7);
+		getTinyMT(tm.data());
+	}
+	else
+	{
+		st.resize((size_t)T_);
+		getStreams(st.data());
+	}
 	std::vector<uint32_t> lat((size_t)words);
 	popSystem(lat.data());
 
@@ -160,8 +174,12 @@ void SchedulerService::saveState(const char* path, uint32_t aux)
 	w.val(words);
 	w.val((uint32_t)tags_.size());
 	w.val((uint32_t)(hasDisorder() ? 1 : 0));
+	w.val((uint32_t)params_.rng);
 	w.put(master.data(), master.size());
-	w.put(st.data(), st.size() * 8);
+	if(tinyMT())
+		w.put(tm.data(), tm.size() * 4);
+	else
+		w.put(st.data(), st.size() * 8);
 	w.put(lat.data(), lat.size() * 4);
 	if(hasDisorder())
 	{
@@ -206,7 +224,8 @@ std::unique_ptr<SchedulerService> SchedulerService::loadState(const char* path, 
 		throw Error(-1, "loadState: checksum mismatch (file damaged)");
 	Reader r{buf};
 	r.pos = sizeof MAGIC;
-	if(r.val<uint32_t>() != VERSION)
+	const uint32_t version = r.val<uint32_t>();
+	if(version != 2 && version != VERSION)
 		throw Error(-1, "loadState: unsupported version");
 	Header h;
 	h.kind = r.val<int32_t>();
@@ -230,12 +249,14 @@ std::unique_ptr<SchedulerService> SchedulerService::loadState(const char* path, 
 	h.words = r.val<uint64_t>();
 	h.tags = r.val<uint32_t>();
 	h.hasMap = r.val<uint32_t>();
+	h.rng = (version >= 3) ? r.val<uint32_t>() : (uint32_t)RNG_SLCG64;
 	if(h.kind < KIND_DT || h.kind > KIND_SCA || h.lx < 2 || h.ly < 2 || h.lx > 30 || h.ly > 30 ||
 	   h.words != (1ull << (h.lx + h.ly - 4)) || h.masterBytes != (uint32_t)MasterRng::STATE_BYTES ||
 	   h.streams == 0 || (h.threads > 0 && (uint64_t)h.threads != h.streams) || h.hasMap > 1 ||
-	   (h.classesSet & ~0xEu))
+	   (h.classesSet & ~0xEu) || h.rng > RNG_TINYMT32)
 		throw Error(-1, "loadState: inconsistent header");
-	const size_t need = r.pos + h.masterBytes + h.streams * 8 + h.words * 4 * (1 + h.hasMap) +
+	const size_t streamBytes = h.rng == RNG_TINYMT32 ? 28 : 8;
+	const size_t need = r.pos + h.masterBytes + h.streams * streamBytes + h.words * 4 * (1 + h.hasMap) +
 	                    (size_t)h.tags * (8 + h.words * 4) + 8;
 	if(need != buf.size())
 		throw Error(-1, "loadState: size does not match the header");
@@ -246,6 +267,7 @@ std::unique_ptr<SchedulerService> SchedulerService::loadState(const char* path, 
 	prm.layout_y = h.layoutY;
 	prm.skip = h.skip;
 	prm.device = device;
+	prm.rng = (int)h.rng;
 	std::unique_ptr<SchedulerService> s = makeScheduler(h.kind, h.lx, h.ly, prm);
 	if(s->threads() != (int64_t)h.streams || s->skip() != h.skip)
 		throw Error(-1, "loadState: stream layout differs");
@@ -258,8 +280,11 @@ std::unique_ptr<SchedulerService> SchedulerService::loadState(const char* path, 
 	}
 	s->master_.load(buf.data() + r.pos);
 	r.pos += h.masterBytes;
-	s->setStreams(reinterpret_cast<const uint64_t*>(buf.data() + r.pos));
-	r.pos += h.streams * 8;
+	if(h.rng == RNG_TINYMT32)
+		s->setTinyMT(reinterpret_cast<const uint32_t*>(buf.data() + r.pos));
+	else
+		s->setStreams(reinterpret_cast<const uint64_t*>(buf.data() + r.pos));
+	r.pos += h.streams * streamBytes;
 	s->pushSystem(reinterpret_cast<const uint32_t*>(buf.data() + r.pos));
 	r.pos += h.words * 4;
 	if(h.hasMap)

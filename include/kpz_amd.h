@@ -50,7 +50,16 @@ typedef struct kpz_params {
 	uint64_t skip;     /* SLCG skip length S (kpz/kpzConst.h:85); 0 = 30721 if T < 30721
 	                      else the smallest odd S > T */
 	int device;        /* HIP device ordinal; -1 = current device */
+	int rng;           /* worker stream generator: KPZ_RNG_SLCG64 (0) or KPZ_RNG_TINYMT32,
+	                      the Rng the reference's frontend gives SchedulerCPUDTr, CB and
+	                      CBBit (kpz/kpzFrontend.cpp:308-331); TinyMT32 for KPZ_DTR,
+	                      KPZ_SCA, KPZ_SCABIT */
 } kpz_params;
+
+enum {
+	KPZ_RNG_SLCG64 = 0,   /* SLCG64 in the skip form, inc/kmcRandom.h:59-124 */
+	KPZ_RNG_TINYMT32 = 1  /* TinyMT32, inc/tinyMT/tinymt32.h; RngIface::TinyMT32 kpz/CPURngIface.h:71-109 */
+};
 
 /* thread-local description of the last failure */
 const char* kpz_last_error(void);
@@ -151,6 +160,18 @@ int kpz_correlate_tags(kpz_scheduler* h, int64_t* out, int max_tags);
 int64_t kpz_threads(const kpz_scheduler* h);
 int kpz_get_streams(kpz_scheduler* h, uint64_t* states);
 int kpz_set_streams(kpz_scheduler* h, const uint64_t* states);
+/* the worker stream generator (KPZ_RNG_*) */
+int kpz_rng(const kpz_scheduler* h);
+/* TinyMT32 streams.  Parameter set t of stream t, as RngIface::TinyMT32::setMat
+ * (kpz/CPURngIface.cpp:37-42) and TinyMT32::initCUDA (kpz/GPUTinyMT.cu:38-61)
+ * take them from tinymt32_param.h (generated by inc/tinyMT/genParam.sh; the
+ * table is not in the reference tree): T words each; required before
+ * kpz_set_seed / kpz_randomize.  State: 7 planes of T words, status[0..3],
+ * mat1, mat2, tmat (the layout of TinyMT32::d_Random and of its `.tinyMT`
+ * dataset, kpz/GPUTinyMT.cu:113-144). */
+int kpz_set_tinymt_params(kpz_scheduler* h, const uint32_t* mat1, const uint32_t* mat2, const uint32_t* tmat);
+int kpz_get_tinymt(kpz_scheduler* h, uint32_t* planes);
+int kpz_set_tinymt(kpz_scheduler* h, const uint32_t* planes);
 int kpz_master_bytes(void);
 int kpz_get_master(kpz_scheduler* h, uint8_t* state);
 int kpz_set_master(kpz_scheduler* h, const uint8_t* state);

@@ -184,6 +184,133 @@ uint32_t kor_slcg_draw(uint64_t* s, uint64_t a, uint64_t c)
 }
 
 /* ======================================================================
+ * TinyMT32 (Saito & Matsumoto 2011, the 127-bit F2-linear generator vendored
+ * at inc/tinyMT/tinymt32.h:70-107 (state transition, tempering) and
+ * inc/tinyMT/tinymt32.c:59-75 (seeding)); RngIface::TinyMT32,
+ * kpz/CPURngIface.h:71-109: random() = the tempered word, randomFloat() =
+ * (tempered >> 8) * 2^-24, generator t takes parameter set t and the seed
+ * (unsigned)(close_open() * UINT_MAX) from the master, in stream order.
+ * ====================================================================== */
+static void tmt_next(kor_tmt* g)
+{
+	uint32_t* st = g->status;
+	uint32_t y = st[3];
+	uint32_t x = (st[0] & 0x7FFFFFFFu) ^ st[1] ^ st[2];
+	x ^= x << 1;
+	y ^= (y >> 1) ^ x;
+	const uint32_t m = (uint32_t)0 - (y & 1u);
+	st[0] = st[1];
+	st[1] = st[2] ^ (m & g->mat1);
+	st[2] = x ^ (y << 10) ^ (m & g->mat2);
+	st[3] = y;
+}
+
+static uint32_t tmt_temper(const kor_tmt* g)
+{
+	const uint32_t t1 = g->status[0] + (g->status[2] >> 8);
+	return (g->status[3] ^ t1) ^ (((uint32_t)0 - (t1 & 1u)) & g->tmat);
+}
+
+void kor_tmt_init(kor_tmt* g, uint32_t seed)
+{
+	uint32_t* st = g->status;
+	st[0] = seed;
+	st[1] = g->mat1;
+	st[2] = g->mat2;
+	st[3] = g->tmat;
+	for(int i = 1; i < 8; ++i)
+	{
+		const uint32_t p = st[(i - 1) & 3];
+		st[i & 3] ^= (uint32_t)i + 1812433253u * (p ^ (p >> 30));
+	}
+	/* period certification: the all-zero state (bit 31 of word 0 aside) */
+	if((st[0] & 0x7FFFFFFFu) == 0 && st[1] == 0 && st[2] == 0 && st[3] == 0)
+	{
+		st[0] = 'T';
+		st[1] = 'I';
+		st[2] = 'N';
+		st[3] = 'Y';
+	}
+	for(int i = 0; i < 8; ++i)
+		tmt_next(g);
+}
+
+uint32_t kor_tmt_u32(kor_tmt* g)
+{
+	tmt_next(g);
+	return tmt_temper(g);
+}
+
+float kor_tmt_float(kor_tmt* g) { return (float)(kor_tmt_u32(g) >> 8) * (1.0f / 16777216.0f); }
+
+void kor_tmt_randomize(kor_dsfmt* master, kor_tmt* g, int64_t T)
+{
+	for(int64_t t = 0; t < T; ++t)
+		kor_tmt_init(&g[t], (uint32_t)(kor_dsfmt_close_open(master) * 4294967295.0));
+}
+
+/* ----------------------------------------------------------------------
+ * A scheduler's T worker streams, of either generator, and one stream's
+ * cursor.  random() is the 32-bit output; randomFloat() is rf * 2^-bits with
+ * rf = the SLCG fold's low 23 bits (kpz/kpzRandom.cu:67-68) or the TinyMT
+ * word's top 24 bits. */
+typedef struct {
+	int tmt;
+	uint64_t* slcg;
+	uint64_t a, c;
+	kor_tmt* g;
+} streams_t;
+
+typedef struct {
+	int tmt;
+	uint64_t s, a, c;
+	kor_tmt g;
+} gen_t;
+
+static inline gen_t gen_get(const streams_t* S, int64_t t)
+{
+	gen_t g;
+	g.tmt = S->tmt;
+	g.a = S->a;
+	g.c = S->c;
+	if(S->tmt)
+	{
+		g.g = S->g[t];
+		g.s = 0;
+	}
+	else
+	{
+		memset(&g.g, 0, sizeof g.g);
+		g.s = S->slcg[t];
+	}
+	return g;
+}
+
+static inline void gen_put(const streams_t* S, int64_t t, const gen_t* g)
+{
+	if(S->tmt)
+		S->g[t] = g->g;
+	else
+		S->slcg[t] = g->s;
+}
+
+static inline uint32_t gen_random(gen_t* g) { return g->tmt ? kor_tmt_u32(&g->g) : kor_slcg_draw(&g->s, g->a, g->c); }
+static inline int gen_bits(int tmt) { return tmt ? 24 : 23; }
+static inline uint32_t gen_rf(gen_t* g) { return g->tmt ? kor_tmt_u32(&g->g) >> 8 : kor_slcg_draw(&g->s, g->a, g->c) & 0x7FFFFFu; }
+
+static streams_t streams_slcg(uint64_t* states, uint64_t a, uint64_t c)
+{
+	streams_t S = {0, states, a, c, NULL};
+	return S;
+}
+
+static streams_t streams_tmt(kor_tmt* g)
+{
+	streams_t S = {1, NULL, 0, 0, g};
+	return S;
+}
+
+/* ======================================================================
  * lattice encodings, kpz/systemSize.h:94-165, kpz/lattice.h:29-96
  * ====================================================================== */
 static inline uint32_t loc_index(int lx, int x, int y) { return ((uint32_t)(y >> 2) << (lx - 2)) + (uint32_t)(x >> 2); }
@@ -362,16 +489,16 @@ int kor_consistent(const uint32_t* lat, int enc, int lx, int ly)
  * update rules
  * ====================================================================== */
 
-/* randomFloat() < p  <=>  (r & 0x7FFFFF) < ceil(p * 2^23), exact because
- * randomFloat is r23 * 2^-23 exactly and the comparison is done in double
- * (kpz/schedulerCPUBaseThreadedRng.inl:34-44). */
-static uint32_t thresh23(double p)
+/* randomFloat() < p  <=>  rf < ceil(p * 2^bits), exact because randomFloat
+ * is rf * 2^-bits exactly (bits = 23 for SLCG64, 24 for TinyMT32) and the
+ * comparison is done in double (kpz/schedulerCPUBaseThreadedRng.inl:34-44). */
+static uint32_t thresh_bits(double p, int bits)
 {
 	if(!(p > 0.0))
 		return 0;
 	if(p >= 1.0)
-		return 1u << 23;
-	return (uint32_t)ceil(p * 8388608.0);
+		return 1u << bits;
+	return (uint32_t)ceil(p * (double)(1u << bits));
 }
 
 /* thresholds per disorder class and the disorder map */
@@ -380,26 +507,30 @@ typedef struct {
 	const uint32_t* dis;
 } thr_t;
 
-static thr_t thr_pq(double p, double q)
+static thr_t thr_pq_bits(double p, double q, int bits)
 {
 	thr_t t;
 	memset(&t, 0, sizeof t);
-	t.tp[0] = thresh23(p);
-	t.tq[0] = thresh23(q);
+	t.tp[0] = thresh_bits(p, bits);
+	t.tq[0] = thresh_bits(q, bits);
 	return t;
 }
 
-static thr_t thr_rates(const kor_rates* r)
+static thr_t thr_pq(double p, double q) { return thr_pq_bits(p, q, 23); }
+
+static thr_t thr_rates_bits(const kor_rates* r, int bits)
 {
 	thr_t t;
 	for(int k = 0; k < 4; ++k)
 	{
-		t.tp[k] = thresh23(r->p[k]);
-		t.tq[k] = thresh23(r->q[k]);
+		t.tp[k] = thresh_bits(r->p[k], bits);
+		t.tq[k] = thresh_bits(r->q[k], bits);
 	}
 	t.dis = r->disorder;
 	return t;
 }
+
+static thr_t thr_rates(const kor_rates* r) { return thr_rates_bits(r, 23); }
 
 /* SchedulerService::generateDisorder2, kpz/schedulerService.cpp:54-69 */
 void kor_generate_disorder2(kor_dsfmt* master, uint32_t* dis, uint64_t size_w, double fill)
@@ -416,11 +547,11 @@ void kor_generate_disorder2(kor_dsfmt* master, uint32_t* dis, uint64_t size_w, d
 
 /* SchedulerCPU::pqOneUpdate, kpz/schedulerCPU.inl:21-33 (upd = KPZ),
  * Func_MCSBase::pqSyncRngUpdate, kpz/schedulerCPUBaseThreadedRng.inl:25-48
- * (upd = PQ; r23 is the 23-bit randomFloat mantissa),
+ * (upd = PQ; rf is the randomFloat numerator, on the thresholds' scale),
  * disorder2SyncRngUpdate :50-75 (upd = DIS: rates of the class c(m_disorder),
  * the 2 bits of the site in the disorder map) and SyncRngEW,
  * kpz/kpzSimulationPrimitivesTemplates.cu:199-234 (upd = EW) */
-static inline void upd_local(uint32_t* lat, int lx, int ly, int x, int y, int upd, uint32_t r23, const thr_t* th)
+static inline void upd_local(uint32_t* lat, int lx, int ly, int x, int y, int upd, uint32_t rf, const thr_t* th)
 {
 	const uint32_t ic = loc_index(lx, x, y);
 	const int sc = loc_shift(x, y);
@@ -444,12 +575,12 @@ static inline void upd_local(uint32_t* lat, int lx, int ly, int x, int y, int up
 		const int k = (upd == KOR_UPD_DIS && th->dis) ? (int)((th->dis[ic] >> sc) & 3) : 0;
 		if(config == 0xC)
 		{
-			if(upd != KOR_UPD_EW && !(r23 < th->tp[k]))
+			if(upd != KOR_UPD_EW && !(rf < th->tp[k]))
 				return;
 		}
 		else if(config == 0x3)
 		{
-			if(upd != KOR_UPD_EW && !(r23 < th->tq[k]))
+			if(upd != KOR_UPD_EW && !(rf < th->tq[k]))
 				return;
 		}
 		else
@@ -476,7 +607,7 @@ void kor_update_local_rates(uint32_t* lat, int lx, int ly, int x, int y, int upd
  * kpz/schedulerCPUCBBit_nested.inl:262-295, and disorder2SyncRngUpdate
  * :298-331 (dis: class = cx(m_disorder), the x-slope bit of the site in the
  * disorder map read with the ENC_CBBIT layout) */
-static inline void upd_cbbit(uint32_t* lat, int lx, int ly, int x, int y, uint32_t r23, const thr_t* th, int dis)
+static inline void upd_cbbit(uint32_t* lat, int lx, int ly, int x, int y, uint32_t rf, const thr_t* th, int dis)
 {
 	const int xr = (x + 1) & ((1 << lx) - 1);
 	const int yu = (y + 1) & ((1 << ly) - 1);
@@ -487,12 +618,12 @@ static inline void upd_cbbit(uint32_t* lat, int lx, int ly, int x, int y, uint32
 	const int k = (dis && th->dis) ? kor_get_slope(th->dis, 1, lx, ly, x, y, 0) : 0;
 	if(config == 0xC)
 	{
-		if(!(r23 < th->tp[k]))
+		if(!(rf < th->tp[k]))
 			return;
 	}
 	else if(config == 0x3)
 	{
-		if(!(r23 < th->tq[k]))
+		if(!(rf < th->tq[k]))
 			return;
 	}
 	else
@@ -539,15 +670,20 @@ void kor_dt_block_offset(int lx, int ly, int lbx, int lby, int domain_set,
  * and SchedulerCPUDTr<Rng>::doMcs / Func_MCS::exec,
  * kpz/schedulerCPUDTr.cpp:66-98, :147-175 (random_origin = 1).
  * Rng = SLCG64 stream t with close_open() = draw * 2^-32 (contract), so
- * (int)(close_open()*2^b) = draw >> (32-b); randomFloat()*2^b = r23 >> (23-b).
+ * (int)(close_open()*2^b) = draw >> (32-b); randomFloat()*2^b = rf >> (bits-b)
+ * (SLCG64 and, for SchedulerCPUDTr<TinyMT32> (kpz/kpzFrontend.cpp:308-310),
+ * TinyMT32; the static DT draws close_open(), which RngIface::TinyMT32 lacks).
  * Blocks of one domain set are disjoint including their +1 halo, so the
  * sequential walk t = 0..T-1 equals any thread interleaving. */
 static int dt_mcs(uint32_t* lat, int lx, int ly, int lbx, int lby,
-		uint64_t* states, int64_t T, uint64_t a, uint64_t c,
+		const streams_t* S, int64_t T,
 		kor_dsfmt* master, int n, int random_origin, int upd, const thr_t* th)
 {
 	if(lbx < 2 || lby < 2 || lx - lbx < 1 || ly - lby < 1 || T < 1 || lbx > 23 || lby > 23)
 		return -1;
+	if(S->tmt && !random_origin)
+		return -1;
+	const int bits = gen_bits(S->tmt);
 	const int64_t nb = 1ll << (lx - lbx + ly - lby - 2);
 	const int64_t bs = 1ll << (lbx + lby);
 	const int mx = (1 << lx) - 1, my = (1 << ly) - 1;
@@ -565,7 +701,7 @@ static int dt_mcs(uint32_t* lat, int lx, int ly, int lbx, int lby,
 		{
 			for(int64_t t = 0; t < T; ++t)
 			{
-				uint64_t s = states[t];
+				gen_t g = gen_get(S, t);
 				for(int64_t i = t; i < nb; i += T)
 				{
 					int off[2];
@@ -575,21 +711,21 @@ static int dt_mcs(uint32_t* lat, int lx, int ly, int lbx, int lby,
 						int x, y;
 						if(random_origin)
 						{
-							x = ((int)((kor_slcg_draw(&s, a, c) & 0x7FFFFF) >> (23 - lbx)) + off[0]) & mx;
-							y = ((int)((kor_slcg_draw(&s, a, c) & 0x7FFFFF) >> (23 - lby)) + off[1]) & my;
+							x = ((int)(gen_rf(&g) >> (bits - lbx)) + off[0]) & mx;
+							y = ((int)(gen_rf(&g) >> (bits - lby)) + off[1]) & my;
 						}
 						else
 						{
-							x = (int)(kor_slcg_draw(&s, a, c) >> (32 - lbx)) + off[0];
-							y = (int)(kor_slcg_draw(&s, a, c) >> (32 - lby)) + off[1];
+							x = (int)(gen_random(&g) >> (32 - lbx)) + off[0];
+							y = (int)(gen_random(&g) >> (32 - lby)) + off[1];
 						}
-						uint32_t r23 = 0;
+						uint32_t rf = 0;
 						if(upd == KOR_UPD_PQ || upd == KOR_UPD_DIS)
-							r23 = kor_slcg_draw(&s, a, c) & 0x7FFFFF;
-						upd_local(lat, lx, ly, x, y, upd, r23, th);
+							rf = gen_rf(&g);
+						upd_local(lat, lx, ly, x, y, upd, rf, th);
 					}
 				}
-				states[t] = s;
+				gen_put(S, t, &g);
 			}
 		}
 	}
@@ -601,7 +737,8 @@ int kor_dt_mcs(uint32_t* lat, int lx, int ly, int lbx, int lby,
 		kor_dsfmt* master, int n, int random_origin, int upd, double p, double q)
 {
 	const thr_t th = thr_pq(p, q);
-	return dt_mcs(lat, lx, ly, lbx, lby, states, T, a, c, master, n, random_origin, upd, &th);
+	const streams_t S = streams_slcg(states, a, c);
+	return dt_mcs(lat, lx, ly, lbx, lby, &S, T, master, n, random_origin, upd, &th);
 }
 
 /* SchedulerCPUDTr / SchedulerCPUDT with per-class rates
@@ -611,7 +748,20 @@ int kor_dt_mcs_rates(uint32_t* lat, int lx, int ly, int lbx, int lby,
 		kor_dsfmt* master, int n, int random_origin, int upd, const kor_rates* r)
 {
 	const thr_t th = thr_rates(r);
-	return dt_mcs(lat, lx, ly, lbx, lby, states, T, a, c, master, n, random_origin, upd, &th);
+	const streams_t S = streams_slcg(states, a, c);
+	return dt_mcs(lat, lx, ly, lbx, lby, &S, T, master, n, random_origin, upd, &th);
+}
+
+/* SchedulerCPUDTr<TinyMT32> (kpz/kpzFrontend.cpp:308-310): upd KPZ / PQ / EW
+ * with (p, q), DIS with the rates r (r may be NULL otherwise) */
+int kor_dtr_mcs_tmt(uint32_t* lat, int lx, int ly, int lbx, int lby, kor_tmt* g, int64_t T,
+		kor_dsfmt* master, int n, int upd, double p, double q, const kor_rates* r)
+{
+	if(upd == KOR_UPD_DIS && !r)
+		return -1;
+	const thr_t th = (upd == KOR_UPD_DIS) ? thr_rates_bits(r, 24) : thr_pq_bits(p, q, 24);
+	const streams_t S = streams_tmt(g);
+	return dt_mcs(lat, lx, ly, lbx, lby, &S, T, master, n, 1, upd, &th);
 }
 
 /* Threaded form of the above for timing: worker w runs streams w, w+nthreads,
@@ -844,7 +994,7 @@ static inline void st64(uint32_t* p, uint64_t i, uint64_t v) { memcpy(p + 2 * i,
  * (r()<<32)^r() of :240) or 32 (L = 64 only: one draw per u32 word --
  * contract; the reference's 64-bit loop does nothing when L < 128). */
 static void cbbit_row_words(uint32_t* lat, int lx, int ly, int s, int y, int wbits,
-		uint64_t* st, uint64_t a, uint64_t c, const uint64_t* masks)
+		gen_t* g, const uint64_t* masks)
 {
 	const int lw = (wbits == 64) ? 7 : 6; /* log2 sites per word pair */
 	const uint64_t Q = size_w(lx, ly) >> (wbits == 64 ? 3 : 2);
@@ -877,12 +1027,12 @@ static void cbbit_row_words(uint32_t* lat, int lx, int ly, int s, int y, int wbi
 			rnd = masks[off];
 		else if(wbits == 64)
 		{
-			const uint64_t hi = kor_slcg_draw(st, a, c);
-			const uint64_t lo = kor_slcg_draw(st, a, c);
+			const uint64_t hi = gen_random(g);
+			const uint64_t lo = gen_random(g);
 			rnd = (hi << 32) ^ lo;
 		}
 		else
-			rnd = kor_slcg_draw(st, a, c);
+			rnd = gen_random(g);
 		m &= rnd;
 		uint64_t xm = m;
 		if(!parity)
@@ -903,7 +1053,7 @@ void kor_cbbit_halfsweep_masked(uint32_t* lat, int lx, int ly, int signum, const
 {
 	const int wbits = (lx >= 7) ? 64 : 32;
 	for(int y = 0; y < (1 << ly); ++y)
-		cbbit_row_words(lat, lx, ly, signum, y, wbits, NULL, 0, 0, masks);
+		cbbit_row_words(lat, lx, ly, signum, y, wbits, NULL, masks);
 }
 
 static void cbbit_rows(int L, int64_t T, int64_t t, int* yMin, int* ySup)
@@ -914,8 +1064,8 @@ static void cbbit_rows(int L, int64_t T, int64_t t, int* yMin, int* ySup)
 	*ySup = (int)(lo + per + (t < rem ? 1 : 0));
 }
 
-static int cbbit_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
-		uint64_t a, uint64_t c, int n, int mode, const thr_t* th)
+static int cbbit_mcs(uint32_t* lat, int lx, int ly, const streams_t* S, int64_t T,
+		int n, int mode, const thr_t* th)
 {
 	if(lx < 6 || ly < 1 || T < 1)
 		return -1;
@@ -927,16 +1077,16 @@ static int cbbit_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
 			{
 				int y0, y1;
 				cbbit_rows(Y, T, t, &y0, &y1);
-				uint64_t st = states[t];
+				gen_t g = gen_get(S, t);
 				for(int y = y0; y < y1; ++y)
 				{
 					if(mode == 0)
-						cbbit_row_words(lat, lx, ly, s, y, wbits, &st, a, c, NULL);
+						cbbit_row_words(lat, lx, ly, s, y, wbits, &g, NULL);
 					else
 						for(int x = ((y & 1) ^ s); x < X; x += 2)
-							upd_cbbit(lat, lx, ly, x, y, kor_slcg_draw(&st, a, c) & 0x7FFFFF, th, mode == 2);
+							upd_cbbit(lat, lx, ly, x, y, gen_rf(&g), th, mode == 2);
 				}
-				states[t] = st;
+				gen_put(S, t, &g);
 			}
 	return 0;
 }
@@ -945,7 +1095,8 @@ int kor_cbbit_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
 		uint64_t a, uint64_t c, int n, int mode, double p, double q)
 {
 	const thr_t th = thr_pq(p, q);
-	return cbbit_mcs(lat, lx, ly, states, T, a, c, n, mode, &th);
+	const streams_t S = streams_slcg(states, a, c);
+	return cbbit_mcs(lat, lx, ly, &S, T, n, mode, &th);
 }
 
 int kor_cbbit_mcs_rates(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
@@ -954,7 +1105,20 @@ int kor_cbbit_mcs_rates(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t
 	if(mode != 1 && mode != 2)
 		return -1;
 	const thr_t th = thr_rates(r);
-	return cbbit_mcs(lat, lx, ly, states, T, a, c, n, mode, &th);
+	const streams_t S = streams_slcg(states, a, c);
+	return cbbit_mcs(lat, lx, ly, &S, T, n, mode, &th);
+}
+
+/* SchedulerCPUCBBit<TinyMT32> (kpz/kpzFrontend.cpp:328-331): mode 0 = VL3,
+ * 1 = per-site pq with (p, q), 2 = disorder2SyncRngUpdate with r */
+int kor_cbbit_mcs_tmt(uint32_t* lat, int lx, int ly, kor_tmt* g, int64_t T,
+		int n, int mode, double p, double q, const kor_rates* r)
+{
+	if(mode < 0 || mode > 2 || (mode == 2 && !r))
+		return -1;
+	const thr_t th = (mode == 2) ? thr_rates_bits(r, 24) : thr_pq_bits(p, q, 24);
+	const streams_t S = streams_tmt(g);
+	return cbbit_mcs(lat, lx, ly, &S, T, n, mode, &th);
 }
 
 /* ======================================================================
@@ -963,20 +1127,20 @@ int kor_cbbit_mcs_rates(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t
  * :153-168 (strip split at xCenter, first half entirely before second).
  * ====================================================================== */
 static void cb_loop(uint32_t* lat, int lx, int ly, int s, int xMin, int xSup,
-		uint64_t* st, uint64_t a, uint64_t c, int upd, const thr_t* th)
+		gen_t* g, int upd, const thr_t* th)
 {
 	for(int yW = 0; yW < (1 << ly); yW += 4)
 		for(int xW = xMin; xW < xSup; xW += 4)
 			for(int y = yW; y < yW + 4; ++y)
 			{
 				const int x = (xW + (y & 1)) ^ s;
-				upd_local(lat, lx, ly, x, y, upd, kor_slcg_draw(st, a, c) & 0x7FFFFF, th);
-				upd_local(lat, lx, ly, x + 2, y, upd, kor_slcg_draw(st, a, c) & 0x7FFFFF, th);
+				upd_local(lat, lx, ly, x, y, upd, gen_rf(g), th);
+				upd_local(lat, lx, ly, x + 2, y, upd, gen_rf(g), th);
 			}
 }
 
-static int cb_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
-		uint64_t a, uint64_t c, int n, int upd, const thr_t* th)
+static int cb_mcs(uint32_t* lat, int lx, int ly, const streams_t* S, int64_t T,
+		int n, int upd, const thr_t* th)
 {
 	if(lx < 2 || ly < 2 || T < 1)
 		return -1;
@@ -992,10 +1156,10 @@ static int cb_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
 				const int xMin = xSup;
 				xSup += per + (t < rem ? 4 : 0);
 				const int xCenter = xMin + (((xSup - xMin) / 2) & ~3);
-				uint64_t st = states[t];
-				cb_loop(lat, lx, ly, s, xMin, xCenter, &st, a, c, upd, th);
-				cb_loop(lat, lx, ly, s, xCenter, xSup, &st, a, c, upd, th);
-				states[t] = st;
+				gen_t g = gen_get(S, t);
+				cb_loop(lat, lx, ly, s, xMin, xCenter, &g, upd, th);
+				cb_loop(lat, lx, ly, s, xCenter, xSup, &g, upd, th);
+				gen_put(S, t, &g);
 			}
 		}
 	return 0;
@@ -1005,7 +1169,8 @@ int kor_cb_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
 		uint64_t a, uint64_t c, int n, double p, double q)
 {
 	const thr_t th = thr_pq(p, q);
-	return cb_mcs(lat, lx, ly, states, T, a, c, n, KOR_UPD_PQ, &th);
+	const streams_t S = streams_slcg(states, a, c);
+	return cb_mcs(lat, lx, ly, &S, T, n, KOR_UPD_PQ, &th);
 }
 
 /* SchedulerCPUCB::mcsDisorder2SyncRng, kpz/schedulerCPUCB.cpp:103-108 (upd = DIS) */
@@ -1015,7 +1180,20 @@ int kor_cb_mcs_rates(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
 	if(upd != KOR_UPD_PQ && upd != KOR_UPD_DIS)
 		return -1;
 	const thr_t th = thr_rates(r);
-	return cb_mcs(lat, lx, ly, states, T, a, c, n, upd, &th);
+	const streams_t S = streams_slcg(states, a, c);
+	return cb_mcs(lat, lx, ly, &S, T, n, upd, &th);
+}
+
+/* SchedulerCPUCB<TinyMT32> (kpz/kpzFrontend.cpp:324-327): upd PQ with (p, q)
+ * or DIS with r */
+int kor_cb_mcs_tmt(uint32_t* lat, int lx, int ly, kor_tmt* g, int64_t T,
+		int n, int upd, double p, double q, const kor_rates* r)
+{
+	if((upd != KOR_UPD_PQ && upd != KOR_UPD_DIS) || (upd == KOR_UPD_DIS && !r))
+		return -1;
+	const thr_t th = (upd == KOR_UPD_DIS) ? thr_rates_bits(r, 24) : thr_pq_bits(p, q, 24);
+	const streams_t S = streams_tmt(g);
+	return cb_mcs(lat, lx, ly, &S, T, n, upd, &th);
 }
 
 /* ======================================================================

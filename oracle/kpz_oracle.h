@@ -21,8 +21,13 @@
  *   update rules         brute-force explicit-height octahedron simulator
  *   half-sweeps          per-site scalar equivalence, order independence
  *   heights/roughness    closed forms (flat IC), brute force on tiny lattices
+ *   kor_tmt_*            golden vectors from the reference's vendored
+ *                        inc/tinyMT/tinymt32.c compiled into oracle/_ref
+ *                        (tests/golden/tinymt32.json)
  *   DT/DTr/CBBit/CB/DB   composition of the pinned pieces above plus
- *                        invariants (consistency, height-change accounting)
+ *                        invariants (consistency, height-change accounting);
+ *                        the TinyMT32 variants share every line with the
+ *                        SLCG64 ones except the stream cursor
  */
 #ifndef KPZ_ORACLE_H
 #define KPZ_ORACLE_H
@@ -60,6 +65,20 @@ void kor_slcg_jump(uint64_t a, uint64_t c, uint64_t k, uint64_t* ak, uint64_t* c
 uint64_t kor_default_skip(uint64_t T);
 void kor_streams_randomize(kor_dsfmt* master, uint64_t* states, int64_t T);
 uint32_t kor_slcg_draw(uint64_t* s, uint64_t a, uint64_t c);
+
+/* ---------------- TinyMT32 ---------------- */
+/* tinymt32_t field order (inc/tinyMT/tinymt32.h:80-85) */
+typedef struct kor_tmt {
+	uint32_t status[4];
+	uint32_t mat1, mat2, tmat;
+} kor_tmt;
+/* tinymt32_init: mat1, mat2, tmat must be set */
+void kor_tmt_init(kor_tmt* g, uint32_t seed);
+uint32_t kor_tmt_u32(kor_tmt* g);     /* tinymt32_generate_uint32 */
+float kor_tmt_float(kor_tmt* g);      /* tinymt32_generate_float */
+/* RngIface::TinyMT32::randomize: generator t seeded with
+ * (unsigned)(close_open() * UINT_MAX), t = 0..T-1 */
+void kor_tmt_randomize(kor_dsfmt* master, kor_tmt* g, int64_t T);
 
 /* ---------------- lattice ---------------- */
 void kor_init_flat_local(uint32_t* lat, int lx, int ly);
@@ -123,6 +142,17 @@ void kor_cbbit_halfsweep_masked(uint32_t* lat, int lx, int ly, int signum, const
 /* SchedulerCPUCB (ENC_LOCAL checkerboard, per-site pq) */
 int kor_cb_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
 		uint64_t a, uint64_t c, int n, double p, double q);
+
+/* the schedulers the reference's frontend builds with TinyMT32 workers
+ * (kpz/kpzFrontend.cpp:308-331): SchedulerCPUDTr (upd KPZ / PQ / EW with p,
+ * q; DIS with r), SchedulerCPUCBBit (mode 0 VL3, 1 per-site pq, 2 disorder),
+ * SchedulerCPUCB (upd PQ / DIS); randomFloat() has 24 bits */
+int kor_dtr_mcs_tmt(uint32_t* lat, int lx, int ly, int lbx, int lby, kor_tmt* g, int64_t T,
+		kor_dsfmt* master, int n, int upd, double p, double q, const kor_rates* r);
+int kor_cbbit_mcs_tmt(uint32_t* lat, int lx, int ly, kor_tmt* g, int64_t T,
+		int n, int mode, double p, double q, const kor_rates* r);
+int kor_cb_mcs_tmt(uint32_t* lat, int lx, int ly, kor_tmt* g, int64_t T,
+		int n, int upd, double p, double q, const kor_rates* r);
 
 /* single updates (for brute-force checks) */
 void kor_update_local(uint32_t* lat, int lx, int ly, int x, int y, int upd, float rnd, double p, double q);

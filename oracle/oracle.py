@@ -97,6 +97,18 @@ def _declare(L):
     L.kor_update_local_rates.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, rp]
     L.kor_update_cbbit_rates.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, rp]
     L.kor_rs_mcs.argtypes = [_u32p, C.c_int, C.c_int, vp, C.c_int, C.c_int, rp]
+    L.kor_tmt_init.argtypes = [_u32p, C.c_uint32]
+    L.kor_tmt_u32.argtypes = [_u32p]
+    L.kor_tmt_u32.restype = C.c_uint32
+    L.kor_tmt_float.argtypes = [_u32p]
+    L.kor_tmt_float.restype = C.c_float
+    L.kor_tmt_randomize.argtypes = [vp, _u32p, C.c_int64]
+    L.kor_dtr_mcs_tmt.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, _u32p, C.c_int64, vp,
+                                  C.c_int, C.c_int, C.c_double, C.c_double, rp]
+    L.kor_cbbit_mcs_tmt.argtypes = [_u32p, C.c_int, C.c_int, _u32p, C.c_int64, C.c_int, C.c_int,
+                                    C.c_double, C.c_double, rp]
+    L.kor_cb_mcs_tmt.argtypes = [_u32p, C.c_int, C.c_int, _u32p, C.c_int64, C.c_int, C.c_int,
+                                 C.c_double, C.c_double, rp]
 
 
 class Rates(C.Structure):
@@ -180,6 +192,67 @@ class Streams:
         r = lib().kor_slcg_draw(C.byref(s), self.a, self.c)
         self.states[t] = s.value
         return r
+
+
+class Tmt:
+    """One TinyMT32 generator (kor_tmt: status[4], mat1, mat2, tmat)."""
+
+    def __init__(self, mat1, mat2, tmat, seed):
+        self.g = np.array([0, 0, 0, 0, mat1, mat2, tmat], np.uint32)
+        lib().kor_tmt_init(self.g, seed & 0xFFFFFFFF)
+
+    def u32(self):
+        return lib().kor_tmt_u32(self.g)
+
+    def float(self):
+        return lib().kor_tmt_float(self.g)
+
+
+class TmtStreams:
+    """T TinyMT32 worker generators, parameter set t for worker t, seeded from
+    the master like RngIface::TinyMT32::randomize.  g[t] = status[4], mat1,
+    mat2, tmat."""
+
+    def __init__(self, T, master, mat1, mat2, tmat):
+        self.T = int(T)
+        self.g = np.zeros((self.T, 7), np.uint32)
+        self.g[:, 4] = mat1
+        self.g[:, 5] = mat2
+        self.g[:, 6] = tmat
+        lib().kor_tmt_randomize(master.ptr, self.g, self.T)
+
+    def u32(self, t):
+        row = np.ascontiguousarray(self.g[t])
+        v = lib().kor_tmt_u32(row)
+        self.g[t] = row
+        return v
+
+    def float(self, t):
+        row = np.ascontiguousarray(self.g[t])
+        v = lib().kor_tmt_float(row)
+        self.g[t] = row
+        return v
+
+
+def dtr_mcs_tmt(lat, lx, ly, lbx, lby, streams, master, n, upd=UPD_KPZ, p=1.0, q=0.0, r=None):
+    rc = lib().kor_dtr_mcs_tmt(lat, lx, ly, lbx, lby, streams.g, streams.T, master.ptr, n, upd, p, q,
+                               None if r is None else C.byref(r))
+    if rc:
+        raise ValueError("invalid DT layout or missing rates")
+
+
+def cbbit_mcs_tmt(lat, lx, ly, streams, n, mode, p=0.5, q=0.0, r=None):
+    rc = lib().kor_cbbit_mcs_tmt(lat, lx, ly, streams.g, streams.T, n, mode, p, q,
+                                 None if r is None else C.byref(r))
+    if rc:
+        raise ValueError("invalid CBBit size, mode or missing rates")
+
+
+def cb_mcs_tmt(lat, lx, ly, streams, n, upd=UPD_PQ, p=0.95, q=0.0, r=None):
+    rc = lib().kor_cb_mcs_tmt(lat, lx, ly, streams.g, streams.T, n, upd, p, q,
+                              None if r is None else C.byref(r))
+    if rc:
+        raise ValueError("invalid CB size, update kind or missing rates")
 
 
 def size_words(lx, ly):

@@ -6,8 +6,12 @@
  * (inc/dSFMT/dSFMT.cpp), Kmc::Shuffle<4> (inc/shuffle.h), the double-tiling
  * block geometry Kmc::DoubleTilingDD<2> / Kmc::DDIterator
  * (inc/doubleTilingDD.h, inc/blockDD.h, inc/ddIterators.h) and the SLCG
- * skip constants (inc/kmcRandom.h, inc/helper/slcgInst.h).  The reference's
- * scheduler classes need libSplash (absent), so they are not built here.
+ * skip constants (inc/kmcRandom.h, inc/helper/slcgInst.h), the vendored
+ * TinyMT32 (inc/tinyMT/tinymt32.c) and the CPU schedulers' TinyMT32 stream
+ * interface (kpz/CPURngIface.h, header part only: its parameter table
+ * tinymt32_param.h is absent, so parameters are set per generator here).
+ * The reference's scheduler classes need libSplash (absent), so they are not
+ * built here.
  *
  * Built by oracle/Makefile straight from the files under the reference tree
  * into oracle/_ref/; used only by tests/golden/make_golden.py to write the
@@ -19,8 +23,12 @@
 #include <doubleTilingDD.h>
 #include <ddIterators.h>
 #include <kmcRandom.h>
+#include <tinyMT/tinymt32.h>
+#include <kpz/CPURngIface.h>
 
 #include <cstdint>
+#include <cstring>
+#include <vector>
 
 extern "C" {
 
@@ -124,5 +132,56 @@ void ref_slcg_skip_tmpl_small(uint64_t* out)
 }
 
 uint64_t ref_slcg_gen(uint64_t s) { return SLCGen(s); }
+
+/* tinymt32_init(seed) with the given parameters: the state after seeding,
+ * then n tinymt32_generate_uint32 outputs; a second generator seeded alike
+ * gives n tinymt32_generate_float outputs */
+void ref_tmt(uint32_t mat1, uint32_t mat2, uint32_t tmat, uint32_t seed, int n,
+		uint32_t* init_state, uint32_t* u32, float* f)
+{
+	tinymt32_t g;
+	g.mat1 = mat1;
+	g.mat2 = mat2;
+	g.tmat = tmat;
+	tinymt32_init(&g, seed);
+	std::memcpy(init_state, g.status, 16);
+	for(int a = 0; a < n; ++a)
+		u32[a] = tinymt32_generate_uint32(&g);
+	tinymt32_t h;
+	h.mat1 = mat1;
+	h.mat2 = mat2;
+	h.tmat = tmat;
+	tinymt32_init(&h, seed);
+	for(int a = 0; a < n; ++a)
+		f[a] = tinymt32_generate_float(&h);
+}
+
+/* Kpz::RngIface::TinyMT32 streams as a CPU scheduler holds them: T
+ * generators with parameters (mat1[t], mat2[t], tmat[t]), randomize()d from
+ * a master dSFMT seeded with init_gen_rand(seed) + `discard` draws; then per
+ * generator n random() and n randomFloat() (alternating: random, randomFloat) */
+void ref_tmt_iface(uint32_t seed, int discard, int T, const uint32_t* mat1, const uint32_t* mat2,
+		const uint32_t* tmat, int n, uint32_t* u32, float* f)
+{
+	dsfmt_t d;
+	dsfmt_init_gen_rand(&d, seed);
+	for(int a = 0; a < discard; ++a)
+		dsfmt_genrand_close_open(&d);
+	std::vector<Kpz::RngIface::TinyMT32> rngs(T);
+	for(int t = 0; t < T; ++t)
+	{
+		tinymt32_t* g = reinterpret_cast<tinymt32_t*>(&rngs[t]);
+		g->mat1 = mat1[t];
+		g->mat2 = mat2[t];
+		g->tmat = tmat[t];
+	}
+	Kpz::RngIface::TinyMT32::randomize(rngs, d);
+	for(int t = 0; t < T; ++t)
+		for(int a = 0; a < n; ++a)
+		{
+			u32[t * n + a] = rngs[t].random();
+			f[t * n + a] = rngs[t].randomFloat();
+		}
+}
 
 }

@@ -3,7 +3,8 @@
 Runs in the development container only: it builds oracle/_ref/libkpzref.so
 from the reference's own sources (inc/dSFMT/dSFMT.cpp plus the header-only
 inc/shuffle.h, inc/doubleTilingDD.h, inc/blockDD.h, inc/ddIterators.h,
-inc/kmcRandom.h, inc/helper/slcgInst.h) via oracle/Makefile and records
+inc/kmcRandom.h, inc/helper/slcgInst.h, inc/tinyMT/tinymt32.c and the
+header part of kpz/CPURngIface.h) via oracle/Makefile and records
 their outputs.  The fixtures are data (inputs and expected outputs) and are
 what the CPU tests compare the oracle against on any machine.
 
@@ -91,6 +92,36 @@ def main():
         "base_step": gen,
     }
 
+    # TinyMT32: the vendored generator, and the CPU schedulers' stream
+    # interface (randomize from a reSeed'ed master, random(), randomFloat())
+    R.ref_tmt.argtypes = [C.c_uint32] * 4 + [C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                             C.POINTER(C.c_float)]
+    R.ref_tmt_iface.argtypes = [C.c_uint32, C.c_int, C.c_int] + [C.POINTER(C.c_uint32)] * 3 + \
+        [C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]
+    params = [(0x8F7011EE, 0xFC78FF1F, 0x3793FDFF), (0, 0, 0), (0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFF),
+              (0x12345678, 0x9ABCDEF0, 0x0F1E2D3C), (0xDEADBEEF, 0x01234567, 0x89ABCDEF)]
+    gens = []
+    n = 64
+    st, u, f = (C.c_uint32 * 4)(), (C.c_uint32 * n)(), (C.c_float * n)()
+    for (m1, m2, tm) in params:
+        for seed in [1, 0, 42, 4294967295, 2718281828]:
+            R.ref_tmt(m1, m2, tm, seed, n, st, u, f)
+            gens.append({"mat1": m1, "mat2": m2, "tmat": tm, "seed": seed, "init": list(st),
+                         "u32": list(u), "float": [float(v).hex() for v in f]})
+    iface = []
+    T, n2 = 6, 8
+    import random
+    rng = random.Random(7)
+    for seed in [42, 3]:
+        m1 = [rng.getrandbits(32) for _ in range(T)]
+        m2 = [rng.getrandbits(32) for _ in range(T)]
+        tm = [rng.getrandbits(32) for _ in range(T)]
+        a1, a2, a3 = ((C.c_uint32 * T)(*v) for v in (m1, m2, tm))
+        u2, f2 = (C.c_uint32 * (T * n2))(), (C.c_float * (T * n2))()
+        R.ref_tmt_iface(seed, 100000, T, a1, a2, a3, n2, u2, f2)
+        iface.append({"seed": seed, "discard": 100000, "mat1": m1, "mat2": m2, "tmat": tm, "n": n2,
+                      "random": list(u2), "randomFloat": [float(v).hex() for v in f2]})
+
     def dump(name, obj):
         with open(os.path.join(HERE, name), "w") as f:
             json.dump(obj, f, separators=(",", ":"))
@@ -99,6 +130,7 @@ def main():
     dump("shuffle4.json", shuffles)
     dump("dt_geometry.json", {"offsets": dt, "walks": walks})
     dump("slcg.json", slcg)
+    dump("tinymt32.json", {"generators": gens, "interface": iface})
 
 
 if __name__ == "__main__":

@@ -135,3 +135,31 @@ def test_seed32_identity_for_32bit_seeds():
     for s in [0, 1, 42, 0xFFFFFFFF]:
         assert O.seed32(s) == s
     assert O.seed32((5 << 32) | 7) == 2
+
+
+def test_tinymt32_matches_reference():
+    recs = load("tinymt32.json")["generators"]
+    assert len(recs) == 25
+    for rec in recs:
+        g = O.Tmt(rec["mat1"], rec["mat2"], rec["tmat"], rec["seed"])
+        assert [int(v) for v in g.g[:4]] == rec["init"], rec["seed"]
+        assert [g.u32() for _ in rec["u32"]] == rec["u32"], (rec["mat1"], rec["seed"])
+        h = O.Tmt(rec["mat1"], rec["mat2"], rec["tmat"], rec["seed"])
+        assert [float(h.float()).hex() for _ in rec["float"]] == rec["float"]
+
+
+def test_tinymt32_scheduler_streams_match_reference():
+    # RngIface::TinyMT32::randomize from a reSeed'ed master, then random() and
+    # randomFloat() alternately, per generator
+    for rec in load("tinymt32.json")["interface"]:
+        m = O.Dsfmt(rec["seed"])
+        assert rec["discard"] == 100000
+        T, n = len(rec["mat1"]), rec["n"]
+        st = O.TmtStreams(T, m, rec["mat1"], rec["mat2"], rec["tmat"])
+        got_u, got_f = [], []
+        for t in range(T):
+            for _ in range(n):
+                got_u.append(st.u32(t))
+                got_f.append(float(st.float(t)).hex())
+        assert got_u == rec["random"]
+        assert got_f == rec["randomFloat"]

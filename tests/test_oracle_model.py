@@ -11,6 +11,35 @@ from oracle import oracle as O
 ENCS = [O.ENC_LOCAL, O.ENC_CBBIT]
 
 
+class Cur:
+    """random() and the randomFloat() numerator of either stream set: SLCG64
+    (fold & 0x7FFFFF, 23 bits) or TinyMT32 (tempered >> 8, 24 bits)"""
+
+    def __init__(self, st):
+        self.st, self.T = st, st.T
+        self.tmt = isinstance(st, O.TmtStreams)
+        self.bits = 24 if self.tmt else 23
+
+    def draw(self, t):
+        return self.st.u32(t) if self.tmt else self.st.draw(t)
+
+    def rf(self, t):
+        return self.draw(t) >> 8 if self.tmt else self.draw(t) & 0x7FFFFF
+
+    def thresh(self, p):
+        if p <= 0:
+            return 0
+        if p >= 1:
+            return 1 << self.bits
+        return math.ceil(p * (1 << self.bits))
+
+
+def tmt_streams(T, master, seed):
+    r = np.random.default_rng(seed)
+    m = [r.integers(0, 1 << 32, T, dtype=np.uint64).astype(np.uint32) for _ in range(3)]
+    return O.TmtStreams(T, master, *m)
+
+
 def evolved_local(lx, ly, seed, mcs=3):
     """a rough consistent ENC_LOCAL lattice from the oracle's SCA dynamics"""
     m = O.Dsfmt(seed)
@@ -194,7 +223,8 @@ def py_dt(hm, lx, ly, lbx, lby, st, master, n, random_origin, upd, p, q):
     X, Y = 1 << lx, 1 << ly
     nb = 1 << (lx - lbx + ly - lby - 2)
     bs = 1 << (lbx + lby)
-    tp, tq = thresh23(p), thresh23(q)
+    st = Cur(st)
+    tp, tq = st.thresh(p), st.thresh(q)
     for _ in range(n):
         seq = master.shuffle4()
         sx = sy = 0
@@ -207,15 +237,15 @@ def py_dt(hm, lx, ly, lbx, lby, st, master, n, random_origin, upd, p, q):
                     ox, oy = _dt_block(lx, ly, lbx, lby, ds, i)
                     for _k in range(bs):
                         if random_origin:
-                            x = (((st.draw(t) & 0x7FFFFF) >> (23 - lbx)) + ox + sx) % X
-                            y = (((st.draw(t) & 0x7FFFFF) >> (23 - lby)) + oy + sy) % Y
+                            x = ((st.rf(t) >> (st.bits - lbx)) + ox + sx) % X
+                            y = ((st.rf(t) >> (st.bits - lby)) + oy + sy) % Y
                         else:
                             x = (st.draw(t) >> (32 - lbx)) + ox
                             y = (st.draw(t) >> (32 - lby)) + oy
                         if upd == O.UPD_KPZ:
                             hm.update(x, y, True, False)
                         else:
-                            r = st.draw(t) & 0x7FFFFF
+                            r = st.rf(t)
                             hm.update(x, y, r < tp, r < tq)
 
 
@@ -288,7 +318,8 @@ def test_db_rejects_bad_layouts():
 def py_cbbit(hm, lx, ly, st, n, mode, p, q):
     X, Y = 1 << lx, 1 << ly
     wbits = 64 if lx >= 7 else 32
-    tp, tq = thresh23(p), thresh23(q)
+    st = Cur(st)
+    tp, tq = st.thresh(p), st.thresh(q)
     per, rem = Y // st.T, Y % st.T
     for _ in range(n):
         for s in range(2):
@@ -310,7 +341,7 @@ def py_cbbit(hm, lx, ly, st, n, mode, p, q):
                                     hm.update(xw * 2 * wbits + 2 * k + par, y, True, False)
                     else:
                         for x in range(par, X, 2):
-                            r = st.draw(t) & 0x7FFFFF
+                            r = st.rf(t)
                             hm.update(x, y, r < tp, r < tq)
                 lo = hi
 
@@ -331,7 +362,8 @@ def test_cbbit_composition(mode, p, q, lx, ly, T):
 
 def py_cb(hm, lx, ly, st, n, p, q):
     X, Y = 1 << lx, 1 << ly
-    tp, tq = thresh23(p), thresh23(q)
+    st = Cur(st)
+    tp, tq = st.thresh(p), st.thresh(q)
     dimxw = X // 4
     per, rem = (dimxw // st.T) * 4, dimxw % st.T
 
@@ -341,7 +373,7 @@ def py_cb(hm, lx, ly, st, n, p, q):
                 for y in range(yw, yw + 4):
                     x = xw + ((y & 1) ^ s)
                     for xx in (x, x + 2):
-                        r = st.draw(t) & 0x7FFFFF
+                        r = st.rf(t)
                         hm.update(xx, y, r < tp, r < tq)
 
     for _ in range(n):
@@ -366,6 +398,69 @@ def test_cb_composition(T):
     py_cb(hm, lx, ly, s2, 2, 0.8, 0.3)
     assert np.array_equal(normalized(O.heights(lat, 0, lx, ly)), hm.normalized())
     assert np.array_equal(s1.states, s2.states)
+
+
+# ---- the TinyMT32 workers of SchedulerCPUDTr / CBBit / CB (kpz/kpzFrontend.cpp:308-331)
+@pytest.mark.parametrize("upd,p,q", [(O.UPD_KPZ, 1.0, 0.0), (O.UPD_PQ, 0.5, 0.5), (O.UPD_PQ, 0.9, 0.2)])
+@pytest.mark.parametrize("T", [1, 3])
+def test_dtr_tinymt_composition(upd, p, q, T):
+    lx, ly, lbx, lby = 6, 5, 2, 2
+    m1, m2 = O.Dsfmt(42), O.Dsfmt(42)
+    s1, s2 = tmt_streams(T, m1, 5), tmt_streams(T, m2, 5)
+    lat = O.flat(lx, ly, 0)
+    hm = HeightLattice(O.heights(lat, 0, lx, ly))
+    O.dtr_mcs_tmt(lat, lx, ly, lbx, lby, s1, m1, 2, upd, p, q)
+    py_dt(hm, lx, ly, lbx, lby, s2, m2, 2, True, upd, p, q)
+    assert np.array_equal(normalized(O.heights(lat, 0, lx, ly)), hm.normalized())
+    assert np.array_equal(s1.g, s2.g)
+    assert O.consistent(lat, 0, lx, ly)
+
+
+def test_tinymt_dtr_disorder_needs_rates():
+    # (the oracle offers TinyMT32 workers for the random-origin DT only:
+    # SchedulerCPUDT draws close_open(), which RngIface::TinyMT32 lacks)
+    m = O.Dsfmt(1)
+    st = tmt_streams(2, m, 1)
+    lat = O.flat(6, 6, 0)
+    rc = O.lib().kor_dtr_mcs_tmt(lat, 6, 6, 2, 2, st.g, st.T, m.ptr, 1, O.UPD_DIS, 1.0, 0.0, None)
+    assert rc == -1  # disorder rule without rates
+
+
+@pytest.mark.parametrize("mode,p,q", [(0, 0.5, 0.0), (1, 0.95, 0.05), (1, 0.5, 0.5)])
+@pytest.mark.parametrize("lx,ly,T", [(7, 4, 1), (8, 3, 3), (6, 4, 5)])
+def test_cbbit_tinymt_composition(mode, p, q, lx, ly, T):
+    m1, m2 = O.Dsfmt(7), O.Dsfmt(7)
+    s1, s2 = tmt_streams(T, m1, 9), tmt_streams(T, m2, 9)
+    lat = O.flat(lx, ly, 1)
+    hm = HeightLattice(O.heights(lat, 1, lx, ly))
+    O.cbbit_mcs_tmt(lat, lx, ly, s1, 2, mode, p, q)
+    py_cbbit(hm, lx, ly, s2, 2, mode, p, q)
+    assert np.array_equal(normalized(O.heights(lat, 1, lx, ly)), hm.normalized())
+    assert np.array_equal(s1.g, s2.g)
+    assert O.consistent(lat, 1, lx, ly)
+
+
+@pytest.mark.parametrize("T", [1, 3, 8])
+def test_cb_tinymt_composition(T):
+    lx, ly = 5, 4
+    m1, m2 = O.Dsfmt(3), O.Dsfmt(3)
+    s1, s2 = tmt_streams(T, m1, 2), tmt_streams(T, m2, 2)
+    lat = O.flat(lx, ly, 0)
+    hm = HeightLattice(O.heights(lat, 0, lx, ly))
+    O.cb_mcs_tmt(lat, lx, ly, s1, 2, O.UPD_PQ, 0.8, 0.3)
+    py_cb(hm, lx, ly, s2, 2, 0.8, 0.3)
+    assert np.array_equal(normalized(O.heights(lat, 0, lx, ly)), hm.normalized())
+    assert np.array_equal(s1.g, s2.g)
+
+
+def test_tinymt_threshold_has_24_bits():
+    # randomFloat() of TinyMT32 is (u >> 8) * 2^-24: a rate on the 2^-24 grid
+    # between two 2^-23 steps separates the two scales
+    p = 0.5 + 2.0 ** -24
+    assert Cur.thresh(type("S", (), {"bits": 24})(), p) == (1 << 23) + 1
+    assert Cur.thresh(type("S", (), {"bits": 23})(), p) == (1 << 22) + 1
+    r24 = 1 << 23  # randomFloat = 0.5 exactly
+    assert (np.float64(np.float32(r24 * 2.0 ** -24)) < p) == (r24 < (1 << 23) + 1)
 
 
 def test_dt_invalid_layout_rejected():
