@@ -92,6 +92,8 @@ struct Args {
 	bool haveSeed = false;
 	uint64_t seed = 0;
 	int device = -1;
+	int rng = KPZ_RNG_SLCG64;     // --rng slcg64|tinymt32
+	std::string tinymtParams;     // --tinymtParams file (tinymt32dc output)
 	Sampling current;
 	std::list<Event> events;
 	bool finished = false;
@@ -119,6 +121,9 @@ void help()
 	             "                            + SLCG stream states)\n"
 	             "  --loadH5 prefix           resume from an HDF5 checkpoint (size from the file)\n"
 	             "  --seed s                  64-bit seed (default: /dev/urandom and time)\n"
+	             "  --rng slcg64|tinymt32     worker generator (tinymt32: dt, sca, scaBit)\n"
+	             "  --tinymtParams file       TinyMT32 parameter sets, tinymt32dc output lines\n"
+	             "                            (characteristic,type,id,mat1,mat2,tmat,...; set t = line t)\n"
 	             "  --device n                HIP device\n";
 }
 
@@ -418,6 +423,24 @@ int parseArg(Args& A, int argc, char** argv, int a)
 		}
 		A.haveSeed = true;
 	}
+	else if(!strcmp(argv[a], "--rng"))
+	{
+		std::string r;
+		if(!getStr(r, ++a, argc, argv) || (r != "slcg64" && r != "tinymt32"))
+		{
+			std::cerr << "Expected slcg64 or tinymt32 for --rng.\n";
+			return -1;
+		}
+		A.rng = (r == "tinymt32") ? KPZ_RNG_TINYMT32 : KPZ_RNG_SLCG64;
+	}
+	else if(!strcmp(argv[a], "--tinymtParams"))
+	{
+		if(!getStr(A.tinymtParams, ++a, argc, argv))
+		{
+			std::cerr << "Expected a file name for --tinymtParams.\n";
+			return -1;
+		}
+	}
 	else if(!strcmp(argv[a], "--device"))
 	{
 		if(!getNum(A.device, ++a, argc, argv))
@@ -432,6 +455,56 @@ int parseArg(Args& A, int argc, char** argv, int a)
 		return -1;
 	}
 	return a;
+}
+
+// TinyMT32 parameter sets from tinymt32dc output, the lines
+// inc/tinyMT/genParam.sh turns into tinymt32_param.h: "characteristic,type,
+// id,mat1,mat2,tmat,weight,delta" with hex words; '#' lines are comments.
+// Set t is the t-th data line, as tinyMTmat1[t] of the generated table.
+bool readTinyMTParams(const std::string& file, size_t T, std::vector<uint32_t> (&m)[3])
+{
+	std::ifstream in(file);
+	if(!in)
+	{
+		std::cerr << "[EE] cannot open TinyMT32 parameter file " << file << '\n';
+		return false;
+	}
+	std::string line;
+	size_t lineNo = 0;
+	while(m[0].size() < T && std::getline(in, line))
+	{
+		++lineNo;
+		if(line.empty() || line[0] == '#')
+			continue;
+		std::vector<std::string> f;
+		size_t b = 0;
+		for(size_t e; (e = line.find(',', b)) != std::string::npos; b = e + 1)
+			f.push_back(line.substr(b, e - b));
+		f.push_back(line.substr(b));
+		if(f.size() < 6)
+		{
+			std::cerr << "[EE] " << file << ":" << lineNo << ": expected tinymt32dc output (>= 6 fields)\n";
+			return false;
+		}
+		for(int k = 0; k < 3; ++k)
+		{
+			char* end = nullptr;
+			const unsigned long v = std::strtoul(f[(size_t)(3 + k)].c_str(), &end, 16);
+			if(end == f[(size_t)(3 + k)].c_str() || v > 0xFFFFFFFFul)
+			{
+				std::cerr << "[EE] " << file << ":" << lineNo << ": field " << (4 + k) << " is not a hex word\n";
+				return false;
+			}
+			m[k].push_back((uint32_t)v);
+		}
+	}
+	if(m[0].size() < T)
+	{
+		std::cerr << "[EE] " << file << " holds " << m[0].size() << " TinyMT32 parameter sets, " << T
+		          << " workers need one each\n";
+		return false;
+	}
+	return true;
 }
 
 // ---------------------------------------------------------------- engine
@@ -678,7 +751,12 @@ int main(int argc, char** argv)
 		kind = KPZ_SCABIT;
 	else if(A.scheduler == "sca")
 		kind = KPZ_SCA;
-	kpz_params prm{A.mcsThreads, A.dd[0], A.dd[1], 0, A.device};
+	kpz_params prm{A.mcsThreads, A.dd[0], A.dd[1], 0, A.device, A.rng};
+	if(A.rng == KPZ_RNG_TINYMT32 && A.loadState.empty() && A.loadH5.empty() && A.tinymtParams.empty())
+	{
+		std::cerr << "[EE][FrontendArgs] --rng tinymt32 needs --tinymtParams (one parameter set per worker).\n";
+		return 1;
+	}
 	Engine e;
 	uint32_t lastMeasurement = 0;
 	if(!A.loadState.empty() && !A.loadH5.empty())
@@ -747,6 +825,16 @@ int main(int argc, char** argv)
 		}
 		else
 			std::cout << "[SchedulerService][supplySeed] seed= " << A.seed << '\n';
+		if(A.rng == KPZ_RNG_TINYMT32)
+		{
+			// RngIface::TinyMT32::initialize: parameter set t for worker t
+			std::vector<uint32_t> m[3];
+			const size_t T = (size_t)kpz_threads(e.h);
+			if(!readTinyMTParams(A.tinymtParams, T, m))
+				return 1;
+			Engine::check(kpz_set_tinymt_params(e.h, m[0].data(), m[1].data(), m[2].data()), "TinyMT32 parameters");
+			std::cout << "TinyMT32: Using parametersets: 0 .. " << T - 1 << '\n';
+		}
 		Engine::check(kpz_set_seed(e.h, A.seed), "seed");
 		Engine::check(kpz_init_flat(e.h), "initHomogeneous");
 		if(!std::isnan(A.disorderFill))

@@ -44,6 +44,9 @@ def test_missing_mcs():
     (["100", "8", "--scheduler", "dt", "--ddBlocksize"], "Expected ddBlocksize"),
     (["100", "8", "--scheduler", "sca", "--1d"], "--1d"),
     (["100", "8", "-f"], "Expected a filename"),
+    (["100", "8", "--rng", "mt19937"], "Expected slcg64 or tinymt32 for --rng"),
+    (["100", "8", "--tinymtParams"], "Expected a file name for --tinymtParams"),
+    (["100", "8", "--scheduler", "dt", "--rng", "tinymt32"], "--rng tinymt32 needs --tinymtParams"),
 ])
 def test_argument_errors(args, msg):
     rc, _, err = run(*args)

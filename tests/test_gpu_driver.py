@@ -7,6 +7,7 @@ import math
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 import cudakpz_amd as K
@@ -155,3 +156,51 @@ def test_driver_linear_sampling_and_save(tmp_path):
     hs = [float(line.split("\t")[3]) for line in xyz[6:]]
     r = s.roughness()
     assert sum(hs) == r.sum_h and sum(h * h for h in hs) == r.sum_h2
+
+
+def write_tinymt_params(path, T, seed):
+    """tinymt32dc output lines (characteristic,type,id,mat1,mat2,tmat,weight,delta),
+    the format inc/tinyMT/genParam.sh reads; random parameter words"""
+    r = np.random.default_rng(seed)
+    m = [r.integers(0, 1 << 32, T, dtype=np.uint64).astype(np.uint32) for _ in range(3)]
+    lines = ["# charactristic, type, id, mat1, mat2, tmat, weight, delta"]
+    for t in range(T):
+        lines.append(f"{t:032x},32,{t},{m[0][t]:x},{m[1][t]:x},{m[2][t]:x},63,0")
+    path.write_text("\n".join(lines) + "\n")
+    return m
+
+
+@pytest.mark.parametrize("sched,cls,pq", [("dt", K.SchedulerDTr, None), ("scaBit", K.SchedulerSCABit, None),
+                                          ("sca", K.SchedulerSCA, (0.8, 0.1))])
+def test_driver_tinymt32_workers(tmp_path, sched, cls, pq):
+    # --rng tinymt32 --tinymtParams: the reference's CPU dt / sca / scaBit
+    # (SchedulerCPUDTr/CB/CBBit<TinyMT32>, kpz/kpzFrontend.cpp:308-331)
+    if not os.path.exists(EXE):
+        pytest.fail("kpz_scaling not built")
+    lx, mcs, seed, T = 8, 60, 13, 6
+    pf = tmp_path / "tinymt32.txt"
+    m = write_tinymt_params(pf, T + 3, 2)  # more sets than workers: the first T are used
+    args = [mcs, lx, "--scheduler", sched, "--mcsThreads", T, "--rng", "tinymt32", "--tinymtParams", pf,
+            "--seed", seed]
+    if pq:
+        args += ["--pq", *pq]
+    rows, stdout = run_driver(tmp_path, args)
+    assert "TinyMT32: Using parametersets: 0 .. 5" in stdout
+    s = cls(lx, lx, threads=T, rng=K.RNG_TINYMT32)
+    s.setTinyMTParams(*(v[:T] for v in m))
+    s.supplySeed(seed)
+    s.initHomogeneous()
+    if pq:
+        s.setPQ(*pq)
+    done = 0
+    for row, (t, _, _) in zip(rows, expected_lines(mcs, [])):
+        assert int(row[0]) == t
+        (s.mcsPQSyncRng if pq else s.mcs)(t - done)
+        done = t
+        assert close(float(row[1]), s.roughness().w2)
+    short = tmp_path / "short.txt"
+    write_tinymt_params(short, T - 1, 3)
+    p = subprocess.run([EXE, *map(str, [10, lx, "--scheduler", sched, "--mcsThreads", T, "--rng", "tinymt32",
+                                        "--tinymtParams", short])], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 1 and "workers need one each" in p.stderr
+

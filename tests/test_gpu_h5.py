@@ -140,3 +140,39 @@ def test_driver_h5_resume_reproduces_table(tmp_path):
     t = lambda line: int(line.split("\t")[0])
     assert [x for x in b if t(x) > 200] == [x for x in full if t(x) > 200]
     assert len([x for x in b if t(x) > 200]) > 5
+
+
+@pytest.mark.parametrize("cls,T", [(K.SchedulerDTr, 9), (K.SchedulerSCABit, 5), (K.SchedulerSCA, 4)])
+def test_tinymt32_round_trip(tmp_path, cls, T):
+    # TinyMT32::writeH5 / readH5 (kpz/GPUTinyMT.cu:113-144): <prefix>.tinyMT,
+    # extents {7, T}, the state planes with the parameter sets
+    r = np.random.default_rng(T)
+    mats = [r.integers(0, 1 << 32, T, dtype=np.uint64).astype(np.uint32) for _ in range(3)]
+    ref, run = (cls(8, 7, threads=T, rng=K.RNG_TINYMT32) for _ in range(2))
+    for s in (ref, run):
+        s.setTinyMTParams(*mats)
+        s.supplySeed(77)
+        s.initHomogeneous()
+        s.mcs(3)
+    base = str(tmp_path / "tm")
+    run.saveH5(base, prefix="k")
+    run.close()
+    res, _, restored = K.load_h5(base, ref.kind, prefix="k", threads=T, rng=K.RNG_TINYMT32)
+    assert restored & K.H5_STREAMS and res.rng == K.RNG_TINYMT32
+    assert np.array_equal(res.tinymt(), ref.tinymt())
+    for s in (ref, res):
+        s.mcs(4)
+    assert np.array_equal(res.popSystem(), ref.popSystem())
+    assert np.array_equal(res.tinymt(), ref.tinymt())
+    h5ls = shutil.which("h5ls") or "/opt/conda/bin/h5ls"
+    if os.path.exists(h5ls):
+        out = subprocess.run([h5ls, "-r", base + "_0_0_0.h5"], capture_output=True, text=True, timeout=60).stdout
+        assert f"/data/3/k.tinyMT Dataset {{7, {T}}}" in [" ".join(x.split()) for x in out.splitlines()], out
+    with pytest.raises(K.KpzError):
+        # an SLCG64 checkpoint has no .tinyMT state to resume TinyMT32 workers from
+        a = cls(8, 7, threads=T)
+        a.supplySeed(1)
+        a.initHomogeneous()
+        a.saveH5(str(tmp_path / "sl"))
+        K.load_h5(str(tmp_path / "sl"), a.kind, threads=T, rng=K.RNG_TINYMT32)
+
