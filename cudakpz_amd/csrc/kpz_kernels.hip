@@ -468,19 +468,26 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 #pragma unroll 2
 	for(int k = 0; k < BX * BY; ++k)
 	{
-		const uint32_t r1 = cur.draw();
+		// x comes as xr = (BX - 1) - x, the same bits of ~r1 (one v_xnor_b32 with
+		// the fold), so that the column mask 1 << x is a right shift: right
+		// shifts issue at twice the rate of left shifts
+		const uint32_t n1 = ~cur.draw();
 		const uint32_t r2 = cur.draw();
-		int x, y;
+		uint32_t xr;
+		int y;
 		if constexpr(!Cur::TOP)
 		{
-			x = (int)((r1 & 0x7FFFFFu) >> (23 - LBX));
+			uint32_t t = n1 >> (23 - LBX);
+			asm("" : "+v"(t)); // a shift and an and, not one v_bfe_u32
+			xr = t & (BX - 1);
 			y = (int)((r2 & 0x7FFFFFu) >> (23 - LBY));
 		}
 		else
 		{
-			x = (int)(r1 >> (32 - LBX));
+			xr = n1 >> (32 - LBX);
 			y = (int)(r2 >> (32 - LBY));
 		}
+		const uint32_t cm = (1u << (BX - 1)) >> xr; // 1 << x
 		// hide y's origin from the optimiser: the LDS address is then one
 		// shift-and-add of the row number instead of a shift/mask/scale chain
 		asm("" : "+v"(y));
@@ -493,41 +500,39 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 		{
 			// at a minimum own X = 0 and right X = 1, own Y = 0 and up Y = 1:
 			// the flip is X - m (bit x set, bit x+1 cleared), Y | m, up - m
-			const uint32_t fm = rx & ~row.x & ~row.y & up & (1u << x);
+			const uint32_t fm = rx & ~row.x & ~row.y & up & cm;
 			my.setX(y, row.x - fm);
 			my.setY(y, row.y | fm);
 			my.setY(y + 1, up - fm);
 		}
-		else if(UPD == 3)
-		{
-			// SyncRngEW (kpz/kpzSimulationPrimitivesTemplates.cu:199-234): p = q = 1,
-			// every minimum and maximum flips, no acceptance draw
-			const uint32_t mn = rx & ~row.x & ~row.y & up;
-			const uint32_t mx = ~rx & row.x & row.y & ~up;
-			const uint32_t fm = (mn | mx) & (1u << x);
-			my.setX(y, row.x ^ (fm | (fm << 1)));
-			my.setY(y, row.y ^ fm);
-			my.setY(y + 1, up ^ fm);
-		}
 		else
 		{
-			const uint32_t r23 = Cur::rf(cur.draw());
-			uint32_t tpv = tp, tqv = tq;
-			if(UPD == 2)
+			// all-ones when accepted: sign of r23 - t (both at most 2^24); the EW
+			// rule (SyncRngEW, kpz/kpzSimulationPrimitivesTemplates.cu:199-234:
+			// p = q = 1) accepts every extremum without a draw
+			uint32_t aMin = ~0u, aMax = ~0u;
+			if(UPD != 3)
 			{
-				const uint32_t cls = (myD[y * DT_WG] >> (2 * x)) & 3u;
-				tpv = (cls & 2) ? ((cls & 1) ? a.tpc[3] : a.tpc[2]) : ((cls & 1) ? a.tpc[1] : a.tpc[0]);
-				tqv = (cls & 2) ? ((cls & 1) ? a.tqc[3] : a.tqc[2]) : ((cls & 1) ? a.tqc[1] : a.tqc[0]);
+				const uint32_t r23 = Cur::rf(cur.draw());
+				uint32_t tpv = tp, tqv = tq;
+				if(UPD == 2)
+				{
+					const uint32_t x = (BX - 1) - xr;
+					const uint32_t cls = (myD[y * DT_WG] >> (2 * x)) & 3u;
+					tpv = (cls & 2) ? ((cls & 1) ? a.tpc[3] : a.tpc[2]) : ((cls & 1) ? a.tpc[1] : a.tpc[0]);
+					tqv = (cls & 2) ? ((cls & 1) ? a.tqc[3] : a.tqc[2]) : ((cls & 1) ? a.tqc[1] : a.tqc[0]);
+				}
+				aMin = (uint32_t)((int32_t)(r23 - tpv) >> 31);
+				aMax = (uint32_t)((int32_t)(r23 - tqv) >> 31);
 			}
-			// all-ones when accepted: sign of r23 - t (both at most 2^24)
-			const uint32_t aMin = (uint32_t)((int32_t)(r23 - tpv) >> 31);
-			const uint32_t aMax = (uint32_t)((int32_t)(r23 - tqv) >> 31);
 			const uint32_t mn = rx & ~row.x & ~row.y & up;
 			const uint32_t mx = ~rx & row.x & row.y & ~up;
-			const uint32_t fm = ((mn & aMin) | (mx & aMax)) & (1u << x);
-			my.setX(y, row.x ^ (fm | (fm << 1)));
-			my.setY(y, row.y ^ fm);
-			my.setY(y + 1, up ^ fm);
+			// a minimum flips as X - m (bit x set, bit x+1 cleared), a maximum
+			// as X + m (the opposite); Y and the Y above toggle either way
+			const uint32_t fn = mn & aMin & cm, fx = mx & aMax & cm;
+			my.setX(y, row.x - fn + fx);
+			my.setY(y, row.y ^ fn ^ fx);
+			my.setY(y + 1, up ^ fn ^ fx);
 		}
 	}
 
