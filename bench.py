@@ -384,15 +384,15 @@ def main():
         slow, fast = K.valu_peak()
         # issue-time lower bound of one launch from the instruction mix: the
         # update loop of dt_phase_kernel<4,4,0,false> (compiled gfx950 code,
-        # per update) issues 7 slow-class instructions (4 v_mul_lo_u32, 2
-        # v_add3_u32, v_lshl_or_b32), 2 v_mad_u64_u32 (1.4 slow-class slots
-        # each, measured) and 11 fast-class ones (xor, xnor, 4 right shifts
-        # incl. the column mask, 2 bitop3, sub, or, sub); the rest of the
+        # per update) issues 6 slow-class instructions (4 v_mul_lo_u32, 2
+        # v_add3_u32), 2 v_mad_u64_u32 (1.4 slow-class slots each, measured)
+        # and 12 fast-class ones (xor, xnor, 4 right shifts incl. the column
+        # mask, 3 bitop3 incl. the row address, sub, or, sub); the rest of the
         # kernel (staging, write-back, stream set-up: lane_ops / update - 20)
         # is counted as fast class, so the bound is a lower bound
         upd = N / 4
         rest = lane_ops / upd - 20.0
-        bound_s = upd * ((7.0 + 2 * 1.4) / slow + (11.0 + rest) / fast)
+        bound_s = upd * ((6.0 + 2 * 1.4) / slow + (12.0 + rest) / fast)
         valu = {"lane_ops_per_launch": lane_ops, "ops_per_update": lane_ops / upd,
                 "achieved_T_lane_ops": rate / 1e12,
                 "peak_T_lane_ops": slow / 1e12, "peak_fast_class_T_lane_ops": fast / 1e12,

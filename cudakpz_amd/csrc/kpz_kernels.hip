@@ -75,10 +75,25 @@ constexpr int DT_WG = 256;
 struct DtTile {
 	uint32_t* x;
 	uint32_t* y;
+	uint32_t* base;   // start of the tile array (X plane row 0, lane 0)
+	uint32_t lane;    // this lane's byte offset in a row (lane * 4 < DT_WG * 4)
 	__device__ __forceinline__ uint32_t X(int r) const { return x[r * DT_WG]; }
 	__device__ __forceinline__ uint32_t Y(int r) const { return y[r * DT_WG]; }
 	__device__ __forceinline__ void setX(int r, uint32_t v) const { x[r * DT_WG] = v; }
 	__device__ __forceinline__ void setY(int r, uint32_t v) const { y[r * DT_WG] = v; }
+	// by byte offset from the tile start (row * DT_WG * 4 + lane; the Y plane
+	// follows BY rows of X)
+	__device__ __forceinline__ static uint32_t* at(uint32_t* p, uint32_t b)
+	{
+		return reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(p) + b);
+	}
+	template<int BY> __device__ __forceinline__ uint32_t Xb(uint32_t b) const { return *at(base, b); }
+	template<int BY> __device__ __forceinline__ uint32_t Yb(uint32_t b) const { return *at(base, BY * DT_WG * 4 + b); }
+	template<int BY> __device__ __forceinline__ void setXb(uint32_t b, uint32_t v) const { *at(base, b) = v; }
+	template<int BY> __device__ __forceinline__ void setYb(uint32_t b, uint32_t v) const
+	{
+		*at(base, BY * DT_WG * 4 + b) = v;
+	}
 };
 
 // First three delta swaps of the outer unshuffle: a 32-bit row of 16
@@ -473,26 +488,27 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 		// shifts issue at twice the rate of left shifts
 		const uint32_t n1 = ~cur.draw();
 		const uint32_t r2 = cur.draw();
-		uint32_t xr;
-		int y;
+		// the row's LDS byte address comes straight from the draw's bits: a right
+		// shift, then one v_bitop3 for (bits & row mask) | lane offset
+		constexpr int LW = 10; // log2 of the row stride in bytes
+		static_assert((1 << LW) == DT_WG * 4, "row stride");
+		uint32_t xr, ys;
 		if constexpr(!Cur::TOP)
 		{
 			uint32_t t = n1 >> (23 - LBX);
 			asm("" : "+v"(t)); // a shift and an and, not one v_bfe_u32
 			xr = t & (BX - 1);
-			y = (int)((r2 & 0x7FFFFFu) >> (23 - LBY));
+			ys = r2 >> (23 - LBY - LW);
 		}
 		else
 		{
 			xr = n1 >> (32 - LBX);
-			y = (int)(r2 >> (32 - LBY));
+			ys = r2 >> (32 - LBY - LW);
 		}
+		const uint32_t yo = __builtin_amdgcn_bitop3_b32(ys, (uint32_t)((BY - 1) << LW), my.lane, 0xEA);
 		const uint32_t cm = (1u << (BX - 1)) >> xr; // 1 << x
-		// hide y's origin from the optimiser: the LDS address is then one
-		// shift-and-add of the row number instead of a shift/mask/scale chain
-		asm("" : "+v"(y));
-		uint2 row = make_uint2(my.X(y), my.Y(y));
-		const uint32_t up = my.Y(y + 1);
+		uint2 row = make_uint2(my.template Xb<BY>(yo), my.template Yb<BY>(yo));
+		const uint32_t up = my.template Yb<BY>(yo + DT_WG * 4);
 		// whole-row bit algebra, then select column x: a local minimum has
 		// own X = 0, right X = 1, own Y = 0, up Y = 1; a maximum the opposite
 		const uint32_t rx = row.x >> 1;
@@ -501,9 +517,9 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 			// at a minimum own X = 0 and right X = 1, own Y = 0 and up Y = 1:
 			// the flip is X - m (bit x set, bit x+1 cleared), Y | m, up - m
 			const uint32_t fm = rx & ~row.x & ~row.y & up & cm;
-			my.setX(y, row.x - fm);
-			my.setY(y, row.y | fm);
-			my.setY(y + 1, up - fm);
+			my.template setXb<BY>(yo, row.x - fm);
+			my.template setYb<BY>(yo, row.y | fm);
+			my.template setYb<BY>(yo + DT_WG * 4, up - fm);
 		}
 		else
 		{
@@ -518,7 +534,7 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 				if(UPD == 2)
 				{
 					const uint32_t x = (BX - 1) - xr;
-					const uint32_t cls = (myD[y * DT_WG] >> (2 * x)) & 3u;
+					const uint32_t cls = (*DtTile::at(myD, yo & ((BY - 1) << LW)) >> (2 * x)) & 3u;
 					tpv = (cls & 2) ? ((cls & 1) ? a.tpc[3] : a.tpc[2]) : ((cls & 1) ? a.tpc[1] : a.tpc[0]);
 					tqv = (cls & 2) ? ((cls & 1) ? a.tqc[3] : a.tqc[2]) : ((cls & 1) ? a.tqc[1] : a.tqc[0]);
 				}
@@ -530,9 +546,9 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 			// a minimum flips as X - m (bit x set, bit x+1 cleared), a maximum
 			// as X + m (the opposite); Y and the Y above toggle either way
 			const uint32_t fn = mn & aMin & cm, fx = mx & aMax & cm;
-			my.setX(y, row.x - fn + fx);
-			my.setY(y, row.y ^ fn ^ fx);
-			my.setY(y + 1, up ^ fn ^ fx);
+			my.template setXb<BY>(yo, row.x - fn + fx);
+			my.template setYb<BY>(yo, row.y ^ fn ^ fx);
+			my.template setYb<BY>(yo + DT_WG * 4, up ^ fn ^ fx);
 		}
 	}
 
@@ -589,7 +605,8 @@ __global__ __launch_bounds__(DT_WG) void dt_phase_kernel(DtPhaseArgs a, int64_t 
 	if(i >= nb)
 		return; // no barrier in this kernel: idle lanes may leave
 	DtSlcg<RO> cur;
-	dt_block<LBX, LBY, UPD, RO>(a, i, nb, DtTile{tile + threadIdx.x, tile + (1 << LBY) * DT_WG + threadIdx.x},
+	dt_block<LBX, LBY, UPD, RO>(a, i, nb,
+	                            DtTile{tile + threadIdx.x, tile + (1 << LBY) * DT_WG + threadIdx.x, tile, threadIdx.x * 4},
 	                            (UPD == 2) ? tileD + threadIdx.x : nullptr, cur);
 }
 
@@ -606,7 +623,7 @@ __global__ __launch_bounds__(DT_WG) void dt_tmt_kernel(DtPhaseArgs a, int64_t nb
 		return; // no barrier in this kernel; workers without a block keep their stream
 	DtTmt cur;
 	cur.g = TinyMT::load(g, a.T, t);
-	const DtTile my{tile + threadIdx.x, tile + (1 << LBY) * DT_WG + threadIdx.x};
+	const DtTile my{tile + threadIdx.x, tile + (1 << LBY) * DT_WG + threadIdx.x, tile, threadIdx.x * 4};
 	for(int64_t i = t; i < nb; i += a.T)
 		dt_block<LBX, LBY, UPD, true>(a, i, nb, my, (UPD == 2) ? tileD + threadIdx.x : nullptr, cur);
 	cur.g.store(g, a.T, t);
@@ -2007,7 +2024,7 @@ __global__ __launch_bounds__(DT_WG) void batch_dt_kernel(BatchDtArgs a)
 	p.ds = (a.order[mi] >> (2 * a.d)) & 3;
 	const int64_t nb = 1ll << (rep.lx + rep.ly - 10);
 	p.T = nb;
-	const DtTile my{tile + threadIdx.x, tile + 16 * DT_WG + threadIdx.x};
+	const DtTile my{tile + threadIdx.x, tile + 16 * DT_WG + threadIdx.x, tile, threadIdx.x * 4};
 	if(rep.kind == BK_DTR)
 	{
 		const uint16_t sh = a.shift[mi];
