@@ -66,7 +66,9 @@ __device__ __forceinline__ uint64_t shflw(uint64_t v, int src, int width) { retu
 // ======================================================================
 // DT phase kernel
 // ======================================================================
-constexpr int DT_WG = 256;
+constexpr int DT_WG = 64;
+constexpr int DT_LOG2_ROW_BYTES = 8; // a tile row: DT_WG lanes x 4 bytes
+static_assert((1 << DT_LOG2_ROW_BYTES) == DT_WG * 4, "row stride");
 
 // A lane's column of the DT LDS tile: the bit-plane rows in two planes,
 // X rows 0..BY-1 and Y rows 0..BY (BY = the halo row above the block), each
@@ -490,8 +492,7 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 		const uint32_t r2 = cur.draw();
 		// the row's LDS byte address comes straight from the draw's bits: a right
 		// shift, then one v_bitop3 for (bits & row mask) | lane offset
-		constexpr int LW = 10; // log2 of the row stride in bytes
-		static_assert((1 << LW) == DT_WG * 4, "row stride");
+		constexpr int LW = DT_LOG2_ROW_BYTES; // log2 of the row stride in bytes
 		uint32_t xr, ys;
 		if constexpr(!Cur::TOP)
 		{
