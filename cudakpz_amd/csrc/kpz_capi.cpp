@@ -2,6 +2,7 @@
 #include "../../include/kpz_amd.h"
 #include "kpz_batch.hpp"
 #include "kpz_capi_internal.hpp"
+#include "kpz_tinymt.hpp"
 #include "kpz_scheduler.hpp"
 #include "kpz_server.hpp"
 
@@ -372,6 +373,18 @@ int kpz_set_tinymt_params(kpz_scheduler* h, const uint32_t* mat1, const uint32_t
 	if(!mat1 || !mat2 || !tmat)
 		return fail(KPZ_EINVAL, "null input");
 	return guard([&] { h->s->setTinyMTParams(mat1, mat2, tmat); });
+}
+
+int kpz_tinymt_jump(const uint32_t gen[7], uint64_t n, uint32_t status[4])
+{
+	if(!gen || !status)
+		return fail(KPZ_EINVAL, "null argument");
+	const kpz::TmtOrbit m = kpz::tmt_orbit(gen, gen[4], gen[5]);
+	uint32_t s[4] = {gen[0], gen[1], gen[2], gen[3]};
+	kpz::tmt_jump4(s, kpz::poly_xpow(n, m), m, gen[4], gen[5]);
+	for(int k = 0; k < 4; ++k)
+		status[k] = s[k];
+	return (int)m.deg;
 }
 
 int kpz_get_tinymt(kpz_scheduler* h, uint32_t* planes)

@@ -662,10 +662,14 @@ __device__ __forceinline__ void cbbit_row(const CbHalfArgs& a, int y, int g)
 		t = a.rem + yy / a.per;
 		r = yy - (t - a.rem) * a.per;
 	}
-	const uint64_t s0 = a.st_in[t];
-	uint64_t s = a.laneJ[g](a.rowJ[r](s0));
-	if(r == 0 && g == 0)
-		a.st_out[t] = (t < a.rem ? a.fullBig : a.fullSmall)(s0);
+	uint64_t s = 0;
+	if constexpr(MODE < 3)
+	{
+		const uint64_t s0 = a.st_in[t];
+		s = a.laneJ[g](a.rowJ[r](s0));
+		if(r == 0 && g == 0)
+			a.st_out[t] = (t < a.rem ? a.fullBig : a.fullSmall)(s0);
+	}
 
 	const bool par = (a.signum == (y & 1));
 	const int64_t thisB = a.signum ? 2 * Q : 0, othB = a.signum ? 0 : 2 * Q;
@@ -692,7 +696,15 @@ __device__ __forceinline__ void cbbit_row(const CbHalfArgs& a, int y, int g)
 		const W xXs = par ? xX : (W)((xX >> 1) | (nbr << (WB - 1)));
 		const W minm = ~(tX | tY) & xXs & yY;
 		W flip;
-		if(MODE == 0)
+		if constexpr(MODE == 3)
+			flip = minm & ((const W*)a.rnd)[row + xw]; // VL3 random words of this half-sweep
+		else if constexpr(MODE == 4)
+		{
+			// per-site acceptance masks of this half-sweep (minima, maxima)
+			const W maxm = tX & tY & ~xXs & ~yY;
+			flip = (minm & ((const W*)a.rnd)[row + xw]) | (maxm & ((const W*)a.rnd)[Q + row + xw]);
+		}
+		else if(MODE == 0)
 		{
 			W rnd;
 			uint64_t u = sW;
@@ -786,7 +798,8 @@ __device__ __forceinline__ void cbbit_row(const CbHalfArgs& a, int y, int g)
 			}
 			flip = (minm & pb) | (maxm & qb);
 		}
-		s = stepJ(sW);
+		if constexpr(MODE < 3)
+			s = stepJ(sW);
 		lat[thisB + row + xw] = tX ^ flip;
 		lat[thisB + Q + row + xw] = tY ^ flip;
 		lat[othB + Q + rowUp + xw] = yY ^ flip;
@@ -1099,91 +1112,111 @@ __global__ __launch_bounds__(256) void cb_local_fix_kernel(CbLocalArgs a)
 // 324-331).  A TinyMT32 stream has no cheap jump ahead, so lane t is worker t
 // and replays the worker's own walk with its own stream.
 // ======================================================================
-// SchedulerCPUCBBit<TinyMT32>, one half-sweep: worker t walks its rows
-// [yMin, ySup) ascending and each row's words ascending
-// (kpz/schedulerCPUCBBit.cpp:43-79); MODE 0 = pSyncRngUpdateVL3 (two draws
-// per u64 word, the first the high half; one per u32 word when L = 64),
-// 1 = per-site pq, 2 = disorder2SyncRngUpdate (one randomFloat per active
-// site, x ascending, kpz/schedulerCPUCBBit_nested.inl:200-331).  A row writes
-// its own words and the other sublattice's Y word of the row above, which no
-// other row of the half-sweep reads or writes: workers never share a word.
-template<class W, int MODE>
-__global__ __launch_bounds__(256) void cbbit_tmt_kernel(CbHalfArgs a, uint32_t* __restrict__ g)
+// worker t of row y and the row's index r in the worker's rows
+// (kpz/schedulerCPUCBBit.cpp:46-62: the first rem workers have per + 1 rows)
+__device__ __forceinline__ void cbbit_worker_row(int64_t y, int per, int rem, int64_t& t, int64_t& r)
 {
-	const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-	if(t >= a.T)
-		return;
-	constexpr int WB = (int)sizeof(W) * 8;
-	constexpr W MSB = (W)1 << (WB - 1);
-	TinyMT r = TinyMT::load(g, a.T, t);
-	const int64_t y0 = t * a.per + (t < a.rem ? t : a.rem);
-	const int64_t y1 = y0 + a.per + (t < a.rem ? 1 : 0);
-	W* __restrict__ lat = reinterpret_cast<W*>(a.lat);
-	const W* __restrict__ dis = reinterpret_cast<const W*>(a.dis);
-	const uint64_t Wn = 1ull << (a.lx - (WB == 64 ? 7 : 6)); // words per row and quadrant
-	const uint64_t Q = Wn << a.ly;                            // words per quadrant
-	const uint64_t thisBase = a.signum ? 2 * Q : 0, othBase = a.signum ? 0 : 2 * Q;
-	const int64_t mY = (1ll << a.ly) - 1;
-	for(int64_t y = y0; y < y1; ++y)
+	const int64_t big = (int64_t)rem * (per + 1);
+	if(y < big)
 	{
-		const bool parity = a.signum == (int)(y & 1);
-		const uint64_t rowOff = (uint64_t)y * Wn, upOff = (uint64_t)((y + 1) & mY) * Wn;
-		for(uint64_t xw = 0; xw < Wn; ++xw)
+		t = y / (per + 1);
+		r = y - t * (per + 1);
+	}
+	else
+	{
+		t = rem + (y - big) / per;
+		r = (y - big) - (t - rem) * per;
+	}
+}
+
+// SchedulerCPUCBBit<TinyMT32>, the random part of one half-sweep: lane
+// (y, c) is chunk c of the 2^lC chunks of site row y; it takes its worker's
+// stream to the chunk's first draw (a jump, rowJ[lane] = x^(r * draws per
+// row + c * draws per chunk) mod the stream's orbit polynomial) and draws the
+// chunk's words in order
+// (kpz/schedulerCPUCBBit.cpp:43-79): MODE 0 = pSyncRngUpdateVL3's random
+// words (two draws per u64 word, the first the high half; one per u32 word
+// when L = 64), 1 = per-site pq, 2 = disorder2SyncRngUpdate (one
+// randomFloat() per active site, x ascending, accepted as (u >> 8) < t; the
+// class is the site's x-slope bit of the map).  Output: the words (MODE 0) or
+// the minimum and maximum acceptance masks (Q words each), which
+// cbbit_half_kernel<W, 3 / 4> applies.  The lane of a worker's last row
+// writes the worker's state after the half-sweep.
+template<class W, int MODE>
+__global__ __launch_bounds__(256) void cbbit_tmt_gen_kernel(CbHalfArgs a, int lC, const uint32_t* __restrict__ gin,
+		uint32_t* __restrict__ gout, const Poly128* __restrict__ rowJ, const TmtOrbit* __restrict__ orb,
+		W* __restrict__ out)
+{
+	const int64_t lane = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if(lane >= (1ll << (a.ly + lC)))
+		return;
+	const int64_t y = lane >> lC;
+	const int c = (int)(lane & ((1 << lC) - 1));
+	constexpr int WB = (int)sizeof(W) * 8;
+	int64_t t, r;
+	cbbit_worker_row(y, a.per, a.rem, t, r);
+	TinyMT g = TinyMT::load(gin, a.T, t);
+	if(r > 0 || c > 0)
+	{
+		uint32_t st[4] = {g.s0, g.s1, g.s2, g.s3};
+		tmt_jump4(st, rowJ[lane], orb[t], g.m1, g.m2);
+		g.s0 = st[0];
+		g.s1 = st[1];
+		g.s2 = st[2];
+		g.s3 = st[3];
+	}
+	const uint64_t Wn = 1ull << (a.lx - (WB == 64 ? 7 : 6)); // words per row and quadrant
+	const uint64_t Q = Wn << a.ly;
+	const uint64_t row = (uint64_t)y * Wn;
+	const W* __restrict__ dis = (MODE == 2) ? reinterpret_cast<const W*>(a.dis) + (a.signum ? 2 * Q : 0) : nullptr;
+	const uint64_t wc = Wn >> lC; // words per chunk
+	for(uint64_t xw = (uint64_t)c * wc; xw < (uint64_t)(c + 1) * wc; ++xw)
+	{
+		if constexpr(MODE == 0)
 		{
-			const uint64_t off = rowOff + xw;
-			const uint64_t tXi = thisBase + off, tYi = thisBase + Q + off;
-			const uint64_t yIdx = othBase + Q + upOff + xw, xIdx = othBase + off;
-			const W tX = lat[tXi], tY = lat[tYi], yY = lat[yIdx];
-			W xX = lat[xIdx];
-			uint64_t nIdx = xIdx;
-			if(!parity)
+			if constexpr(WB == 64)
 			{
-				nIdx = (xw + 1 == Wn) ? xIdx - xw : xIdx + 1;
-				xX = ((xX >> 1) & ~MSB) | ((W)(lat[nIdx] << (WB - 1)) & MSB);
-			}
-			W m;
-			if constexpr(MODE == 0)
-			{
-				W rnd;
-				if constexpr(WB == 64)
-				{
-					const uint64_t hi = r.u32();
-					const uint64_t lo = r.u32();
-					rnd = (W)((hi << 32) ^ lo);
-				}
-				else
-					rnd = (W)r.u32();
-				m = ~(tX | tY) & xX & yY & rnd;
+				const uint64_t hi = g.u32();
+				const uint64_t lo = g.u32();
+				out[row + xw] = (W)((hi << 32) ^ lo);
 			}
 			else
+				out[row + xw] = (W)g.u32();
+		}
+		else
+		{
+			const W cl = (MODE == 2) ? dis[row + xw] : (W)0;
+			W pb = 0, qb = 0;
+			for(int k = 0; k < WB; ++k)
 			{
-				// minima (own X = Y = 0, right X = up Y = 1) and maxima, per site
-				const W mn = ~(tX | tY) & xX & yY;
-				const W mx = tX & tY & ~xX & ~yY;
-				const W cl = (MODE == 2) ? dis[tXi] : (W)0; // class: the site's x-slope bit of the map
-				m = 0;
-				for(int k = 0; k < WB; ++k)
-				{
-					const uint32_t rf = r.u32() >> 8;
-					const bool c1 = (cl >> k) & 1;
-					const uint32_t tp = c1 ? a.tp1 : a.tp, tq = c1 ? a.tq1 : a.tq;
-					const W acc = (W)(rf < tp ? mn : 0) | (W)(rf < tq ? mx : 0);
-					m |= acc & ((W)1 << k);
-				}
+				const uint32_t rf = g.u32() >> 8;
+				const bool c1 = (cl >> k) & 1;
+				const uint32_t tp = c1 ? a.tp1 : a.tp, tq = c1 ? a.tq1 : a.tq;
+				pb |= (W)(rf < tp ? 1u : 0u) << k;
+				qb |= (W)(rf < tq ? 1u : 0u) << k;
 			}
-			W xm = m;
-			if(!parity)
-			{
-				lat[nIdx] = lat[nIdx] ^ (W)((m >> (WB - 1)) & 1);
-				xm = (W)(m << 1);
-			}
-			lat[xIdx] = lat[xIdx] ^ xm;
-			lat[tXi] = tX ^ m;
-			lat[tYi] = tY ^ m;
-			lat[yIdx] = yY ^ m;
+			out[row + xw] = pb;
+			out[Q + row + xw] = qb;
 		}
 	}
-	r.store(g, a.T, t);
+	const int64_t rows = a.per + (t < a.rem ? 1 : 0);
+	if(r == rows - 1 && c == (1 << lC) - 1)
+		g.store(gout, a.T, t);
+}
+
+// out[(y << lC) + c] = x^(r * drawsPerRow + c * drawsPerRow / 2^lC) mod the
+// orbit polynomial of row y's worker
+__global__ __launch_bounds__(256) void tmt_row_polys_kernel(const TmtOrbit* __restrict__ orb, int64_t nrows, int lC,
+		int per, int rem, uint64_t drawsPerRow, Poly128* __restrict__ out)
+{
+	const int64_t lane = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if(lane >= (nrows << lC))
+		return;
+	const int64_t y = lane >> lC;
+	const uint64_t c = (uint64_t)(lane & ((1 << lC) - 1));
+	int64_t t, r;
+	cbbit_worker_row(y, per, rem, t, r);
+	out[lane] = poly_xpow((uint64_t)r * drawsPerRow + c * (drawsPerRow >> lC), orb[t]);
 }
 
 // SchedulerCPUCB<TinyMT32>, one half-sweep: worker t walks the first half of
@@ -2186,6 +2219,57 @@ hipError_t dt_dispatch_y(const DtPhaseArgs& a, int64_t nb, dim3 grid, hipStream_
 
 bool dt_layout_supported(int lbx, int lby) { return lbx >= 2 && lbx <= 4 && lby >= 2 && lby <= 4; }
 
+hipError_t launch_tmt_row_polys(const TmtOrbit* orb, int64_t nrows, int lC, int per, int rem, uint64_t drawsPerRow,
+		Poly128* out, hipStream_t stream)
+{
+	if(!orb || !out || nrows < 1 || lC < 0 || lC > 8 || per < 0 || rem < 0 || (drawsPerRow & ((1ull << lC) - 1)))
+		return hipErrorInvalidValue;
+	const int64_t lanes = nrows << lC;
+	hipLaunchKernelGGL(tmt_row_polys_kernel, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, stream, orb, nrows, lC,
+	                   per, rem, drawsPerRow, out);
+	return hipGetLastError();
+}
+
+hipError_t launch_cbbit_tmt_gen(const CbHalfArgs& a, int mode, int lC, const uint32_t* gin, uint32_t* gout,
+		const Poly128* rowJ, const TmtOrbit* orb, void* out, hipStream_t stream)
+{
+	const int lWn = a.lx - (a.lx >= 7 ? 7 : 6);
+	if(a.lx < 6 || a.ly < 1 || a.T < 1 || a.T > (1ll << a.ly) || mode < 0 || mode > 2 || (mode == 2 && !a.dis) ||
+	   lC < 0 || lC > lWn || !gin || !gout || !out ||
+	   ((lC > 0 || a.per > 1 || (a.per == 1 && a.rem > 0)) && (!rowJ || !orb)))
+		return hipErrorInvalidValue;
+	const dim3 grid((unsigned)(((1ll << (a.ly + lC)) + 255) / 256));
+	const bool wide = a.lx >= 7;
+	switch(mode * 2 + (wide ? 1 : 0))
+	{
+		case 0:
+			hipLaunchKernelGGL((cbbit_tmt_gen_kernel<uint32_t, 0>), grid, dim3(256), 0, stream, a, lC, gin, gout, rowJ, orb,
+			                   (uint32_t*)out);
+			break;
+		case 1:
+			hipLaunchKernelGGL((cbbit_tmt_gen_kernel<uint64_t, 0>), grid, dim3(256), 0, stream, a, lC, gin, gout, rowJ, orb,
+			                   (uint64_t*)out);
+			break;
+		case 2:
+			hipLaunchKernelGGL((cbbit_tmt_gen_kernel<uint32_t, 1>), grid, dim3(256), 0, stream, a, lC, gin, gout, rowJ, orb,
+			                   (uint32_t*)out);
+			break;
+		case 3:
+			hipLaunchKernelGGL((cbbit_tmt_gen_kernel<uint64_t, 1>), grid, dim3(256), 0, stream, a, lC, gin, gout, rowJ, orb,
+			                   (uint64_t*)out);
+			break;
+		case 4:
+			hipLaunchKernelGGL((cbbit_tmt_gen_kernel<uint32_t, 2>), grid, dim3(256), 0, stream, a, lC, gin, gout, rowJ, orb,
+			                   (uint32_t*)out);
+			break;
+		default:
+			hipLaunchKernelGGL((cbbit_tmt_gen_kernel<uint64_t, 2>), grid, dim3(256), 0, stream, a, lC, gin, gout, rowJ, orb,
+			                   (uint64_t*)out);
+			break;
+	}
+	return hipGetLastError();
+}
+
 hipError_t launch_tmt_seed(uint32_t* g, int64_t T, hipStream_t stream)
 {
 	if(!g || T < 1)
@@ -2211,20 +2295,28 @@ hipError_t launch_dt_phase(const DtPhaseArgs& a, hipStream_t stream)
 
 hipError_t launch_cbbit_half(const CbHalfArgs& a, hipStream_t stream)
 {
-	if(a.tmt)
+	if(a.mode == 3 || a.mode == 4)
 	{
-		if(a.lx < 6 || a.ly < 1 || a.T < 1 || a.mode < 0 || a.mode > 2 || (a.mode == 2 && !a.dis))
+		// random words / acceptance masks from a buffer (the TinyMT32 workers)
+		if(!a.rnd)
 			return hipErrorInvalidValue;
-		const dim3 grid((unsigned)((a.T + 255) / 256));
+		const int64_t threads = (int64_t)1 << (a.ly + a.lG);
+		const int block = (int)(threads < 256 ? threads : 256);
+		const dim3 grid((unsigned)(threads / block));
 		const bool wide = a.lx >= 7;
-		switch(a.mode * 2 + (wide ? 1 : 0))
+		if(a.mode == 3)
 		{
-			case 0: hipLaunchKernelGGL((cbbit_tmt_kernel<uint32_t, 0>), grid, dim3(256), 0, stream, a, a.tmt); break;
-			case 1: hipLaunchKernelGGL((cbbit_tmt_kernel<uint64_t, 0>), grid, dim3(256), 0, stream, a, a.tmt); break;
-			case 2: hipLaunchKernelGGL((cbbit_tmt_kernel<uint32_t, 1>), grid, dim3(256), 0, stream, a, a.tmt); break;
-			case 3: hipLaunchKernelGGL((cbbit_tmt_kernel<uint64_t, 1>), grid, dim3(256), 0, stream, a, a.tmt); break;
-			case 4: hipLaunchKernelGGL((cbbit_tmt_kernel<uint32_t, 2>), grid, dim3(256), 0, stream, a, a.tmt); break;
-			default: hipLaunchKernelGGL((cbbit_tmt_kernel<uint64_t, 2>), grid, dim3(256), 0, stream, a, a.tmt); break;
+			if(wide)
+				hipLaunchKernelGGL((cbbit_half_kernel<uint64_t, 3>), grid, dim3(block), 0, stream, a);
+			else
+				hipLaunchKernelGGL((cbbit_half_kernel<uint32_t, 3>), grid, dim3(block), 0, stream, a);
+		}
+		else
+		{
+			if(wide)
+				hipLaunchKernelGGL((cbbit_half_kernel<uint64_t, 4>), grid, dim3(block), 0, stream, a);
+			else
+				hipLaunchKernelGGL((cbbit_half_kernel<uint32_t, 4>), grid, dim3(block), 0, stream, a);
 		}
 		return hipGetLastError();
 	}

@@ -4,6 +4,7 @@
 #pragma once
 
 #include "kpz_common.hpp"
+#include "kpz_tinymt.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -57,11 +58,24 @@ struct CbHalfArgs {
 	// class = x-slope bit of the site in the map (ENC_CBBIT layout), class-1 thresholds
 	const uint32_t* dis;
 	uint32_t tp1, tq1;
-	// TinyMT32 streams (7 planes of T words), or null for SLCG64: then one lane
-	// per worker, thresholds on the 2^24 scale, the jump tables unused
-	uint32_t* tmt;
+	// modes 3 (VL3 random words) and 4 (per-site acceptance masks, minima then
+	// maxima, Q words each) take the half-sweep's random part from this
+	// buffer (the TinyMT32 workers, launch_cbbit_tmt_gen)
+	const void* rnd;
 };
 hipError_t launch_cbbit_half(const CbHalfArgs& a, hipStream_t stream);
+// TinyMT32 workers of SCABit: the random part of one half-sweep (mode 0 VL3,
+// 1 per-site pq, 2 disorder) into `out`, one lane per 1/2^lC of a row;
+// streams read from gin, the state after the half-sweep written to gout (7
+// planes of T words each); rowJ (per lane) / orb (per worker) are needed when
+// a lane starts mid-stream (lC > 0 or a worker with several rows)
+hipError_t launch_cbbit_tmt_gen(const CbHalfArgs& a, int mode, int lC, const uint32_t* gin, uint32_t* gout,
+		const Poly128* rowJ, const TmtOrbit* orb, void* out, hipStream_t stream);
+// rowJ for launch_cbbit_tmt_gen: x^(r * drawsPerRow + c * drawsPerRow / 2^lC)
+// mod the orbit polynomial of row y's worker, r the row's index among the
+// worker's rows, for lane (y << lC) + c
+hipError_t launch_tmt_row_polys(const TmtOrbit* orb, int64_t nrows, int lC, int per, int rem, uint64_t drawsPerRow,
+		Poly128* out, hipStream_t stream);
 
 // ---- SCA on ENC_LOCAL: one checkerboard half-sweep (SchedulerCPUCB) --------
 // Worker t owns cell columns [c0_t, c0_t + w_t), w_t = pc (+1 for t < rem);

@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <thread>
 
 namespace kpz {
 
@@ -79,7 +80,8 @@ void SchedulerService::allocStreams(bool twoBuffers)
 {
 	if(tinyMT())
 	{
-		tmt_.alloc((size_t)T_ * 7 * 4);
+		tmt_[0].alloc((size_t)T_ * 7 * 4);
+		tmt_[1].alloc((size_t)T_ * 7 * 4);
 		return;
 	}
 	st_[0].alloc((size_t)T_ * 8);
@@ -99,9 +101,10 @@ void SchedulerService::randomize()
 		std::vector<uint32_t> seed((size_t)T_);
 		for(auto& v : seed)
 			v = (uint32_t)(master_.close_open() * 4294967295.0);
-		check(hipMemcpyAsync(tmt_.get(), seed.data(), seed.size() * 4, hipMemcpyHostToDevice, stream_), "seeds H2D");
-		check(launch_tmt_seed(tmt_.as<uint32_t>(), T_, stream_), "tmt_seed");
+		check(hipMemcpyAsync(tmtIn(), seed.data(), seed.size() * 4, hipMemcpyHostToDevice, stream_), "seeds H2D");
+		check(launch_tmt_seed(tmtIn(), T_, stream_), "tmt_seed");
 		sync();
+		tmtStreamsChanged();
 		return;
 	}
 	const double m1 = master_.close_open();
@@ -239,18 +242,20 @@ void SchedulerService::setTinyMTParams(const uint32_t* mat1, const uint32_t* mat
 		throw Error(-1, "setTinyMTParams: this scheduler uses SLCG64 streams");
 	const size_t T = (size_t)T_;
 	const uint32_t* src[3] = {mat1, mat2, tmat};
-	for(int k = 0; k < 3; ++k)
-		check(hipMemcpyAsync(tmt_.as<uint32_t>() + (4 + k) * T, src[k], T * 4, hipMemcpyHostToDevice, stream_),
-				"TinyMT parameters H2D");
+	for(int b = 0; b < 2; ++b)
+		for(int k = 0; k < 3; ++k)
+			check(hipMemcpyAsync(tmt_[b].as<uint32_t>() + (4 + k) * T, src[k], T * 4, hipMemcpyHostToDevice, stream_),
+					"TinyMT parameters H2D");
 	sync();
 	tmtParams_ = true;
+	tmtStreamsChanged();
 }
 
 void SchedulerService::getTinyMT(uint32_t* out)
 {
 	if(!tinyMT())
 		throw Error(-1, "getTinyMT: this scheduler uses SLCG64 streams");
-	check(hipMemcpyAsync(out, tmt_.get(), (size_t)T_ * 28, hipMemcpyDeviceToHost, stream_), "getTinyMT");
+	check(hipMemcpyAsync(out, tmtIn(), (size_t)T_ * 28, hipMemcpyDeviceToHost, stream_), "getTinyMT");
 	sync();
 }
 
@@ -258,9 +263,40 @@ void SchedulerService::setTinyMT(const uint32_t* in)
 {
 	if(!tinyMT())
 		throw Error(-1, "setTinyMT: this scheduler uses SLCG64 streams");
-	check(hipMemcpyAsync(tmt_.get(), in, (size_t)T_ * 28, hipMemcpyHostToDevice, stream_), "setTinyMT");
+	for(int b = 0; b < 2; ++b)
+		check(hipMemcpyAsync(tmt_[b].get(), in, (size_t)T_ * 28, hipMemcpyHostToDevice, stream_), "setTinyMT");
 	sync();
 	tmtParams_ = true;
+	tmtStreamsChanged();
+}
+
+// orbit polynomial of every stream's current state (host threads; the
+// elimination takes ~40 us per stream)
+const TmtOrbit* SchedulerService::tmtOrbits()
+{
+	if(orbitValid_)
+		return orbit_.as<TmtOrbit>();
+	const size_t T = (size_t)T_;
+	std::vector<uint32_t> g(T * 7);
+	getTinyMT(g.data());
+	std::vector<TmtOrbit> o(T);
+	const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+	std::vector<std::thread> pool;
+	for(unsigned w = 0; w < nth; ++w)
+		pool.emplace_back([&, w] {
+			for(size_t t = w; t < T; t += nth)
+			{
+				const uint32_t st[4] = {g[t], g[T + t], g[2 * T + t], g[3 * T + t]};
+				o[t] = tmt_orbit(st, g[4 * T + t], g[5 * T + t]);
+			}
+		});
+	for(auto& th : pool)
+		th.join();
+	orbit_.alloc(T * sizeof(TmtOrbit));
+	check(hipMemcpyAsync(orbit_.get(), o.data(), T * sizeof(TmtOrbit), hipMemcpyHostToDevice, stream_), "orbits H2D");
+	sync();
+	orbitValid_ = true;
+	return orbit_.as<TmtOrbit>();
 }
 
 const Affine* SchedulerService::upload(std::vector<Affine>& host, DevBuf& buf)
@@ -418,7 +454,7 @@ void SchedulerDT::run(int n, int upd)
 	a.lbx = lbx_;
 	a.lby = lby_;
 	a.T = T_;
-	a.tmt = tinyMT() ? tmt_.as<uint32_t>() : nullptr;
+	a.tmt = tinyMT() ? tmtIn() : nullptr;
 	a.jumps = tinyMT() ? nullptr : jumps(du);
 	a.step = step_;
 	a.blockJ = affine_pow(step_, (uint64_t)du << (lbx_ + lby_));
@@ -528,12 +564,41 @@ void SchedulerSCABit::run(int n, int mode)
 		throw Error(-4, "mcsDisorder2SyncRng needs a disorder map (generateDisorder2 / setDisorder)");
 	if(tinyMT())
 	{
-		// one lane per worker with its own TinyMT32 stream
+		// TinyMT32 workers: per half-sweep the random part first (one lane per
+		// row, a stream jumped to the row's first draw), then the bit-parallel
+		// kernel applies it
+		const bool wide = size_.lx >= 7;
+		const uint64_t WB = wide ? 64 : 32;
+		const uint64_t Wn = 1ull << (size_.lx - (wide ? 7 : 6));
+		const uint64_t Q = Wn << size_.ly;
+		const uint64_t drawsPerRow = Wn * (mode == 0 ? (wide ? 2 : 1) : WB);
+		// rows of many draws are split into chunks of >= 4096 draws (more lanes,
+		// each jumped to its chunk); a chunk is at least one word
+		int lC = 0;
+		while(lC < 4 && (drawsPerRow >> (lC + 1)) >= 4096 && (Wn >> (lC + 1)) >= 1)
+			++lC;
+		const bool split = lC > 0 || per_ > 1 || (per_ == 1 && rem_ > 0); // lanes starting mid-stream
+		const TmtOrbit* orb = split ? tmtOrbits() : nullptr;
+		const Poly128* rowJ = nullptr;
+		if(split)
+		{
+			auto& tab = tmtTab_[(drawsPerRow << 4) | (uint64_t)lC];
+			if(!tab)
+			{
+				tab = std::make_unique<DevBuf>(((size_t)size_.dimY() << lC) * sizeof(Poly128));
+				check(launch_tmt_row_polys(orb, size_.dimY(), lC, per_, rem_, drawsPerRow, tab->as<Poly128>(), stream_),
+						"tmt_row_polys");
+			}
+			rowJ = tab->as<Poly128>();
+		}
+		const size_t need = (size_t)(mode == 0 ? Q : 2 * Q) * (WB / 8);
+		if(rndBuf_.bytes() < need)
+			rndBuf_.alloc(need);
 		CbHalfArgs a{};
 		a.lat = deviceLattice();
 		a.lx = size_.lx;
 		a.ly = size_.ly;
-		a.mode = mode;
+		a.mode = mode == 0 ? 3 : 4;
 		a.T = T_;
 		a.per = per_;
 		a.rem = rem_;
@@ -542,13 +607,17 @@ void SchedulerSCABit::run(int n, int mode)
 		a.dis = dis_.as<uint32_t>();
 		a.tp1 = thresh(rateP(1));
 		a.tq1 = thresh(rateQ(1));
-		a.tmt = tmt_.as<uint32_t>();
+		a.lG = lG_;
+		a.rnd = rndBuf_.get();
 		for(int m = 0; m < n; ++m)
 		{
 			for(int s = 0; s < 2; ++s)
 			{
 				a.signum = s;
+				check(launch_cbbit_tmt_gen(a, mode, lC, tmtIn(), tmtOut(), rowJ, orb, rndBuf_.get(), stream_),
+						"cbbit_tmt_gen");
 				check(launch_cbbit_half(a, stream_), "cbbit_half");
+				tmtCur_ ^= 1;
 			}
 			++mcs_;
 		}
@@ -708,7 +777,7 @@ void SchedulerSCA::run(int n, bool disorder)
 	a.tp = thresh(p_);
 	a.tq = thresh(q_);
 	a.dis = disorder ? dis_.as<uint32_t>() : nullptr;
-	a.tmt = tinyMT() ? tmt_.as<uint32_t>() : nullptr;
+	a.tmt = tinyMT() ? tmtIn() : nullptr;
 	classThresholds(a.tpc, a.tqc);
 	for(int m = 0; m < n; ++m)
 	{

@@ -198,8 +198,25 @@ protected:
 	std::vector<Tag> tags_;
 	DevBuf st_[2];
 	int cur_ = 0;
-	DevBuf tmt_;
+	// TinyMT32 streams: two sets of 7 planes (the current one and the one a
+	// split launch writes the states after it into); orbit polynomials and
+	// jump tables (x^n mod m per lane) for launches that start a stream
+	// mid-sequence, both rebuilt when the streams are set
+	DevBuf tmt_[2];
+	int tmtCur_ = 0;
 	bool tmtParams_ = false;
+	uint32_t* tmtIn() const { return tmt_[tmtCur_].as<uint32_t>(); }
+	uint32_t* tmtOut() const { return tmt_[tmtCur_ ^ 1].as<uint32_t>(); }
+	DevBuf orbit_;
+	bool orbitValid_ = false;
+	std::map<uint64_t, std::unique_ptr<DevBuf>> tmtTab_;
+	DevBuf rndBuf_;
+	const TmtOrbit* tmtOrbits();
+	void tmtStreamsChanged()
+	{
+		orbitValid_ = false;
+		tmtTab_.clear();
+	}
 	MasterRng master_;
 	double p_ = 1.0, q_ = 0.0;
 	double cp_[4] = {0, 0, 0, 0}, cq_[4] = {0, 0, 0, 0};

@@ -170,6 +170,11 @@ int kpz_rng(const kpz_scheduler* h);
  * mat1, mat2, tmat (the layout of TinyMT32::d_Random and of its `.tinyMT`
  * dataset, kpz/GPUTinyMT.cu:113-144). */
 int kpz_set_tinymt_params(kpz_scheduler* h, const uint32_t* mat1, const uint32_t* mat2, const uint32_t* tmat);
+/* Host utility, no counterpart in the reference: jump one TinyMT32 generator
+ * (gen = status[0..3], mat1, mat2, tmat) n steps ahead with the polynomial of
+ * its orbit, as the engine's TinyMT32 kernels do; writes the 4 status words
+ * and returns the orbit polynomial's degree (<= 128).  Needs no device. */
+int kpz_tinymt_jump(const uint32_t gen[7], uint64_t n, uint32_t status[4]);
 int kpz_get_tinymt(kpz_scheduler* h, uint32_t* planes);
 int kpz_set_tinymt(kpz_scheduler* h, const uint32_t* planes);
 int kpz_master_bytes(void);

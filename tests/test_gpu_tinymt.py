@@ -131,7 +131,7 @@ def test_scabit_bitparallel(lx, ly, T):
     check(s, lat, st, master)
 
 
-@pytest.mark.parametrize("lx,ly,T", CB_CASES)
+@pytest.mark.parametrize("lx,ly,T", CB_CASES + [(14, 4, 3), (15, 3, 0)])  # long rows: split into chunks
 @pytest.mark.parametrize("p,q", [(0.95, 0.05), (0.3, 0.6)])
 def test_scabit_pq(lx, ly, T, p, q):
     s, m = engine(K.SchedulerSCABit, lx, ly, 11, T)
@@ -143,7 +143,7 @@ def test_scabit_pq(lx, ly, T, p, q):
     check(s, lat, st, master)
 
 
-@pytest.mark.parametrize("lx,ly,T", [(8, 6, 3), (7, 7, 0)])
+@pytest.mark.parametrize("lx,ly,T", [(8, 6, 3), (7, 7, 0), (14, 3, 5)])
 def test_scabit_disorder2(lx, ly, T):
     s, m = engine(K.SchedulerSCABit, lx, ly, 5, T)
     for n in (0, 1):
