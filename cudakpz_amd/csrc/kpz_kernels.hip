@@ -611,23 +611,52 @@ __global__ __launch_bounds__(DT_WG) void dt_phase_kernel(DtPhaseArgs a, int64_t 
 	                            (UPD == 2) ? tileD + threadIdx.x : nullptr, cur);
 }
 
-// SchedulerCPUDTr<TinyMT32>: lane t is worker t and updates its blocks
-// t, t+T, ... of the domain set in order with its own stream
-// (SchedulerCPUDTr::Func_MCS::exec, kpz/schedulerCPUDTr.cpp:147-175)
+// SchedulerCPUDTr<TinyMT32> (SchedulerCPUDTr::Func_MCS::exec,
+// kpz/schedulerCPUDTr.cpp:147-175: worker t updates blocks t, t+T, ... of the
+// domain set in order with its own stream).  One lane per block: block i is
+// the j = i / T-th block of worker i % T, whose stream is jumped j blocks of
+// draws ahead (a.tmtJ[i] = x^(j * draws per block) mod the orbit polynomial);
+// the lane of a worker's last block writes the worker's state to a.tmtOut.
+// With T >= blocks every worker has one block at most and its lane works in
+// place.
 template<int LBX, int LBY, int UPD>
 __global__ __launch_bounds__(DT_WG) void dt_tmt_kernel(DtPhaseArgs a, int64_t nb, uint32_t* __restrict__ g)
 {
 	__shared__ uint32_t tile[(2 * (1 << LBY) + 1) * DT_WG];
 	__shared__ uint32_t tileD[(UPD == 2) ? (1 << LBY) * DT_WG : 1];
-	const int64_t t = (int64_t)blockIdx.x * DT_WG + threadIdx.x;
-	if(t >= a.T || t >= nb)
+	const int64_t i = (int64_t)blockIdx.x * DT_WG + threadIdx.x;
+	if(i >= nb)
 		return; // no barrier in this kernel; workers without a block keep their stream
+	const int64_t t = (a.T >= nb) ? i : i % a.T;
+	const int64_t j = (a.T >= nb) ? 0 : i / a.T;
 	DtTmt cur;
 	cur.g = TinyMT::load(g, a.T, t);
+	if(j > 0)
+	{
+		uint32_t st[4] = {cur.g.s0, cur.g.s1, cur.g.s2, cur.g.s3};
+		tmt_jump4(st, a.tmtJ[i], a.tmtOrb[t], cur.g.m1, cur.g.m2);
+		cur.g.s0 = st[0];
+		cur.g.s1 = st[1];
+		cur.g.s2 = st[2];
+		cur.g.s3 = st[3];
+	}
 	const DtTile my{tile + threadIdx.x, tile + (1 << LBY) * DT_WG + threadIdx.x, tile, threadIdx.x * 4};
-	for(int64_t i = t; i < nb; i += a.T)
-		dt_block<LBX, LBY, UPD, true>(a, i, nb, my, (UPD == 2) ? tileD + threadIdx.x : nullptr, cur);
-	cur.g.store(g, a.T, t);
+	dt_block<LBX, LBY, UPD, true>(a, i, nb, my, (UPD == 2) ? tileD + threadIdx.x : nullptr, cur);
+	if(a.T >= nb)
+		cur.g.store(g, a.T, t);
+	else if(i + a.T >= nb)
+		cur.g.store(a.tmtOut, a.T, t);
+}
+
+// tmtJ for dt_tmt_kernel: block i gets x^((i / T) * drawsPerBlock) mod the
+// orbit polynomial of worker i % T
+__global__ __launch_bounds__(256) void tmt_block_polys_kernel(const TmtOrbit* __restrict__ orb, int64_t T, int64_t nb,
+		uint64_t drawsPerBlock, Poly128* __restrict__ out)
+{
+	const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if(i >= nb)
+		return;
+	out[i] = poly_xpow((uint64_t)(i / T) * drawsPerBlock, orb[i % T]);
 }
 
 // ======================================================================
@@ -877,6 +906,10 @@ struct CbCell {
 	                             S ? 18 : 16, S ? 22 : 20, S ? 24 : 26, S ? 28 : 30};
 };
 
+template<int S>
+__device__ __forceinline__ void cb_flips(uint32_t ap, uint32_t aq, uint32_t own, uint32_t right, uint32_t up,
+                                         uint32_t& fOwn, uint32_t& fRight, uint32_t& fUp);
+
 // one cell: 8 draws (stream state at the cell's first draw in sc), flips of
 // the own word, of column 0 of the right word and of row 0 of the up word.
 // DIS: disorder2SyncRngUpdate (kpz/schedulerCPUBaseThreadedRng.inl:50-75),
@@ -931,8 +964,15 @@ __device__ __forceinline__ void cb_cell(uint64_t sc, uint32_t aLo, uint32_t aHi,
 			ap = __builtin_amdgcn_alignbit(ap, r23 - tp, 32 - CbCell<S>::K[i]);
 		}
 	}
-	ap = __brev(ap);
-	aq = __brev(aq);
+	cb_flips<S>(__brev(ap), __brev(aq), own, right, up, fOwn, fRight, fUp);
+}
+
+// the flips of one cell from its acceptance bits (bit 2x + 8y of ap / aq:
+// the draw of active site (x, y) accepts a minimum / a maximum)
+template<int S>
+__device__ __forceinline__ void cb_flips(uint32_t ap, uint32_t aq, uint32_t own, uint32_t right, uint32_t up,
+                                         uint32_t& fOwn, uint32_t& fRight, uint32_t& fUp)
+{
 	// slopes at the X positions 2x + 8y: own X, own Y, X of x+1, Y of y+1
 	const uint32_t X = own & 0x55555555u;
 	const uint32_t Yb = (own >> 1) & 0x55555555u;
@@ -957,7 +997,10 @@ __device__ __forceinline__ bool cb_right_local(int lG, int64_t grp, int cg, int 
 	return cb_mail(lG, ngc) && cg + 1 < ngc && (grp & 3) != 3;
 }
 
-template<int S, bool DIS>
+// DIS: disorder2SyncRngUpdate; ACC: the acceptance bits of every active site
+// come from a.acc (a word per cell: bit 2x + 8y accepts a minimum, the bit
+// above it a maximum; the TinyMT32 workers), no stream is drawn
+template<int S, bool DIS, bool ACC = false>
 __global__ __launch_bounds__(256) void cb_local_band_kernel(CbLocalArgs a)
 {
 	__shared__ uint32_t mail[4][2]; // [group in the workgroup][row parity]
@@ -995,17 +1038,22 @@ __global__ __launch_bounds__(256) void cb_local_band_kernel(CbLocalArgs a)
 	const bool h = cc >= cells0;
 	const int ci = h ? cc - cells0 : cc;
 	const int cellsH = h ? w - cells0 : cells0;
-	const int clsH = cb_class(a, cellsH);
-	const uint64_t s0 = a.st_in[t];
-	uint64_t s = s0;
-	if(h)
-		s = a.baseJ[cb_class(a, cells0)](s);
-	s = a.rowJ[clsH * a.nyb + yb0](s);
-	s = a.cellJ[ci](s);
-	if(yb0 == 0 && cc == 0)
-		a.st_out[t] = (t < a.rem ? a.fullBig : a.fullSmall)(s0);
-	// the same cell of the next row: 8 draws per cell of the half strip later
-	const Affine rowStep = a.rowJ[clsH * a.nyb + 1];
+	uint64_t s = 0;
+	Affine rowStep{0, 0};
+	if constexpr(!ACC)
+	{
+		const int clsH = cb_class(a, cellsH);
+		const uint64_t s0 = a.st_in[t];
+		s = s0;
+		if(h)
+			s = a.baseJ[cb_class(a, cells0)](s);
+		s = a.rowJ[clsH * a.nyb + yb0](s);
+		s = a.cellJ[ci](s);
+		if(yb0 == 0 && cc == 0)
+			a.st_out[t] = (t < a.rem ? a.fullBig : a.fullSmall)(s0);
+		// the same cell of the next row: 8 draws per cell of the half strip later
+		rowStep = a.rowJ[clsH * a.nyb + 1];
+	}
 	const uint32_t rLo = (uint32_t)rowStep.a, rHi = (uint32_t)(rowStep.a >> 32);
 
 	const uint32_t tp = a.tp, tq = a.tq;
@@ -1034,8 +1082,16 @@ __global__ __launch_bounds__(256) void cb_local_band_kernel(CbLocalArgs a)
 		const uint32_t right = *(const uint32_t*)(base + (rowOff + cr4));
 		const uint32_t up = *(const uint32_t*)(base + (upOff + c4));
 		uint32_t fOwn, fRight, fUp;
-		const uint32_t dw = DIS ? *(const uint32_t*)((const char*)a.dis + (rowOff + c4)) : 0u;
-		cb_cell<S, DIS>(s, aLo, aHi, stepC, tp, tq, own, right, up, fOwn, fRight, fUp, dw, a.tpc, a.tqc);
+		if constexpr(ACC)
+		{
+			const uint32_t acc = *(const uint32_t*)((const char*)a.acc + (rowOff + c4));
+			cb_flips<S>(acc, acc >> 1, own, right, up, fOwn, fRight, fUp);
+		}
+		else
+		{
+			const uint32_t dw = DIS ? *(const uint32_t*)((const char*)a.dis + (rowOff + c4)) : 0u;
+			cb_cell<S, DIS>(s, aLo, aHi, stepC, tp, tq, own, right, up, fOwn, fRight, fUp, dw, a.tpc, a.tqc);
+		}
 		uint32_t fromLeft = (uint32_t)__shfl((int)fRight, (g - 1) & (G - 1), G);
 		if(g == 0 && !wrapG)
 			fromLeft = 0;
@@ -1055,12 +1111,15 @@ __global__ __launch_bounds__(256) void cb_local_band_kernel(CbLocalArgs a)
 		fromBelow = fUp;
 		own = up; // no other lane writes the bits this lane reads from it
 		rowOff = upOff;
-		// the jump to the next row's cell, kept exact in the low state word
-		// and mod 2^24 in the high one: all cb_cell uses (the stream's final
-		// state comes from s0 above)
-		const uint64_t pr = (uint64_t)(uint32_t)s * rLoV + rowC;
-		const uint32_t h24 = mad_u24((uint32_t)(s >> 32), rLoV, mad_u24((uint32_t)s, rHiV, (uint32_t)(pr >> 32)));
-		s = ((uint64_t)h24 << 32) | (uint32_t)pr;
+		if constexpr(!ACC)
+		{
+			// the jump to the next row's cell, kept exact in the low state word
+			// and mod 2^24 in the high one: all cb_cell uses (the stream's final
+			// state comes from s0 above)
+			const uint64_t pr = (uint64_t)(uint32_t)s * rLoV + rowC;
+			const uint32_t h24 = mad_u24((uint32_t)(s >> 32), rLoV, mad_u24((uint32_t)s, rHiV, (uint32_t)(pr >> 32)));
+			s = ((uint64_t)h24 << 32) | (uint32_t)pr;
+		}
 	}
 	a.edgeU[(size_t)band * XC + c] = fromBelow;
 }
@@ -1219,101 +1278,116 @@ __global__ __launch_bounds__(256) void tmt_row_polys_kernel(const TmtOrbit* __re
 	out[lane] = poly_xpow((uint64_t)r * drawsPerRow + c * (drawsPerRow >> lC), orb[t]);
 }
 
-// SchedulerCPUCB<TinyMT32>, one half-sweep: worker t walks the first half of
-// its strip of cell columns, then the second, cell row by cell row and cell by
-// cell, two active sites per site row, one randomFloat() each
-// (kpz/schedulerCPUCB.cpp:128-168).  Every update of a half-sweep reads only
-// slope bits that no other update of the half-sweep flips, so a cell's flips
-// follow from its words as loaded, and the result does not depend on the
-// order across workers.  A strip's first cell column also takes the right-edge
-// flips of the strip to its left: it is updated with atomics, every other
-// word by its worker alone.
+// SchedulerCPUCB<TinyMT32>, the random part of one half-sweep: lane (t, k)
+// draws chunk k (CB_CHUNK cells, 8 draws each) of worker t's walk -- the
+// first half of its strip of cell columns, then the second, cell row by cell
+// row, cell by cell, two active sites per site row, one randomFloat() each
+// (kpz/schedulerCPUCB.cpp:128-168) -- with the worker's stream jumped to the
+// chunk's first draw (chunkJ[lane] = x^(8 CB_CHUNK k) mod its orbit
+// polynomial).  Output: one word per cell, bit 2x + 8y set when the draw of
+// active site (x, y) accepts a minimum ((u >> 8) < tp of the site's class),
+// the bit above it when it accepts a maximum; cb_local_band_kernel<S, *, ACC>
+// applies them.  The lane of a worker's last chunk writes the worker's state.
+constexpr int CB_CHUNK = 256;
+static_assert(CB_TMT_CHUNK_DRAWS == (uint64_t)CB_CHUNK * 8, "chunk of the SCA TinyMT32 walk");
+
 template<bool DIS>
-__global__ __launch_bounds__(256) void cb_local_tmt_kernel(CbLocalArgs a, int64_t T, uint32_t* __restrict__ g)
+__global__ __launch_bounds__(256) void cb_local_tmt_gen_kernel(CbLocalArgs a, int chunks, const uint32_t* __restrict__ gin,
+		uint32_t* __restrict__ gout, const Poly128* __restrict__ chunkJ, const TmtOrbit* __restrict__ orb,
+		uint32_t* __restrict__ acc)
 {
-	const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-	if(t >= T)
+	const int64_t lane = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if(lane >= a.T * chunks)
 		return;
-	TinyMT r = TinyMT::load(g, T, t);
+	const int64_t t = lane / chunks;
+	const int k = (int)(lane - t * chunks);
 	const int64_t XC = 1ll << (a.lx - 2), YC = 1ll << (a.ly - 2);
 	const int64_t w = a.pc + (t < a.rem ? 1 : 0);
 	const int64_t c0 = t * a.pc + (t < a.rem ? t : a.rem);
-	const int64_t wl = w / 2;
-	const int s = a.signum;
-	uint32_t* __restrict__ lat = a.lat;
-	for(int half = 0; half < 2; ++half)
+	const int64_t wl = w / 2; // xCenter = xMin + ((w*4/2) & ~3)
+	const int64_t L = w * YC;  // cells of the worker's walk
+	const int64_t q0 = (int64_t)k * CB_CHUNK;
+	if(q0 >= L)
+		return;
+	const int64_t q1 = (q0 + CB_CHUNK < L) ? q0 + CB_CHUNK : L;
+	TinyMT g = TinyMT::load(gin, a.T, t);
+	if(k > 0)
 	{
-		const int64_t cb = half ? c0 + wl : c0, ce = half ? c0 + w : c0 + wl;
-		for(int64_t cy = 0; cy < YC; ++cy)
+		uint32_t st[4] = {g.s0, g.s1, g.s2, g.s3};
+		tmt_jump4(st, chunkJ[lane], orb[t], g.m1, g.m2);
+		g.s0 = st[0];
+		g.s1 = st[1];
+		g.s2 = st[2];
+		g.s3 = st[3];
+	}
+	// cell q0 of the walk: half, cell row, cell in the half row
+	int half;
+	int64_t r, ci;
+	if(q0 < wl * YC)
+	{
+		half = 0;
+		r = q0 / wl;
+		ci = q0 - r * wl;
+	}
+	else
+	{
+		half = 1;
+		const int64_t q = q0 - wl * YC;
+		r = q / (w - wl);
+		ci = q - r * (w - wl);
+	}
+	const int sg = a.signum;
+	for(int64_t q = q0; q < q1; ++q)
+	{
+		const int64_t hw = half ? w - wl : wl;
+		const size_t io = (size_t)(r * XC + c0 + (half ? wl : 0) + ci);
+		const uint32_t dw = DIS ? a.dis[io] : 0u;
+		uint32_t out = 0;
+#pragma unroll
+		for(int rr = 0; rr < 4; ++rr)
 		{
-			const size_t rowBase = (size_t)(cy * XC), upBase = (size_t)(((cy + 1) & (YC - 1)) * XC);
-			for(int64_t cx = cb; cx < ce; ++cx)
+#pragma unroll
+			for(int kk = 0; kk < 2; ++kk)
 			{
-				const size_t io = rowBase + (size_t)cx, iu = upBase + (size_t)cx;
-				const size_t ir = rowBase + (size_t)((cx + 1) & (XC - 1));
-				const uint32_t wo = lat[io], wr = lat[ir], wu = lat[iu];
-				const uint32_t wd = DIS ? a.dis[io] : 0u;
-				uint32_t fo = 0, fr = 0, fu = 0;
-#pragma unroll
-				for(int rr = 0; rr < 4; ++rr)
+				const int sh = 8 * rr + 2 * (((rr & 1) ^ sg) + 2 * kk);
+				const uint32_t rf = g.u32() >> 8;
+				uint32_t tp = a.tp, tq = a.tq;
+				if(DIS)
 				{
-#pragma unroll
-					for(int k = 0; k < 2; ++k)
-					{
-						// x & 3 of the active sites of site row rr: ((y & 1) ^ s) and + 2
-						const int c = ((rr & 1) ^ s) + 2 * k;
-						const int sh = 8 * rr + 2 * c;
-						const uint32_t rf = r.u32() >> 8;
-						const uint32_t rx = (c < 3) ? ((wo >> (sh + 2)) & 1u) : ((wr >> (8 * rr)) & 1u);
-						const uint32_t uy = (rr < 3) ? ((wo >> (sh + 9)) & 1u) : ((wu >> (2 * c + 1)) & 1u);
-						const uint32_t cfg = ((wo >> sh) & 3u) | (rx << 2) | (uy << 3);
-						uint32_t tp = a.tp, tq = a.tq;
-						if(DIS)
-						{
-							const uint32_t cls = (wd >> sh) & 3u;
-							tp = a.tpc[cls];
-							tq = a.tqc[cls];
-						}
-						if((cfg == 0xCu && rf < tp) || (cfg == 0x3u && rf < tq))
-						{
-							fo ^= 3u << sh;
-							if(c < 3)
-								fo ^= 1u << (sh + 2);
-							else
-								fr ^= 1u << (8 * rr);
-							if(rr < 3)
-								fo ^= 1u << (sh + 9);
-							else
-								fu ^= 1u << (2 * c + 1);
-						}
-					}
+					const uint32_t cls = (dw >> sh) & 3u;
+					tp = a.tpc[cls];
+					tq = a.tqc[cls];
 				}
-				const bool first = cx == c0;
-				if(fo)
-				{
-					if(first)
-						atomicXor(lat + io, fo);
-					else
-						lat[io] = wo ^ fo;
-				}
-				if(fr)
-				{
-					if(cx + 1 == c0 + w)
-						atomicXor(lat + ir, fr); // the next strip's first column
-					else
-						lat[ir] = wr ^ fr;
-				}
-				if(fu)
-				{
-					if(first)
-						atomicXor(lat + iu, fu);
-					else
-						lat[iu] = wu ^ fu;
-				}
+				out |= (rf < tp ? 1u : 0u) << sh;
+				out |= (rf < tq ? 1u : 0u) << (sh + 1);
+			}
+		}
+		acc[io] = out;
+		if(++ci == hw)
+		{
+			ci = 0;
+			if(++r == YC)
+			{
+				r = 0;
+				half = 1;
 			}
 		}
 	}
-	r.store(g, T, t);
+	if(q1 == L)
+		g.store(gout, a.T, t);
+}
+
+// chunkJ for cb_local_tmt_gen_kernel: lane t * chunks + k gets
+// x^(k * drawsPerChunk) mod the orbit polynomial of worker t
+__global__ __launch_bounds__(256) void tmt_chunk_polys_kernel(const TmtOrbit* __restrict__ orb, int64_t T, int chunks,
+		uint64_t drawsPerChunk, Poly128* __restrict__ out)
+{
+	const int64_t lane = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if(lane >= T * chunks)
+		return;
+	const int64_t t = lane / chunks;
+	const uint64_t k = (uint64_t)(lane - t * chunks);
+	out[lane] = poly_xpow(k * drawsPerChunk, orb[t]);
 }
 
 // ======================================================================
@@ -2174,11 +2248,7 @@ template<int LBX, int LBY, int UPD>
 void dt_launch(const DtPhaseArgs& a, int64_t nb, dim3 grid, hipStream_t stream)
 {
 	if(a.tmt)
-	{
-		const int64_t lanes = a.T < nb ? a.T : nb;
-		hipLaunchKernelGGL((dt_tmt_kernel<LBX, LBY, UPD>), dim3((unsigned)((lanes + DT_WG - 1) / DT_WG)), dim3(DT_WG), 0,
-		                   stream, a, nb, a.tmt);
-	}
+		hipLaunchKernelGGL((dt_tmt_kernel<LBX, LBY, UPD>), grid, dim3(DT_WG), 0, stream, a, nb, a.tmt);
 	else if(a.random_origin)
 		hipLaunchKernelGGL((dt_phase_kernel<LBX, LBY, UPD, true>), grid, dim3(DT_WG), 0, stream, a, nb);
 	else
@@ -2270,6 +2340,51 @@ hipError_t launch_cbbit_tmt_gen(const CbHalfArgs& a, int mode, int lC, const uin
 	return hipGetLastError();
 }
 
+int cb_tmt_chunks(int lx, int ly, int64_t T)
+{
+	const int64_t cells = (((1ll << (lx - 2)) + T - 1) / T) << (ly - 2); // the longest walk
+	return (int)((cells + CB_CHUNK - 1) / CB_CHUNK);
+}
+
+hipError_t launch_tmt_chunk_polys(const TmtOrbit* orb, int64_t T, int chunks, uint64_t drawsPerChunk, Poly128* out,
+		hipStream_t stream)
+{
+	if(!orb || !out || T < 1 || chunks < 1)
+		return hipErrorInvalidValue;
+	const int64_t lanes = T * chunks;
+	hipLaunchKernelGGL(tmt_chunk_polys_kernel, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, stream, orb, T, chunks,
+	                   drawsPerChunk, out);
+	return hipGetLastError();
+}
+
+hipError_t launch_cb_local_tmt_gen(const CbLocalArgs& a, const uint32_t* gin, uint32_t* gout, const Poly128* chunkJ,
+		const TmtOrbit* orb, uint32_t* acc, hipStream_t stream)
+{
+	const int chunks = (a.T >= 1 && a.lx >= 3 && a.ly >= 3) ? cb_tmt_chunks(a.lx, a.ly, a.T) : 0;
+	if(a.lx < 3 || a.ly < 3 || a.T < 1 || a.T > (1ll << (a.lx - 2)) || a.pc != (int)((1ll << (a.lx - 2)) / a.T) ||
+	   a.rem != (int)((1ll << (a.lx - 2)) % a.T) || !gin || !gout || !acc || (chunks > 1 && (!chunkJ || !orb)))
+		return hipErrorInvalidValue;
+	const int64_t lanes = a.T * chunks;
+	const dim3 grid((unsigned)((lanes + 255) / 256));
+	if(a.dis)
+		hipLaunchKernelGGL((cb_local_tmt_gen_kernel<true>), grid, dim3(256), 0, stream, a, chunks, gin, gout, chunkJ, orb,
+		                   acc);
+	else
+		hipLaunchKernelGGL((cb_local_tmt_gen_kernel<false>), grid, dim3(256), 0, stream, a, chunks, gin, gout, chunkJ, orb,
+		                   acc);
+	return hipGetLastError();
+}
+
+hipError_t launch_tmt_block_polys(const TmtOrbit* orb, int64_t T, int64_t nb, uint64_t drawsPerBlock, Poly128* out,
+		hipStream_t stream)
+{
+	if(!orb || !out || T < 1 || nb < 1)
+		return hipErrorInvalidValue;
+	hipLaunchKernelGGL(tmt_block_polys_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, stream, orb, T, nb,
+	                   drawsPerBlock, out);
+	return hipGetLastError();
+}
+
 hipError_t launch_tmt_seed(uint32_t* g, int64_t T, hipStream_t stream)
 {
 	if(!g || T < 1)
@@ -2283,6 +2398,8 @@ hipError_t launch_dt_phase(const DtPhaseArgs& a, hipStream_t stream)
 	if(!dt_layout_supported(a.lbx, a.lby) || a.T < 1 || (a.tmt && !a.random_origin))
 		return hipErrorInvalidValue;
 	const int64_t nb = 1ll << (a.lx - a.lbx + a.ly - a.lby - 2);
+	if(a.tmt && a.T < nb && (!a.tmtOut || !a.tmtJ || !a.tmtOrb))
+		return hipErrorInvalidValue;
 	const dim3 grid((unsigned)((nb + DT_WG - 1) / DT_WG));
 	switch(a.lbx)
 	{
@@ -2409,19 +2526,6 @@ hipError_t launch_fill_flat(uint32_t* lat, int enc, int lx, int ly, hipStream_t 
 
 hipError_t launch_cb_local_half(const CbLocalArgs& a, hipStream_t stream)
 {
-	if(a.tmt)
-	{
-		// one lane per worker; strips of at least one cell column
-		if(a.lx < 3 || a.ly < 3 || a.T < 1 || a.T > (1ll << (a.lx - 2)) || a.pc != (int)((1ll << (a.lx - 2)) / a.T) ||
-		   a.rem != (int)((1ll << (a.lx - 2)) % a.T))
-			return hipErrorInvalidValue;
-		const dim3 grid((unsigned)((a.T + 255) / 256));
-		if(a.dis)
-			hipLaunchKernelGGL((cb_local_tmt_kernel<true>), grid, dim3(256), 0, stream, a, a.T, a.tmt);
-		else
-			hipLaunchKernelGGL((cb_local_tmt_kernel<false>), grid, dim3(256), 0, stream, a, a.T, a.tmt);
-		return hipGetLastError();
-	}
 	// the band kernel's shape: groups of 2^lG <= 64 columns inside a row of
 	// 2^(lx-2) cells, bands of 2^lNB cell rows, at least 2 cell rows
 	// (and a lattice below 4 GiB: the band kernel addresses it with 32-bit byte offsets)
@@ -2430,7 +2534,14 @@ hipError_t launch_cb_local_half(const CbLocalArgs& a, hipStream_t stream)
 		return hipErrorInvalidValue;
 	const int64_t lanes = 1ll << (a.lx - 2 + a.ly - 2 - a.lNB);
 	const unsigned grid = (unsigned)((lanes + 255) / 256);
-	if(a.dis)
+	if(a.acc)
+	{
+		if(a.signum)
+			hipLaunchKernelGGL((cb_local_band_kernel<1, false, true>), dim3(grid), dim3(256), 0, stream, a);
+		else
+			hipLaunchKernelGGL((cb_local_band_kernel<0, false, true>), dim3(grid), dim3(256), 0, stream, a);
+	}
+	else if(a.dis)
 	{
 		if(a.signum)
 			hipLaunchKernelGGL((cb_local_band_kernel<1, true>), dim3(grid), dim3(256), 0, stream, a);

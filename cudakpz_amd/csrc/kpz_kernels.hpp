@@ -30,8 +30,17 @@ struct DtPhaseArgs {
 	uint32_t tpc[4], tqc[4]; // upd 2: thresholds per class
 	uint32_t* tmt;          // TinyMT32 streams (7 planes of T words; random origin only;
 	                        // thresholds then on the 2^24 scale), or null for SLCG64
+	// TinyMT32 with fewer streams than blocks: the states after the phase go
+	// to tmtOut, block i's stream is jumped by tmtJ[i] (launch_tmt_block_polys)
+	uint32_t* tmtOut;
+	const Poly128* tmtJ;
+	const TmtOrbit* tmtOrb;
 };
 hipError_t launch_dt_phase(const DtPhaseArgs& a, hipStream_t stream);
+// tmtJ for a DT phase with T < nb streams: block i -> x^((i / T) * drawsPerBlock)
+// mod the orbit polynomial of stream i % T
+hipError_t launch_tmt_block_polys(const TmtOrbit* orb, int64_t T, int64_t nb, uint64_t drawsPerBlock, Poly128* out,
+		hipStream_t stream);
 
 // ---- TinyMT32 stream sets: tinymt32_init of every stream from the seed in
 // plane 0 and the parameters in planes 4..6 (7 planes of T words)
@@ -104,14 +113,25 @@ struct CbLocalArgs {
 	// layout (null: plain pq) and thresholds per class
 	const uint32_t* dis;
 	uint32_t tpc[4], tqc[4];
-	// TinyMT32 streams (7 planes of T words), or null for SLCG64: then one
-	// launch, one lane per worker, thresholds on the 2^24 scale
-	uint32_t* tmt;
+	// acceptance words from a buffer instead of drawn (the TinyMT32 workers,
+	// launch_cb_local_tmt_gen), or null
+	const uint32_t* acc;
 	int64_t T;
 };
-// one half-sweep: the band kernel plus the edge fix-up (two launches), or
-// the TinyMT32 worker kernel (one launch)
+// one half-sweep: the band kernel plus the edge fix-up (two launches)
 hipError_t launch_cb_local_half(const CbLocalArgs& a, hipStream_t stream);
+// TinyMT32 workers of SCA: the acceptance words of one half-sweep into `acc`
+// (lattice layout; thresholds on the 2^24 scale), one lane per chunk of a
+// worker's walk; streams read from gin, the states after the half-sweep
+// written to gout; chunkJ / orb needed when a walk has several chunks
+hipError_t launch_cb_local_tmt_gen(const CbLocalArgs& a, const uint32_t* gin, uint32_t* gout, const Poly128* chunkJ,
+		const TmtOrbit* orb, uint32_t* acc, hipStream_t stream);
+// chunks per worker for launch_cb_local_tmt_gen, and its jump table:
+// out[t * chunks + k] = x^(k * drawsPerChunk) mod the orbit polynomial of worker t
+int cb_tmt_chunks(int lx, int ly, int64_t T);
+constexpr uint64_t CB_TMT_CHUNK_DRAWS = 256 * 8;
+hipError_t launch_tmt_chunk_polys(const TmtOrbit* orb, int64_t T, int chunks, uint64_t drawsPerChunk, Poly128* out,
+		hipStream_t stream);
 
 // ---- reductions ----------------------------------------------------------
 // Per site row y, relative to the row's start height (see kpz_kernels.hip):

@@ -456,6 +456,19 @@ void SchedulerDT::run(int n, int upd)
 	a.T = T_;
 	a.tmt = tinyMT() ? tmtIn() : nullptr;
 	a.jumps = tinyMT() ? nullptr : jumps(du);
+	if(tinyMT() && T_ < nb_)
+	{
+		// blocks start mid-stream: jump tables per block (draws per block = du << (lbx + lby))
+		a.tmtOrb = tmtOrbits();
+		const uint64_t dpb = (uint64_t)du << (lbx_ + lby_);
+		auto& tab = tmtTab_[dpb];
+		if(!tab)
+		{
+			tab = std::make_unique<DevBuf>((size_t)nb_ * sizeof(Poly128));
+			check(launch_tmt_block_polys(a.tmtOrb, T_, nb_, dpb, tab->as<Poly128>(), stream_), "tmt_block_polys");
+		}
+		a.tmtJ = tab->as<Poly128>();
+	}
 	a.step = step_;
 	a.blockJ = affine_pow(step_, (uint64_t)du << (lbx_ + lby_));
 	a.upd = upd;
@@ -479,7 +492,11 @@ void SchedulerDT::run(int n, int upd)
 			a.ds = seq[d];
 			if(tinyMT())
 			{
+				a.tmt = tmtIn();
+				a.tmtOut = tmtOut();
 				check(launch_dt_phase(a, stream_), "dt_phase");
+				if(T_ < nb_)
+					tmtCur_ ^= 1;
 				continue;
 			}
 			a.st_in = stIn();
@@ -697,8 +714,6 @@ SchedulerSCA::SchedulerSCA(int lx, int ly, const Params& p) : SchedulerService(l
 	args_.T = T_;
 	p_ = 0.95;
 	q_ = 0.0;
-	if(tinyMT())
-		return; // one lane per worker: no jump tables, no edge buffers
 	std::vector<int> V;
 	auto addV = [&](int v) { // v = 0: an empty first half (one-cell strips)
 		if(std::find(V.begin(), V.end(), v) == V.end())
@@ -777,8 +792,44 @@ void SchedulerSCA::run(int n, bool disorder)
 	a.tp = thresh(p_);
 	a.tq = thresh(q_);
 	a.dis = disorder ? dis_.as<uint32_t>() : nullptr;
-	a.tmt = tinyMT() ? tmtIn() : nullptr;
 	classThresholds(a.tpc, a.tqc);
+	if(tinyMT())
+	{
+		// TinyMT32 workers: per half-sweep the acceptance words first (one lane
+		// per chunk of a worker's walk, its stream jumped to the chunk), then
+		// the band kernel applies them
+		const int chunks = cb_tmt_chunks(size_.lx, size_.ly, T_);
+		const TmtOrbit* orb = chunks > 1 ? tmtOrbits() : nullptr;
+		const Poly128* chunkJ = nullptr;
+		if(chunks > 1)
+		{
+			auto& tab = tmtTab_[CB_TMT_CHUNK_DRAWS];
+			if(!tab)
+			{
+				tab = std::make_unique<DevBuf>((size_t)T_ * chunks * sizeof(Poly128));
+				check(launch_tmt_chunk_polys(orb, T_, chunks, CB_TMT_CHUNK_DRAWS, tab->as<Poly128>(), stream_),
+						"tmt_chunk_polys");
+			}
+			chunkJ = tab->as<Poly128>();
+		}
+		if(rndBuf_.bytes() < size_.sizeW() * 4)
+			rndBuf_.alloc(size_.sizeW() * 4);
+		CbLocalArgs b = a;
+		b.acc = rndBuf_.as<uint32_t>();
+		for(int m = 0; m < n; ++m)
+		{
+			for(int s = 0; s < 2; ++s)
+			{
+				a.signum = b.signum = s;
+				check(launch_cb_local_tmt_gen(a, tmtIn(), tmtOut(), chunkJ, orb, rndBuf_.as<uint32_t>(), stream_),
+						"cb_local_tmt_gen");
+				check(launch_cb_local_half(b, stream_), "cb_local_half");
+				tmtCur_ ^= 1;
+			}
+			++mcs_;
+		}
+		return;
+	}
 	for(int m = 0; m < n; ++m)
 	{
 		for(int s = 0; s < 2; ++s)
@@ -787,8 +838,7 @@ void SchedulerSCA::run(int n, bool disorder)
 			a.st_in = stIn();
 			a.st_out = stOut();
 			check(launch_cb_local_half(a, stream_), "cb_local_half");
-			if(!tinyMT())
-				swapStreams();
+			swapStreams();
 		}
 		++mcs_;
 	}

@@ -232,9 +232,11 @@ def test_state_file_resumes_bit_exact(tmp_path):
     assert a.master_draw() == b.master_draw()
 
 
-def test_medium_dtr_against_oracle():
-    # 1024^2, 16x16 blocks, 2^10 workers (4 blocks per worker and domain set)
-    s, m = engine(K.SchedulerDTr, 10, 10, 77, 1024)
+@pytest.mark.parametrize("T", [1024, 300])
+def test_medium_dtr_against_oracle(T):
+    # 1024^2, 16x16 blocks, 1024 blocks per domain set: one block per worker,
+    # or up to 4 blocks per worker (streams jumped to each block)
+    s, m = engine(K.SchedulerDTr, 10, 10, 77, T)
     s.mcs(2)
     master, st = oracle(s, 77, m)
     lat = O.flat(10, 10, O.ENC_LOCAL)
