@@ -95,11 +95,11 @@ def run_dt(K, args, ev):
     return s, total_ms, mcs_ms, w2
 
 
-def run_scabit(K, lx, p, q, steps, seed, ev, cls=None, kpz=False, rng=0):
+def run_scabit(K, lx, p, q, steps, seed, ev, cls=None, kpz=False, rng=0, threads=0):
     """steps MCS of mcsPQSyncRng (kpz: mcs) after 2 warm-up MCS, hipEvent time.
     rng = TinyMT32: the engine's default worker count, parameter set t of
     stream t = random words (the reference's table is not in its tree)"""
-    s = (cls or K.SchedulerSCABit)(lx, lx, rng=rng)
+    s = (cls or K.SchedulerSCABit)(lx, lx, rng=rng, threads=threads)
     if rng == K.RNG_TINYMT32:
         r = np.random.default_rng(seed)
         s.setTinyMTParams(*(r.integers(0, 1 << 32, s.threads, dtype=np.uint64).astype(np.uint32)
@@ -440,12 +440,18 @@ def main():
                             "hbm_frac": ups * 0.5 / 1e9 / HBM_PEAK_GBS, "ms_per_mcs": ms / 100}
         # TinyMT32 workers, the generator the reference's frontend gives dt, sca
         # and scaBit (kpz/kpzFrontend.cpp:308-331): one lane per worker stream
-        for name, cls, kpz, p, q, steps, bpu in (
-                ("dtr_tinymt32_p1_q0", K.SchedulerDTr, True, 1.0, 0.0, 40, 0.5),
-                ("scabit_tinymt32_p0.5_q0", K.SchedulerSCABit, True, 0.5, 0.0, 20, 1.0),
-                ("scabit_tinymt32_p0.95_q0.05", K.SchedulerSCABit, False, 0.95, 0.05, 10, 1.0),
-                ("sca_tinymt32_p0.95_q0", K.SchedulerSCA, True, 0.95, 0.0, 4, 1.0)):
-            ms = run_scabit(K, args.lx, p, q, steps, args.seed, ev, cls, kpz=kpz, rng=K.RNG_TINYMT32)
+        # (T = 0: the engine's default worker count; T = 32: a reference CPU
+        # run's thread count, whose results the engine reproduces bit for bit
+        # with streams jumped ahead to every lane's start)
+        for name, cls, kpz, p, q, steps, bpu, T in (
+                ("dtr_tinymt32_p1_q0", K.SchedulerDTr, True, 1.0, 0.0, 40, 0.5, 0),
+                ("scabit_tinymt32_p0.5_q0", K.SchedulerSCABit, True, 0.5, 0.0, 20, 1.0, 0),
+                ("scabit_tinymt32_p0.95_q0.05", K.SchedulerSCABit, False, 0.95, 0.05, 10, 1.0, 0),
+                ("sca_tinymt32_p0.95_q0", K.SchedulerSCA, True, 0.95, 0.0, 10, 1.0, 0),
+                ("dtr_tinymt32_T32_p1_q0", K.SchedulerDTr, True, 1.0, 0.0, 20, 0.5, 32),
+                ("scabit_tinymt32_T32_p0.5_q0", K.SchedulerSCABit, True, 0.5, 0.0, 20, 1.0, 32),
+                ("sca_tinymt32_T32_p0.95_q0", K.SchedulerSCA, True, 0.95, 0.0, 10, 1.0, 32)):
+            ms = run_scabit(K, args.lx, p, q, steps, args.seed, ev, cls, kpz=kpz, rng=K.RNG_TINYMT32, threads=T)
             ups = N * steps / (ms * 1e-3)
             sec[name] = {"site_updates_per_s": ups, "hbm_GBs_algorithmic": ups * bpu / 1e9,
                          "hbm_frac": ups * bpu / 1e9 / HBM_PEAK_GBS, "ms_per_mcs": ms / steps}
