@@ -1189,58 +1189,65 @@ __device__ __forceinline__ void cbbit_worker_row(int64_t y, int per, int rem, in
 }
 
 // SchedulerCPUCBBit<TinyMT32>, the random part of one half-sweep: lane
-// (y, c) is chunk c of the 2^lC chunks of site row y; it takes its worker's
-// stream to the chunk's first draw (a jump, rowJ[lane] = x^(r * draws per
-// row + c * draws per chunk) mod the stream's orbit polynomial) and draws the
-// chunk's words in order
+// c * Y + y is chunk c of the 2^lC chunks of site row y; it takes its
+// worker's stream to the chunk's first draw (a jump, rowJ[(y << lC) + c] =
+// x^(r * draws per row + c * draws per chunk) mod the stream's orbit
+// polynomial) and draws the chunk's words in order
 // (kpz/schedulerCPUCBBit.cpp:43-79): MODE 0 = pSyncRngUpdateVL3's random
 // words (two draws per u64 word, the first the high half; one per u32 word
 // when L = 64), 1 = per-site pq, 2 = disorder2SyncRngUpdate (one
 // randomFloat() per active site, x ascending, accepted as (u >> 8) < t; the
 // class is the site's x-slope bit of the map).  Output: the words (MODE 0) or
 // the minimum and maximum acceptance masks (Q words each), which
-// cbbit_half_kernel<W, 3 / 4> applies.  The lane of a worker's last row
-// writes the worker's state after the half-sweep.
+// cbbit_half_kernel<W, 3 / 4> applies.  A workgroup is 256 rows of one
+// chunk: blocks of B words per row are staged in LDS and stored row segment
+// by row segment (a lane per row would store 256 scattered words).  The lane
+// of a worker's last row writes the worker's state after the half-sweep.
 template<class W, int MODE>
 __global__ __launch_bounds__(256) void cbbit_tmt_gen_kernel(CbHalfArgs a, int lC, const uint32_t* __restrict__ gin,
 		uint32_t* __restrict__ gout, const Poly128* __restrict__ rowJ, const TmtOrbit* __restrict__ orb,
 		W* __restrict__ out)
 {
-	const int64_t lane = (int64_t)blockIdx.x * 256 + threadIdx.x;
-	if(lane >= (1ll << (a.ly + lC)))
-		return;
-	const int64_t y = lane >> lC;
-	const int c = (int)(lane & ((1 << lC) - 1));
 	constexpr int WB = (int)sizeof(W) * 8;
+	constexpr int NO = (MODE == 0) ? 1 : 2;        // output planes
+	constexpr int B = (MODE == 0) ? 16 : 8;        // staged words per row
+	__shared__ W stage[NO][256][B + 1];
+	const int64_t Y = 1ll << a.ly;
+	const int64_t lane = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	const uint64_t Wn = 1ull << (a.lx - (WB == 64 ? 7 : 6)); // words per row and quadrant
+	const uint64_t wc = Wn >> lC;                             // words per chunk
+	const bool staged = Y >= 256 && wc >= (uint64_t)B;        // launch-uniform
+	if(!staged && lane >= (Y << lC))
+		return; // no barrier on this path
+	const int c = (int)(lane >> a.ly);
+	const int64_t y = lane & (Y - 1);
 	int64_t t, r;
 	cbbit_worker_row(y, a.per, a.rem, t, r);
 	TinyMT g = TinyMT::load(gin, a.T, t);
 	if(r > 0 || c > 0)
 	{
 		uint32_t st[4] = {g.s0, g.s1, g.s2, g.s3};
-		tmt_jump4(st, rowJ[lane], orb[t], g.m1, g.m2);
+		tmt_jump4(st, rowJ[(y << lC) + c], orb[t], g.m1, g.m2);
 		g.s0 = st[0];
 		g.s1 = st[1];
 		g.s2 = st[2];
 		g.s3 = st[3];
 	}
-	const uint64_t Wn = 1ull << (a.lx - (WB == 64 ? 7 : 6)); // words per row and quadrant
 	const uint64_t Q = Wn << a.ly;
 	const uint64_t row = (uint64_t)y * Wn;
 	const W* __restrict__ dis = (MODE == 2) ? reinterpret_cast<const W*>(a.dis) + (a.signum ? 2 * Q : 0) : nullptr;
-	const uint64_t wc = Wn >> lC; // words per chunk
-	for(uint64_t xw = (uint64_t)c * wc; xw < (uint64_t)(c + 1) * wc; ++xw)
-	{
+	auto word = [&](uint64_t xw, W& v0, W& v1) {
 		if constexpr(MODE == 0)
 		{
 			if constexpr(WB == 64)
 			{
 				const uint64_t hi = g.u32();
 				const uint64_t lo = g.u32();
-				out[row + xw] = (W)((hi << 32) ^ lo);
+				v0 = (W)((hi << 32) ^ lo);
 			}
 			else
-				out[row + xw] = (W)g.u32();
+				v0 = (W)g.u32();
+			v1 = 0;
 		}
 		else
 		{
@@ -1254,8 +1261,46 @@ __global__ __launch_bounds__(256) void cbbit_tmt_gen_kernel(CbHalfArgs a, int lC
 				pb |= (W)(rf < tp ? 1u : 0u) << k;
 				qb |= (W)(rf < tq ? 1u : 0u) << k;
 			}
-			out[row + xw] = pb;
-			out[Q + row + xw] = qb;
+			v0 = pb;
+			v1 = qb;
+		}
+	};
+	const uint64_t x0 = (uint64_t)c * wc, x1 = x0 + wc;
+	if(!staged)
+	{
+		for(uint64_t xw = x0; xw < x1; ++xw)
+		{
+			W v0, v1;
+			word(xw, v0, v1);
+			out[row + xw] = v0;
+			if(NO == 2)
+				out[Q + row + xw] = v1;
+		}
+	}
+	else
+	{
+		const int tid = (int)threadIdx.x;
+		const uint64_t y0 = (uint64_t)(y - tid); // first row of the workgroup
+		for(uint64_t xb = x0; xb < x1; xb += B)
+		{
+			for(int w = 0; w < B; ++w)
+			{
+				W v0, v1;
+				word(xb + w, v0, v1);
+				stage[0][tid][w] = v0;
+				if(NO == 2)
+					stage[NO - 1][tid][w] = v1;
+			}
+			__syncthreads();
+			for(int s = 0; s < B; ++s)
+			{
+				const int e = s * 256 + tid, rr = e / B, w = e % B;
+				const uint64_t o = (y0 + (uint64_t)rr) * Wn + xb + (uint64_t)w;
+				out[o] = stage[0][rr][w];
+				if(NO == 2)
+					out[Q + o] = stage[NO - 1][rr][w];
+			}
+			__syncthreads();
 		}
 	}
 	const int64_t rows = a.per + (t < a.rem ? 1 : 0);
@@ -1296,11 +1341,13 @@ __global__ __launch_bounds__(256) void cb_local_tmt_gen_kernel(CbLocalArgs a, in
 		uint32_t* __restrict__ gout, const Poly128* __restrict__ chunkJ, const TmtOrbit* __restrict__ orb,
 		uint32_t* __restrict__ acc)
 {
+	// lane k * T + t: neighbouring lanes are neighbouring workers at the same
+	// place of their walks, so their stores of a step land side by side
 	const int64_t lane = (int64_t)blockIdx.x * 256 + threadIdx.x;
 	if(lane >= a.T * chunks)
 		return;
-	const int64_t t = lane / chunks;
-	const int k = (int)(lane - t * chunks);
+	const int k = (int)(lane / a.T);
+	const int64_t t = lane - (int64_t)k * a.T;
 	const int64_t XC = 1ll << (a.lx - 2), YC = 1ll << (a.ly - 2);
 	const int64_t w = a.pc + (t < a.rem ? 1 : 0);
 	const int64_t c0 = t * a.pc + (t < a.rem ? t : a.rem);
@@ -1314,7 +1361,7 @@ __global__ __launch_bounds__(256) void cb_local_tmt_gen_kernel(CbLocalArgs a, in
 	if(k > 0)
 	{
 		uint32_t st[4] = {g.s0, g.s1, g.s2, g.s3};
-		tmt_jump4(st, chunkJ[lane], orb[t], g.m1, g.m2);
+		tmt_jump4(st, chunkJ[t * chunks + k], orb[t], g.m1, g.m2);
 		g.s0 = st[0];
 		g.s1 = st[1];
 		g.s2 = st[2];

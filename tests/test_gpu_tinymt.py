@@ -121,7 +121,7 @@ CB_CASES = [
 ]
 
 
-@pytest.mark.parametrize("lx,ly,T", CB_CASES)
+@pytest.mark.parametrize("lx,ly,T", CB_CASES + [(11, 8, 0), (12, 9, 5)])  # >= 256 rows: staged stores
 def test_scabit_bitparallel(lx, ly, T):
     s, m = engine(K.SchedulerSCABit, lx, ly, 42, T)
     s.mcs(3)
@@ -143,7 +143,7 @@ def test_scabit_pq(lx, ly, T, p, q):
     check(s, lat, st, master)
 
 
-@pytest.mark.parametrize("lx,ly,T", [(8, 6, 3), (7, 7, 0), (14, 3, 5)])
+@pytest.mark.parametrize("lx,ly,T", [(8, 6, 3), (7, 7, 0), (14, 3, 5), (10, 9, 3)])
 def test_scabit_disorder2(lx, ly, T):
     s, m = engine(K.SchedulerSCABit, lx, ly, 5, T)
     for n in (0, 1):
