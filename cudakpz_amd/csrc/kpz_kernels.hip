@@ -1251,15 +1251,39 @@ __global__ __launch_bounds__(256) void cbbit_tmt_gen_kernel(CbHalfArgs a, int lC
 		}
 		else
 		{
-			const W cl = (MODE == 2) ? dis[row + xw] : (W)0;
-			W pb = 0, qb = 0;
-			for(int k = 0; k < WB; ++k)
+			// acceptance bits: the sign of rf - t (both at most 2^24) funnel-shifted
+			// into an accumulator per 32 sites, which ends bit-reversed; the
+			// disorder mode keeps both classes' bits and selects per site
+			W pb = 0, qb = 0, pb1 = 0, qb1 = 0;
+#pragma unroll
+			for(int h = 0; h < WB / 32; ++h)
 			{
-				const uint32_t rf = g.u32() >> 8;
-				const bool c1 = (cl >> k) & 1;
-				const uint32_t tp = c1 ? a.tp1 : a.tp, tq = c1 ? a.tq1 : a.tq;
-				pb |= (W)(rf < tp ? 1u : 0u) << k;
-				qb |= (W)(rf < tq ? 1u : 0u) << k;
+				uint32_t pa = 0, qa = 0, pa1 = 0, qa1 = 0;
+#pragma unroll 8
+				for(int k = 0; k < 32; ++k)
+				{
+					const uint32_t rf = g.u32() >> 8;
+					pa = __builtin_amdgcn_alignbit(pa, rf - a.tp, 31);
+					qa = __builtin_amdgcn_alignbit(qa, rf - a.tq, 31);
+					if(MODE == 2)
+					{
+						pa1 = __builtin_amdgcn_alignbit(pa1, rf - a.tp1, 31);
+						qa1 = __builtin_amdgcn_alignbit(qa1, rf - a.tq1, 31);
+					}
+				}
+				pb |= (W)__brev(pa) << (32 * h);
+				qb |= (W)__brev(qa) << (32 * h);
+				if(MODE == 2)
+				{
+					pb1 |= (W)__brev(pa1) << (32 * h);
+					qb1 |= (W)__brev(qa1) << (32 * h);
+				}
+			}
+			if(MODE == 2)
+			{
+				const W cl = dis[row + xw]; // class: the site's x-slope bit of the map
+				pb = (pb & ~cl) | (pb1 & cl);
+				qb = (qb & ~cl) | (qb1 & cl);
 			}
 			v0 = pb;
 			v1 = qb;
@@ -1336,7 +1360,7 @@ __global__ __launch_bounds__(256) void tmt_row_polys_kernel(const TmtOrbit* __re
 constexpr int CB_CHUNK = 256;
 static_assert(CB_TMT_CHUNK_DRAWS == (uint64_t)CB_CHUNK * 8, "chunk of the SCA TinyMT32 walk");
 
-template<bool DIS>
+template<int S, bool DIS>
 __global__ __launch_bounds__(256) void cb_local_tmt_gen_kernel(CbLocalArgs a, int chunks, const uint32_t* __restrict__ gin,
 		uint32_t* __restrict__ gout, const Poly128* __restrict__ chunkJ, const TmtOrbit* __restrict__ orb,
 		uint32_t* __restrict__ acc)
@@ -1383,33 +1407,29 @@ __global__ __launch_bounds__(256) void cb_local_tmt_gen_kernel(CbLocalArgs a, in
 		r = q / (w - wl);
 		ci = q - r * (w - wl);
 	}
-	const int sg = a.signum;
 	for(int64_t q = q0; q < q1; ++q)
 	{
 		const int64_t hw = half ? w - wl : wl;
 		const size_t io = (size_t)(r * XC + c0 + (half ? wl : 0) + ci);
 		const uint32_t dw = DIS ? a.dis[io] : 0u;
-		uint32_t out = 0;
+		// the 8 draws in site order (CbCell<S>::P[i] = 2x + 8y of draw i): the
+		// sign of rf - t funnel-shifted in K[i] times, bit-reversed at the end
+		uint32_t ap = 0, aq = 0;
 #pragma unroll
-		for(int rr = 0; rr < 4; ++rr)
+		for(int i = 0; i < 8; ++i)
 		{
-#pragma unroll
-			for(int kk = 0; kk < 2; ++kk)
+			const uint32_t rf = g.u32() >> 8;
+			uint32_t tp = a.tp, tq = a.tq;
+			if(DIS)
 			{
-				const int sh = 8 * rr + 2 * (((rr & 1) ^ sg) + 2 * kk);
-				const uint32_t rf = g.u32() >> 8;
-				uint32_t tp = a.tp, tq = a.tq;
-				if(DIS)
-				{
-					const uint32_t cls = (dw >> sh) & 3u;
-					tp = a.tpc[cls];
-					tq = a.tqc[cls];
-				}
-				out |= (rf < tp ? 1u : 0u) << sh;
-				out |= (rf < tq ? 1u : 0u) << (sh + 1);
+				const uint32_t cls = (dw >> CbCell<S>::P[i]) & 3u;
+				tp = a.tpc[cls];
+				tq = a.tqc[cls];
 			}
+			ap = __builtin_amdgcn_alignbit(ap, rf - tp, 32 - CbCell<S>::K[i]);
+			aq = __builtin_amdgcn_alignbit(aq, rf - tq, 32 - CbCell<S>::K[i]);
 		}
-		acc[io] = out;
+		acc[io] = (__brev(ap) & CbCell<S>::MS) | ((__brev(aq) & CbCell<S>::MS) << 1);
 		if(++ci == hw)
 		{
 			ci = 0;
@@ -2413,12 +2433,25 @@ hipError_t launch_cb_local_tmt_gen(const CbLocalArgs& a, const uint32_t* gin, ui
 		return hipErrorInvalidValue;
 	const int64_t lanes = a.T * chunks;
 	const dim3 grid((unsigned)((lanes + 255) / 256));
-	if(a.dis)
-		hipLaunchKernelGGL((cb_local_tmt_gen_kernel<true>), grid, dim3(256), 0, stream, a, chunks, gin, gout, chunkJ, orb,
-		                   acc);
-	else
-		hipLaunchKernelGGL((cb_local_tmt_gen_kernel<false>), grid, dim3(256), 0, stream, a, chunks, gin, gout, chunkJ, orb,
-		                   acc);
+	switch((a.dis ? 2 : 0) + (a.signum ? 1 : 0))
+	{
+		case 0:
+			hipLaunchKernelGGL((cb_local_tmt_gen_kernel<0, false>), grid, dim3(256), 0, stream, a, chunks, gin, gout, chunkJ,
+			                   orb, acc);
+			break;
+		case 1:
+			hipLaunchKernelGGL((cb_local_tmt_gen_kernel<1, false>), grid, dim3(256), 0, stream, a, chunks, gin, gout, chunkJ,
+			                   orb, acc);
+			break;
+		case 2:
+			hipLaunchKernelGGL((cb_local_tmt_gen_kernel<0, true>), grid, dim3(256), 0, stream, a, chunks, gin, gout, chunkJ,
+			                   orb, acc);
+			break;
+		default:
+			hipLaunchKernelGGL((cb_local_tmt_gen_kernel<1, true>), grid, dim3(256), 0, stream, a, chunks, gin, gout, chunkJ,
+			                   orb, acc);
+			break;
+	}
 	return hipGetLastError();
 }
 
