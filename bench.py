@@ -354,8 +354,9 @@ def main():
                          "FETCH_SIZE/WRITE_SIZE calibrate the counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
-              "(see DESIGN.md for how the lattice would shard)", file=sys.stderr)
+        print("bench.py: this engine runs on one GPU; runs on several GPUs (row stripes of the "
+              "lattice with one halo row exchanged per phase, or whole replicas per GPU) are not "
+              "part of it", file=sys.stderr)
         sys.exit(2)
 
     import cudakpz_amd as K

@@ -269,7 +269,7 @@ int parseArg(Args& A, int argc, char** argv, int a)
 			return -1;
 		}
 #else
-		std::cerr << "[EE][FrontendArgs] " << argv[a] << ": this build has no HDF5 library; use --saveState / "
+		std::cerr << "[EE][FrontendArgs] " << argv[a] << ": kpz_scaling was compiled without HDF5; use --saveState / "
 		          << "--loadState.\n";
 		return -1;
 #endif
