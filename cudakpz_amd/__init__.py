@@ -99,6 +99,7 @@ def lib():
         L.kpz_set_tinymt_params.argtypes = [h, u32p, u32p, u32p]
         L.kpz_get_tinymt.argtypes = [h, u32p]
         L.kpz_set_tinymt.argtypes = [h, u32p]
+        L.kpz_consistency_check.argtypes = [h, C.c_int, i64p]
         _LIB = L
     return _LIB
 
@@ -332,6 +333,16 @@ class SchedulerService:
         out = (C.c_double * 7)()
         _check(lib().kpz_moments(self._h, out))
         return list(out)
+
+    def consistencyCheck(self, full=True):
+        """SchedulerService::consitencyCheck (kpz/schedulerService.cpp:721-854):
+        (ok, (bad rows, bad columns, bad loops)) of the lattice and, with full,
+        of every live correlator tag's snapshot as well."""
+        out = (C.c_int64 * 3)()
+        rc = lib().kpz_consistency_check(self._h, int(bool(full)), out)
+        if rc < 0:
+            _check(rc)
+        return rc == 1, (int(out[0]), int(out[1]), int(out[2]))
 
     def correlateTag(self, stop=None):
         if stop is None:

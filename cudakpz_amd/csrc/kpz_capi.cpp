@@ -289,6 +289,16 @@ int kpz_moments(kpz_scheduler* h, double out[7])
 	});
 }
 
+int kpz_consistency_check(kpz_scheduler* h, int full, int64_t out[3])
+{
+	HANDLE(h);
+	if(!out)
+		return fail(KPZ_EINVAL, "null output");
+	bool ok = false;
+	const int r = guard([&] { ok = h->s->consistencyCheck(full != 0, out); });
+	return r < 0 ? r : (ok ? 1 : 0);
+}
+
 int kpz_corr_tag(kpz_scheduler* h)
 {
 	HANDLE(h);

@@ -1523,6 +1523,136 @@ __device__ __forceinline__ int ybit0(const uint32_t* lat, int lx, int ly, int y)
 	return lat[(sw >> 2) + ((y & 1) ? (sw >> 1) : 0) + ((size_t)y << (lx - 6))] & 1;
 }
 
+// x- and y-slope bits of sites x = 64j .. 64j+63 of row y (bit i <-> x = 64j+i)
+template<int ENC>
+__device__ __forceinline__ void xybits(const uint32_t* lat, int lx, int ly, int j, int y, uint64_t& X, uint64_t& Yb)
+{
+	X = 0;
+	Yb = 0;
+	if(ENC == ENC_LOCAL)
+	{
+		const size_t base = ((size_t)(y >> 2) << (lx - 2)) + (size_t)j * 16;
+		const int sh = (y & 3) << 3;
+		if(lx >= 6)
+		{
+			const uint4* v = reinterpret_cast<const uint4*>(lat + base);
+#pragma unroll
+			for(int k = 0; k < 4; ++k)
+			{
+				const uint4 w = v[k];
+				const uint32_t b = ((w.x >> sh) & 0xFFu) | (((w.y >> sh) & 0xFFu) << 8) | (((w.z >> sh) & 0xFFu) << 16)
+				                 | (((w.w >> sh) & 0xFFu) << 24);
+				X |= (uint64_t)even16(b) << (16 * k);
+				Yb |= (uint64_t)even16(b >> 1) << (16 * k);
+			}
+		}
+		else
+		{
+			for(int k = 0; k < (1 << (lx - 2)); ++k)
+			{
+				const uint32_t b = (lat[base + k] >> sh) & 0xFFu;
+				X |= (uint64_t)(even16(b) & 0xFu) << (4 * k);
+				Yb |= (uint64_t)(even16(b >> 1) & 0xFu) << (4 * k);
+			}
+		}
+	}
+	else
+	{
+		X = xbits<ENC>(lat, lx, ly, j, y);
+		const size_t sw = (size_t)1 << (lx + ly - 4);
+		const size_t w = (sw >> 2) + ((size_t)y << (lx - 6)) + j;
+		const uint32_t e = lat[w + ((y & 1) ? (sw >> 1) : 0)];
+		const uint32_t o = lat[w + ((y & 1) ? 0 : (sw >> 1))];
+		const uint64_t se = (uint64_t)spread16(e & 0xFFFFu) | ((uint64_t)spread16(e >> 16) << 32);
+		const uint64_t so = (uint64_t)spread16(o & 0xFFFFu) | ((uint64_t)spread16(o >> 16) << 32);
+		Yb = se | (so << 1);
+	}
+}
+
+// the y slope of site (x, y)
+template<int ENC>
+__device__ __forceinline__ uint32_t yslope(const uint32_t* lat, int lx, int ly, int x, int y)
+{
+	if(ENC == ENC_LOCAL)
+		return (lat[((size_t)(y >> 2) << (lx - 2)) + (size_t)(x >> 2)] >> ((((x & 3) << 1) | ((y & 3) << 3)) + 1)) & 1u;
+	const size_t sw = (size_t)1 << (lx + ly - 4);
+	const size_t w = (sw >> 2) + (((x ^ y) & 1) ? (sw >> 1) : 0) + ((size_t)y << (lx - 6)) + (size_t)(x >> 6);
+	return (lat[w] >> ((x >> 1) & 31)) & 1u;
+}
+
+// SchedulerService::consitencyCheck (kpz/schedulerService.cpp:721-854) on the
+// device: every row's x slopes and every column's y slopes sum to zero as
+// +-1, and every loop closes, sX(x,y) + sY(x-1,y) = sY(x,y) + sX(x,y-1).
+// Lane (band, j) takes the CK-site chunk j of the rows of one band of CB rows
+// (chunks fastest, NC chunks per row): rowOnes[y] += ones of the row's x
+// slopes (summed over the row's lanes by shuffles), colOnes[x] += ones of the
+// column's y slopes in the band (bit-sliced counters), badLoops += loops that
+// do not close.  Every lane runs every shuffle; lanes past the end carry zeros.
+constexpr int CK_BAND = 256;
+template<int ENC>
+__global__ __launch_bounds__(256) void consistency_kernel(const uint32_t* __restrict__ lat, int lx, int ly,
+		int* __restrict__ rowOnes, int* __restrict__ colOnes, unsigned long long* __restrict__ badLoops)
+{
+	const int L = 1 << lx, Y = 1 << ly;
+	const int CK = L < 64 ? L : 64;               // sites per chunk
+	const int NC = L / CK;                        // chunks per row (a power of two)
+	const int CB = Y < CK_BAND ? Y : CK_BAND;     // rows per band
+	const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const int64_t lanes = (int64_t)NC * (Y / CB);
+	const bool live = lane < lanes;
+	const int j = (int)(lane % NC), band = (int)(lane / NC);
+	const uint64_t wm = (CK == 64) ? ~0ull : ((1ull << CK) - 1);
+	const int y0 = band * CB;
+	uint64_t c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}; // bit-sliced column counts (CB <= 256)
+	unsigned long long bad = 0;
+	uint64_t xPrev = 0, yDummy;
+	if(live)
+		xybits<ENC>(lat, lx, ly, j, (y0 - 1) & (Y - 1), xPrev, yDummy);
+	const int xm1 = (j * CK - 1) & (L - 1); // the site left of the chunk
+	for(int r = 0; r < CB; ++r)
+	{
+		const int y = y0 + r;
+		uint64_t X = 0, Yb = 0;
+		if(live)
+		{
+			xybits<ENC>(lat, lx, ly, j, y, X, Yb);
+			const uint64_t q = ((Yb << 1) | yslope<ENC>(lat, lx, ly, xm1, y)) & wm; // sY(x-1, y)
+			const uint64_t e = ((X ^ q) ^ (Yb ^ xPrev)) | ((X & q) ^ (Yb & xPrev));
+			bad += (unsigned long long)__popcll(e & wm);
+			xPrev = X;
+			uint64_t carry = Yb;
+#pragma unroll
+			for(int k = 0; k < 9; ++k)
+			{
+				const uint64_t t = c[k] & carry;
+				c[k] ^= carry;
+				carry = t;
+			}
+		}
+		// the row's ones: the NC lanes of one row are consecutive, NC-aligned
+		int ones = __popcll(X);
+		for(int o = 1; o < NC && o < 64; o <<= 1)
+			ones += __shfl_xor(ones, o, 64);
+		if(live && (j & 63) == 0)
+			atomicAdd(rowOnes + y, ones);
+	}
+	if(live)
+		for(int i = 0; i < CK; ++i)
+		{
+			int n = 0;
+#pragma unroll
+			for(int k = 0; k < 9; ++k)
+				n |= (int)((c[k] >> i) & 1) << k;
+			if(n)
+				atomicAdd(colOnes + j * CK + i, n);
+		}
+	// one atomic per wave for the loops
+	for(int o = 32; o > 0; o >>= 1)
+		bad += __shfl_xor(bad, o, 64);
+	if((threadIdx.x & 63) == 0 && bad)
+		atomicAdd(badLoops, bad);
+}
+
 // local prefix statistics of one chunk: q = sum d, sq = sum Q_i, sq2 = sum Q_i^2
 // over the n counted sites (site x = 0 is excluded from the row prefix)
 __device__ __forceinline__ void chunk_prefix(uint64_t bits, int nbits, bool skip0, int& q, int64_t& sq, int64_t& sq2)
@@ -2555,6 +2685,26 @@ hipError_t launch_row_moments(const uint32_t* lat, int enc, int lx, int ly, cons
 		hipLaunchKernelGGL((row_moments_kernel<ENC_LOCAL>), grid, dim3(256), 0, stream, lat, lx, ly, hs, out);
 	else
 		hipLaunchKernelGGL((row_moments_kernel<ENC_CBBIT>), grid, dim3(256), 0, stream, lat, lx, ly, hs, out);
+	return hipGetLastError();
+}
+
+hipError_t launch_consistency(const uint32_t* lat, int enc, int lx, int ly, int* rowOnes, int* colOnes,
+		unsigned long long* badLoops, hipStream_t stream)
+{
+	if((enc == ENC_LOCAL ? (lx < 2 || ly < 2) : (lx < 6 || ly < 1)) || lx > 30 || ly > 30 || !rowOnes || !colOnes
+	   || !badLoops)
+		return hipErrorInvalidValue;
+	const int L = 1 << lx, Y = 1 << ly;
+	const int CK = L < 64 ? L : 64;
+	const int CB = Y < CK_BAND ? Y : CK_BAND;
+	const int64_t lanes = (int64_t)(L / CK) * (Y / CB);
+	const dim3 grid((unsigned)((lanes + 255) / 256));
+	if(enc == ENC_LOCAL)
+		hipLaunchKernelGGL((consistency_kernel<ENC_LOCAL>), grid, dim3(256), 0, stream, lat, lx, ly, rowOnes, colOnes,
+		                   badLoops);
+	else
+		hipLaunchKernelGGL((consistency_kernel<ENC_CBBIT>), grid, dim3(256), 0, stream, lat, lx, ly, rowOnes, colOnes,
+		                   badLoops);
 	return hipGetLastError();
 }
 

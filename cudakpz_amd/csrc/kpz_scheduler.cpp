@@ -333,6 +333,40 @@ Sums SchedulerService::totals(const uint32_t* snap, int64_t* hs)
 
 Sums SchedulerService::roughness() { return totals(nullptr, nullptr); }
 
+// counts of one lattice (device or snapshot), added to out[0..2]
+void SchedulerService::consistencyOf(const uint32_t* lat, int64_t out[3])
+{
+	const size_t nx = (size_t)1 << size_.lx, ny = (size_t)1 << size_.ly;
+	const size_t bytes = (ny + nx) * sizeof(int) + sizeof(unsigned long long);
+	if(check_.bytes() < bytes)
+		check_.alloc(bytes);
+	int* rowOnes = check_.as<int>();
+	int* colOnes = rowOnes + ny;
+	unsigned long long* bad = (unsigned long long*)(colOnes + nx);
+	check(hipMemsetAsync(check_.get(), 0, bytes, stream_), "consistency clear");
+	check(launch_consistency(lat, enc_, size_.lx, size_.ly, rowOnes, colOnes, bad, stream_), "consistency");
+	std::vector<int> h(ny + nx + 2);
+	check(hipMemcpyAsync(h.data(), check_.get(), bytes, hipMemcpyDeviceToHost, stream_), "consistency D2H");
+	sync();
+	unsigned long long nbad;
+	std::memcpy(&nbad, h.data() + ny + nx, sizeof nbad);
+	for(size_t y = 0; y < ny; ++y)
+		out[0] += (size_t)h[y] * 2 != nx;
+	for(size_t x = 0; x < nx; ++x)
+		out[1] += (size_t)h[ny + x] * 2 != ny;
+	out[2] += (int64_t)nbad;
+}
+
+bool SchedulerService::consistencyCheck(bool full, int64_t out[3])
+{
+	out[0] = out[1] = out[2] = 0;
+	consistencyOf(deviceLattice(), out);
+	if(full)
+		for(const Tag& t : tags_)
+			consistencyOf(t.snap->as<uint32_t>(), out);
+	return !out[0] && !out[1] && !out[2];
+}
+
 void SchedulerService::moments(long double out[7])
 {
 	const size_t ny = (size_t)1 << size_.ly;

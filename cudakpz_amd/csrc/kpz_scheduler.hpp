@@ -139,6 +139,12 @@ public:
 	// tags whose stop time has passed are dropped first
 	std::vector<TagResult> correlateAll();
 
+	// SchedulerService::consitencyCheck (kpz/schedulerService.cpp:800-854):
+	// out[0] rows whose x slopes, out[1] columns whose y slopes do not sum to
+	// zero, out[2] loops that do not close; with full, every live correlator
+	// tag's snapshot is checked too and its counts added in.  True if all are 0.
+	bool consistencyCheck(bool full, int64_t out[3]);
+
 	// ---- RNG state access (parity checks, checkpoints)
 	int rng() const { return params_.rng; }
 	bool tinyMT() const { return params_.rng == RNG_TINYMT32; }
@@ -194,7 +200,8 @@ protected:
 	};
 	Sums correlateWith(const DevBuf& snap);
 	Sums totals(const uint32_t* snap, int64_t* hs);
-	DevBuf lat_, rows_, totals_, rowStart_, rowMoments_;
+	DevBuf lat_, rows_, totals_, rowStart_, rowMoments_, check_;
+	void consistencyOf(const uint32_t* lat, int64_t out[3]);
 	std::vector<Tag> tags_;
 	DevBuf st_[2];
 	int cur_ = 0;

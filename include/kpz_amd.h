@@ -153,6 +153,13 @@ int kpz_correlate(kpz_scheduler* h, int64_t out[7]);
  * values of kpz_correlate}; returns the number of tags written (<= max_tags)
  * or a negative error */
 int kpz_correlate_tags(kpz_scheduler* h, int64_t* out, int max_tags);
+/* SchedulerService::consitencyCheck(full, out) (kpz/schedulerService.cpp:721-854,
+ * declared at kpz/schedulerService.h:148), on the device: out = {rows whose x
+ * slopes do not sum to zero, columns whose y slopes do not, loops
+ * sX(x,y) + sY(x-1,y) != sY(x,y) + sX(x,y-1)}; full != 0 adds the counts of
+ * every live correlator tag's snapshot.  Returns 1 if all three are zero, 0
+ * if not, or a negative error. */
+int kpz_consistency_check(kpz_scheduler* h, int full, int64_t out[3]);
 
 /* RNG state: stream states (T u64) and the master dSFMT.  The master state
  * is kpz_master_bytes() bytes laid out as the reference's dsfmt_t
