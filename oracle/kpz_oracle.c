@@ -1064,30 +1064,35 @@ static void cbbit_rows(int L, int64_t T, int64_t t, int* yMin, int* ySup)
 	*ySup = (int)(lo + per + (t < rem ? 1 : 0));
 }
 
+/* worker t's rows of half-sweep s (kpz/schedulerCPUCBBit_nested.inl:200-295) */
+static void cbbit_worker(uint32_t* lat, int lx, int ly, const streams_t* S, int64_t T, int64_t t,
+		int s, int mode, const thr_t* th)
+{
+	const int X = 1 << lx, Y = 1 << ly;
+	const int wbits = (lx >= 7) ? 64 : 32;
+	int y0, y1;
+	cbbit_rows(Y, T, t, &y0, &y1);
+	gen_t g = gen_get(S, t);
+	for(int y = y0; y < y1; ++y)
+	{
+		if(mode == 0)
+			cbbit_row_words(lat, lx, ly, s, y, wbits, &g, NULL);
+		else
+			for(int x = ((y & 1) ^ s); x < X; x += 2)
+				upd_cbbit(lat, lx, ly, x, y, gen_rf(&g), th, mode == 2);
+	}
+	gen_put(S, t, &g);
+}
+
 static int cbbit_mcs(uint32_t* lat, int lx, int ly, const streams_t* S, int64_t T,
 		int n, int mode, const thr_t* th)
 {
 	if(lx < 6 || ly < 1 || T < 1)
 		return -1;
-	const int X = 1 << lx, Y = 1 << ly;
-	const int wbits = (lx >= 7) ? 64 : 32;
 	for(int m = 0; m < n; ++m)
 		for(int s = 0; s < 2; ++s)
 			for(int64_t t = 0; t < T; ++t)
-			{
-				int y0, y1;
-				cbbit_rows(Y, T, t, &y0, &y1);
-				gen_t g = gen_get(S, t);
-				for(int y = y0; y < y1; ++y)
-				{
-					if(mode == 0)
-						cbbit_row_words(lat, lx, ly, s, y, wbits, &g, NULL);
-					else
-						for(int x = ((y & 1) ^ s); x < X; x += 2)
-							upd_cbbit(lat, lx, ly, x, y, gen_rf(&g), th, mode == 2);
-				}
-				gen_put(S, t, &g);
-			}
+				cbbit_worker(lat, lx, ly, S, T, t, s, mode, th);
 	return 0;
 }
 
@@ -1139,29 +1144,31 @@ static void cb_loop(uint32_t* lat, int lx, int ly, int s, int xMin, int xSup,
 			}
 }
 
+/* worker t's strip of half-sweep s: columns [xMin, xSup), a whole number of
+ * 4-site cells, split at xCenter (kpz/schedulerCPUCB.cpp:153-168) */
+static void cb_worker(uint32_t* lat, int lx, int ly, const streams_t* S, int64_t T, int64_t t,
+		int s, int upd, const thr_t* th)
+{
+	const int64_t dimXW = (int64_t)1 << (lx - 2);
+	const int64_t per = dimXW / T, rem = dimXW % T;
+	const int xMin = (int)(4 * (t * per + (t < rem ? t : rem)));
+	const int xSup = xMin + (int)(4 * (per + (t < rem ? 1 : 0)));
+	const int xCenter = xMin + (((xSup - xMin) / 2) & ~3);
+	gen_t g = gen_get(S, t);
+	cb_loop(lat, lx, ly, s, xMin, xCenter, &g, upd, th);
+	cb_loop(lat, lx, ly, s, xCenter, xSup, &g, upd, th);
+	gen_put(S, t, &g);
+}
+
 static int cb_mcs(uint32_t* lat, int lx, int ly, const streams_t* S, int64_t T,
 		int n, int upd, const thr_t* th)
 {
 	if(lx < 2 || ly < 2 || T < 1)
 		return -1;
-	const int dimXW = 1 << (lx - 2);
-	const int per = (int)(dimXW / T) << 2;
-	const int rem = (int)(dimXW % T);
 	for(int m = 0; m < n; ++m)
 		for(int s = 0; s < 2; ++s)
-		{
-			int xSup = 0;
 			for(int64_t t = 0; t < T; ++t)
-			{
-				const int xMin = xSup;
-				xSup += per + (t < rem ? 4 : 0);
-				const int xCenter = xMin + (((xSup - xMin) / 2) & ~3);
-				gen_t g = gen_get(S, t);
-				cb_loop(lat, lx, ly, s, xMin, xCenter, &g, upd, th);
-				cb_loop(lat, lx, ly, s, xCenter, xSup, &g, upd, th);
-				gen_put(S, t, &g);
-			}
-		}
+				cb_worker(lat, lx, ly, S, T, t, s, upd, th);
 	return 0;
 }
 
@@ -1194,6 +1201,109 @@ int kor_cb_mcs_tmt(uint32_t* lat, int lx, int ly, kor_tmt* g, int64_t T,
 	const thr_t th = (upd == KOR_UPD_DIS) ? thr_rates_bits(r, 24) : thr_pq_bits(p, q, 24);
 	const streams_t S = streams_tmt(g);
 	return cb_mcs(lat, lx, ly, &S, T, n, upd, &th);
+}
+
+/* ======================================================================
+ * Threaded forms of the two checkerboard loops, for checks at large sizes.
+ * Within a half-sweep no update reads a bit that another update writes
+ * (sites of one sublattice own disjoint bits), and each worker draws from its
+ * own stream in its own order, so any schedule of the workers gives the
+ * sequential result.  What threads must not share is a word written by two
+ * of them at once:
+ *  - CBBit: worker t writes rows y0..y1-1 of quadrants s and the other
+ *    sublattice's x slopes, and row y1 of the other sublattice's y slopes,
+ *    which no other worker writes in this half-sweep: any split is safe;
+ *  - CB: a strip writes its own cells and the x slopes of the first column
+ *    right of it, a cell of the next strip.  Each thread takes a contiguous
+ *    run of at least two workers and runs its first half of them, then (after
+ *    all threads) its second half, so that concurrent writes are always at
+ *    least one whole worker strip apart.
+ * ====================================================================== */
+typedef struct {
+	uint32_t* lat;
+	int lx, ly, s, mode, cb;
+	const streams_t* S;
+	int64_t T, t0, t1;
+	const thr_t* th;
+} sweep_work;
+
+static void* sweep_run(void* arg)
+{
+	const sweep_work* W = (const sweep_work*)arg;
+	for(int64_t t = W->t0; t < W->t1; ++t)
+	{
+		if(W->cb)
+			cb_worker(W->lat, W->lx, W->ly, W->S, W->T, t, W->s, W->mode, W->th);
+		else
+			cbbit_worker(W->lat, W->lx, W->ly, W->S, W->T, t, W->s, W->mode, W->th);
+	}
+	return NULL;
+}
+
+static int sweep_threaded(uint32_t* lat, int lx, int ly, const streams_t* S, int64_t T, int n,
+		int cb, int mode, const thr_t* th, int nthreads)
+{
+	if(nthreads < 1)
+		return -1;
+	int64_t nt = nthreads;
+	if(cb && nt > T / 2)
+		nt = T / 2;
+	if(nt > T)
+		nt = T;
+	if(nt < 1)
+		nt = 1;
+	pthread_t* th_ = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nt);
+	sweep_work* wk = (sweep_work*)malloc(sizeof(sweep_work) * (size_t)nt);
+	const int phases = cb ? 2 : 1;
+	for(int m = 0; m < n; ++m)
+		for(int s = 0; s < 2; ++s)
+			for(int ph = 0; ph < phases; ++ph)
+			{
+				for(int64_t k = 0; k < nt; ++k)
+				{
+					const int64_t a = T * k / nt, b = T * (k + 1) / nt, mid = (a + b) / 2;
+					sweep_work* W = &wk[k];
+					W->lat = lat;
+					W->lx = lx;
+					W->ly = ly;
+					W->s = s;
+					W->mode = mode;
+					W->cb = cb;
+					W->S = S;
+					W->T = T;
+					W->t0 = (phases == 1) ? a : (ph == 0 ? a : mid);
+					W->t1 = (phases == 1) ? b : (ph == 0 ? mid : b);
+					W->th = th;
+					pthread_create(&th_[k], NULL, sweep_run, W);
+				}
+				for(int64_t k = 0; k < nt; ++k)
+					pthread_join(th_[k], NULL);
+			}
+	free(wk);
+	free(th_);
+	return 0;
+}
+
+/* SchedulerCPUCBBit (mode 0 VL3, 1 per-site pq) on nthreads threads */
+int kor_cbbit_mcs_threaded(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
+		uint64_t a, uint64_t c, int n, int mode, double p, double q, int nthreads)
+{
+	if(lx < 6 || ly < 1 || T < 1 || mode < 0 || mode > 1)
+		return -1;
+	const thr_t th = thr_pq(p, q);
+	const streams_t S = streams_slcg(states, a, c);
+	return sweep_threaded(lat, lx, ly, &S, T, n, 0, mode, &th, nthreads);
+}
+
+/* SchedulerCPUCB (pqSyncRngUpdate) on nthreads threads */
+int kor_cb_mcs_threaded(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
+		uint64_t a, uint64_t c, int n, double p, double q, int nthreads)
+{
+	if(lx < 2 || ly < 2 || T < 1)
+		return -1;
+	const thr_t th = thr_pq(p, q);
+	const streams_t S = streams_slcg(states, a, c);
+	return sweep_threaded(lat, lx, ly, &S, T, n, 1, KOR_UPD_PQ, &th, nthreads);
 }
 
 /* ======================================================================

@@ -142,6 +142,12 @@ void kor_cbbit_halfsweep_masked(uint32_t* lat, int lx, int ly, int signum, const
 /* SchedulerCPUCB (ENC_LOCAL checkerboard, per-site pq) */
 int kor_cb_mcs(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
 		uint64_t a, uint64_t c, int n, double p, double q);
+/* the same two on nthreads pthreads (the sequential result: a half-sweep's
+ * updates own disjoint bits; threads never write one word at once) */
+int kor_cbbit_mcs_threaded(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
+		uint64_t a, uint64_t c, int n, int mode, double p, double q, int nthreads);
+int kor_cb_mcs_threaded(uint32_t* lat, int lx, int ly, uint64_t* states, int64_t T,
+		uint64_t a, uint64_t c, int n, double p, double q, int nthreads);
 
 /* the schedulers the reference's frontend builds with TinyMT32 workers
  * (kpz/kpzFrontend.cpp:308-331): SchedulerCPUDTr (upd KPZ / PQ / EW with p,

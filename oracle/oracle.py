@@ -82,6 +82,8 @@ def _declare(L):
     L.kor_cbbit_halfsweep_masked.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, _u64p]
     L.kor_cb_mcs.argtypes = [_u32p, C.c_int, C.c_int, _u64p, C.c_int64, C.c_uint64, C.c_uint64,
                              C.c_int, C.c_double, C.c_double]
+    L.kor_cbbit_mcs_threaded.argtypes = L.kor_cbbit_mcs.argtypes + [C.c_int]
+    L.kor_cb_mcs_threaded.argtypes = L.kor_cb_mcs.argtypes + [C.c_int]
     L.kor_update_local.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
                                    C.c_double, C.c_double]
     L.kor_update_cbbit.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
@@ -346,6 +348,20 @@ def cbbit_mcs(lat, lx, ly, streams, n, mode, p=0.5, q=0.0):
     rc = lib().kor_cbbit_mcs(lat, lx, ly, streams.states, streams.T, streams.a, streams.c, n, mode, p, q)
     if rc:
         raise ValueError("invalid CBBit size")
+
+
+def cbbit_mcs_threaded(lat, lx, ly, streams, n, mode, p=0.5, q=0.0, nthreads=1):
+    rc = lib().kor_cbbit_mcs_threaded(lat, lx, ly, streams.states, streams.T, streams.a, streams.c, n, mode, p, q,
+                                      int(nthreads))
+    if rc:
+        raise ValueError("invalid CBBit size or mode")
+
+
+def cb_mcs_threaded(lat, lx, ly, streams, n, p=0.95, q=0.0, nthreads=1):
+    rc = lib().kor_cb_mcs_threaded(lat, lx, ly, streams.states, streams.T, streams.a, streams.c, n, p, q,
+                                   int(nthreads))
+    if rc:
+        raise ValueError("invalid CB size")
 
 
 def cbbit_halfsweep_masked(lat, lx, ly, signum, masks):

@@ -2,9 +2,11 @@
 of lattice, 2^32 sites), against the oracle: one SchedulerDT MCS with one
 SLCG64 stream per 16 x 16 block (2^22 streams, the configuration the
 site-update rate is measured on), one SchedulerDTr MCS, and two SchedulerSCABit
-VL3 MCS with one stream per row (65536 streams).  The DT oracle runs a domain
-set's blocks on several threads (blocks of one set are disjoint; threaded ==
-sequential is pinned by tests/test_oracle_model.py).  Beyond word equality,
+VL3 MCS and one per-site p/q MCS with one stream per row (65536 streams), and
+one SchedulerSCA MCS with one stream per cell column (16384 streams).  The
+oracle runs them on several threads (DT: a domain set's blocks are disjoint;
+checkerboard: a half-sweep's updates own disjoint bits; threaded == sequential
+is pinned by tests/test_oracle_model.py).  Beyond word equality,
 the device consistency check and the exact roughness sums must agree with the
 oracle's."""
 import os
@@ -62,4 +64,32 @@ def test_scabit_vl3_two_mcs():
     lat = O.flat(LX, LY, O.ENC_CBBIT)
     O.cbbit_mcs(lat, LX, LY, st, 2, 0)
     compare(s, lat, st.states, m, O.ENC_CBBIT)
+    s.close()
+
+
+def test_scabit_pq_one_mcs():
+    s = K.SchedulerSCABit(LX, LY)
+    s.supplySeed(11)
+    s.initHomogeneous()
+    s.setPQ(0.95, 0.05)
+    s.mcsPQSyncRng(1)
+    m = O.Dsfmt(O.seed32(11))
+    st = O.Streams(s.threads, m)
+    lat = O.flat(LX, LY, O.ENC_CBBIT)
+    O.cbbit_mcs_threaded(lat, LX, LY, st, 1, 1, 0.95, 0.05, THREADS)
+    compare(s, lat, st.states, m, O.ENC_CBBIT)
+    s.close()
+
+
+def test_sca_one_mcs():
+    s = K.SchedulerSCA(LX, LY)
+    assert s.threads == 1 << (LX - 2)  # one stream per 4-site cell column
+    s.supplySeed(3)
+    s.initHomogeneous()
+    s.mcs(1)  # p = .95, q = 0
+    m = O.Dsfmt(O.seed32(3))
+    st = O.Streams(s.threads, m)
+    lat = O.flat(LX, LY, O.ENC_LOCAL)
+    O.cb_mcs_threaded(lat, LX, LY, st, 1, 0.95, 0.0, THREADS)
+    compare(s, lat, st.states, m, O.ENC_LOCAL)
     s.close()

@@ -482,3 +482,28 @@ def test_threaded_dt_equals_sequential(T, nthreads):
     O.dt_mcs(a, lx, ly, 3, 3, s1, m1, 3, True, O.UPD_PQ, 0.8, 0.3)
     O.dt_mcs_threaded(b, lx, ly, 3, 3, s2, m2, 3, nthreads, True, O.UPD_PQ, 0.8, 0.3)
     assert np.array_equal(a, b) and np.array_equal(s1.states, s2.states)
+
+
+@pytest.mark.parametrize("lx,ly,T,nthreads", [(7, 5, 32, 4), (8, 6, 5, 3), (10, 9, 512, 8), (6, 4, 16, 16)])
+@pytest.mark.parametrize("mode,p,q", [(0, 0.5, 0.0), (1, 0.9, 0.07)])
+def test_threaded_cbbit_equals_sequential(lx, ly, T, nthreads, mode, p, q):
+    m1, m2 = O.Dsfmt(4), O.Dsfmt(4)
+    s1, s2 = O.Streams(T, m1), O.Streams(T, m2)
+    a, b = O.flat(lx, ly, O.ENC_CBBIT), O.flat(lx, ly, O.ENC_CBBIT)
+    O.cbbit_mcs(a, lx, ly, s1, 3, mode, p, q)
+    O.cbbit_mcs_threaded(b, lx, ly, s2, 3, mode, p, q, nthreads)
+    assert np.array_equal(a, b) and np.array_equal(s1.states, s2.states)
+    assert not np.array_equal(a, O.flat(lx, ly, O.ENC_CBBIT))
+
+
+# T = 2 and 3 leave one thread; 256 workers of one cell column each over 8
+# threads is the shape of the default SCA engine
+@pytest.mark.parametrize("lx,ly,T,nthreads", [(5, 4, 2, 4), (6, 5, 3, 2), (10, 8, 256, 8), (9, 7, 13, 5)])
+def test_threaded_cb_equals_sequential(lx, ly, T, nthreads):
+    m1, m2 = O.Dsfmt(6), O.Dsfmt(6)
+    s1, s2 = O.Streams(T, m1), O.Streams(T, m2)
+    a, b = O.flat(lx, ly, O.ENC_LOCAL), O.flat(lx, ly, O.ENC_LOCAL)
+    O.cb_mcs(a, lx, ly, s1, 3, 0.8, 0.15)
+    O.cb_mcs_threaded(b, lx, ly, s2, 3, 0.8, 0.15, nthreads)
+    assert np.array_equal(a, b) and np.array_equal(s1.states, s2.states)
+    assert not np.array_equal(a, O.flat(lx, ly, O.ENC_LOCAL))
