@@ -1407,13 +1407,11 @@ __global__ __launch_bounds__(256) void cb_local_tmt_gen_kernel(CbLocalArgs a, in
 		r = q / (w - wl);
 		ci = q - r * (w - wl);
 	}
-	for(int64_t q = q0; q < q1; ++q)
-	{
-		const int64_t hw = half ? w - wl : wl;
-		const size_t io = (size_t)(r * XC + c0 + (half ? wl : 0) + ci);
+	// the acceptance word of cell io: its 8 draws in site order
+	// (CbCell<S>::P[i] = 2x + 8y of draw i), the sign of rf - t funnel-shifted
+	// in K[i] times, bit-reversed at the end
+	auto cell = [&](size_t io) {
 		const uint32_t dw = DIS ? a.dis[io] : 0u;
-		// the 8 draws in site order (CbCell<S>::P[i] = 2x + 8y of draw i): the
-		// sign of rf - t funnel-shifted in K[i] times, bit-reversed at the end
 		uint32_t ap = 0, aq = 0;
 #pragma unroll
 		for(int i = 0; i < 8; ++i)
@@ -1429,8 +1427,35 @@ __global__ __launch_bounds__(256) void cb_local_tmt_gen_kernel(CbLocalArgs a, in
 			ap = __builtin_amdgcn_alignbit(ap, rf - tp, 32 - CbCell<S>::K[i]);
 			aq = __builtin_amdgcn_alignbit(aq, rf - tq, 32 - CbCell<S>::K[i]);
 		}
-		acc[io] = (__brev(ap) & CbCell<S>::MS) | ((__brev(aq) & CbCell<S>::MS) << 1);
-		if(++ci == hw)
+		return (__brev(ap) & CbCell<S>::MS) | ((__brev(aq) & CbCell<S>::MS) << 1);
+	};
+	// Sixteen cells of one half row at a time where they fill an aligned
+	// 64-byte segment (four 16-byte stores): neighbouring lanes' walks are a
+	// strip apart, and single 4-byte stores would reach HBM as partial writes
+	constexpr int V = 16;
+	int64_t q = q0;
+	while(q < q1)
+	{
+		const int64_t hw = half ? w - wl : wl;
+		const size_t io = (size_t)(r * XC + c0 + (half ? wl : 0) + ci);
+		int64_t n = 1;
+		if((io & (V - 1)) == 0 && ci + V <= hw && q + V <= q1)
+		{
+			uint32_t v[V];
+#pragma unroll
+			for(int u = 0; u < V; ++u)
+				v[u] = cell(io + u);
+			uint4* d = reinterpret_cast<uint4*>(acc + io);
+#pragma unroll
+			for(int u = 0; u < V / 4; ++u)
+				d[u] = make_uint4(v[4 * u], v[4 * u + 1], v[4 * u + 2], v[4 * u + 3]);
+			n = V;
+		}
+		else
+			acc[io] = cell(io);
+		q += n;
+		ci += n;
+		if(ci == hw)
 		{
 			ci = 0;
 			if(++r == YC)
