@@ -45,6 +45,9 @@ def test_null_arguments_are_rejected(lib):
     assert lib.kpz_create(0, 8, 8, None, None) == -1
     assert lib.kpz_destroy(None) == 0
     assert lib.kpz_master_bytes() > 3000
+    out = (C.c_int64 * 3)()
+    assert lib.kpz_consistency_check(None, 1, out) == -1  # no scheduler: an error, not "inconsistent"
+    assert b"null" in lib.kpz_last_error()
 
 
 def test_python_mirror_fails_loudly_without_library(monkeypatch, tmp_path):
