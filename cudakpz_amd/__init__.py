@@ -344,6 +344,8 @@ class SchedulerService:
             _check(rc)
         return rc == 1, (int(out[0]), int(out[1]), int(out[2]))
 
+    consitencyCheck = consistencyCheck  # the reference's spelling (kpz/schedulerService.h:148)
+
     def correlateTag(self, stop=None):
         if stop is None:
             _check(lib().kpz_corr_tag(self._h))

@@ -80,6 +80,7 @@ def test_closed_forms(cls, lx, ly):
     e.s.initHomogeneous()
     flat = e.pop()
     assert e.s.consistencyCheck() == (True, (0, 0, 0))
+    assert e.s.consitencyCheck(False) == (True, (0, 0, 0))
     L, Y = 1 << lx, 1 << ly
     rng = np.random.default_rng(lx * 100 + ly)
     for _ in range(3):
