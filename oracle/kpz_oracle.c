@@ -883,9 +883,33 @@ int kor_dt_mcs_threaded(uint32_t* lat, int lx, int ly, int lbx, int lby,
  * reference derives it from the host's L1 size (layoutFromL1,
  * inc/cacheLayout.h) unless setLayout is called.  Blocks are disjoint
  * including the one-site border every update reaches, so the order of the
- * workers does not matter.  close_open() of an SLCG stream is draw * 2^-32
+ * workers does not matter (threads toggle bits atomically: blocks off the
+ * word grid share words).  close_open() of an SLCG stream is draw * 2^-32
  * (contract), so (int)(close_open()*m) = (draw * m) >> 32 exactly.
  * ====================================================================== */
+/* pqOneUpdate with word-atomic loads and bit toggles: the blocks own
+ * disjoint sites, but with lorest > 0 their origins are not on the 4-site
+ * word grid, so two threads' blocks can share a lattice word */
+static inline void upd_local_kpz_atomic(uint32_t* lat, int lx, int ly, int x, int y)
+{
+	const uint32_t ic = loc_index(lx, x, y);
+	const int sc = loc_shift(x, y);
+	const int xr = (x + 1) & ((1 << lx) - 1);
+	const int yu = (y + 1) & ((1 << ly) - 1);
+	const uint32_t ir = loc_index(lx, xr, y);
+	const int sr = loc_shift(xr, y);
+	const uint32_t il = loc_index(lx, x, yu);
+	const int sl = loc_shift(x, yu) | 1;
+	const uint32_t c = (__atomic_load_n(&lat[ic], __ATOMIC_RELAXED) >> sc) & 3;
+	const uint32_t r = (__atomic_load_n(&lat[ir], __ATOMIC_RELAXED) >> sr) & 1;
+	const uint32_t l = (__atomic_load_n(&lat[il], __ATOMIC_RELAXED) >> sl) & 1;
+	if(!(c == 0 && r && l))
+		return;
+	__atomic_fetch_xor(&lat[ic], 3u << sc, __ATOMIC_RELAXED);
+	__atomic_fetch_xor(&lat[ir], 1u << sr, __ATOMIC_RELAXED);
+	__atomic_fetch_xor(&lat[il], 1u << sl, __ATOMIC_RELAXED);
+}
+
 typedef struct {
 	uint32_t* lat;
 	int lx, ly, lbx, lby;
@@ -913,7 +937,10 @@ static void* db_worker(void* arg)
 			{
 				const int x = ((int)(((uint64_t)kor_slcg_draw(&s, W->a, W->c) * ax) >> 32) + offx) & mx;
 				const int y = ((int)(((uint64_t)kor_slcg_draw(&s, W->a, W->c) * ay) >> 32) + offy) & my;
-				upd_local(W->lat, W->lx, W->ly, x, y, KOR_UPD_KPZ, 0, &W->th);
+				if(W->nw > 1)
+					upd_local_kpz_atomic(W->lat, W->lx, W->ly, x, y);
+				else
+					upd_local(W->lat, W->lx, W->ly, x, y, KOR_UPD_KPZ, 0, &W->th);
 			}
 		}
 		W->states[t] = s;

@@ -507,3 +507,16 @@ def test_threaded_cb_equals_sequential(lx, ly, T, nthreads):
     O.cb_mcs_threaded(b, lx, ly, s2, 3, 0.8, 0.15, nthreads)
     assert np.array_equal(a, b) and np.array_equal(s1.states, s2.states)
     assert not np.array_equal(a, O.flat(lx, ly, O.ENC_LOCAL))
+
+
+def test_threaded_db_off_the_word_grid_equals_sequential():
+    # lorest = 1: block origins on even sites, so neighbouring blocks of two
+    # threads share lattice words; repeated to give a lost update a chance
+    lx, ly, lbx, lby, T = 9, 9, 3, 3, 64
+    for rep in range(20):
+        m1, m2 = O.Dsfmt(rep), O.Dsfmt(rep)
+        s1, s2 = O.Streams(T, m1), O.Streams(T, m2)
+        a, b = O.flat(lx, ly, 0), O.flat(lx, ly, 0)
+        O.db_mcs(a, lx, ly, lbx, lby, s1, m1, 2, 1, 1)
+        O.db_mcs(b, lx, ly, lbx, lby, s2, m2, 2, 1, 8)
+        assert np.array_equal(a, b) and np.array_equal(s1.states, s2.states)
