@@ -1610,13 +1610,14 @@ __device__ __forceinline__ uint32_t yslope(const uint32_t* lat, int lx, int ly, 
 // +-1, and every loop closes, sX(x,y) + sY(x-1,y) = sY(x,y) + sX(x,y-1).
 // Lane (band, j) takes the CK-site chunk j of the rows of one band of CB rows
 // (chunks fastest, NC chunks per row): rowOnes[y] += ones of the row's x
-// slopes (summed over the row's lanes by shuffles), colOnes[x] += ones of the
-// column's y slopes in the band (bit-sliced counters), badLoops += loops that
-// do not close.  Every lane runs every shuffle; lanes past the end carry zeros.
+// slopes (summed over the row's lanes by shuffles), part[band][x] = ones of
+// column x's y slopes in the band (bit-sliced counters, one 16-byte-store
+// run per lane; column_sum_kernel adds the bands), badLoops += loops that do
+// not close.  Every lane runs every shuffle; lanes past the end carry zeros.
 constexpr int CK_BAND = 256;
 template<int ENC>
 __global__ __launch_bounds__(256) void consistency_kernel(const uint32_t* __restrict__ lat, int lx, int ly,
-		int* __restrict__ rowOnes, int* __restrict__ colOnes, unsigned long long* __restrict__ badLoops)
+		int* __restrict__ rowOnes, uint16_t* __restrict__ part, unsigned long long* __restrict__ badLoops)
 {
 	const int L = 1 << lx, Y = 1 << ly;
 	const int CK = L < 64 ? L : 64;               // sites per chunk
@@ -1662,20 +1663,45 @@ __global__ __launch_bounds__(256) void consistency_kernel(const uint32_t* __rest
 			atomicAdd(rowOnes + y, ones);
 	}
 	if(live)
-		for(int i = 0; i < CK; ++i)
+	{
+		uint16_t* dst = part + ((size_t)band << lx) + (size_t)j * CK;
+		for(int i0 = 0; i0 < CK; i0 += 8)
 		{
-			int n = 0;
+			uint32_t n[8];
 #pragma unroll
-			for(int k = 0; k < 9; ++k)
-				n |= (int)((c[k] >> i) & 1) << k;
-			if(n)
-				atomicAdd(colOnes + j * CK + i, n);
+			for(int u = 0; u < 8; ++u)
+			{
+				n[u] = 0;
+#pragma unroll
+				for(int k = 0; k < 9; ++k)
+					n[u] |= (uint32_t)((c[k] >> (i0 + u)) & 1) << k;
+			}
+			if(CK >= 8)
+				*reinterpret_cast<uint4*>(dst + i0) =
+					make_uint4(n[0] | (n[1] << 16), n[2] | (n[3] << 16), n[4] | (n[5] << 16), n[6] | (n[7] << 16));
+			else
+				for(int u = 0; u < CK; ++u)
+					dst[u] = (uint16_t)n[u];
 		}
+	}
 	// one atomic per wave for the loops
 	for(int o = 32; o > 0; o >>= 1)
 		bad += __shfl_xor(bad, o, 64);
 	if((threadIdx.x & 63) == 0 && bad)
 		atomicAdd(badLoops, bad);
+}
+
+// colOnes[x] = sum over the bands of part[band][x]
+__global__ __launch_bounds__(256) void column_sum_kernel(const uint16_t* __restrict__ part, int nbands, int lx,
+		int* __restrict__ colOnes)
+{
+	const int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if(x >= (1ll << lx))
+		return;
+	int n = 0;
+	for(int b = 0; b < nbands; ++b)
+		n += part[((size_t)b << lx) + (size_t)x];
+	colOnes[x] = n;
 }
 
 // local prefix statistics of one chunk: q = sum d, sq = sum Q_i, sq2 = sum Q_i^2
@@ -2713,11 +2739,17 @@ hipError_t launch_row_moments(const uint32_t* lat, int enc, int lx, int ly, cons
 	return hipGetLastError();
 }
 
+int64_t consistency_part_words(int lx, int ly)
+{
+	const int64_t Y = 1ll << ly;
+	return (Y < CK_BAND ? 1 : Y / CK_BAND) << lx;
+}
+
 hipError_t launch_consistency(const uint32_t* lat, int enc, int lx, int ly, int* rowOnes, int* colOnes,
-		unsigned long long* badLoops, hipStream_t stream)
+		uint16_t* part, unsigned long long* badLoops, hipStream_t stream)
 {
 	if((enc == ENC_LOCAL ? (lx < 2 || ly < 2) : (lx < 6 || ly < 1)) || lx > 30 || ly > 30 || !rowOnes || !colOnes
-	   || !badLoops)
+	   || !part || !badLoops)
 		return hipErrorInvalidValue;
 	const int L = 1 << lx, Y = 1 << ly;
 	const int CK = L < 64 ? L : 64;
@@ -2725,11 +2757,13 @@ hipError_t launch_consistency(const uint32_t* lat, int enc, int lx, int ly, int*
 	const int64_t lanes = (int64_t)(L / CK) * (Y / CB);
 	const dim3 grid((unsigned)((lanes + 255) / 256));
 	if(enc == ENC_LOCAL)
-		hipLaunchKernelGGL((consistency_kernel<ENC_LOCAL>), grid, dim3(256), 0, stream, lat, lx, ly, rowOnes, colOnes,
+		hipLaunchKernelGGL((consistency_kernel<ENC_LOCAL>), grid, dim3(256), 0, stream, lat, lx, ly, rowOnes, part,
 		                   badLoops);
 	else
-		hipLaunchKernelGGL((consistency_kernel<ENC_CBBIT>), grid, dim3(256), 0, stream, lat, lx, ly, rowOnes, colOnes,
+		hipLaunchKernelGGL((consistency_kernel<ENC_CBBIT>), grid, dim3(256), 0, stream, lat, lx, ly, rowOnes, part,
 		                   badLoops);
+	hipLaunchKernelGGL(column_sum_kernel, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, stream, part, Y / CB, lx,
+	                   colOnes);
 	return hipGetLastError();
 }
 

@@ -143,10 +143,12 @@ constexpr int ROW_STATS = 10;
 hipError_t launch_row_moments(const uint32_t* lat, int enc, int lx, int ly, const int64_t* hs, double* out,
 		hipStream_t stream);
 // consistency of a lattice (SchedulerService::consitencyCheck): rowOnes[y] +=
-// ones among row y's x slopes, colOnes[x] += ones among column x's y slopes,
-// badLoops += loops that do not close (counters cleared by the caller)
+// ones among row y's x slopes, colOnes[x] = ones among column x's y slopes,
+// badLoops += loops that do not close (rowOnes and badLoops cleared by the
+// caller); part: consistency_part_words(lx, ly) u16 of per-band column counts
+int64_t consistency_part_words(int lx, int ly);
 hipError_t launch_consistency(const uint32_t* lat, int enc, int lx, int ly, int* rowOnes, int* colOnes,
-		unsigned long long* badLoops, hipStream_t stream);
+		uint16_t* part, unsigned long long* badLoops, hipStream_t stream);
 hipError_t launch_row_stats(const uint32_t* lat, const uint32_t* snap, int enc, int lx, int ly,
 		int64_t* out, hipStream_t stream);
 // exact totals from the row statistics: res[7] = {sum h, sum h^2 lo, hi,

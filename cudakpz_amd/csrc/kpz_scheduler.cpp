@@ -338,13 +338,16 @@ void SchedulerService::consistencyOf(const uint32_t* lat, int64_t out[3])
 {
 	const size_t nx = (size_t)1 << size_.lx, ny = (size_t)1 << size_.ly;
 	const size_t bytes = (ny + nx) * sizeof(int) + sizeof(unsigned long long);
-	if(check_.bytes() < bytes)
-		check_.alloc(bytes);
+	const size_t partOff = (bytes + 255) & ~(size_t)255; // 16-byte stores into the per-band counts
+	const size_t partBytes = (size_t)consistency_part_words(size_.lx, size_.ly) * sizeof(uint16_t);
+	if(check_.bytes() < partOff + partBytes)
+		check_.alloc(partOff + partBytes);
 	int* rowOnes = check_.as<int>();
 	int* colOnes = rowOnes + ny;
 	unsigned long long* bad = (unsigned long long*)(colOnes + nx);
+	uint16_t* part = (uint16_t*)(check_.as<char>() + partOff);
 	check(hipMemsetAsync(check_.get(), 0, bytes, stream_), "consistency clear");
-	check(launch_consistency(lat, enc_, size_.lx, size_.ly, rowOnes, colOnes, bad, stream_), "consistency");
+	check(launch_consistency(lat, enc_, size_.lx, size_.ly, rowOnes, colOnes, part, bad, stream_), "consistency");
 	std::vector<int> h(ny + nx + 2);
 	check(hipMemcpyAsync(h.data(), check_.get(), bytes, hipMemcpyDeviceToHost, stream_), "consistency D2H");
 	sync();
