@@ -183,6 +183,85 @@ uint32_t kor_slcg_draw(uint64_t* s, uint64_t a, uint64_t c)
 	return (uint32_t)(*s ^ (*s >> 32));
 }
 
+/* Statistics of n folded SLCG64 draws, as the reference's profiling program
+ * inc/slcgProfile.cpp:41-60 prints them: the base generator (A, C) is
+ * stepped n times from `state`; of every draw, bits[b] counts bit b of the
+ * fold random() = lo ^ hi, j[] is the 12-bucket histogram of
+ * (random() >> 24) / 256 * 12 and j2[] the one of randomFloat() * 12, with
+ * randomFloat() = (random() & 0x7FFFFF) / 2^23 (kpz/kpzRandom.cu:62-68; float
+ * arithmetic throughout, as there).  random() and randomFloat() are this
+ * file's own stream cursor functions (gen_random / gen_rf) on two copies of
+ * the stream, so the counts check exactly what the schedulers draw.  The n
+ * draws are split over nthreads chunks, each started by kor_slcg_jump. */
+static uint32_t gen_random_slcg(uint64_t* s, uint64_t a, uint64_t c);
+static uint32_t gen_rf_slcg(uint64_t* s, uint64_t a, uint64_t c);
+
+typedef struct {
+	uint64_t state, n;
+	uint64_t byte[4][256];
+	uint64_t j[12], j2[12];
+} prof_work;
+
+static void* prof_run(void* arg)
+{
+	prof_work* W = (prof_work*)arg;
+	uint64_t sa = W->state, sb = W->state;
+	for(uint64_t i = 0; i < W->n; ++i)
+	{
+		const uint32_t t = gen_random_slcg(&sa, KOR_SLCG_A, KOR_SLCG_C);
+		const uint32_t r = gen_rf_slcg(&sb, KOR_SLCG_A, KOR_SLCG_C);
+		++W->byte[0][t & 0xFF];
+		++W->byte[1][(t >> 8) & 0xFF];
+		++W->byte[2][(t >> 16) & 0xFF];
+		++W->byte[3][t >> 24];
+		const float f = (float)(t >> 24) / 256.0f * 12.0f;
+		++W->j[(int)f];
+		const float f2 = (float)r / 8388608.0f * 12.0f;
+		++W->j2[(int)f2];
+	}
+	return NULL;
+}
+
+#include <pthread.h>
+
+void kor_slcg_profile(uint64_t state, uint64_t n, int nthreads, uint64_t bits[32], uint64_t j[12], uint64_t j2[12])
+{
+	if(nthreads < 1)
+		nthreads = 1;
+	prof_work* wk = (prof_work*)calloc((size_t)nthreads, sizeof(prof_work));
+	pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+	uint64_t start = 0;
+	for(int w = 0; w < nthreads; ++w)
+	{
+		const uint64_t cnt = n / (uint64_t)nthreads + ((uint64_t)w < n % (uint64_t)nthreads ? 1 : 0);
+		uint64_t ja, jc;
+		kor_slcg_jump(KOR_SLCG_A, KOR_SLCG_C, start, &ja, &jc);
+		wk[w].state = ja * state + jc;
+		wk[w].n = cnt;
+		start += cnt;
+		pthread_create(&th[w], NULL, prof_run, &wk[w]);
+	}
+	memset(bits, 0, 32 * sizeof(uint64_t));
+	memset(j, 0, 12 * sizeof(uint64_t));
+	memset(j2, 0, 12 * sizeof(uint64_t));
+	for(int w = 0; w < nthreads; ++w)
+	{
+		pthread_join(th[w], NULL);
+		for(int k = 0; k < 4; ++k)
+			for(int v = 0; v < 256; ++v)
+				for(int b = 0; b < 8; ++b)
+					if(v >> b & 1)
+						bits[8 * k + b] += wk[w].byte[k][v];
+		for(int k = 0; k < 12; ++k)
+		{
+			j[k] += wk[w].j[k];
+			j2[k] += wk[w].j2[k];
+		}
+	}
+	free(th);
+	free(wk);
+}
+
 /* ======================================================================
  * TinyMT32 (Saito & Matsumoto 2011, the 127-bit F2-linear generator vendored
  * at inc/tinyMT/tinymt32.h:70-107 (state transition, tempering) and
@@ -297,6 +376,31 @@ static inline void gen_put(const streams_t* S, int64_t t, const gen_t* g)
 static inline uint32_t gen_random(gen_t* g) { return g->tmt ? kor_tmt_u32(&g->g) : kor_slcg_draw(&g->s, g->a, g->c); }
 static inline int gen_bits(int tmt) { return tmt ? 24 : 23; }
 static inline uint32_t gen_rf(gen_t* g) { return g->tmt ? kor_tmt_u32(&g->g) >> 8 : kor_slcg_draw(&g->s, g->a, g->c) & 0x7FFFFFu; }
+
+/* the SLCG64 cursor functions above on a bare state (kor_slcg_profile) */
+static uint32_t gen_random_slcg(uint64_t* s, uint64_t a, uint64_t c)
+{
+	gen_t g;
+	g.tmt = 0;
+	g.s = *s;
+	g.a = a;
+	g.c = c;
+	const uint32_t v = gen_random(&g);
+	*s = g.s;
+	return v;
+}
+
+static uint32_t gen_rf_slcg(uint64_t* s, uint64_t a, uint64_t c)
+{
+	gen_t g;
+	g.tmt = 0;
+	g.s = *s;
+	g.a = a;
+	g.c = c;
+	const uint32_t v = gen_rf(&g);
+	*s = g.s;
+	return v;
+}
 
 static streams_t streams_slcg(uint64_t* states, uint64_t a, uint64_t c)
 {

@@ -18,6 +18,10 @@
  *   kor_slcg_skip        known-answer constants of inc/helper/slcgInst.h and
  *                        the reference's own SLCGskipA/SLCGskipC
  *   kor_slcg_jump        closed form vs. step-by-step iteration
+ *   kor_slcg_draw fold,  bit counts and histograms of 2^30 draws printed by
+ *   23-bit randomFloat   the reference's own inc/slcgProfile.cpp, built
+ *                        unchanged into oracle/_ref
+ *                        (tests/golden/slcg_profile.txt)
  *   update rules         brute-force explicit-height octahedron simulator
  *   half-sweeps          per-site scalar equivalence, order independence
  *   heights/roughness    closed forms (flat IC), brute force on tiny lattices
@@ -28,6 +32,14 @@
  *                        invariants (consistency, height-change accounting);
  *                        the TinyMT32 variants share every line with the
  *                        SLCG64 ones except the stream cursor
+ *
+ * Parity unpinned (no reference-built code, fixture or closed form reaches
+ * them in this image; each restates its cited lines and is tested only
+ * against that reading):
+ *   kor_streams_randomize  kpz/GPURandom.cu:42-54 (CUDA source)
+ *   kor_reseed's 100000    kpz/schedulerService.cpp:460-470 (needs libSplash)
+ *   discarded draws
+ *   kor_generate_disorder2 kpz/schedulerService.cpp:54-69 (needs libSplash)
  */
 #ifndef KPZ_ORACLE_H
 #define KPZ_ORACLE_H
@@ -65,6 +77,9 @@ void kor_slcg_jump(uint64_t a, uint64_t c, uint64_t k, uint64_t* ak, uint64_t* c
 uint64_t kor_default_skip(uint64_t T);
 void kor_streams_randomize(kor_dsfmt* master, uint64_t* states, int64_t T);
 uint32_t kor_slcg_draw(uint64_t* s, uint64_t a, uint64_t c);
+/* bit counts and histograms of n folded base-generator draws from `state`,
+ * as inc/slcgProfile.cpp:41-60 prints them */
+void kor_slcg_profile(uint64_t state, uint64_t n, int nthreads, uint64_t bits[32], uint64_t j[12], uint64_t j2[12]);
 
 /* ---------------- TinyMT32 ---------------- */
 /* tinymt32_t field order (inc/tinyMT/tinymt32.h:80-85) */

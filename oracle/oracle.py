@@ -58,6 +58,7 @@ def _declare(L):
     L.kor_default_skip.restype = C.c_uint64
     L.kor_streams_randomize.argtypes = [vp, _u64p, C.c_int64]
     L.kor_slcg_draw.argtypes = [C.POINTER(C.c_uint64), C.c_uint64, C.c_uint64]
+    L.kor_slcg_profile.argtypes = [C.c_uint64, C.c_uint64, C.c_int, _u64p, _u64p, _u64p]
     L.kor_slcg_draw.restype = C.c_uint32
     for f in ("kor_init_flat_local", "kor_init_flat_cbbit"):
         getattr(L, f).argtypes = [_u32p, C.c_int, C.c_int]
@@ -173,6 +174,16 @@ def slcg_jump(a, c, k):
     ak, ck = C.c_uint64(), C.c_uint64()
     lib().kor_slcg_jump(a, c, k, C.byref(ak), C.byref(ck))
     return ak.value, ck.value
+
+
+def slcg_profile(state, n, nthreads=8):
+    """bit counts and the two 12-bucket histograms of n folded draws of the
+    base SLCG64 from `state` (inc/slcgProfile.cpp:41-60)"""
+    bits = np.zeros(32, np.uint64)
+    j = np.zeros(12, np.uint64)
+    j2 = np.zeros(12, np.uint64)
+    lib().kor_slcg_profile(state, n, nthreads, bits, j, j2)
+    return bits, j, j2
 
 
 def default_skip(T):
