@@ -8,6 +8,7 @@
 #include "kpz_oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -553,6 +554,136 @@ void kor_correlate(const uint32_t* sys, const uint32_t* snap, int enc, int lx, i
 	out[4] = (int64_t)(uint64_t)s11;
 	out[5] = (int64_t)(s11 >> 64);
 	out[6] = tcs;
+}
+
+/* The same sums as kor_roughness / kor_correlate, row by row on nthreads
+ * threads, with no lattice-sized height array (a 65536^2 lattice would need
+ * 16 GiB per height field).  Row y is walked exactly as kor_heights walks it,
+ * starting from 0 instead of the height carried in from row y-1: h_rel(x,y) =
+ * sY(0,y) + sum_{x'=1..x} sX(x',y) for x >= 1 and h_rel(0,y) = that plus
+ * sX(0,y) =: D(y), the row's carry.  With base(y) = sum_{y'<y} D(y') the true
+ * height is base(y) + h_rel(x,y), so per row only sum h_rel, sum h_rel^2
+ * (and against the snapshot sum g_rel, sum h_rel g_rel, the slope overlap)
+ * are kept; the host combines rows in 128-bit arithmetic. */
+typedef struct {
+	const uint32_t *sys, *snap;
+	int enc, lx, ly, y0, y1;
+	int64_t *D, *H1, *H2, *DG, *G1, *HG, *CS;
+} rows_work;
+
+static void* rows_run(void* arg)
+{
+	rows_work* W = (rows_work*)arg;
+	const int X = 1 << W->lx;
+	for(int y = W->y0; y < W->y1; ++y)
+	{
+		int64_t h = kor_get_slope(W->sys, W->enc, W->lx, W->ly, 0, y, 1) ? 1 : -1;
+		int64_t g = 0, h1 = 0, h2 = 0, g1 = 0, hg = 0, cs = 0;
+		if(W->snap)
+			g = kor_get_slope(W->snap, W->enc, W->lx, W->ly, 0, y, 1) ? 1 : -1;
+		for(int x = 1; x <= X; ++x)
+		{
+			const int xs = x & (X - 1); /* x = X adds the slope at x = 0 last */
+			h += kor_get_slope(W->sys, W->enc, W->lx, W->ly, xs, y, 0) ? 1 : -1;
+			h1 += h;
+			h2 += h * h;
+			if(W->snap)
+			{
+				g += kor_get_slope(W->snap, W->enc, W->lx, W->ly, xs, y, 0) ? 1 : -1;
+				g1 += g;
+				hg += h * g;
+			}
+		}
+		W->D[y] = h;
+		W->H1[y] = h1;
+		W->H2[y] = h2;
+		if(W->snap)
+		{
+			for(int x = 0; x < X; ++x)
+				for(int d = 0; d < 2; ++d)
+					cs += (kor_get_slope(W->sys, W->enc, W->lx, W->ly, x, y, d)
+						== kor_get_slope(W->snap, W->enc, W->lx, W->ly, x, y, d)) ? 1 : -1;
+			W->DG[y] = g;
+			W->G1[y] = g1;
+			W->HG[y] = hg;
+			W->CS[y] = cs;
+		}
+	}
+	return NULL;
+}
+
+static void rows_sums(const uint32_t* sys, const uint32_t* snap, int enc, int lx, int ly, int nthreads, int64_t out[7])
+{
+	const int Y = 1 << ly;
+	const int64_t X = 1ll << lx;
+	if(nthreads < 1)
+		nthreads = 1;
+	if(nthreads > Y)
+		nthreads = Y;
+	int64_t* buf = (int64_t*)calloc((size_t)Y * 7, sizeof(int64_t));
+	rows_work* wk = (rows_work*)malloc(sizeof(rows_work) * (size_t)nthreads);
+	pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+	for(int w = 0; w < nthreads; ++w)
+	{
+		rows_work* W = &wk[w];
+		W->sys = sys;
+		W->snap = snap;
+		W->enc = enc;
+		W->lx = lx;
+		W->ly = ly;
+		W->y0 = (int)((int64_t)Y * w / nthreads);
+		W->y1 = (int)((int64_t)Y * (w + 1) / nthreads);
+		W->D = buf;
+		W->H1 = buf + Y;
+		W->H2 = buf + 2 * (size_t)Y;
+		W->DG = buf + 3 * (size_t)Y;
+		W->G1 = buf + 4 * (size_t)Y;
+		W->HG = buf + 5 * (size_t)Y;
+		W->CS = buf + 6 * (size_t)Y;
+		pthread_create(&th[w], NULL, rows_run, W);
+	}
+	for(int w = 0; w < nthreads; ++w)
+		pthread_join(th[w], NULL);
+	const rows_work* R = &wk[0];
+	int64_t bh = 0, bg = 0, s1 = 0, t1 = 0, tcs = 0;
+	__int128 s2 = 0, s11 = 0;
+	for(int y = 0; y < Y; ++y)
+	{
+		s1 += X * bh + R->H1[y];
+		s2 += (__int128)X * bh * bh + (__int128)2 * bh * R->H1[y] + R->H2[y];
+		if(snap)
+		{
+			t1 += X * bg + R->G1[y];
+			s11 += (__int128)X * bh * bg + (__int128)bh * R->G1[y] + (__int128)bg * R->H1[y] + R->HG[y];
+			tcs += R->CS[y];
+			bg += R->DG[y];
+		}
+		bh += R->D[y];
+	}
+	free(th);
+	free(wk);
+	free(buf);
+	out[0] = s1;
+	out[1] = t1;
+	out[2] = (int64_t)(uint64_t)s2;
+	out[3] = (int64_t)(s2 >> 64);
+	out[4] = (int64_t)(uint64_t)s11;
+	out[5] = (int64_t)(s11 >> 64);
+	out[6] = tcs;
+}
+
+void kor_roughness_rows(const uint32_t* lat, int enc, int lx, int ly, int nthreads, int64_t out[3])
+{
+	int64_t o[7];
+	rows_sums(lat, NULL, enc, lx, ly, nthreads, o);
+	out[0] = o[0];
+	out[1] = o[2];
+	out[2] = o[3];
+}
+
+void kor_correlate_rows(const uint32_t* sys, const uint32_t* snap, int enc, int lx, int ly, int nthreads, int64_t out[7])
+{
+	rows_sums(sys, snap, enc, lx, ly, nthreads, out);
 }
 
 /* SchedulerService::consitencyCheck, kpz/schedulerService.cpp:721-854:

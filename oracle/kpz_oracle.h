@@ -108,6 +108,10 @@ void kor_heights(const uint32_t* lat, int enc, int lx, int ly, int32_t* h);
 void kor_roughness(const uint32_t* lat, int enc, int lx, int ly, int64_t out[3]);
 /* out = {sum h_sys, sum h_snap, sum h_sys^2 lo, hi, sum h_sys*h_snap lo, hi, tcs} */
 void kor_correlate(const uint32_t* sys, const uint32_t* snap, int enc, int lx, int ly, int64_t out[7]);
+/* the same sums row by row on nthreads threads, without lattice-sized height
+ * arrays (any lattice size) */
+void kor_roughness_rows(const uint32_t* lat, int enc, int lx, int ly, int nthreads, int64_t out[3]);
+void kor_correlate_rows(const uint32_t* sys, const uint32_t* snap, int enc, int lx, int ly, int nthreads, int64_t out[7]);
 /* 1 if every row/column slope sum is zero and every plaquette closes */
 int kor_consistent(const uint32_t* lat, int enc, int lx, int ly);
 

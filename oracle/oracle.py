@@ -68,6 +68,8 @@ def _declare(L):
     L.kor_heights.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, _i32p]
     L.kor_roughness.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, _i64p]
     L.kor_correlate.argtypes = [_u32p, _u32p, C.c_int, C.c_int, C.c_int, _i64p]
+    L.kor_roughness_rows.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, _i64p]
+    L.kor_correlate_rows.argtypes = [_u32p, _u32p, C.c_int, C.c_int, C.c_int, C.c_int, _i64p]
     L.kor_consistent.argtypes = [_u32p, C.c_int, C.c_int, C.c_int]
     L.kor_dt_mcs.argtypes = [_u32p, C.c_int, C.c_int, C.c_int, C.c_int, _u64p, C.c_int64,
                              C.c_uint64, C.c_uint64, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
@@ -318,6 +320,20 @@ def correlate(sys, snap, enc, lx, ly):
     """(sum h, sum h_snap, sum h^2, sum h*h_snap, tcs) as exact ints."""
     out = np.zeros(7, np.int64)
     lib().kor_correlate(np.ascontiguousarray(sys), np.ascontiguousarray(snap), enc, lx, ly, out)
+    return int(out[0]), int(out[1]), _i128(out[2], out[3]), _i128(out[4], out[5]), int(out[6])
+
+
+def roughness_rows(lat, enc, lx, ly, nthreads=1):
+    """roughness() row by row on nthreads threads, no height array"""
+    out = np.zeros(3, np.int64)
+    lib().kor_roughness_rows(np.ascontiguousarray(lat), enc, lx, ly, nthreads, out)
+    return int(out[0]), _i128(out[1], out[2])
+
+
+def correlate_rows(sys, snap, enc, lx, ly, nthreads=1):
+    """correlate() row by row on nthreads threads, no height arrays"""
+    out = np.zeros(7, np.int64)
+    lib().kor_correlate_rows(np.ascontiguousarray(sys), np.ascontiguousarray(snap), enc, lx, ly, nthreads, out)
     return int(out[0]), int(out[1]), _i128(out[2], out[3]), _i128(out[4], out[5]), int(out[6])
 
 

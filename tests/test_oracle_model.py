@@ -130,6 +130,25 @@ def test_correlate_brute_force():
     assert O.correlate(ca, cb, 1, lx, ly) == (s1, t1, s2, s11, tcs)
 
 
+@pytest.mark.parametrize("lx,ly,nthreads", [(7, 7, 1), (8, 6, 3), (6, 9, 8), (9, 8, 5)])
+def test_row_streamed_reductions_equal_height_field_sums(lx, ly, nthreads):
+    # the row-by-row threaded sums (used where a height field would not fit in
+    # memory) against the explicit-height ones, on evolved lattices of both
+    # encodings and on random words (inconsistent lattices: the row carry
+    # then differs from the y slope, and both walks must still agree)
+    a = evolved_local(lx, ly, 5, 3)
+    b = evolved_local(lx, ly, 6, 5)
+    r = np.random.default_rng(lx * 100 + ly)
+    ra = r.integers(0, 1 << 32, a.size, dtype=np.uint64).astype(np.uint32)
+    rb = r.integers(0, 1 << 32, a.size, dtype=np.uint64).astype(np.uint32)
+    for x, y in [(a, b), (ra, rb), (a, a)]:
+        for enc in ENCS:
+            u = x if enc == O.ENC_LOCAL else O.local_to_cbbit(x, lx, ly)
+            v = y if enc == O.ENC_LOCAL else O.local_to_cbbit(y, lx, ly)
+            assert O.roughness_rows(u, enc, lx, ly, nthreads) == O.roughness(u, enc, lx, ly)
+            assert O.correlate_rows(u, v, enc, lx, ly, nthreads) == O.correlate(u, v, enc, lx, ly)
+
+
 def test_inconsistent_lattice_detected():
     lx, ly = 6, 6
     lat = O.flat(lx, ly, 0)
