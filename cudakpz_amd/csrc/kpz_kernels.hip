@@ -51,17 +51,59 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src, int width)
 	const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, width);
 	return ((uint64_t)hi << 32) | lo;
 }
-// a[23:0] * b[23:0] + c, low 32 bits, in one VALU instruction (left to
-// itself the compiler widens a product used mod 2^23 to v_mul_lo_u32 + add)
-__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c)
+
+// The high state word of an SLCG64 step mod 2^24: h aLo + lo aHi + carry, as
+// two v_mad_u32_u24 (a[23:0] * b[23:0] + c) in one asm block.  Left to itself
+// the compiler widens products used mod 2^23 to v_mul_lo_u32 + v_add3_u32;
+// one block instead of two halves the wait states it pads asm blocks with.
+__device__ __forceinline__ uint32_t hi24_step(uint32_t h, uint32_t aLo, uint32_t lo, uint32_t aHi, uint32_t carry)
 {
 	uint32_t r;
-	asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+	asm("v_mad_u32_u24 %0, %3, %4, %5\n\tv_mad_u32_u24 %0, %1, %2, %0"
+	    : "=&v"(r) : "v"(h), "v"(aLo), "v"(lo), "v"(aHi), "v"(carry));
 	return r;
 }
 
 __device__ __forceinline__ uint32_t shflw(uint32_t v, int src, int width) { return (uint32_t)__shfl((int)v, src, width); }
 __device__ __forceinline__ uint64_t shflw(uint64_t v, int src, int width) { return shfl64(v, src, width); }
+
+// Acceptance bits of 32 consecutive draws, draw k -> bit k.  The difference
+// d = r - t of a draw's r < 2^23 (or 2^24) and a threshold t <= 2^23 (2^24)
+// has bits 24..31 all equal to its sign, which is the acceptance r < t.  One
+// v_bitop3 (a | (d & m)) copies that sign into any single bit m of those:
+// draws 0-7, 8-15 and 16-23 go to bits 24-31 of three accumulators, shifted
+// down once at the end, draws 24-31 straight to their own bits of a fourth.
+// Three fast-class instructions per draw and threshold fewer slow ones than
+// funnel-shifting the sign in (v_alignbit) and reversing the word.
+struct SignBits32 {
+	uint32_t a = 0, b = 0, c = 0, d = 0;
+	template<int K>
+	__device__ __forceinline__ void put(uint32_t diff)
+	{
+		static_assert(K >= 0 && K < 32, "draw index");
+		constexpr uint32_t m = 1u << (24 + (K & 7));
+		if constexpr(K < 8)
+			a = __builtin_amdgcn_bitop3_b32(a, diff, m, 0xF8);
+		else if constexpr(K < 16)
+			b = __builtin_amdgcn_bitop3_b32(b, diff, m, 0xF8);
+		else if constexpr(K < 24)
+			c = __builtin_amdgcn_bitop3_b32(c, diff, m, 0xF8);
+		else
+			d = __builtin_amdgcn_bitop3_b32(d, diff, m, 0xF8);
+	}
+	__device__ __forceinline__ uint32_t bits() const { return (a >> 24) | (b >> 16) | (c >> 8) | d; }
+};
+
+// compile-time loop: f(integral_constant<int, K>) for K = 0..N-1
+template<int N, int K = 0, class F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+	if constexpr(K < N)
+	{
+		f(std::integral_constant<int, K>{});
+		static_for<N, K + 1>(f);
+	}
+}
 
 // ======================================================================
 // DT phase kernel
@@ -339,7 +381,7 @@ struct DtSlcg {
 		if constexpr(RO)
 		{
 			const uint64_t pr = (uint64_t)lo * aLo + step.c;
-			h24 = mad_u24(h24, aLo, mad_u24(lo, aHi, (uint32_t)(pr >> 32)));
+			h24 = hi24_step(h24, aLo, lo, aHi, (uint32_t)(pr >> 32));
 			lo = (uint32_t)pr;
 			return lo ^ h24; // bits 0..22 are those of the full fold
 		}
@@ -763,61 +805,47 @@ __device__ __forceinline__ void cbbit_row(const CbHalfArgs& a, int y, int g)
 			const uint32_t aLo = (uint32_t)step.a, aHi = (uint32_t)(step.a >> 32);
 			const uint64_t cc = step.c;
 			uint32_t lo = (uint32_t)sW, h24 = (uint32_t)(sW >> 32);
-			// acceptance bits, site k of the word -> bit k: the sign bit of
-			// r23 - t is funnel-shifted into an accumulator (v_alignbit), 32
-			// sites per accumulator, which ends bit-reversed
+			// acceptance bits, site k of the word -> bit k (SignBits32: the sign of
+			// r23 - t of every draw copied into its own bit)
 			W pb = 0, qb = 0, pb1 = 0, qb1 = 0;
+			const uint32_t tp1 = a.tp1, tq1 = a.tq1;
+			// NT = thresholds per draw: 4 (two disorder classes), 2 (p and q), 1 (q = 0)
+			auto half = [&](auto ntc, int h) {
+				constexpr int NT = decltype(ntc)::value;
+				SignBits32 P, Q, P1, Q1;
+				static_for<32>([&](auto kc) {
+					constexpr int k = decltype(kc)::value;
+					const uint64_t pr = (uint64_t)lo * aLo + cc;
+					h24 = hi24_step(h24, aLo, lo, aHi, (uint32_t)(pr >> 32));
+					lo = (uint32_t)pr;
+					const uint32_t r23 = (lo ^ h24) & 0x7FFFFFu;
+					P.template put<k>(r23 - tp);
+					if constexpr(NT >= 2)
+						Q.template put<k>(r23 - tq);
+					if constexpr(NT == 4)
+					{
+						// both classes' acceptance bits; the map selects per site below
+						P1.template put<k>(r23 - tp1);
+						Q1.template put<k>(r23 - tq1);
+					}
+				});
+				pb |= (W)P.bits() << (32 * h);
+				qb |= (W)Q.bits() << (32 * h);
+				if constexpr(NT == 4)
+				{
+					pb1 |= (W)P1.bits() << (32 * h);
+					qb1 |= (W)Q1.bits() << (32 * h);
+				}
+			};
 #pragma unroll
 			for(int h = 0; h < WB / 32; ++h)
 			{
-				uint32_t pa = 0, qa = 0;
-				if(MODE == 2)
-				{
-					// both classes' acceptance bits; the map selects per site below
-					uint32_t pa1 = 0, qa1 = 0;
-					const uint32_t tp1 = a.tp1, tq1 = a.tq1;
-#pragma unroll
-					for(int k = 0; k < 32; ++k)
-					{
-						const uint64_t pr = (uint64_t)lo * aLo + cc;
-						h24 = mad_u24(h24, aLo, mad_u24(lo, aHi, (uint32_t)(pr >> 32)));
-						lo = (uint32_t)pr;
-						const uint32_t r23 = (lo ^ h24) & 0x7FFFFFu;
-						pa = __builtin_amdgcn_alignbit(pa, r23 - tp, 31);
-						qa = __builtin_amdgcn_alignbit(qa, r23 - tq, 31);
-						pa1 = __builtin_amdgcn_alignbit(pa1, r23 - tp1, 31);
-						qa1 = __builtin_amdgcn_alignbit(qa1, r23 - tq1, 31);
-					}
-					pb1 |= (W)__brev(pa1) << (32 * h);
-					qb1 |= (W)__brev(qa1) << (32 * h);
-				}
+				if constexpr(MODE == 2)
+					half(std::integral_constant<int, 4>{}, h);
 				else if(tq)
-				{
-#pragma unroll
-					for(int k = 0; k < 32; ++k)
-					{
-						const uint64_t pr = (uint64_t)lo * aLo + cc;
-						h24 = mad_u24(h24, aLo, mad_u24(lo, aHi, (uint32_t)(pr >> 32)));
-						lo = (uint32_t)pr;
-						const uint32_t r23 = (lo ^ h24) & 0x7FFFFFu;
-						pa = __builtin_amdgcn_alignbit(pa, r23 - tp, 31);
-						qa = __builtin_amdgcn_alignbit(qa, r23 - tq, 31);
-					}
-				}
+					half(std::integral_constant<int, 2>{}, h);
 				else
-				{
-#pragma unroll
-					for(int k = 0; k < 32; ++k)
-					{
-						const uint64_t pr = (uint64_t)lo * aLo + cc;
-						h24 = mad_u24(h24, aLo, mad_u24(lo, aHi, (uint32_t)(pr >> 32)));
-						lo = (uint32_t)pr;
-						const uint32_t r23 = (lo ^ h24) & 0x7FFFFFu;
-						pa = __builtin_amdgcn_alignbit(pa, r23 - tp, 31);
-					}
-				}
-				pb |= (W)__brev(pa) << (32 * h);
-				qb |= (W)__brev(qa) << (32 * h);
+					half(std::integral_constant<int, 1>{}, h);
 			}
 			if(MODE == 2)
 			{
@@ -895,13 +923,11 @@ __device__ __forceinline__ int cb_class(const CbLocalArgs& a, int v)
 // active sites of a cell in draw order (yy = 0..3, x = x0, x0 + 2 with
 // x0 = (yy & 1) ^ signum), X-bit positions 2x + 8yy:
 //   signum 0: 0 4 10 14 16 20 26 30      signum 1: 2 6 8 12 18 22 24 28
-// The acceptance bit of draw i is funnel-shifted in as K[i] copies of the
-// sign of r23 - t (bits 31..23 of that difference are all sign), so after a
-// bit reversal draw i covers positions [P[i], P[i+1]) and lands on its site.
+// The acceptance bit of draw i (the sign of r23 - t) lands on bit P[i]
+// (SignBits32::put<P[i]>).
 template<int S>
 struct CbCell {
 	static constexpr uint32_t MS = S ? 0x11441144u : 0x44114411u; // active X positions
-	static constexpr int K[8] = {S ? 6 : 4, S ? 2 : 6, 4, S ? 6 : 2, 4, S ? 2 : 6, 4, S ? 4 : 2};
 	static constexpr int P[8] = {S ? 2 : 0, S ? 6 : 4, S ? 8 : 10, S ? 12 : 14,
 	                             S ? 18 : 16, S ? 22 : 20, S ? 24 : 26, S ? 28 : 30};
 };
@@ -922,49 +948,40 @@ __device__ __forceinline__ void cb_cell(uint64_t sc, uint32_t aLo, uint32_t aHi,
                                         uint32_t& fUp, uint32_t dw, const uint32_t* tpc, const uint32_t* tqc)
 {
 	uint32_t lo = (uint32_t)sc, h24 = (uint32_t)(sc >> 32);
-	uint32_t ap = 0, aq = 0;
+	SignBits32 ap, aq;
+	// NT = thresholds per draw: 3 (the site's class picks them), 2 (p, q), 1 (q = 0)
+	auto draws = [&](auto ntc) {
+		constexpr int NT = decltype(ntc)::value;
+		static_for<8>([&](auto ic) {
+			constexpr int i = decltype(ic)::value;
+			constexpr int P = CbCell<S>::P[i];
+			const uint64_t pr = (uint64_t)lo * aLo + cc;
+			h24 = hi24_step(h24, aLo, lo, aHi, (uint32_t)(pr >> 32));
+			lo = (uint32_t)pr;
+			const uint32_t r23 = (lo ^ h24) & 0x7FFFFFu;
+			if constexpr(NT == 3)
+			{
+				const uint32_t cls = (dw >> P) & 3u;
+				const uint32_t tpv = (cls & 2) ? ((cls & 1) ? tpc[3] : tpc[2]) : ((cls & 1) ? tpc[1] : tpc[0]);
+				const uint32_t tqv = (cls & 2) ? ((cls & 1) ? tqc[3] : tqc[2]) : ((cls & 1) ? tqc[1] : tqc[0]);
+				ap.template put<P>(r23 - tpv);
+				aq.template put<P>(r23 - tqv);
+			}
+			else
+			{
+				ap.template put<P>(r23 - tp);
+				if constexpr(NT == 2)
+					aq.template put<P>(r23 - tq);
+			}
+		});
+	};
 	if(DIS)
-	{
-#pragma unroll
-		for(int i = 0; i < 8; ++i)
-		{
-			const uint64_t pr = (uint64_t)lo * aLo + cc;
-			h24 = mad_u24(h24, aLo, mad_u24(lo, aHi, (uint32_t)(pr >> 32)));
-			lo = (uint32_t)pr;
-			const uint32_t r23 = (lo ^ h24) & 0x7FFFFFu;
-			const uint32_t cls = (dw >> CbCell<S>::P[i]) & 3u;
-			const uint32_t tpv = (cls & 2) ? ((cls & 1) ? tpc[3] : tpc[2]) : ((cls & 1) ? tpc[1] : tpc[0]);
-			const uint32_t tqv = (cls & 2) ? ((cls & 1) ? tqc[3] : tqc[2]) : ((cls & 1) ? tqc[1] : tqc[0]);
-			ap = __builtin_amdgcn_alignbit(ap, r23 - tpv, 32 - CbCell<S>::K[i]);
-			aq = __builtin_amdgcn_alignbit(aq, r23 - tqv, 32 - CbCell<S>::K[i]);
-		}
-	}
+		draws(std::integral_constant<int, 3>{});
 	else if(tq)
-	{
-#pragma unroll
-		for(int i = 0; i < 8; ++i)
-		{
-			const uint64_t pr = (uint64_t)lo * aLo + cc;
-			h24 = mad_u24(h24, aLo, mad_u24(lo, aHi, (uint32_t)(pr >> 32)));
-			lo = (uint32_t)pr;
-			const uint32_t r23 = (lo ^ h24) & 0x7FFFFFu;
-			ap = __builtin_amdgcn_alignbit(ap, r23 - tp, 32 - CbCell<S>::K[i]);
-			aq = __builtin_amdgcn_alignbit(aq, r23 - tq, 32 - CbCell<S>::K[i]);
-		}
-	}
+		draws(std::integral_constant<int, 2>{});
 	else
-	{
-#pragma unroll
-		for(int i = 0; i < 8; ++i)
-		{
-			const uint64_t pr = (uint64_t)lo * aLo + cc;
-			h24 = mad_u24(h24, aLo, mad_u24(lo, aHi, (uint32_t)(pr >> 32)));
-			lo = (uint32_t)pr;
-			const uint32_t r23 = (lo ^ h24) & 0x7FFFFFu;
-			ap = __builtin_amdgcn_alignbit(ap, r23 - tp, 32 - CbCell<S>::K[i]);
-		}
-	}
-	cb_flips<S>(__brev(ap), __brev(aq), own, right, up, fOwn, fRight, fUp);
+		draws(std::integral_constant<int, 1>{});
+	cb_flips<S>(ap.bits(), aq.bits(), own, right, up, fOwn, fRight, fUp);
 }
 
 // the flips of one cell from its acceptance bits (bit 2x + 8y of ap / aq:
@@ -1117,7 +1134,7 @@ __global__ __launch_bounds__(256) void cb_local_band_kernel(CbLocalArgs a)
 			// and mod 2^24 in the high one: all cb_cell uses (the stream's final
 			// state comes from s0 above)
 			const uint64_t pr = (uint64_t)(uint32_t)s * rLoV + rowC;
-			const uint32_t h24 = mad_u24((uint32_t)(s >> 32), rLoV, mad_u24((uint32_t)s, rHiV, (uint32_t)(pr >> 32)));
+			const uint32_t h24 = hi24_step((uint32_t)(s >> 32), rLoV, (uint32_t)s, rHiV, (uint32_t)(pr >> 32));
 			s = ((uint64_t)h24 << 32) | (uint32_t)pr;
 		}
 	}
@@ -1407,27 +1424,26 @@ __global__ __launch_bounds__(256) void cb_local_tmt_gen_kernel(CbLocalArgs a, in
 		r = q / (w - wl);
 		ci = q - r * (w - wl);
 	}
-	// the acceptance word of cell io: its 8 draws in site order
-	// (CbCell<S>::P[i] = 2x + 8y of draw i), the sign of rf - t funnel-shifted
-	// in K[i] times, bit-reversed at the end
+	// the acceptance word of cell io: its 8 draws in site order, the sign of
+	// rf - t of draw i on bit CbCell<S>::P[i] = 2x + 8y (minimum), the bit
+	// above it for a maximum
 	auto cell = [&](size_t io) {
 		const uint32_t dw = DIS ? a.dis[io] : 0u;
-		uint32_t ap = 0, aq = 0;
-#pragma unroll
-		for(int i = 0; i < 8; ++i)
-		{
+		SignBits32 ap, aq;
+		static_for<8>([&](auto ic) {
+			constexpr int P = CbCell<S>::P[decltype(ic)::value];
 			const uint32_t rf = g.u32() >> 8;
 			uint32_t tp = a.tp, tq = a.tq;
 			if(DIS)
 			{
-				const uint32_t cls = (dw >> CbCell<S>::P[i]) & 3u;
+				const uint32_t cls = (dw >> P) & 3u;
 				tp = a.tpc[cls];
 				tq = a.tqc[cls];
 			}
-			ap = __builtin_amdgcn_alignbit(ap, rf - tp, 32 - CbCell<S>::K[i]);
-			aq = __builtin_amdgcn_alignbit(aq, rf - tq, 32 - CbCell<S>::K[i]);
-		}
-		return (__brev(ap) & CbCell<S>::MS) | ((__brev(aq) & CbCell<S>::MS) << 1);
+			ap.template put<P>(rf - tp);
+			aq.template put<P>(rf - tq);
+		});
+		return (ap.bits() & CbCell<S>::MS) | ((aq.bits() & CbCell<S>::MS) << 1);
 	};
 	// Sixteen cells of one half row at a time where they fill an aligned
 	// 64-byte segment (four 16-byte stores): neighbouring lanes' walks are a
@@ -2108,104 +2124,247 @@ __global__ __launch_bounds__(256) void row_stats_kernel(const uint32_t* __restri
 	row_stats_row<ENC, CORR>(lat, snap, lx, ly, y, threadIdx.x & 63, out + (size_t)y * ROW_STATS, false);
 }
 
-// Exact lattice totals from the row statistics, on the device: one 1024-lane
-// workgroup; lane i takes rows [i R, (i+1) R).  Row y starts at height
-// hs_y = hs_{y-1} + D_{y-1} +- 1 (its y slope at x = 0, Correlator::set,
-// kpz/correlator.cpp:50-64), so the start of a lane's range is a workgroup
-// scan of the ranges' height changes.  res = {sum h, sum h^2 lo, hi, sum h_s,
-// sum h h_s lo, hi, tcs}; hs (may be null) receives every row's start height.
-__global__ __launch_bounds__(1024) void row_combine_kernel(const int64_t* __restrict__ rows, int lx, int ly, int corr,
-		int64_t* __restrict__ hs, int64_t* __restrict__ res)
+// Exact lattice totals from the row statistics, on the device, in two passes.
+// Row y starts at height hs_y = hs_{y-1} + D_{y-1} +- 1 (its y slope at x = 0,
+// Correlator::set, kpz/correlator.cpp:50-64).  Pass 1: one workgroup per
+// RC_ROWS rows, one lane per row; a workgroup scan of the rows' height changes
+// gives each row's start height h relative to the workgroup's first row, and
+// the workgroup's sums relative to that start are written as one record
+// (with H the true start: sum h = A1 + H L n, sum h^2 = A2 + 2 H A1 + H^2 L n,
+// and against a snapshot starting at G: sum h g = B12 + H T1 + G A1 + H G L n).
+// Pass 2: one workgroup scans the records' height changes for every H and G
+// and adds them up in int128.  res = {sum h, sum h^2 lo, hi, sum h_s,
+// sum h h_s lo, hi, tcs}; hs (may be null) receives every row's start height
+// (pass 1 writes it relative, pass 3 adds H).
+constexpr int RC_ROWS = 256;
+enum { RC_DA, RC_DB, RC_A1, RC_A2LO, RC_A2HI, RC_T1, RC_B12LO, RC_B12HI, RC_TCS, RC_N, RC_H, RC_G, RC_REC };
+
+__device__ __forceinline__ __int128 wave_sum128(__int128 v)
 {
-	__shared__ int64_t sa[1024], sb[1024];
-	__shared__ __int128 r2[1024];
-	const int i = threadIdx.x;
-	const int ny = 1 << ly;
-	const int R = ny > 1024 ? ny / 1024 : 1;
-	const int y0 = i * R, y1 = (y0 + R < ny) ? y0 + R : ny;
-	int64_t da = 0, db = 0;
-	for(int y = y0; y < y1; ++y)
+	uint64_t lo = (uint64_t)v;
+	int64_t hi = (int64_t)(v >> 64);
+#pragma unroll
+	for(int o = 32; o > 0; o >>= 1)
+	{
+		const int src = (threadIdx.x & 63) ^ o;
+		const uint64_t lo2 = shfl64(lo, src, 64);
+		const int64_t hi2 = (int64_t)shfl64((uint64_t)hi, src, 64);
+		const uint64_t s = lo + lo2;
+		hi = hi + hi2 + (s < lo ? 1 : 0);
+		lo = s;
+	}
+	return ((__int128)hi << 64) | lo;
+}
+
+__global__ __launch_bounds__(RC_ROWS) void row_part_kernel(const int64_t* __restrict__ rows, int lx, int ly, int corr,
+		int64_t* __restrict__ hsRel, int64_t* __restrict__ part)
+{
+	constexpr int NW = RC_ROWS / 64;
+	__shared__ int64_t wa[NW], wb[NW], w1[NW], wt[NW], wc[NW];
+	__shared__ __int128 w2[NW], w12[NW];
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	const int64_t ny = (int64_t)1 << ly;
+	const int64_t y = (int64_t)blockIdx.x * RC_ROWS + threadIdx.x;
+	const bool live = y < ny;
+	int64_t o0 = 0, o1 = 0, o2 = 0, o3 = 0, o4 = 0, o5 = 0, o6 = 0, o7 = 0, o8 = 0;
+	if(live)
 	{
 		const int64_t* o = rows + (size_t)y * ROW_STATS;
-		da += (o[3] ? 1 : -1) + o[2];
+		o0 = o[0];
+		o1 = o[1];
+		o2 = o[2];
+		o3 = o[3];
 		if(corr)
-			db += (o[6] ? 1 : -1) + o[5];
+		{
+			o4 = o[4];
+			o5 = o[5];
+			o6 = o[6];
+			o7 = o[7];
+			o8 = o[8];
+		}
 	}
-	// inclusive scan of the range changes, then exclusive starts
-	sa[i] = da;
-	sb[i] = db;
-	__syncthreads();
-	for(int off = 1; off < 1024; off <<= 1)
+	const int64_t sy = o3 ? 1 : -1, syS = o6 ? 1 : -1;
+	const int64_t da = live ? sy + o2 : 0, db = (live && corr) ? syS + o5 : 0;
+	// workgroup exclusive scan of the row changes
+	const int64_t ia = wave_incl_scan(da, lane), ib = corr ? wave_incl_scan(db, lane) : 0;
+	if(lane == 63)
 	{
-		const int64_t ua = (i >= off) ? sa[i - off] : 0, ub = (i >= off) ? sb[i - off] : 0;
-		__syncthreads();
-		sa[i] += ua;
-		sb[i] += ub;
-		__syncthreads();
+		wa[wv] = ia;
+		wb[wv] = ib;
 	}
-	int64_t prev = sa[i] - da, prevS = sb[i] - db;
+	__syncthreads();
+	int64_t offA = 0, offB = 0, totA = 0, totB = 0;
+#pragma unroll
+	for(int k = 0; k < NW; ++k)
+	{
+		if(k < wv)
+		{
+			offA += wa[k];
+			offB += wb[k];
+		}
+		totA += wa[k];
+		totB += wb[k];
+	}
+	const int64_t L = (int64_t)1 << lx;
+	const int64_t h = offA + ia - da + sy; // this row's start height, relative
+	int64_t c1 = 0, t1 = 0, tcs = 0;
+	__int128 c2 = 0, c12 = 0;
+	if(live)
+	{
+		if(hsRel)
+			hsRel[y] = h;
+		c1 = L * h + o0;
+		c2 = (__int128)L * h * h + (__int128)2 * h * o0 + o1;
+		if(corr)
+		{
+			const int64_t g = offB + ib - db + syS;
+			t1 = L * g + o4;
+			c12 = (__int128)L * h * g + (__int128)h * o4 + (__int128)g * o0 + o7;
+			tcs = o8;
+		}
+	}
+	c1 = wave_sum(c1);
+	c2 = wave_sum128(c2);
+	if(corr)
+	{
+		t1 = wave_sum(t1);
+		c12 = wave_sum128(c12);
+		tcs = wave_sum(tcs);
+	}
+	if(lane == 0)
+	{
+		w1[wv] = c1;
+		w2[wv] = c2;
+		wt[wv] = t1;
+		w12[wv] = c12;
+		wc[wv] = tcs;
+	}
+	__syncthreads();
+	if(threadIdx.x == 0)
+	{
+		int64_t A1 = 0, T1 = 0, TC = 0;
+		__int128 A2 = 0, B12 = 0;
+#pragma unroll
+		for(int k = 0; k < NW; ++k)
+		{
+			A1 += w1[k];
+			A2 += w2[k];
+			T1 += wt[k];
+			B12 += w12[k];
+			TC += wc[k];
+		}
+		const int64_t n = (ny - (int64_t)blockIdx.x * RC_ROWS) < RC_ROWS ? ny - (int64_t)blockIdx.x * RC_ROWS : RC_ROWS;
+		int64_t* r = part + (size_t)blockIdx.x * RC_REC;
+		r[RC_DA] = totA;
+		r[RC_DB] = totB;
+		r[RC_A1] = A1;
+		r[RC_A2LO] = (int64_t)(uint64_t)A2;
+		r[RC_A2HI] = (int64_t)(A2 >> 64);
+		r[RC_T1] = T1;
+		r[RC_B12LO] = (int64_t)(uint64_t)B12;
+		r[RC_B12HI] = (int64_t)(B12 >> 64);
+		r[RC_TCS] = TC;
+		r[RC_N] = n;
+	}
+}
+
+// pass 2: one 1024-lane workgroup over the nb records; lane i takes records
+// [i R, (i+1) R); writes every record's start heights H, G (for pass 3)
+__global__ __launch_bounds__(1024) void row_final_kernel(int64_t* __restrict__ part, int64_t nb, int lx, int corr,
+		int64_t* __restrict__ res)
+{
+	__shared__ int64_t sa[16], sb[16];
+	__shared__ int64_t q1[16], qt[16], qc[16];
+	__shared__ __int128 q2[16], q12[16];
+	const int i = threadIdx.x, lane = i & 63, wv = i >> 6;
+	const int64_t R = (nb + 1023) / 1024;
+	const int64_t b0 = i * R, b1 = (b0 + R < nb) ? b0 + R : nb;
+	int64_t da = 0, db = 0;
+	for(int64_t b = b0; b < b1; ++b)
+	{
+		da += part[b * RC_REC + RC_DA];
+		db += part[b * RC_REC + RC_DB];
+	}
+	const int64_t ia = wave_incl_scan(da, lane), ib = wave_incl_scan(db, lane);
+	if(lane == 63)
+	{
+		sa[wv] = ia;
+		sb[wv] = ib;
+	}
+	__syncthreads();
+	int64_t H = ia - da, G = ib - db;
+	for(int k = 0; k < wv; ++k)
+	{
+		H += sa[k];
+		G += sb[k];
+	}
 	const int64_t L = (int64_t)1 << lx;
 	int64_t s1 = 0, t1 = 0, tcs = 0;
 	__int128 s2 = 0, s12 = 0;
-	for(int y = y0; y < y1; ++y)
+	for(int64_t b = b0; b < b1; ++b)
 	{
-		const int64_t* o = rows + (size_t)y * ROW_STATS;
-		const int64_t h = prev + (o[3] ? 1 : -1);
-		if(hs)
-			hs[y] = h;
-		s1 += L * h + o[0];
-		s2 += (__int128)L * h * h + (__int128)2 * h * o[0] + o[1];
-		prev = h + o[2];
+		int64_t* r = part + b * RC_REC;
+		const int64_t n = r[RC_N], A1 = r[RC_A1];
+		const __int128 A2 = ((__int128)r[RC_A2HI] << 64) | (uint64_t)r[RC_A2LO];
+		r[RC_H] = H;
+		r[RC_G] = G;
+		s1 += A1 + H * L * n;
+		s2 += A2 + (__int128)2 * H * A1 + (__int128)H * H * L * n;
 		if(corr)
 		{
-			const int64_t g = prevS + (o[6] ? 1 : -1);
-			t1 += L * g + o[4];
-			s12 += (__int128)L * h * g + (__int128)h * o[4] + (__int128)g * o[0] + o[7];
-			prevS = g + o[5];
-			tcs += o[8];
+			const int64_t T1 = r[RC_T1];
+			const __int128 B12 = ((__int128)r[RC_B12HI] << 64) | (uint64_t)r[RC_B12LO];
+			t1 += T1 + G * L * n;
+			s12 += B12 + (__int128)H * T1 + (__int128)G * A1 + (__int128)H * G * L * n;
+			tcs += r[RC_TCS];
 		}
+		H += r[RC_DA];
+		G += r[RC_DB];
 	}
-	__syncthreads();
-	sa[i] = s1;
-	sb[i] = t1;
-	r2[i] = s2;
-	__syncthreads();
-	for(int off = 512; off > 0; off >>= 1)
+	s1 = wave_sum(s1);
+	s2 = wave_sum128(s2);
+	t1 = wave_sum(t1);
+	s12 = wave_sum128(s12);
+	tcs = wave_sum(tcs);
+	if(lane == 0)
 	{
-		if(i < off)
-		{
-			sa[i] += sa[i + off];
-			sb[i] += sb[i + off];
-			r2[i] += r2[i + off];
-		}
-		__syncthreads();
+		q1[wv] = s1;
+		q2[wv] = s2;
+		qt[wv] = t1;
+		q12[wv] = s12;
+		qc[wv] = tcs;
 	}
+	__syncthreads();
 	if(i == 0)
 	{
-		res[0] = sa[0];
-		res[1] = (int64_t)(uint64_t)r2[0];
-		res[2] = (int64_t)(r2[0] >> 64);
-		res[3] = sb[0];
-	}
-	__syncthreads();
-	sa[i] = tcs;
-	r2[i] = s12;
-	__syncthreads();
-	for(int off = 512; off > 0; off >>= 1)
-	{
-		if(i < off)
+		int64_t S1 = 0, T1 = 0, TC = 0;
+		__int128 S2 = 0, S12 = 0;
+		for(int k = 0; k < 16; ++k)
 		{
-			sa[i] += sa[i + off];
-			r2[i] += r2[i + off];
+			S1 += q1[k];
+			S2 += q2[k];
+			T1 += qt[k];
+			S12 += q12[k];
+			TC += qc[k];
 		}
-		__syncthreads();
+		res[0] = S1;
+		res[1] = (int64_t)(uint64_t)S2;
+		res[2] = (int64_t)(S2 >> 64);
+		res[3] = corr ? T1 : 0;
+		res[4] = corr ? (int64_t)(uint64_t)S12 : 0;
+		res[5] = corr ? (int64_t)(S12 >> 64) : 0;
+		res[6] = corr ? TC : 0;
 	}
-	if(i == 0)
-	{
-		res[4] = (int64_t)(uint64_t)r2[0];
-		res[5] = (int64_t)(r2[0] >> 64);
-		res[6] = sa[0];
-	}
+}
+
+// pass 3 (hs requested): every row's start height = its workgroup's H plus
+// the relative start pass 1 wrote
+__global__ __launch_bounds__(256) void row_heights_kernel(const int64_t* __restrict__ part, int ly,
+		int64_t* __restrict__ hs)
+{
+	const int64_t y = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if(y < ((int64_t)1 << ly))
+		hs[y] += part[(y / RC_ROWS) * RC_REC + RC_H];
 }
 
 // Height moments of site row y: out[y*7 + k-1] = sum_x h(x, y)^k, k = 1..7, in
@@ -2795,10 +2954,23 @@ hipError_t launch_row_stats(const uint32_t* lat, const uint32_t* snap, int enc, 
 	return hipGetLastError();
 }
 
-hipError_t launch_row_combine(const int64_t* rows, int lx, int ly, bool corr, int64_t* hs, int64_t* res,
-		hipStream_t stream)
+int64_t row_combine_part_words(int ly)
 {
-	hipLaunchKernelGGL(row_combine_kernel, dim3(1), dim3(1024), 0, stream, rows, lx, ly, corr ? 1 : 0, hs, res);
+	return ((((int64_t)1 << ly) + RC_ROWS - 1) / RC_ROWS) * RC_REC;
+}
+
+hipError_t launch_row_combine(const int64_t* rows, int lx, int ly, bool corr, int64_t* hs, int64_t* part,
+		int64_t* res, hipStream_t stream)
+{
+	if(!rows || !part || !res || lx < 0 || ly < 0 || lx > 30 || ly > 30)
+		return hipErrorInvalidValue;
+	const int64_t nb = (((int64_t)1 << ly) + RC_ROWS - 1) / RC_ROWS;
+	hipLaunchKernelGGL(row_part_kernel, dim3((unsigned)nb), dim3(RC_ROWS), 0, stream, rows, lx, ly, corr ? 1 : 0, hs,
+	                   part);
+	hipLaunchKernelGGL(row_final_kernel, dim3(1), dim3(1024), 0, stream, part, nb, lx, corr ? 1 : 0, res);
+	if(hs)
+		hipLaunchKernelGGL(row_heights_kernel, dim3((unsigned)((((int64_t)1 << ly) + 255) / 256)), dim3(256), 0,
+		                   stream, part, ly, hs);
 	return hipGetLastError();
 }
 

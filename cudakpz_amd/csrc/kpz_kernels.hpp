@@ -153,9 +153,11 @@ hipError_t launch_row_stats(const uint32_t* lat, const uint32_t* snap, int enc, 
 		int64_t* out, hipStream_t stream);
 // exact totals from the row statistics: res[7] = {sum h, sum h^2 lo, hi,
 // sum h_snap, sum h h_snap lo, hi, tcs} (the last four with corr only); hs
-// (may be null): every row's start height
-hipError_t launch_row_combine(const int64_t* rows, int lx, int ly, bool corr, int64_t* hs, int64_t* res,
-		hipStream_t stream);
+// (may be null): every row's start height; part: row_combine_part_words(ly)
+// int64 of per-workgroup partial sums
+int64_t row_combine_part_words(int ly);
+hipError_t launch_row_combine(const int64_t* rows, int lx, int ly, bool corr, int64_t* hs, int64_t* part,
+		int64_t* res, hipStream_t stream);
 
 // ---- batched replicas ------------------------------------------------------
 // One device-table entry per replica of a batch.  DT replicas use one stream

@@ -51,7 +51,8 @@ SchedulerService::SchedulerService(int lx, int ly, Encoding enc, const Params& p
 		check(hipGetDevice(&device_), "hipGetDevice");
 	check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
 	lat_.alloc(size_.sizeW() * sizeof(uint32_t));
-	rows_.alloc((size_t)size_.dimY() * ROW_STATS * sizeof(int64_t));
+	// row statistics, then the combine's per-workgroup partial sums
+	rows_.alloc(((size_t)size_.dimY() * ROW_STATS + (size_t)row_combine_part_words(size_.ly)) * sizeof(int64_t));
 	totals_.alloc(8 * sizeof(int64_t));
 	master_.reseed(0);
 }
@@ -314,8 +315,9 @@ const Affine* SchedulerService::upload(std::vector<Affine>& host, DevBuf& buf)
 Sums SchedulerService::totals(const uint32_t* snap, int64_t* hs)
 {
 	check(launch_row_stats(deviceLattice(), snap, enc_, size_.lx, size_.ly, rows_.as<int64_t>(), stream_), "row_stats");
-	check(launch_row_combine(rows_.as<int64_t>(), size_.lx, size_.ly, snap != nullptr, hs, totals_.as<int64_t>(), stream_),
-			"row_combine");
+	check(launch_row_combine(rows_.as<int64_t>(), size_.lx, size_.ly, snap != nullptr, hs,
+	                         rows_.as<int64_t>() + (size_t)size_.dimY() * ROW_STATS, totals_.as<int64_t>(), stream_),
+	      "row_combine");
 	int64_t r[7];
 	check(hipMemcpyAsync(r, totals_.get(), sizeof r, hipMemcpyDeviceToHost, stream_), "totals D2H");
 	sync();
