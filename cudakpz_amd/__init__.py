@@ -702,6 +702,15 @@ class Server:
         return {"queued": out[0], "active": out[1], "hbm_used": out[2], "hbm_capacity": out[3],
                 "site_updates": out[4]}
 
+    def busy(self):
+        """(device seconds spent in ticks, seconds from the first tick's start
+        to the last finished tick's end)"""
+        L = lib()
+        L.kpz_server_busy.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        g, w = C.c_double(), C.c_double()
+        _check(L.kpz_server_busy(self._h, C.byref(g), C.byref(w)))
+        return g.value, w.value
+
 
 def make_scheduler(name, lx, ly, **kw):
     """SchedulerConfig::mkScheduler by scheduler id (kpz/kpzFrontend.cpp:93-250)."""

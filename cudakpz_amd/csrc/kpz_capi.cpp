@@ -718,6 +718,18 @@ int kpz_server_stats(kpz_server* s, uint64_t out[5])
 
 double kpz_server_now(const kpz_server* s) { return (s && s->s) ? s->s->now() : -1.0; }
 
+int kpz_server_busy(const kpz_server* s, double* gpu_seconds, double* span_seconds)
+{
+	SHANDLE(s);
+	double g = 0, w = 0;
+	s->s->busy(g, w);
+	if(gpu_seconds)
+		*gpu_seconds = g;
+	if(span_seconds)
+		*span_seconds = w;
+	return KPZ_OK;
+}
+
 int kpz_batch_autotune(const kpz_replica* reps, int n, int mcs, int device, kpz_batch_tuning* tuning, double* ms_per_mcs)
 {
 	std::vector<kpz::ReplicaSpec> specs;

@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace kpz {
 
@@ -83,6 +84,41 @@ void BuddyArena::release(uint64_t off)
 	free_[(size_t)o].push_back(b);
 }
 
+// ---------------------------------------------------------------- pinned staging
+PinnedStage::~PinnedStage()
+{
+	for(auto& b : blocks_)
+		(void)hipHostFree(b.p);
+}
+
+void PinnedStage::reset()
+{
+	for(auto& b : blocks_)
+		b.used = 0;
+	cur_ = 0;
+}
+
+void* PinnedStage::take(size_t bytes)
+{
+	bytes = (bytes + 255) & ~(size_t)255;
+	for(; cur_ < blocks_.size(); ++cur_)
+	{
+		Block& b = blocks_[cur_];
+		if(b.used + bytes <= b.size)
+		{
+			void* p = b.p + b.used;
+			b.used += bytes;
+			return p;
+		}
+	}
+	const size_t sz = std::max<size_t>(bytes, blocks_.empty() ? ((size_t)4 << 20) : 2 * blocks_.back().size);
+	Block b{nullptr, sz, bytes};
+	check(hipHostMalloc((void**)&b.p, sz, hipHostMallocDefault), "hipHostMalloc");
+	blocks_.push_back(b);
+	cur_ = blocks_.size() - 1;
+	return b.p;
+}
+
 // ---------------------------------------------------------------- server
 Server::Server(size_t hbmBudget, int maxSlots, int maxTickMcs, int device)
 	: maxSlots_(maxSlots), maxTick_(maxTickMcs < 1 ? 1 : maxTickMcs)
@@ -100,12 +136,15 @@ Server::Server(size_t hbmBudget, int maxSlots, int maxTickMcs, int device)
 	identity_.alloc(sizeof(Affine));
 	const Affine id{1, 0};
 	check(hipMemcpy(identity_.get(), &id, sizeof id, hipMemcpyHostToDevice), "identity");
-	orderHost_.assign((size_t)maxTick_ * (size_t)maxSlots_, 0);
-	shiftHost_.assign((size_t)maxTick_ * (size_t)maxSlots_, 0);
-	orderBuf_.alloc(orderHost_.size());
-	shiftBuf_.alloc(shiftHost_.size() * 2);
-	limitHost_.assign((size_t)maxSlots_, 0);
-	limitBuf_.alloc(limitHost_.size() * sizeof(int));
+	orderBuf_.alloc((size_t)maxTick_ * (size_t)maxSlots_);
+	shiftBuf_.alloc((size_t)maxTick_ * (size_t)maxSlots_ * 2);
+	limitBuf_.alloc((size_t)maxSlots_ * sizeof(int));
+	for(Tick& t : ticks_)
+	{
+		check(hipEventCreate(&t.begin), "hipEventCreate");
+		check(hipEventCreate(&t.end), "hipEventCreate");
+	}
+	check(hipEventCreate(&first_), "hipEventCreate");
 	t0_ = std::chrono::steady_clock::now();
 }
 
@@ -117,6 +156,15 @@ Server::~Server()
 		delete j;
 	for(Job* j : slots_)
 		delete j;
+	for(Tick& t : ticks_)
+	{
+		if(t.begin)
+			(void)hipEventDestroy(t.begin);
+		if(t.end)
+			(void)hipEventDestroy(t.end);
+	}
+	if(first_)
+		(void)hipEventDestroy(first_);
 	if(s0_)
 		(void)hipStreamDestroy(s0_);
 }
@@ -126,10 +174,20 @@ double Server::now() const
 	return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
 }
 
+void Server::busy(double& gpuSeconds, double& spanSeconds) const
+{
+	gpuSeconds = gpuMs_ * 1e-3;
+	spanSeconds = spanMs_ * 1e-3;
+}
+
+// device buffers grow geometrically; a tick may still read the old one
 void Server::ensure(DevBuf& b, size_t bytes)
 {
 	if(b.bytes() < bytes)
+	{
+		check(hipStreamSynchronize(s0_), "grow sync");
 		b.alloc(std::max(bytes, 2 * b.bytes()));
+	}
 }
 
 size_t Server::footprint(const JobSpec& s, size_t* parts)
@@ -173,7 +231,9 @@ uint64_t Server::submit(const JobSpec& j)
 	return job->id;
 }
 
-bool Server::admit(Job& j)
+// reserve the job's memory, seed it and queue its set-up on the server
+// stream (host data staged in pinned memory: nothing here waits for the GPU)
+bool Server::admit(Job& j, PinnedStage& stage)
 {
 	size_t parts[3];
 	footprint(j.spec, parts);
@@ -221,28 +281,28 @@ bool Server::admit(Job& j)
 	const double m1 = j.master.close_open();
 	const double m2 = j.master.close_open();
 	uint64_t x = (uint64_t)(m1 * 4294967295.0) ^ ((uint64_t)(m2 * 4294967295.0) << 32);
-	std::vector<uint64_t> st((size_t)j.T);
-	for(auto& v : st)
+	uint64_t* st = (uint64_t*)stage.take((size_t)j.T * 8);
+	for(int64_t t = 0; t < j.T; ++t)
 	{
-		v = x;
+		st[t] = x;
 		x = x * SLCG_A + SLCG_C;
 	}
-	check(hipMemcpyAsync(d.st[0], st.data(), st.size() * 8, hipMemcpyHostToDevice, s0_), "job streams");
+	check(hipMemcpyAsync(d.st[0], st, (size_t)j.T * 8, hipMemcpyHostToDevice, s0_), "job streams");
 	if(d.kind == BK_SCABIT)
 	{
 		const uint64_t du = d.mode == 0 ? (d.wide ? 2 : 1) : (d.wide ? 64 : 32);
 		const uint64_t Wn = 1ull << (s.lx - (d.wide ? 7 : 6));
 		const int G = 1 << d.lG;
-		std::vector<Affine> lane((size_t)G);
+		Affine* lane = (Affine*)stage.take((size_t)G * sizeof(Affine));
 		const Affine word = affine_pow(d.step, du);
 		Affine acc{1, 0};
-		for(auto& a : lane)
+		for(int g = 0; g < G; ++g)
 		{
-			a = acc;
+			lane[g] = acc;
 			acc = compose(word, acc);
 		}
 		d.laneJ = (const Affine*)(base + j.off[2]);
-		check(hipMemcpyAsync((void*)d.laneJ, lane.data(), lane.size() * sizeof(Affine), hipMemcpyHostToDevice, s0_), "lane table");
+		check(hipMemcpyAsync((void*)d.laneJ, lane, (size_t)G * sizeof(Affine), hipMemcpyHostToDevice, s0_), "lane table");
 		d.stepJ = affine_pow(d.step, du * (uint64_t)G);
 		d.full = affine_pow(d.step, du * Wn);
 	}
@@ -259,24 +319,26 @@ bool Server::admit(Job& j)
 		j.samples.push_back((uint32_t)t);
 	}
 	j.samples.push_back((uint32_t)s.mcs);
-	j.next = 0;
+	j.nextPlan = 0;
+	j.planned = 0;
 	j.done = 0;
 	j.res.tStart = now();
-	// the host vectors above are pageable: make sure the copies are done
-	check(hipStreamSynchronize(s0_), "admit sync");
 	return true;
 }
 
-void Server::rebuild()
+// device tables of the jobs that still have MCS to run: DT lanes (one per
+// block) grouped by (kind, mode), SCABit waves (rows)
+void Server::rebuild(PinnedStage& stage)
 {
 	std::vector<uint2> lanes, waves;
+	auto runs = [](const Job* j) { return j && j->planned < (uint32_t)j->spec.mcs; };
 	for(int kind : {BK_DT, BK_DTR})
 		for(int mode : {0, 1})
 		{
 			for(int sl = 0; sl < maxSlots_; ++sl)
 			{
 				Job* j = slots_[(size_t)sl];
-				if(!j || j->dev.kind != kind || j->dev.mode != mode)
+				if(!runs(j) || j->dev.kind != kind || j->dev.mode != mode)
 					continue;
 				for(int64_t b = 0; b < j->T; ++b)
 					lanes.push_back(make_uint2((unsigned)sl, (unsigned)b));
@@ -287,8 +349,10 @@ void Server::rebuild()
 	for(int sl = 0; sl < maxSlots_; ++sl)
 	{
 		Job* j = slots_[(size_t)sl];
-		hostReps_[(size_t)sl] = j ? j->dev : BatchRep{};
-		if(!j || j->dev.kind != BK_SCABIT)
+		hostReps_[(size_t)sl] = runs(j) ? j->dev : BatchRep{};
+		if(j)
+			j->inTable = runs(j);
+		if(!runs(j) || j->dev.kind != BK_SCABIT)
 			continue;
 		const int rpw = 64 >> j->dev.lG;
 		for(int y = 0; y < (1 << j->dev.ly); y += rpw)
@@ -296,24 +360,42 @@ void Server::rebuild()
 	}
 	nLanes_ = (int64_t)lanes.size();
 	nWaves_ = (int64_t)waves.size();
-	check(hipMemcpyAsync(repTable_.get(), hostReps_.data(), sizeof(BatchRep) * hostReps_.size(), hipMemcpyHostToDevice, s0_), "reps");
 	if(nLanes_)
-	{
 		ensure(laneTable_, lanes.size() * sizeof(uint2));
-		check(hipMemcpyAsync(laneTable_.get(), lanes.data(), lanes.size() * sizeof(uint2), hipMemcpyHostToDevice, s0_), "lanes");
-	}
 	if(nWaves_)
-	{
 		ensure(waveTable_, waves.size() * sizeof(uint2));
-		check(hipMemcpyAsync(waveTable_.get(), waves.data(), waves.size() * sizeof(uint2), hipMemcpyHostToDevice, s0_), "waves");
-	}
-	check(hipStreamSynchronize(s0_), "rebuild sync");
+	auto put = [&](DevBuf& dst, const void* src, size_t bytes, const char* what) {
+		void* p = stage.take(bytes);
+		std::memcpy(p, src, bytes);
+		check(hipMemcpyAsync(dst.get(), p, bytes, hipMemcpyHostToDevice, s0_), what);
+	};
+	put(repTable_, hostReps_.data(), sizeof(BatchRep) * hostReps_.size(), "reps");
+	if(nLanes_)
+		put(laneTable_, lanes.data(), lanes.size() * sizeof(uint2), "lanes");
+	if(nWaves_)
+		put(waveTable_, waves.data(), waves.size() * sizeof(uint2), "waves");
 	dirty_ = false;
 }
 
-size_t Server::step()
+// plan the next tick and queue all of its device work; nothing here waits
+// for the tick that may still be running (all work is ordered on s0_)
+bool Server::enqueue(Tick& t)
 {
-	// admission: FIFO with bounded backfill by HBM footprint
+	t.stage.reset();
+	t.plan.clear();
+	t.ran.clear();
+	t.live = false;
+	if(nActive_ == 0 && queue_.empty())
+		return false;
+	if(!started_)
+	{
+		check(hipEventRecord(first_, s0_), "first event");
+		started_ = true;
+	}
+	check(hipEventRecord(t.begin, s0_), "tick begin");
+	t.live = true;
+
+	// admission: FIFO with bounded backfill by HBM footprint, into free slots
 	int looked = 0;
 	for(auto it = queue_.begin(); it != queue_.end() && looked < 64; ++looked)
 	{
@@ -327,7 +409,7 @@ size_t Server::step()
 		if(sl < 0)
 			break;
 		Job* j = *it;
-		if(admit(*j))
+		if(admit(*j, t.stage))
 		{
 			j->slot = sl;
 			slots_[(size_t)sl] = j;
@@ -338,32 +420,43 @@ size_t Server::step()
 		else
 			++it;
 	}
-	if(dirty_)
-		rebuild();
-	if(nActive_ == 0)
-		return queue_.size();
 
 	// fixed-length tick: job j runs lim_j = min(maxTick, MCS left) MCS; the
-	// kernels skip a job once m >= lim_j.  Samples inside the tick are taken
-	// in stream order right after their MCS; one synchronisation per tick.
+	// kernels skip a job once m >= lim_j
 	const size_t R = (size_t)maxSlots_;
-	std::fill(limitHost_.begin(), limitHost_.end(), 0);
+	int* limit = (int*)t.stage.take(R * sizeof(int));
+	std::fill(limit, limit + R, 0);
 	int k = 0;
 	for(Job* j : slots_)
-		if(j)
+		if(j && j->planned < (uint32_t)j->spec.mcs)
 		{
-			const int lim = std::min<int>(maxTick_, j->spec.mcs - (int)j->done);
-			limitHost_[(size_t)j->slot] = lim;
+			const int lim = std::min<int>(maxTick_, j->spec.mcs - (int)j->planned);
+			limit[j->slot] = lim;
 			k = std::max(k, lim);
+			if(!j->inTable)
+				dirty_ = true;
 		}
+	if(k == 0)
+	{
+		check(hipEventRecord(t.end, s0_), "tick end");
+		return true;
+	}
+	if(dirty_)
+		rebuild(t.stage);
+
+	// master draws of the DT jobs, per MCS of the tick (domain order, DTr shift)
+	uint8_t* order = (uint8_t*)t.stage.take((size_t)k * R);
+	uint16_t* shift = (uint16_t*)t.stage.take((size_t)k * R * 2);
+	std::memset(order, 0, (size_t)k * R);
+	std::memset(shift, 0, (size_t)k * R * 2);
 	for(int m = 0; m < k; ++m)
 		for(Job* j : slots_)
 		{
-			if(!j || j->dev.kind == BK_SCABIT || m >= limitHost_[(size_t)j->slot])
+			if(!j || j->dev.kind == BK_SCABIT || m >= limit[j->slot])
 				continue;
 			int seq[4];
 			j->master.shuffle4(seq);
-			orderHost_[(size_t)m * R + (size_t)j->slot] = (uint8_t)(seq[0] | (seq[1] << 2) | (seq[2] << 4) | (seq[3] << 6));
+			order[(size_t)m * R + (size_t)j->slot] = (uint8_t)(seq[0] | (seq[1] << 2) | (seq[2] << 4) | (seq[3] << 6));
 			uint16_t sh = 0;
 			if(j->dev.kind == BK_DTR)
 			{
@@ -371,51 +464,56 @@ size_t Server::step()
 				const unsigned sy = (unsigned)(j->master.close_open() * 16.0);
 				sh = (uint16_t)(sx | (sy << 8));
 			}
-			shiftHost_[(size_t)m * R + (size_t)j->slot] = sh;
+			shift[(size_t)m * R + (size_t)j->slot] = sh;
 		}
-	check(hipMemcpyAsync(orderBuf_.get(), orderHost_.data(), (size_t)k * R, hipMemcpyHostToDevice, s0_), "order");
-	check(hipMemcpyAsync(shiftBuf_.get(), shiftHost_.data(), (size_t)k * R * 2, hipMemcpyHostToDevice, s0_), "shift");
-	check(hipMemcpyAsync(limitBuf_.get(), limitHost_.data(), R * sizeof(int), hipMemcpyHostToDevice, s0_), "limit");
+	check(hipMemcpyAsync(orderBuf_.get(), order, (size_t)k * R, hipMemcpyHostToDevice, s0_), "order");
+	check(hipMemcpyAsync(shiftBuf_.get(), shift, (size_t)k * R * 2, hipMemcpyHostToDevice, s0_), "shift");
+	check(hipMemcpyAsync(limitBuf_.get(), limit, R * sizeof(int), hipMemcpyHostToDevice, s0_), "limit");
 
-	// sample plan: (m, job) for every sample time t in (done, done + lim]
-	std::vector<std::pair<int, Job*>> plan;
+	// sample plan: (m, job) for every sample time in (planned, planned + lim]
 	for(Job* j : slots_)
-		if(j)
-			for(size_t s = j->next; s < j->samples.size() && j->samples[s] <= j->done + (uint32_t)limitHost_[(size_t)j->slot]; ++s)
-				plan.push_back({(int)(j->samples[s] - j->done) - 1, j});
-	std::stable_sort(plan.begin(), plan.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
-	std::vector<BatchRep> sreps(plan.size());
-	std::vector<uint2> srows;
-	std::vector<int64_t> rowStart(plan.size() + 1);
+		if(j && limit[j->slot] > 0)
+			for(size_t s = j->nextPlan; s < j->samples.size() && j->samples[s] <= j->planned + (uint32_t)limit[j->slot]; ++s)
+				t.plan.push_back({(int)(j->samples[s] - j->planned) - 1, j});
+	std::stable_sort(t.plan.begin(), t.plan.end(), [](const Sample& a, const Sample& b) { return a.m < b.m; });
+	const size_t np = t.plan.size();
+	std::vector<int64_t> rowStart(np + 1);
 	std::vector<size_t> groupStart; // plan index where each m-group starts
-	int64_t rb = 0;
-	for(size_t i = 0; i < plan.size(); ++i)
+	int64_t nrows = 0;
+	for(size_t i = 0; i < np; ++i)
 	{
-		if(i == 0 || plan[i].first != plan[i - 1].first)
+		rowStart[i] = nrows;
+		nrows += 1 << t.plan[i].job->spec.ly;
+	}
+	rowStart[np] = nrows;
+	BatchRep* sreps = (BatchRep*)t.stage.take(std::max<size_t>(np, 1) * sizeof(BatchRep));
+	uint2* srows = (uint2*)t.stage.take((size_t)std::max<int64_t>(nrows, 1) * sizeof(uint2));
+	int64_t rb = 0, maxGroupRows = 0;
+	for(size_t i = 0; i < np; ++i)
+	{
+		if(i == 0 || t.plan[i].m != t.plan[i - 1].m)
 		{
 			groupStart.push_back(i);
 			rb = 0; // scratch rows restart per group (groups run in stream order)
 		}
-		sreps[i] = plan[i].second->dev;
+		sreps[i] = t.plan[i].job->dev;
 		sreps[i].rowBase = rb;
-		rowStart[i] = (int64_t)srows.size();
-		for(int y = 0; y < (1 << sreps[i].ly); ++y)
-			srows.push_back(make_uint2((unsigned)(i - groupStart.back()), (unsigned)y));
-		rb += 1 << sreps[i].ly;
+		const int ny = 1 << sreps[i].ly;
+		for(int y = 0; y < ny; ++y)
+			srows[rowStart[i] + y] = make_uint2((unsigned)(i - groupStart.back()), (unsigned)y);
+		rb += ny;
+		maxGroupRows = std::max(maxGroupRows, rb);
 	}
-	rowStart[plan.size()] = (int64_t)srows.size();
-	groupStart.push_back(plan.size());
-	int64_t maxGroupRows = 0;
-	for(size_t g = 0; g + 1 < groupStart.size(); ++g)
-		maxGroupRows = std::max(maxGroupRows, rowStart[groupStart[g + 1]] - rowStart[groupStart[g]]);
-	if(!plan.empty())
+	groupStart.push_back(np);
+	t.out = (int64_t*)t.stage.take(std::max<size_t>(np, 1) * 3 * sizeof(int64_t));
+	if(np)
 	{
-		ensure(sampleReps_, sreps.size() * sizeof(BatchRep));
-		ensure(rowTable_, srows.size() * sizeof(uint2));
+		ensure(sampleReps_, np * sizeof(BatchRep));
+		ensure(rowTable_, (size_t)nrows * sizeof(uint2));
 		ensure(rowScratch_, (size_t)maxGroupRows * 4 * sizeof(int64_t));
-		ensure(outBuf_, plan.size() * 3 * sizeof(int64_t));
-		check(hipMemcpyAsync(sampleReps_.get(), sreps.data(), sreps.size() * sizeof(BatchRep), hipMemcpyHostToDevice, s0_), "sample reps");
-		check(hipMemcpyAsync(rowTable_.get(), srows.data(), srows.size() * sizeof(uint2), hipMemcpyHostToDevice, s0_), "rows");
+		ensure(outBuf_, np * 3 * sizeof(int64_t));
+		check(hipMemcpyAsync(sampleReps_.get(), sreps, np * sizeof(BatchRep), hipMemcpyHostToDevice, s0_), "sample reps");
+		check(hipMemcpyAsync(rowTable_.get(), srows, (size_t)nrows * sizeof(uint2), hipMemcpyHostToDevice, s0_), "rows");
 	}
 
 	BatchDtArgs da{};
@@ -448,42 +546,70 @@ size_t Server::step()
 			check(launch_batch_cbbit(ca, s0_), "batch_cbbit");
 		}
 		// exact roughness of the jobs whose sample time is this MCS
-		if(g + 1 < groupStart.size() && plan[groupStart[g]].first == m)
+		if(g + 1 < groupStart.size() && t.plan[groupStart[g]].m == m)
 		{
 			const size_t i0 = groupStart[g], i1 = groupStart[g + 1];
-			check(launch_batch_roughness(sampleReps_.as<BatchRep>() + i0, (int)(i1 - i0), rowTable_.as<uint2>() + rowStart[i0],
-					rowStart[i1] - rowStart[i0], rowScratch_.as<int64_t>(), outBuf_.as<int64_t>() + 3 * i0, s0_), "batch_roughness");
+			check(launch_batch_roughness(sampleReps_.as<BatchRep>() + i0, (int)(i1 - i0),
+			                             rowTable_.as<uint2>() + rowStart[i0], rowStart[i1] - rowStart[i0],
+			                             rowScratch_.as<int64_t>(), outBuf_.as<int64_t>() + 3 * i0, s0_),
+			      "batch_roughness");
 			++g;
 		}
 	}
-	std::vector<int64_t> out(plan.size() * 3);
-	if(!plan.empty())
-		check(hipMemcpyAsync(out.data(), outBuf_.get(), out.size() * 8, hipMemcpyDeviceToHost, s0_), "roughness D2H");
-	check(hipStreamSynchronize(s0_), "tick sync");
-	for(size_t i = 0; i < plan.size(); ++i)
-	{
-		Job* j = plan[i].second;
-		JobSample smp;
-		smp.mcs = j->done + (uint32_t)plan[i].first + 1;
-		smp.sumH = out[3 * i];
-		smp.sumH2 = ((__int128)out[3 * i + 2] << 64) | (__int128)(uint64_t)out[3 * i + 1];
-		j->res.series.push_back(smp);
-		++j->next;
-	}
+	if(np)
+		check(hipMemcpyAsync(t.out, outBuf_.get(), np * 3 * sizeof(int64_t), hipMemcpyDeviceToHost, s0_), "roughness D2H");
+	check(hipEventRecord(t.end, s0_), "tick end");
+
+	// bookkeeping of what this tick runs; a job that has now planned all its
+	// MCS leaves the tables of the next tick
 	for(Job* j : slots_)
+		if(j && limit[j->slot] > 0)
+		{
+			const int lim = limit[j->slot];
+			t.ran.push_back({j, lim});
+			while(j->nextPlan < j->samples.size() && j->samples[j->nextPlan] <= j->planned + (uint32_t)lim)
+				++j->nextPlan;
+			j->planned += (uint32_t)lim;
+			if(j->planned == (uint32_t)j->spec.mcs)
+				dirty_ = true;
+		}
+	return true;
+}
+
+// wait for a tick, record its samples, retire the jobs it finished
+void Server::finish(Tick& t)
+{
+	if(!t.live)
+		return;
+	check(hipEventSynchronize(t.end), "tick sync");
+	float ms = 0, span = 0;
+	check(hipEventElapsedTime(&ms, t.begin, t.end), "tick time");
+	check(hipEventElapsedTime(&span, first_, t.end), "span time");
+	gpuMs_ += ms;
+	spanMs_ = span;
+	for(size_t i = 0; i < t.plan.size(); ++i)
 	{
-		if(!j)
-			continue;
-		const int lim = limitHost_[(size_t)j->slot];
-		j->done += (uint32_t)lim;
-		siteUpdates_ += (uint64_t)lim << (j->spec.lx + j->spec.ly);
+		Job* j = t.plan[i].job;
+		JobSample smp;
+		smp.mcs = j->done + (uint32_t)t.plan[i].m + 1;
+		smp.sumH = t.out[3 * i];
+		smp.sumH2 = ((__int128)t.out[3 * i + 2] << 64) | (__int128)(uint64_t)t.out[3 * i + 1];
+		j->res.series.push_back(smp);
+	}
+	for(auto& r : t.ran)
+	{
+		Job* j = r.first;
+		j->done += (uint32_t)r.second;
+		siteUpdates_ += (uint64_t)r.second << (j->spec.lx + j->spec.ly);
 		if(j->done != (uint32_t)j->spec.mcs)
 			continue;
-		// retire: results out, memory back to the arena, slot free
+		// retire: results out, memory back to the arena, slot free (no tick
+		// still queued touches this job: it left the tables after its last MCS)
 		if(j->spec.keepLattice)
 		{
 			j->res.lattice.resize((size_t)1 << (j->spec.lx + j->spec.ly - 4));
-			check(hipMemcpy(j->res.lattice.data(), j->dev.lat, j->res.lattice.size() * 4, hipMemcpyDeviceToHost), "final lattice");
+			check(hipMemcpy(j->res.lattice.data(), j->dev.lat, j->res.lattice.size() * 4, hipMemcpyDeviceToHost),
+			      "final lattice");
 		}
 		size_t parts[3];
 		footprint(j->spec, parts);
@@ -495,8 +621,17 @@ size_t Server::step()
 		slots_[(size_t)j->slot] = nullptr;
 		delete j;
 		--nActive_;
-		dirty_ = true;
 	}
+	t.ran.clear();
+	t.plan.clear();
+	t.live = false;
+}
+
+size_t Server::step()
+{
+	enqueue(ticks_[cur_]);
+	finish(ticks_[cur_ ^ 1]);
+	cur_ ^= 1;
 	return queue_.size() + nActive_;
 }
 

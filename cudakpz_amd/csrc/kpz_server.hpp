@@ -59,6 +59,26 @@ struct JobResult {
 	double tSubmit = 0, tStart = 0, tEnd = 0; // seconds since server start
 };
 
+// pinned host memory handed out by a bump pointer; blocks are kept (and
+// reused after reset()) so that the steady state allocates nothing
+class PinnedStage {
+public:
+	PinnedStage() = default;
+	~PinnedStage();
+	PinnedStage(const PinnedStage&) = delete;
+	PinnedStage& operator=(const PinnedStage&) = delete;
+	void reset();
+	void* take(size_t bytes);
+
+private:
+	struct Block {
+		char* p;
+		size_t size, used;
+	};
+	std::vector<Block> blocks_;
+	size_t cur_ = 0;
+};
+
 class Server {
 public:
 	Server(size_t hbmBudget, int maxSlots, int maxTickMcs = 16, int device = -1);
@@ -67,8 +87,10 @@ public:
 	Server& operator=(const Server&) = delete;
 
 	uint64_t submit(const JobSpec& j);
-	// one tick: admit, advance every active job up to its next sample, sample,
-	// retire; returns the number of jobs still queued or active
+	// one tick, pipelined: plan and enqueue the next tick (admission, tables,
+	// master draws, launches) while the tick enqueued by the previous call is
+	// still running, then wait for that previous tick and finish it (samples,
+	// retirement).  Returns the number of jobs still queued or active.
 	size_t step();
 	bool pop(JobResult& out);
 	size_t queued() const { return queue_.size(); }
@@ -76,6 +98,9 @@ public:
 	size_t hbmUsed() const { return arena_.used(); }
 	size_t hbmCapacity() const { return arena_.capacity(); }
 	uint64_t siteUpdates() const { return siteUpdates_; }
+	// device time spent in ticks and the span from the first tick's start to
+	// the last finished tick's end, in seconds (hipEvents on the server stream)
+	void busy(double& gpuSeconds, double& spanSeconds) const;
 	double now() const;
 
 private:
@@ -86,16 +111,33 @@ private:
 		uint64_t off[3] = {0, 0, 0}; // lattice, streams, lane table
 		int64_t T = 0;
 		MasterRng master;
-		uint32_t done = 0;
+		uint32_t planned = 0;        // MCS enqueued
+		uint32_t done = 0;           // MCS finished (device work complete)
 		std::vector<uint32_t> samples;
-		size_t next = 0;
+		size_t nextPlan = 0;         // next sample time to plan
+		bool inTable = false;        // in the current device tables
 		JobResult res;
 		BatchRep dev{};
 	};
+	struct Sample {
+		int m;      // MCS index inside its tick
+		Job* job;
+	};
+	// one tick in flight: its pinned staging, sample plan, per-job MCS and events
+	struct Tick {
+		bool live = false;
+		PinnedStage stage;
+		std::vector<Sample> plan;
+		int64_t* out = nullptr; // pinned: 3 int64 per planned sample
+		std::vector<std::pair<Job*, int>> ran; // (job, MCS it runs in this tick)
+		hipEvent_t begin = nullptr, end = nullptr;
+	};
 	static size_t footprint(const JobSpec& s, size_t* parts);
-	bool admit(Job& j);
-	void rebuild();
+	bool admit(Job& j, PinnedStage& stage);
+	void rebuild(PinnedStage& stage);
 	void ensure(DevBuf& b, size_t bytes);
+	bool enqueue(Tick& t);
+	void finish(Tick& t);
 
 	BuddyArena arena_;
 	int maxSlots_, maxTick_;
@@ -111,10 +153,12 @@ private:
 	DevBuf repTable_, laneTable_, waveTable_, rowTable_, rowScratch_, outBuf_, orderBuf_, shiftBuf_, identity_, sampleReps_,
 			limitBuf_;
 	std::vector<BatchRep> hostReps_;
-	std::vector<int> limitHost_;
 	int64_t nLanes_ = 0, nWaves_ = 0;
-	std::vector<uint8_t> orderHost_;
-	std::vector<uint16_t> shiftHost_;
+	Tick ticks_[2];
+	int cur_ = 0;
+	double gpuMs_ = 0, spanMs_ = 0;
+	bool started_ = false;
+	hipEvent_t first_ = nullptr;
 	std::chrono::steady_clock::time_point t0_;
 };
 

@@ -275,7 +275,9 @@ typedef struct kpz_job {
 int kpz_server_create(uint64_t hbm_budget_bytes, int max_slots, int max_tick_mcs, int device, kpz_server** out);
 int kpz_server_destroy(kpz_server* s);
 int kpz_server_submit(kpz_server* s, const kpz_job* job, uint64_t* id);
-/* one tick; returns queued + active jobs (>= 0) or a negative error code */
+/* one tick: queues the next tick's work, then waits for the tick queued by
+ * the previous call and finishes it (samples, retirement); returns queued +
+ * active jobs (>= 0) or a negative error code */
 int64_t kpz_server_step(kpz_server* s);
 /* 1 if a finished job is waiting: its sample count and lattice words (0 if not kept) */
 int kpz_server_peek(kpz_server* s, int* nsamples, uint64_t* lattice_words);
@@ -287,6 +289,12 @@ int kpz_server_pop(kpz_server* s, uint64_t* id, double times[3], uint32_t* mcs, 
 int kpz_server_stats(kpz_server* s, uint64_t out[5]);
 /* the server's clock (s since creation; monotonic) */
 double kpz_server_now(const kpz_server* s);
+/* device time of the server: gpu_seconds = time the device spent in ticks
+ * (hipEvents around each tick's work on the server stream), span_seconds =
+ * from the first tick's start to the end of the last finished tick; their
+ * ratio is the fraction of time the device was busy.  Ticks are pipelined:
+ * kpz_server_step queues the next tick before it waits for the previous one. */
+int kpz_server_busy(const kpz_server* s, double* gpu_seconds, double* span_seconds);
 
 #ifdef __cplusplus
 }

@@ -83,6 +83,8 @@ def test_server_results_equal_standalone(budget, slots, tick):
         assert res.t_submit <= res.t_start <= res.t_end
     st = srv.stats()
     assert st["queued"] == 0 and st["active"] == 0 and st["hbm_used"] == 0
+    gpu_s, span_s = srv.busy()
+    assert 0 < gpu_s <= span_s * 1.001 + 1e-4
 
 
 def test_server_rejects_oversized_job():
