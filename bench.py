@@ -232,11 +232,14 @@ def run_serve_bench(K, window_s, rate=200.0, seed=5, kind_name="scaBit", budget=
         if left == 0 and i < n:
             time.sleep(max(0.0, arr[i] - srv.now()))
     st = srv.stats()
+    gpu_s, gpu_span = srv.busy()
     srv.close()
     lat = np.array(lat)
     span = last_done - base
     return {"jobs": n, "kind": kind_name, "rate_per_s": rate, "window_s": window_s,
             "offered_site_updates": work, "site_updates_per_s": st["site_updates"] / span,
+            "site_updates_per_gpu_s": st["site_updates"] / gpu_s if gpu_s > 0 else None,
+            "gpu_busy_frac": gpu_s / gpu_span if gpu_span > 0 else None,
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
             "latency_max_s": float(lat.max()), "makespan_s": span,
             "offered_load_vs_capacity": (work / window_s) / (st["site_updates"] / span)}
@@ -265,27 +268,80 @@ def l1_layout(lx, ly, lmin=2):
     return lm[0], lm[1]
 
 
-def cpu_baseline(seconds=10.0, lx=12):
-    """The oracle's restatement of SchedulerCPUDT on the host cores: T worker
-    threads = T SLCG streams, spawned and joined per domain set, 16x16-site
-    blocks, p=1 q=0, flat IC, on a 4096^2 sample for a bounded time; plus the
-    same on one core."""
-    from oracle import oracle as O
-    cores = max(1, min(16, os.cpu_count() or 1))
+def host_cpu():
+    """The host CPU as this run sees it: model, sockets, physical cores and
+    logical CPUs of the machine (lscpu), and the CPUs this process may run on
+    (its affinity mask, capped by a cgroup CPU quota where one is set)."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {k.strip(): v.strip() for k, v in (ln.split(":", 1) for ln in out.splitlines() if ":" in ln)}
+        info["model"] = kv.get("Model name")
+        sockets = int(kv.get("Socket(s)", "0") or 0)
+        cores = int(kv.get("Core(s) per socket", "0") or 0)
+        info["sockets"] = sockets
+        info["physical_cores"] = sockets * cores if sockets and cores else None
+        info["threads_per_core"] = int(kv.get("Thread(s) per core", "0") or 0) or None
+        info["logical_cpus"] = int(kv.get("CPU(s)", "0") or 0) or None
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    avail = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = max(1, int(int(q) // int(per)))
+    except (OSError, ValueError):
+        pass
+    info["affinity_cpus"] = avail
+    info["cgroup_cpu_quota"] = quota
+    info["usable_cpus"] = min(avail, quota) if quota else avail
+    return info
 
-    def timed(nthreads, budget):
+
+def cpu_baseline(lx=12):
+    """The oracle's restatements of the reference's threaded CPU schedulers,
+    timed on this host with T = every CPU this run may use and with T = 1:
+    SchedulerCPUDT (pthreads spawned and joined per domain set, 16x16-site
+    blocks, p=1 q=0) and SchedulerCPUCBBit (rows split over T workers, per
+    half-sweep; the per-site p=.95 q=.05 path and the bit-parallel p=.5 path),
+    each on a 4096^2 flat sample for a bounded time with T SLCG64 streams;
+    plus, on one core, the reference's CPU-runnable 512^2 configuration, the
+    plain random-sequential SchedulerCPU and SchedulerCPUDB."""
+    from oracle import oracle as O
+    cpu = host_cpu()
+    cores = max(1, cpu["usable_cpus"])
+
+    def timed(run, enc, nthreads, budget, L=lx):
         m = O.Dsfmt(42)
         st = O.Streams(nthreads, m)
-        lat = O.flat(lx, lx, O.ENC_LOCAL)
+        lat = O.flat(L, L, enc)
         n, t0 = 0, time.perf_counter()
         while time.perf_counter() - t0 < budget:
-            O.dt_mcs_threaded(lat, lx, lx, 4, 4, st, m, 1, nthreads)
+            run(lat, L, st, m, nthreads)
             n += 1
-        return n, time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        return {"site_updates_per_s": n * (1 << (2 * L)) / dt, "mcs": n, "seconds": dt, "threads": nthreads,
+                "lattice": f"{1 << L}x{1 << L}"}
 
-    n, dt = timed(cores, seconds)
-    n1, dt1 = timed(1, seconds / 2)
-    N = 1 << (2 * lx)
+    def dt_run(lat, L, st, m, T):
+        O.dt_mcs_threaded(lat, L, L, 4, 4, st, m, 1, T)
+
+    def cb_pq(lat, L, st, m, T):
+        O.cbbit_mcs_threaded(lat, L, L, st, 1, 1, 0.95, 0.05, T)
+
+    def cb_vl3(lat, L, st, m, T):
+        O.cbbit_mcs_threaded(lat, L, L, st, 1, 0, 0.5, 0.0, T)
+
+    dt_all = timed(dt_run, O.ENC_LOCAL, cores, 8.0)
+    dt_one = timed(dt_run, O.ENC_LOCAL, 1, 4.0)
+    pq_all = timed(cb_pq, O.ENC_CBBIT, cores, 5.0)
+    pq_one = timed(cb_pq, O.ENC_CBBIT, 1, 3.0)
+    # the bit-parallel path streams the lattice: a 16384^2 sample (64 MiB)
+    # keeps it out of the caches, as the 1 GiB benchmark lattice is
+    vl3_all = timed(cb_vl3, O.ENC_CBBIT, cores, 2.0, lx + 2)
+    vl3_one = timed(cb_vl3, O.ENC_CBBIT, 1, 2.0, lx + 2)
 
     # the reference's CPU-runnable configuration: SchedulerCPUDT on 512^2,
     # 16x16 blocks, one worker, seed 42, 100 MCS, exact w^2 after every MCS
@@ -321,8 +377,17 @@ def cpu_baseline(seconds=10.0, lx=12):
         ndb += 1
     tdb_s = time.perf_counter() - t0
     db_upd = ((1 << lbx) - 1) * ((1 << lby) - 1) << (18 - lbx - lby)
-    return {"value": n * N / dt, "unit": "site updates/s", "cores": cores, "kind": "port",
-            "value_1core": n1 * N / dt1,
+    side = f"{1 << lx}x{1 << lx}"
+    return {"value": dt_all["site_updates_per_s"], "unit": "site updates/s", "cores": cores, "kind": "port",
+            "value_1core": dt_one["site_updates_per_s"],
+            "host": cpu,
+            "dt": {"all_cores": dt_all, "one_core": dt_one,
+                   "workload": f"SchedulerCPUDT {side}, 16x16 blocks, p=1 q=0, T streams = T threads"},
+            "cbbit_pq": {"all_cores": pq_all, "one_core": pq_one,
+                         "workload": f"SchedulerCPUCBBit {side} ENC_CBBIT, per-site p=.95 q=.05, T row workers"},
+            "cbbit_vl3": {"all_cores": vl3_all, "one_core": vl3_one,
+                          "workload": f"SchedulerCPUCBBit {vl3_all['lattice']} ENC_CBBIT, bit-parallel p=.5 q=0, "
+                                      f"T row workers"},
             "c1_512_dt_1core": {"site_updates_per_s": 100 * (1 << 18) / c1, "seconds": c1,
                                 "workload": "SchedulerCPUDT 512x512, 16x16 blocks, T=1, 100 MCS, w2 every MCS"},
             "plain_rs_512_1core": {"site_updates_per_s": nrs * (1 << 18) / rs,
@@ -330,8 +395,51 @@ def cpu_baseline(seconds=10.0, lx=12):
             "db_512": {"site_updates_per_s": ndb * db_upd / tdb_s, "threads": tdb,
                        "workload": f"SchedulerCPUDB 512x512, {1 << lbx}x{1 << lby} blocks (layoutFromL1 on this "
                                    f"host), dead border, dynamic origin, T={tdb}"},
-            "sample": f"oracle SchedulerCPUDT restatement (pthreads per phase), {1 << lx}x{1 << lx}, 16x16 blocks, "
-                      f"T={cores} streams: {n} MCS in {dt:.1f} s; T=1: {n1} MCS in {dt1:.1f} s"}
+            "sample": f"oracle restatements on flat lattices, T={cores} threads (the CPUs this run may use) "
+                      f"and T=1, each for a bounded time: SchedulerCPUDT {side} {dt_all['mcs']} + {dt_one['mcs']} "
+                      f"MCS, SchedulerCPUCBBit p=.95/q=.05 {side} {pq_all['mcs']} + {pq_one['mcs']} MCS, VL3 "
+                      f"{vl3_all['lattice']} {vl3_all['mcs']} + {vl3_one['mcs']} MCS"}
+
+
+def kernel_roofline(K, key, avg_launch_ms, algo_bytes, fast, slow):
+    """Roofline object of one kernel.  The kernels here are bound by integer
+    VALU issue, not HBM, so the binding roofline is VALU: achieved = the
+    lane operations of one launch (rocprof SQ_INSTS_VALU per launch x 64
+    lanes, profiles/traffic.json) / the live average launch time; peak = the
+    highest issue rate of any VALU instruction measured on this device in this
+    run (v_xor_b32 class, kpz_valu_peak), which no instruction mix can
+    exceed.  Beside it: the HBM roofline (algorithmic bytes / launch time vs
+    8 TB/s), the counted HBM traffic (FETCH_SIZE + WRITE_SIZE, calibrated),
+    and the issue-time bound of the instruction mix (slow-class instructions
+    issue at about half the fast-class rate)."""
+    prof = {}
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        prof = json.load(open(tfile)).get(key, {})
+    t = avg_launch_ms * 1e-3
+    hbm = algo_bytes / t / 1e9
+    out = {"bound": "valu", "achieved": None, "peak": fast / 1e12, "unit": "T lane-ops/s", "frac": None,
+           "traffic": prof.get("bytes_per_launch"), "kernel": key.split("@")[0], "avg_launch_ms": avg_launch_ms,
+           "hbm": {"achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm / HBM_PEAK_GBS,
+                   "bytes_per_launch": algo_bytes},
+           "valu_peaks_measured": {"fast_class_T": fast / 1e12, "slow_class_T": slow / 1e12,
+                                   "nominal_T": VALU_PEAK_LANE_OPS / 1e12}}
+    if prof.get("valu_insts_per_launch"):
+        lane_ops = prof["valu_insts_per_launch"] * 64.0
+        out["achieved"] = lane_ops / t / 1e12
+        out["frac"] = out["achieved"] / out["peak"]
+        out["lane_ops_per_launch"] = lane_ops
+        mix = prof.get("issue_mix")
+        if mix:
+            # issue-time lower bound from the compiled loop's instruction mix
+            # (per unit: slow-class, v_mad_u64_u32 at 1.4 slow slots, fast-class;
+            # everything outside the loop counted as fast class)
+            units = prof["units_per_launch"]
+            rest = lane_ops / units - (mix["slow"] + mix["mad64"] + mix["fast"])
+            bound = units * ((mix["slow"] + 1.4 * mix["mad64"]) / slow + (mix["fast"] + max(rest, 0.0)) / fast)
+            out["issue_bound_ms"] = bound * 1e3
+            out["issue_bound_frac"] = bound / t
+    return out
 
 
 def main():
@@ -366,39 +474,10 @@ def main():
     launches = 4 * args.steps
     avg_launch_ms = mcs_ms / launches
     bytes_per_launch = 0.5 * N / 4  # 0.5 B per site update, N/4 updates per phase
-    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     value = N * args.steps / (total_ms * 1e-3)
-    prof = {}
-    tfile = os.path.join(ROOT, "profiles", "traffic.json")
-    key = f"dt_phase_kernel<4,4,KPZ,static>@{1 << args.lx}"
-    if os.path.exists(tfile):
-        prof = json.load(open(tfile)).get(key, {})
-    traffic = prof.get("bytes_per_launch")
-    # VALU issue: wave instructions per launch (rocprof SQ_INSTS_VALU) x 64
-    # lanes over the live launch time, against the integer VALU rate measured
-    # here (independent v_xad_u32 chains) and the nominal 256 CU x 4 SIMD x 16
-    # lanes x 2.4 GHz
-    valu = None
-    if prof.get("valu_insts_per_launch"):
-        lane_ops = prof["valu_insts_per_launch"] * 64.0
-        rate = lane_ops / (avg_launch_ms * 1e-3)
-        slow, fast = K.valu_peak()
-        # issue-time lower bound of one launch from the instruction mix: the
-        # update loop of dt_phase_kernel<4,4,0,false> (compiled gfx950 code,
-        # per update) issues 6 slow-class instructions (4 v_mul_lo_u32, 2
-        # v_add3_u32), 2 v_mad_u64_u32 (1.4 slow-class slots each, measured)
-        # and 12 fast-class ones (xor, xnor, 4 right shifts incl. the column
-        # mask, 3 bitop3 incl. the row address, sub, or, sub); the rest of the
-        # kernel (staging, write-back, stream set-up: lane_ops / update - 20)
-        # is counted as fast class, so the bound is a lower bound
-        upd = N / 4
-        rest = lane_ops / upd - 20.0
-        bound_s = upd * ((6.0 + 2 * 1.4) / slow + (12.0 + rest) / fast)
-        valu = {"lane_ops_per_launch": lane_ops, "ops_per_update": lane_ops / upd,
-                "achieved_T_lane_ops": rate / 1e12,
-                "peak_T_lane_ops": slow / 1e12, "peak_fast_class_T_lane_ops": fast / 1e12,
-                "nominal_T_lane_ops": VALU_PEAK_LANE_OPS / 1e12,
-                "issue_bound_ms": bound_s * 1e3, "frac": bound_s / (avg_launch_ms * 1e-3)}
+    slow, fast = K.valu_peak()
+    roof = kernel_roofline(K, f"dt_phase_kernel<4,4,KPZ,static>@{1 << args.lx}", avg_launch_ms,
+                           bytes_per_launch, fast, slow)
     line = {
         "metric": METRIC,
         "value": value,
@@ -416,10 +495,7 @@ def main():
                                f"16x16-site DT blocks, one SLCG64 stream per block ({s.threads} streams), "
                                f"{args.steps} MCS, exact w2 at {len(w2)} log-spaced times",
                    "lattice": f"{1 << args.lx}x{1 << args.lx}", "mcs": args.steps, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "dt_phase_kernel<4,4,KPZ,static>", "avg_launch_ms": avg_launch_ms,
-                     "bytes_per_launch": bytes_per_launch, "valu": valu},
+        "roofline": roof,
         "w2_final": w2[-1][1] if w2 else None,
     }
     s.close()
@@ -433,6 +509,11 @@ def main():
             ups = N * steps / (ms * 1e-3)
             sec[name] = {"site_updates_per_s": ups, "hbm_GBs_algorithmic": ups * 1.0 / 1e9,
                          "hbm_frac": ups / 1e9 / HBM_PEAK_GBS, "ms_per_mcs": ms / steps}
+            if name == "scabit_p0.95_q0.05":
+                # the autocorrelator configuration's kernel: one launch per
+                # half-sweep, 2 launches per MCS, 2^31 site updates each
+                line["roofline_c3"] = kernel_roofline(
+                    K, f"cbbit_half_kernel<u64,per-site>@{1 << args.lx}", ms / steps / 2, 1.0 * N / 2, fast, slow)
         # the reference's default "dt" scheduler: DT with a random origin per
         # MCS (SchedulerCPUDTr semantics), p = 1, q = 0, 0.5 B per update
         ms = run_scabit(K, args.lx, 1.0, 0.0, 100, args.seed, ev, K.SchedulerDTr, kpz=True)
@@ -461,7 +542,12 @@ def main():
     if args.batch:
         line["batched"] = run_batch_bench(K, ev, args.batch, args.serial_mcs)
     if args.serve > 0:
-        line["serving"] = run_serve_bench(K, args.serve)
+        over = run_serve_bench(K, args.serve)
+        # a second point below capacity: the same job mix at 0.8x the offered
+        # load the engine sustained in the first run
+        rate = 200.0 * 0.8 / over["offered_load_vs_capacity"]
+        under = run_serve_bench(K, max(args.serve, 4.0), rate=rate, seed=6)
+        line["serving"] = dict(over, at_0p8_capacity=under)
     if args.copy_calibration:
         src = K.SchedulerDT(args.lx, args.lx, layout=(4, 4), threads=1)
         dst = K.SchedulerDT(args.lx, args.lx, layout=(4, 4), threads=1)
