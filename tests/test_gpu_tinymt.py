@@ -166,6 +166,12 @@ SCA_CASES = [
     (8, 6, 0),     # one cell column per worker: every word is shared, all atomics
     (9, 7, 13),
     (6, 9, 2),
+    # at least 16 cells per half strip, so the generate kernel takes its
+    # 16-cell path (four aligned 16-byte stores where a walk runs along a half
+    # row): strips of 64 cells starting on the 16-cell grid, and strips of
+    # 86/85 cells whose halves start off it (scalar and vector stores mix)
+    (9, 5, 2),
+    (10, 4, 3),
 ]
 
 
