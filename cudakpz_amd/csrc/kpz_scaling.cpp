@@ -52,14 +52,6 @@ bool getNum(T& v, int a, int argc, char** argv)
 	return true;
 }
 
-bool getStr(std::string& v, int a, int argc, char** argv)
-{
-	if(a >= argc)
-		return false;
-	v = argv[a];
-	return true;
-}
-
 // SamplingParam (kpz/sampling.h:55-96)
 struct Sampling {
 	enum Kind { EXP, LINEAR, RANDOM } kind = EXP;
@@ -127,26 +119,22 @@ void help()
 	             "  --device n                HIP device\n";
 }
 
-// parseSysArgs (kpz/kpzFrontend.cpp:470-488): mcs [x [y]] first
-int parseSysArgs(Args& A, int argc, char** argv)
+// the leading positional words "mcs [x [y]]" (kpz/kpzFrontend.cpp:470-488);
+// returns the index of the first option word, -1 without an MCS count
+int parsePositional(Args& A, int argc, char** argv)
 {
-	int a = 1;
-	if(!getNum(A.mcs, a, argc, argv))
+	if(!getNum(A.mcs, 1, argc, argv))
 	{
-		std::cerr << "Number of mcs was not supplied.\n";
+		std::cerr << "[EE][kpz_scaling] Number of mcs was not supplied (kpz_scaling mcs [x [y]] ...)\n";
 		return -1;
 	}
-	if(!getNum(A.lx, ++a, argc, argv))
-	{
-		A.lx = A.ly = 10;
-		return a;
-	}
-	if(!getNum(A.ly, ++a, argc, argv))
-	{
-		A.ly = A.lx;
-		return a;
-	}
-	return a + 1;
+	int sizes[2];
+	int n = 0;
+	while(n < 2 && getNum(sizes[n], 2 + n, argc, argv))
+		++n;
+	A.lx = n > 0 ? sizes[0] : 10;
+	A.ly = n > 1 ? sizes[1] : A.lx;
+	return 2 + n;
 }
 
 void pushSampling(Args& A, unsigned start, const Sampling& s)
@@ -157,304 +145,276 @@ void pushSampling(Args& A, unsigned start, const Sampling& s)
 		A.events.push_back(Event{Event::SAMPLING, start, true, s, 0});
 }
 
-// SchedulerConfig::parseArgs (kpz/kpzFrontend.cpp:93-232): the scheduler id
-// and the scheduler options that follow it
-int parseScheduler(Args& A, int argc, char** argv, int a)
+// ---------------------------------------------------------------- option table
+// Options are rows of a table: names and a handler.  A handler reads its
+// values through the cursor: need() takes the next word or fails with the
+// given message, maybe() takes it only when it parses (else the default
+// stays).  Option names and meanings are those of kpz/kpzFrontend.cpp:93-232
+// (scheduler options) and :494-733 (front-end options); scheduler options are
+// looked up first while the words after --scheduler are read, as there.
+struct Cursor {
+	int argc;
+	char** argv;
+	int a;              // index of the option word being handled
+	bool sched = false; // inside the scheduler's own options
+
+	const char* word() const { return argv[a]; }
+	template<class T>
+	bool maybe(T& v)
+	{
+		if(!getNum(v, a + 1, argc, argv))
+			return false;
+		++a;
+		return true;
+	}
+	bool maybeStr(std::string& v)
+	{
+		if(a + 1 >= argc)
+			return false;
+		v = argv[++a];
+		return true;
+	}
+	template<class T>
+	bool need(T& v, const std::string& err)
+	{
+		if(maybe(v))
+			return true;
+		std::cerr << err << '\n';
+		return false;
+	}
+	bool needStr(std::string& v, const std::string& err)
+	{
+		if(maybeStr(v))
+			return true;
+		std::cerr << err << '\n';
+		return false;
+	}
+};
+
+struct Option {
+	std::vector<std::string> names;
+	bool (*run)(Cursor&, Args&);
+};
+
+bool warn(const std::string& msg)
 {
-	if(!getStr(A.scheduler, a, argc, argv))
-	{
-		std::cerr << "Expected scheduler id.\n";
-		return -1;
-	}
-	if(A.scheduler == "cb")
-		A.scheduler = "sca";
-	if(A.scheduler != "dt" && A.scheduler != "dtStatic" && A.scheduler != "scaBit" && A.scheduler != "sca" &&
-	   A.scheduler != "default")
-	{
-		std::cerr << "[EE][SchedulerConfig] No such scheduler: " << A.scheduler
-		          << " (this engine runs dt, dtStatic, scaBit, sca/cb)\n";
-		return -1;
-	}
-	for(++a; a < argc; ++a)
-	{
-		if(!strcmp(argv[a], "--yield") || !strcmp(argv[a], "--anaGPU"))
-			std::cerr << "[WW][SchedulerConfig] " << argv[a] << " has no effect on this engine.\n";
-		else if(!strcmp(argv[a], "--byRank"))
-		{
-			int tmp;
-			if(!getNum(tmp, ++a, argc, argv))
-				--a;
-			std::cerr << "[WW][SchedulerConfig] --byRank: this engine runs on one GPU (use --device).\n";
-		}
-		else if(!strcmp(argv[a], "-l") || !strcmp(argv[a], "--localLayout"))
-		{
-			std::string tmp;
-			if(!getStr(tmp, ++a, argc, argv))
-			{
-				std::cerr << "[EE][SchedulerConfig] expected local layout preset id for --localLayout\n";
-				return -1;
-			}
-			std::cerr << "[WW][SchedulerConfig] --localLayout has no effect on this engine.\n";
-		}
-		else if(!strcmp(argv[a], "--1d"))
-		{
-			std::cerr << "[EE][SchedulerConfig] --1d: the 1+1d schedulers are not part of this engine.\n";
-			return -1;
-		}
-		else if(!strcmp(argv[a], "--mcsThreads"))
-		{
-			if(!getNum(A.mcsThreads, ++a, argc, argv))
-			{
-				std::cerr << "Expected number of threads for --mcsThreads.\n";
-				return -1;
-			}
-			if(A.mcsThreads < 1)
-			{
-				std::cerr << "Invalid number of threads given, irgnoring.\n";
-				A.mcsThreads = 1;
-			}
-		}
-		else if(!strcmp(argv[a], "--ddBlocksize"))
-		{
-			if(!getNum(A.dd[0], ++a, argc, argv))
-			{
-				std::cerr << "Expected ddBlocksize for --ddBlocksize.\n";
-				return -1;
-			}
-			if(!getNum(A.dd[1], ++a, argc, argv))
-			{
-				--a;
-				A.dd[1] = A.dd[0];
-			}
-		}
-		else
-		{
-			--a;
-			break;
-		}
-	}
-	if(a >= argc)
-		return --a;
-	return a;
+	std::cerr << "[WW][kpz_scaling] " << msg << '\n';
+	return true;
 }
 
-// FrontendArgs::parseArg (kpz/kpzFrontend.cpp:494-733)
-int parseArg(Args& A, int argc, char** argv, int a)
+bool refuse(const std::string& msg)
 {
-	if(!strcmp(argv[a], "--scheduler") || !strcmp(argv[a], "--gpuscheduler") || !strcmp(argv[a], "--cpuscheduler"))
-	{
-		int tmp;
-		if(!getNum(tmp, ++a, argc, argv))
-			--a;
-		else if(tmp != 0)
-		{
-			std::cerr << "[EE][FrontendArgs] only scheduler config 0 exists in this engine.\n";
-			return -1;
-		}
-		return parseScheduler(A, argc, argv, ++a);
-	}
-	else if(!strcmp(argv[a], "--nschedulers") || !strcmp(argv[a], "--mcsFct") ||
-	        !strcmp(argv[a], "--retryH5") || !strcmp(argv[a], "--prevMeasureOverride"))
-	{
-		std::cerr << "[EE][FrontendArgs] " << argv[a] << " is not supported by this engine.\n";
-		return -1;
-	}
-	else if(!strcmp(argv[a], "--saveH5") || !strcmp(argv[a], "--loadH5"))
-	{
+	std::cerr << "[EE][kpz_scaling] " << msg << '\n';
+	return false;
+}
+
+// a start time after a sampling option: optional, 0 = replace the current rule
+bool samplingStart(Cursor& c, Args& A, const Sampling& s)
+{
+	unsigned start = 0;
+	c.maybe(start);
+	pushSampling(A, start, s);
+	return true;
+}
+
+const std::vector<Option>& schedulerOptions()
+{
+	static const std::vector<Option> t = {
+		{{"--yield", "--anaGPU"}, [](Cursor& c, Args&) { return warn(std::string(c.word()) + " does nothing on this engine"); }},
+		{{"--byRank"}, [](Cursor& c, Args&) {
+			 int rank;
+			 c.maybe(rank);
+			 return warn("--byRank: one GPU per process here; pick it with --device");
+		 }},
+		{{"-l", "--localLayout"}, [](Cursor& c, Args&) {
+			 std::string id;
+			 if(!c.needStr(id, "--localLayout needs a layout preset id"))
+				 return false;
+			 return warn("--localLayout does nothing on this engine");
+		 }},
+		{{"--1d"}, [](Cursor&, Args&) { return refuse("--1d: this engine has no 1+1d schedulers"); }},
+		{{"--mcsThreads"}, [](Cursor& c, Args& A) {
+			 if(!c.need(A.mcsThreads, "--mcsThreads needs a thread count"))
+				 return false;
+			 if(A.mcsThreads < 1)
+			 {
+				 A.mcsThreads = 1;
+				 warn("--mcsThreads below 1, using 1");
+			 }
+			 return true;
+		 }},
+		{{"--ddBlocksize"}, [](Cursor& c, Args& A) {
+			 if(!c.need(A.dd[0], "--ddBlocksize needs a block size (log2 x [log2 y])"))
+				 return false;
+			 if(!c.maybe(A.dd[1]))
+				 A.dd[1] = A.dd[0];
+			 return true;
+		 }},
+	};
+	return t;
+}
+
+const std::vector<Option>& frontendOptions()
+{
+	static const std::vector<Option> t = {
+		{{"--scheduler", "--gpuscheduler", "--cpuscheduler"}, [](Cursor& c, Args& A) {
+			 int config = 0;
+			 if(c.maybe(config) && config != 0)
+				 return refuse("scheduler configuration " + std::to_string(config) + " does not exist (only 0)");
+			 if(!c.needStr(A.scheduler, "--scheduler needs a scheduler id"))
+				 return false;
+			 if(A.scheduler == "cb")
+				 A.scheduler = "sca";
+			 static const char* const known[] = {"dt", "dtStatic", "scaBit", "sca", "default"};
+			 if(std::find_if(std::begin(known), std::end(known), [&](const char* k) { return A.scheduler == k; }) ==
+			    std::end(known))
+				 return refuse("No such scheduler: " + A.scheduler + " (available: dt, dtStatic, scaBit, sca/cb)");
+			 c.sched = true;
+			 return true;
+		 }},
+		{{"--nschedulers", "--mcsFct", "--retryH5", "--prevMeasureOverride"},
+		 [](Cursor& c, Args&) { return refuse(std::string(c.word()) + " is not supported by this engine"); }},
+		{{"--saveH5", "--loadH5"}, [](Cursor& c, Args& A) {
 #ifdef KPZ_WITH_H5
-		std::string& dst = (argv[a][2] == 's') ? A.saveH5 : A.loadH5;
-		if(!getStr(dst, ++a, argc, argv))
-		{
-			std::cerr << "Expected a HDF5 filname-prefix for " << argv[a - 1] << ".\n";
-			return -1;
-		}
+			 const std::string opt = c.word();
+			 return c.needStr(opt == "--saveH5" ? A.saveH5 : A.loadH5, opt + " needs an HDF5 file-name prefix");
 #else
-		std::cerr << "[EE][FrontendArgs] " << argv[a] << ": kpz_scaling was compiled without HDF5; use --saveState / "
-		          << "--loadState.\n";
-		return -1;
+			 (void)A;
+			 return refuse(std::string(c.word()) + ": built without HDF5; --saveState / --loadState do the same");
 #endif
-	}
-	else if(!strcmp(argv[a], "--saveState") || !strcmp(argv[a], "--loadState"))
+		 }},
+		{{"--saveState", "--loadState"}, [](Cursor& c, Args& A) {
+			 const std::string opt = c.word();
+			 return c.needStr(opt == "--saveState" ? A.saveState : A.loadState, opt + " needs a file name");
+		 }},
+		{{"--disorder2"}, [](Cursor& c, Args& A) {
+			 A.disorderFill = 0.5;
+			 c.maybe(A.disorderFill);
+			 return true;
+		 }},
+		{{"--pq", "--pq0", "--pq1", "--pq2", "--pq3"}, [](Cursor& c, Args& A) {
+			 const std::string opt = c.word();
+			 const int n = opt.size() == 5 ? opt[4] - '0' : 0;
+			 if(!c.need(A.p[n], opt + " needs p (and optionally q)"))
+				 return false;
+			 A.q[n] = 0.;
+			 c.maybe(A.q[n]);
+			 return true;
+		 }},
+		{{"--anaThreads"}, [](Cursor& c, Args& A) {
+			 if(!c.need(A.anaThreads, "--anaThreads needs a thread count"))
+				 return false;
+			 if(A.anaThreads < 1)
+			 {
+				 A.anaThreads = 1;
+				 warn("--anaThreads below 1, using 1");
+			 }
+			 return true;
+		 }},
+		{{"--save", "--noSave"}, [](Cursor& c, Args& A) {
+			 A.save = !strcmp(c.word(), "--save");
+			 return true;
+		 }},
+		{{"--countTagsAsMeasurement", "--noCountTagsAsMeasurement"}, [](Cursor& c, Args& A) {
+			 A.countTags = !strcmp(c.word(), "--countTagsAsMeasurement");
+			 return true;
+		 }},
+		{{"--maxH"}, [](Cursor& c, Args& A) {
+			 float h = 0;
+			 if(!c.maybe(h) || h <= 0.f)
+				 return refuse("--maxH needs a positive number of hours");
+			 A.maxH = h;
+			 return true;
+		 }},
+		{{"--version"}, [](Cursor&, Args&) {
+			 std::cout << VERSION << '\n';
+			 return true;
+		 }},
+		{{"--help"}, [](Cursor&, Args&) {
+			 help();
+			 std::cout << VERSION << '\n';
+			 return true;
+		 }},
+		{{"--sampleRate"}, [](Cursor& c, Args& A) {
+			 Sampling s;
+			 return c.need(s.density, "--sampleRate needs a rate [start MCS]") && samplingStart(c, A, s);
+		 }},
+		{{"--linSample"}, [](Cursor& c, Args& A) {
+			 Sampling s;
+			 s.kind = Sampling::LINEAR;
+			 if(!c.maybe(s.interval) || s.interval == 0)
+				 return refuse("--linSample needs an interval of at least 1 MCS [start MCS]");
+			 return samplingStart(c, A, s);
+		 }},
+		{{"--randomSample"}, [](Cursor& c, Args& A) {
+			 Sampling s;
+			 s.kind = Sampling::RANDOM;
+			 double mean;
+			 if(!c.need(mean, "--randomSample needs a mean interval [start MCS]"))
+				 return false;
+			 s.mean = mean / 2.; // intervals uniform in [mean/2, 3 mean/2]
+			 return samplingStart(c, A, s);
+		 }},
+		{{"--corrStart"}, [](Cursor& c, Args& A) {
+			 unsigned t, stop = 0xFFFFFFFFu;
+			 if(!c.need(t, "--corrStart needs the tag time in MCS [stop MCS]"))
+				 return false;
+			 c.maybe(stop);
+			 A.events.push_back(Event{Event::TAG, t, A.countTags, Sampling{}, stop});
+			 return true;
+		 }},
+		{{"--seed"}, [](Cursor& c, Args& A) {
+			 A.haveSeed = c.need(A.seed, "--seed needs an unsigned 64-bit integer");
+			 return A.haveSeed;
+		 }},
+		{{"--rng"}, [](Cursor& c, Args& A) {
+			 std::string r;
+			 if(!c.maybeStr(r) || (r != "slcg64" && r != "tinymt32"))
+				 return refuse("--rng takes slcg64 or tinymt32");
+			 A.rng = (r == "tinymt32") ? KPZ_RNG_TINYMT32 : KPZ_RNG_SLCG64;
+			 return true;
+		 }},
+		{{"--tinymtParams"}, [](Cursor& c, Args& A) {
+			 return c.needStr(A.tinymtParams, "--tinymtParams needs a file name");
+		 }},
+		{{"--device"}, [](Cursor& c, Args& A) { return c.need(A.device, "--device needs a device ordinal"); }},
+		{{"-f", "--file"}, [](Cursor& c, Args& A) {
+			 return c.needStr(A.file, "-f / --file needs a file name (-- for stdout)");
+		 }},
+	};
+	return t;
+}
+
+const Option* lookup(const std::vector<Option>& table, const char* w)
+{
+	for(const Option& o : table)
+		if(std::find(o.names.begin(), o.names.end(), w) != o.names.end())
+			return &o;
+	return nullptr;
+}
+
+// every word after "mcs [x [y]]"; false on the first bad option
+bool parseOptions(Args& A, int argc, char** argv, int first)
+{
+	Cursor c{argc, argv, first};
+	for(; c.a < argc; ++c.a)
 	{
-		std::string& dst = (argv[a][2] == 's') ? A.saveState : A.loadState;
-		if(!getStr(dst, ++a, argc, argv))
+		const Option* o = c.sched ? lookup(schedulerOptions(), c.word()) : nullptr;
+		if(!o)
 		{
-			std::cerr << "Expected a file name for " << argv[a - 1] << ".\n";
-			return -1;
+			c.sched = false;
+			o = lookup(frontendOptions(), c.word());
 		}
+		if(!o)
+		{
+			if(!strncmp(c.word(), "--pq", 4))
+				return refuse(std::string(c.word()) + ": rate sets are --pq, --pq0 .. --pq3");
+			return refuse(std::string("Unknown argument: ") + c.word());
+		}
+		if(!o->run(c, A))
+			return false;
 	}
-	else if(!strcmp(argv[a], "--disorder2"))
-	{
-		if(!getNum(A.disorderFill, ++a, argc, argv))
-		{
-			A.disorderFill = 0.5;
-			--a;
-		}
-	}
-	else if(!memcmp(argv[a], "--pq", 4))
-	{
-		int n = 0;
-		if(strlen(argv[a]) > 4)
-		{
-			n = argv[a][4] - '0';
-			if(n < 0 || n > 3 || strlen(argv[a]) > 5)
-			{
-				std::cerr << "[EE][FrontendArgs] " << argv[a] << ": rate sets are --pq, --pq0 .. --pq3.\n";
-				return -1;
-			}
-		}
-		if(!getNum(A.p[n], ++a, argc, argv))
-		{
-			std::cerr << "--pq: Expected a double value for p.\n";
-			return -1;
-		}
-		if(!getNum(A.q[n], ++a, argc, argv))
-		{
-			A.q[n] = 0.;
-			--a;
-		}
-	}
-	else if(!strcmp(argv[a], "--anaThreads"))
-	{
-		if(!getNum(A.anaThreads, ++a, argc, argv))
-		{
-			std::cerr << "Expected number of threads for --anaThreads.\n";
-			return -1;
-		}
-		if(A.anaThreads < 1)
-		{
-			std::cerr << "Invalid number of threads given, irgnoring. (--anaThreads)\n";
-			A.anaThreads = 1;
-		}
-	}
-	else if(!strcmp(argv[a], "--noSave"))
-		A.save = false;
-	else if(!strcmp(argv[a], "--save"))
-		A.save = true;
-	else if(!strcmp(argv[a], "--countTagsAsMeasurement"))
-		A.countTags = true;
-	else if(!strcmp(argv[a], "--noCountTagsAsMeasurement"))
-		A.countTags = false;
-	else if(!strcmp(argv[a], "--maxH"))
-	{
-		float maxH;
-		if(!getNum(maxH, ++a, argc, argv) || maxH <= 0.)
-		{
-			std::cerr << "Expected a positive floating point value for --maxH.\n";
-			return -1;
-		}
-		A.maxH = maxH;
-	}
-	else if(!strcmp(argv[a], "--version"))
-		std::cout << VERSION << '\n';
-	else if(!strcmp(argv[a], "--sampleRate"))
-	{
-		Sampling s;
-		if(!getNum(s.density, ++a, argc, argv))
-		{
-			std::cerr << "Expected a floating point value for --sampleRate [rate] [startTime=0].\n";
-			return -1;
-		}
-		unsigned start;
-		if(!getNum(start, ++a, argc, argv))
-		{
-			--a;
-			start = 0;
-		}
-		pushSampling(A, start, s);
-	}
-	else if(!strcmp(argv[a], "--linSample"))
-	{
-		Sampling s;
-		s.kind = Sampling::LINEAR;
-		if(!getNum(s.interval, ++a, argc, argv) || s.interval == 0)
-		{
-			std::cerr << "Expected an unsigned int value as first arg for --linSample.\n";
-			return -1;
-		}
-		unsigned start;
-		if(!getNum(start, ++a, argc, argv))
-		{
-			--a;
-			start = 0;
-		}
-		pushSampling(A, start, s);
-	}
-	else if(!strcmp(argv[a], "--randomSample"))
-	{
-		Sampling s;
-		s.kind = Sampling::RANDOM;
-		double mean;
-		if(!getNum(mean, ++a, argc, argv))
-		{
-			std::cerr << "Expected a floating point value for --randomSample.\n";
-			return -1;
-		}
-		s.mean = mean / 2.; // uniform in [mean/2, mean + mean/2]
-		unsigned start;
-		if(!getNum(start, ++a, argc, argv))
-		{
-			--a;
-			start = 0;
-		}
-		pushSampling(A, start, s);
-	}
-	else if(!strcmp(argv[a], "--corrStart"))
-	{
-		unsigned t, s;
-		if(!getNum(t, ++a, argc, argv))
-		{
-			std::cerr << "[EE][FrontendArgs] Expected a unsigned int for --corrStart.\n";
-			return -1;
-		}
-		if(!getNum(s, ++a, argc, argv))
-		{
-			--a;
-			s = 0xFFFFFFFFu;
-		}
-		A.events.push_back(Event{Event::TAG, t, A.countTags, Sampling{}, s});
-	}
-	else if(!strcmp(argv[a], "--seed"))
-	{
-		if(!getNum(A.seed, ++a, argc, argv))
-		{
-			std::cerr << "Expected an unsigned 64-bit integer for --seed.\n";
-			return -1;
-		}
-		A.haveSeed = true;
-	}
-	else if(!strcmp(argv[a], "--rng"))
-	{
-		std::string r;
-		if(!getStr(r, ++a, argc, argv) || (r != "slcg64" && r != "tinymt32"))
-		{
-			std::cerr << "Expected slcg64 or tinymt32 for --rng.\n";
-			return -1;
-		}
-		A.rng = (r == "tinymt32") ? KPZ_RNG_TINYMT32 : KPZ_RNG_SLCG64;
-	}
-	else if(!strcmp(argv[a], "--tinymtParams"))
-	{
-		if(!getStr(A.tinymtParams, ++a, argc, argv))
-		{
-			std::cerr << "Expected a file name for --tinymtParams.\n";
-			return -1;
-		}
-	}
-	else if(!strcmp(argv[a], "--device"))
-	{
-		if(!getNum(A.device, ++a, argc, argv))
-		{
-			std::cerr << "Expected a device ordinal for --device.\n";
-			return -1;
-		}
-	}
-	else
-	{
-		std::cerr << "[EE][FrontendArgs] Unknown argument: " << argv[a] << '\n';
-		return -1;
-	}
-	return a;
+	return true;
 }
 
 // TinyMT32 parameter sets from tinymt32dc output, the lines
@@ -715,27 +675,11 @@ int main(int argc, char** argv)
 		return 1;
 	}
 	Args A;
-	int a = parseSysArgs(A, argc, argv);
+	int a = parsePositional(A, argc, argv);
 	if(a < 0)
 		return 1;
-	for(; a < argc; ++a)
-	{
-		if(!strcmp(argv[a], "-f") || !strcmp(argv[a], "--file"))
-		{
-			if(!getStr(A.file, ++a, argc, argv))
-			{
-				std::cerr << "Expected a filename for -f / --file.\n";
-				return 1;
-			}
-		}
-		else if(!strcmp(argv[a], "--help"))
-		{
-			help();
-			std::cout << VERSION << '\n';
-		}
-		else if((a = parseArg(A, argc, argv, a)) < 0)
-			return 1;
-	}
+	if(!parseOptions(A, argc, argv, a))
+		return 1;
 	// FrontendArgs::init (kpz/kpzFrontend.cpp:735-745)
 	std::cout << "[MM][FrontendArgs] running " << A.anaThreads << " threads for analysis (per process)\n";
 	std::cout << std::setprecision(12);

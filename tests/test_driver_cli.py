@@ -30,22 +30,22 @@ def test_missing_mcs():
 @pytest.mark.parametrize("args,msg", [
     (["100", "8", "--scheduler", "db4"], "No such scheduler: db4"),
     (["100", "8", "--bogus"], "Unknown argument: --bogus"),
-    (["100", "8", "--pq"], "Expected a double value for p"),
+    (["100", "8", "--pq"], "--pq needs p"),
     (["100", "8", "--pq5", "0.5"], "rate sets are --pq"),
     (["100", "8", "--mcsFct", "nop"], "not supported"),
-    (["100", "8", "--saveH5"], "Expected a HDF5 filname-prefix for --saveH5"),
+    (["100", "8", "--saveH5"], "--saveH5 needs an HDF5 file-name prefix"),
     (["100", "8", "--loadH5", "/nonexistent/ckpt"], "cannot open /nonexistent/ckpt_0_0_0.h5"),
     (["100", "8", "--loadH5", "/nonexistent/ckpt.h5"], "cannot open /nonexistent/ckpt.h5"),
     (["100", "8", "--loadH5", "a", "--loadState", "b"], "exclude each other"),
-    (["100", "8", "--corrStart"], "Expected a unsigned int for --corrStart"),
+    (["100", "8", "--corrStart"], "--corrStart needs the tag time"),
     (["100", "8", "--linSample", "0"], "--linSample"),
-    (["100", "8", "--maxH", "-1"], "positive floating point value for --maxH"),
-    (["100", "8", "--scheduler", "dt", "--mcsThreads"], "Expected number of threads for --mcsThreads"),
-    (["100", "8", "--scheduler", "dt", "--ddBlocksize"], "Expected ddBlocksize"),
+    (["100", "8", "--maxH", "-1"], "--maxH needs a positive number of hours"),
+    (["100", "8", "--scheduler", "dt", "--mcsThreads"], "--mcsThreads needs a thread count"),
+    (["100", "8", "--scheduler", "dt", "--ddBlocksize"], "--ddBlocksize needs a block size"),
     (["100", "8", "--scheduler", "sca", "--1d"], "--1d"),
-    (["100", "8", "-f"], "Expected a filename"),
-    (["100", "8", "--rng", "mt19937"], "Expected slcg64 or tinymt32 for --rng"),
-    (["100", "8", "--tinymtParams"], "Expected a file name for --tinymtParams"),
+    (["100", "8", "-f"], "-f / --file needs a file name"),
+    (["100", "8", "--rng", "mt19937"], "--rng takes slcg64 or tinymt32"),
+    (["100", "8", "--tinymtParams"], "--tinymtParams needs a file name"),
     (["100", "8", "--scheduler", "dt", "--rng", "tinymt32"], "--rng tinymt32 needs --tinymtParams"),
 ])
 def test_argument_errors(args, msg):
