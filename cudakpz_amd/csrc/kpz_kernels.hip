@@ -707,6 +707,50 @@ __global__ __launch_bounds__(256) void tmt_block_polys_kernel(const TmtOrbit* __
 // Site row y of one half-sweep, lane g of the row's group of G = 1 << a.lG
 // lanes (groups are aligned to G within the wave; every lane of the wave
 // must call this, the shuffles are wave-wide).
+// Per-site acceptance bits of one SCABit word: the WB draws of the word from
+// stream state sW, site k -> bit k of pb (minimum accepted: r23 < tp) and qb
+// (maximum accepted: r23 < tq); NT = thresholds per draw: 1 (q = 0), 2 (p and
+// q), 4 (two disorder classes: pb1, qb1 from tp1, tq1).  Only the low 23 bits
+// of lo ^ hi are used: the low state word evolves on its own (lo' = A_lo lo +
+// C mod 2^32) and hi' mod 2^24 needs only hi mod 2^24 and lo, so inside the
+// word the high half is carried mod 2^24 with two 24-bit multiply-adds (the
+// caller rebuilds the full state from the word's start by a jump).  The sign
+// of r23 - t of every draw is copied into its own bit (SignBits32).
+template<typename W, int NT>
+__device__ __forceinline__ void cbbit_word_draws(uint64_t sW, const Affine& step, uint32_t tp, uint32_t tq,
+                                                 uint32_t tp1, uint32_t tq1, W& pb, W& qb, W& pb1, W& qb1)
+{
+	const uint32_t aLo = (uint32_t)step.a, aHi = (uint32_t)(step.a >> 32);
+	const uint64_t cc = step.c;
+	uint32_t lo = (uint32_t)sW, h24 = (uint32_t)(sW >> 32);
+	static_for<(int)sizeof(W) / 4>([&](auto hc) {
+		constexpr int h = decltype(hc)::value;
+		SignBits32 P, Q, P1, Q1;
+		static_for<32>([&](auto kc) {
+			constexpr int k = decltype(kc)::value;
+			const uint64_t pr = (uint64_t)lo * aLo + cc;
+			h24 = hi24_step(h24, aLo, lo, aHi, (uint32_t)(pr >> 32));
+			lo = (uint32_t)pr;
+			const uint32_t r23 = (lo ^ h24) & 0x7FFFFFu;
+			P.template put<k>(r23 - tp);
+			if constexpr(NT >= 2)
+				Q.template put<k>(r23 - tq);
+			if constexpr(NT == 4)
+			{
+				P1.template put<k>(r23 - tp1);
+				Q1.template put<k>(r23 - tq1);
+			}
+		});
+		pb |= (W)P.bits() << (32 * h);
+		qb |= (W)Q.bits() << (32 * h);
+		if constexpr(NT == 4)
+		{
+			pb1 |= (W)P1.bits() << (32 * h);
+			qb1 |= (W)Q1.bits() << (32 * h);
+		}
+	});
+}
+
 template<typename W, int MODE>
 __device__ __forceinline__ void cbbit_row(const CbHalfArgs& a, int y, int g)
 {
@@ -758,13 +802,36 @@ __device__ __forceinline__ void cbbit_row(const CbHalfArgs& a, int y, int g)
 		const W tY = lat[thisB + Q + row + xw];
 		const W xX = lat[othB + row + xw];
 		const W yY = lat[othB + Q + rowUp + xw];
+		// the last lane of a group takes its LSB source from memory (below);
+		// the other lanes load their own xX word again (same cache lines, no
+		// extra HBM traffic) so that the load needs no divergent branch and
+		// stays in flight behind the draws.  Only bit 0 of the source is used,
+		// so only its low 32 bits are loaded.
+		uint32_t nbrMem = 0;
+		if(!par) // uniform: the row's parity
+			nbrMem = reinterpret_cast<const uint32_t*>(lat + othB + row)[(size_t)((g == G - 1) ? ((xw + 1) & (Wn - 1)) : xw) *
+			                                                             (sizeof(W) / 4)];
+		W D = 0; // disorder classes of the word's sites (mode 2)
+		if constexpr(MODE == 2)
+			D = ((const W*)a.dis)[thisB + row + xw];
+		// per-site p/q mode: the word's acceptance bits depend on the stream
+		// alone, so they are drawn before the loaded words are first used and
+		// the loads complete behind the draws
+		W pb = 0, qb = 0, pb1 = 0, qb1 = 0;
+		if constexpr(MODE == 1)
+		{
+			if(tq)
+				cbbit_word_draws<W, 2>(sW, step, tp, tq, 0, 0, pb, qb, pb1, qb1);
+			else
+				cbbit_word_draws<W, 1>(sW, step, tp, tq, 0, 0, pb, qb, pb1, qb1);
+		}
 		// LSB source for the shifted neighbour slope (pSyncRngUpdateVL3,
 		// kpz/schedulerCPUCBBit_nested.inl:226-233): lane g+1, or for the last
 		// lane the first word of the next stride (wrapping to word 0)
-		W nbr = shflw(xX, gUp, G);
+		uint32_t nbr = shflw((uint32_t)xX, gUp, G);
 		if(g == G - 1)
-			nbr = lat[othB + row + ((xw + 1) & (Wn - 1))];
-		const W xXs = par ? xX : (W)((xX >> 1) | (nbr << (WB - 1)));
+			nbr = nbrMem;
+		const W xXs = par ? xX : (W)((xX >> 1) | ((W)nbr << (WB - 1)));
 		const W minm = ~(tX | tY) & xXs & yY;
 		W flip;
 		if constexpr(MODE == 3)
@@ -797,59 +864,11 @@ __device__ __forceinline__ void cbbit_row(const CbHalfArgs& a, int y, int g)
 		else
 		{
 			const W maxm = tX & tY & ~xXs & ~yY;
-			// Only the low 23 bits of lo ^ hi are used.  The low state word
-			// evolves on its own (lo' = A_lo lo + C mod 2^32) and hi' mod 2^24
-			// needs only hi mod 2^24 and lo, so inside the word the high half
-			// is carried mod 2^24 with two 24-bit multiply-adds; the full
-			// state is rebuilt from the word's start by the stepJ jump below.
-			const uint32_t aLo = (uint32_t)step.a, aHi = (uint32_t)(step.a >> 32);
-			const uint64_t cc = step.c;
-			uint32_t lo = (uint32_t)sW, h24 = (uint32_t)(sW >> 32);
-			// acceptance bits, site k of the word -> bit k (SignBits32: the sign of
-			// r23 - t of every draw copied into its own bit)
-			W pb = 0, qb = 0, pb1 = 0, qb1 = 0;
-			const uint32_t tp1 = a.tp1, tq1 = a.tq1;
-			// NT = thresholds per draw: 4 (two disorder classes), 2 (p and q), 1 (q = 0)
-			auto half = [&](auto ntc, int h) {
-				constexpr int NT = decltype(ntc)::value;
-				SignBits32 P, Q, P1, Q1;
-				static_for<32>([&](auto kc) {
-					constexpr int k = decltype(kc)::value;
-					const uint64_t pr = (uint64_t)lo * aLo + cc;
-					h24 = hi24_step(h24, aLo, lo, aHi, (uint32_t)(pr >> 32));
-					lo = (uint32_t)pr;
-					const uint32_t r23 = (lo ^ h24) & 0x7FFFFFu;
-					P.template put<k>(r23 - tp);
-					if constexpr(NT >= 2)
-						Q.template put<k>(r23 - tq);
-					if constexpr(NT == 4)
-					{
-						// both classes' acceptance bits; the map selects per site below
-						P1.template put<k>(r23 - tp1);
-						Q1.template put<k>(r23 - tq1);
-					}
-				});
-				pb |= (W)P.bits() << (32 * h);
-				qb |= (W)Q.bits() << (32 * h);
-				if constexpr(NT == 4)
-				{
-					pb1 |= (W)P1.bits() << (32 * h);
-					qb1 |= (W)Q1.bits() << (32 * h);
-				}
-			};
-#pragma unroll
-			for(int h = 0; h < WB / 32; ++h)
+			if constexpr(MODE == 2)
 			{
-				if constexpr(MODE == 2)
-					half(std::integral_constant<int, 4>{}, h);
-				else if(tq)
-					half(std::integral_constant<int, 2>{}, h);
-				else
-					half(std::integral_constant<int, 1>{}, h);
-			}
-			if(MODE == 2)
-			{
-				const W D = ((const W*)a.dis)[thisB + row + xw];
+				// the disorder mode draws here, after the lattice logic: its four
+				// accumulators then do not share the live range of the loads
+				cbbit_word_draws<W, 4>(sW, step, tp, tq, a.tp1, a.tq1, pb, qb, pb1, qb1);
 				pb = (pb & ~D) | (pb1 & D);
 				qb = (qb & ~D) | (qb1 & D);
 			}
@@ -861,7 +880,7 @@ __device__ __forceinline__ void cbbit_row(const CbHalfArgs& a, int y, int g)
 		lat[thisB + Q + row + xw] = tY ^ flip;
 		lat[othB + Q + rowUp + xw] = yY ^ flip;
 		// carry of the shifted flip into the next word's LSB
-		const W recv = shflw((W)(flip >> (WB - 1)), gDn, G);
+		const W recv = shflw((uint32_t)(flip >> (WB - 1)), gDn, G);
 		W cin;
 		if(g == 0)
 		{

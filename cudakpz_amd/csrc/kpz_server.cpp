@@ -7,6 +7,9 @@
 namespace kpz {
 
 namespace {
+// log2 of the most lanes that share one SCABit row of a job
+constexpr int SERVE_MAX_LANES_LOG2 = 3;
+
 int ceilLog2(size_t v)
 {
 	int l = 0;
@@ -260,7 +263,11 @@ bool Server::admit(Job& j, PinnedStage& stage)
 		d.kind = BK_SCABIT;
 		d.mode = (s.p == 0.5 && s.q == 0.0) ? 0 : 1;
 		d.wide = s.lx >= 7 ? 1 : 0;
-		d.lG = std::min(s.lx - (d.wide ? 7 : 6), 6);
+		// at most 8 lanes per row: a lane then walks >= 8 words of a wide row,
+		// so its stream set-up (dependent loads of the job table, the stream
+		// and its jumps) is spread over >= 512 sites (the choice the batch
+		// autotuner makes on the 4096-replica mix)
+		d.lG = std::min(s.lx - (d.wide ? 7 : 6), SERVE_MAX_LANES_LOG2);
 		j.T = 1ll << s.ly;
 	}
 	else
