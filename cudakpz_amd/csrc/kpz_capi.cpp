@@ -215,6 +215,25 @@ int kpz_roughness(kpz_scheduler* h, int64_t out[3])
 	});
 }
 
+int kpz_roughness_noalloc(const uint32_t* lattice, int lx, int ly, int encoding, int64_t* out, void* stream)
+{
+	if(!lattice || !out)
+		return fail(KPZ_EINVAL, "null lattice or output pointer");
+	if(encoding != KPZ_ENC_LOCAL && encoding != KPZ_ENC_CBBIT)
+		return fail(KPZ_EINVAL, "encoding must be KPZ_ENC_LOCAL or KPZ_ENC_CBBIT");
+	const bool cb = encoding == KPZ_ENC_CBBIT;
+	if(lx < (cb ? 6 : 2) || lx > 30 || ly < (cb ? 1 : 2))
+		return fail(KPZ_EINVAL, "lattice size out of range: lx " + std::to_string(lx) + ", ly " + std::to_string(ly));
+	if(ly > kpz::NOALLOC_MAX_LY)
+		return fail(KPZ_EINVAL, "ly " + std::to_string(ly) + " exceeds the fixed workspace (" +
+		                            std::to_string(kpz::noalloc_workspace_bytes()) + " bytes): at most 2^" +
+		                            std::to_string(kpz::NOALLOC_MAX_LY) + " site rows");
+	const hipError_t e = kpz::launch_roughness_noalloc(lattice, encoding, lx, ly, out, static_cast<hipStream_t>(stream));
+	if(e != hipSuccess)
+		return fail(KPZ_EDEVICE, std::string("kpz_roughness_noalloc: ") + hipGetErrorString(e));
+	return KPZ_OK;
+}
+
 int kpz_set_pq_class(kpz_scheduler* h, double p, double q, unsigned n)
 {
 	HANDLE(h);

@@ -18,6 +18,8 @@
 //    relative to the row's start height; the host combines rows in int128.
 #include "kpz_kernels.hpp"
 
+#include <mutex>
+
 namespace kpz {
 
 namespace {
@@ -2991,6 +2993,69 @@ hipError_t launch_row_combine(const int64_t* rows, int lx, int ly, bool corr, in
 		hipLaunchKernelGGL(row_heights_kernel, dim3((unsigned)((((int64_t)1 << ly) + 255) / 256)), dim3(256), 0,
 		                   stream, part, ly, hs);
 	return hipGetLastError();
+}
+
+// ---- allocation-free roughness -------------------------------------------
+// The row statistics, the combine's per-workgroup records and the totals of
+// launch_roughness_noalloc live in this file-scope device array instead of
+// buffers the library allocates.  In host code `scratch` is only a stand-in
+// for the symbol: its device address (one copy per GPU) comes from
+// hipGetSymbolAddress with that GPU current, resolved on the first call per
+// device and kept in g_scratchDev; kernels and copies get that address only.
+namespace {
+constexpr int64_t NOALLOC_ROW_WORDS = ((int64_t)1 << NOALLOC_MAX_LY) * ROW_STATS;
+constexpr int64_t NOALLOC_PART_WORDS = (((int64_t)1 << NOALLOC_MAX_LY) + RC_ROWS - 1) / RC_ROWS * RC_REC;
+constexpr int64_t NOALLOC_WORDS = NOALLOC_ROW_WORDS + NOALLOC_PART_WORDS + 8;
+static __device__ __align__(256) int64_t scratch[NOALLOC_WORDS];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratchMu;
+int64_t* g_scratchDev[NOALLOC_MAX_DEVICES] = {};
+
+hipError_t scratch_on_current_device(int64_t** out)
+{
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if(e != hipSuccess)
+		return e;
+	if(dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+		return hipErrorInvalidDevice;
+	std::lock_guard<std::mutex> lock(g_scratchMu);
+	if(!g_scratchDev[dev])
+	{
+		void* p = nullptr;
+		e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if(e != hipSuccess)
+			return e;
+		g_scratchDev[dev] = static_cast<int64_t*>(p);
+	}
+	*out = g_scratchDev[dev];
+	return hipSuccess;
+}
+} // namespace
+
+int64_t noalloc_workspace_bytes() { return NOALLOC_WORDS * (int64_t)sizeof(int64_t); }
+
+hipError_t launch_roughness_noalloc(const uint32_t* lat, int enc, int lx, int ly, int64_t* out, hipStream_t stream)
+{
+	if(!lat || !out || ly > NOALLOC_MAX_LY)
+		return hipErrorInvalidValue;
+	int64_t* ws = nullptr;
+	hipError_t e = scratch_on_current_device(&ws);
+	if(e != hipSuccess)
+		return e;
+	int64_t* rows = ws;
+	int64_t* part = ws + NOALLOC_ROW_WORDS;
+	int64_t* res = part + NOALLOC_PART_WORDS;
+	// every word of rows, part and res that the combine reads is written first
+	// by this call's own kernels: nothing in the workspace needs clearing
+	e = launch_row_stats(lat, nullptr, enc, lx, ly, rows, stream);
+	if(e != hipSuccess)
+		return e;
+	e = launch_row_combine(rows, lx, ly, false, nullptr, part, res, stream);
+	if(e != hipSuccess)
+		return e;
+	return hipMemcpyAsync(out, res, 3 * sizeof(int64_t), hipMemcpyDeviceToDevice, stream);
 }
 
 hipError_t launch_fill_flat(uint32_t* lat, int enc, int lx, int ly, hipStream_t stream)

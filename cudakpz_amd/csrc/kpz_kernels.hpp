@@ -158,6 +158,14 @@ hipError_t launch_row_stats(const uint32_t* lat, const uint32_t* snap, int enc, 
 int64_t row_combine_part_words(int ly);
 hipError_t launch_row_combine(const int64_t* rows, int lx, int ly, bool corr, int64_t* hs, int64_t* part,
 		int64_t* res, hipStream_t stream);
+// Roughness totals {sum h, sum h^2 lo, hi} of a lattice in caller-owned device
+// memory into out (device, 3 int64), queued on stream, with every intermediate
+// in a fixed per-device workspace of noalloc_workspace_bytes() (a static device
+// array): no allocation.  Rows up to 2^NOALLOC_MAX_LY.  Calls that overlap on
+// one device share the workspace and race.
+constexpr int NOALLOC_MAX_LY = 16;
+int64_t noalloc_workspace_bytes();
+hipError_t launch_roughness_noalloc(const uint32_t* lat, int enc, int lx, int ly, int64_t* out, hipStream_t stream);
 
 // ---- batched replicas ------------------------------------------------------
 // One device-table entry per replica of a batch.  DT replicas use one stream

@@ -134,6 +134,21 @@ int kpz_copy_system(kpz_scheduler* dst, const kpz_scheduler* src);
 /* exact height sums over all sites, heights as Correlator::set
  * (kpz/correlator.cpp:37-106): out = {sum h, sum h^2 low 64, sum h^2 high 64} */
 int kpz_roughness(kpz_scheduler* h, int64_t out[3]);
+/* No counterpart in the reference: the variant of kpz_roughness for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same sums, of a lattice of
+ * 2^(lx+ly-4) u32 words in `encoding` (KPZ_ENC_*) in the caller's device
+ * memory on the current device; writes {sum h, sum h^2 low 64, high 64} to
+ * `out` (device memory, 3 int64).  Only queues work on `stream` (a
+ * hipStream_t; NULL = the null stream) and returns: no allocation, no
+ * synchronisation.  Intermediate sums use a fixed per-device workspace of
+ * the library (a static device array of about 5.3 MB, held by each GPU the
+ * library runs on until the process ends), resolved on the first call on a
+ * device (make that first call outside stream capture), so at most 2^16 site
+ * rows (ly <= 16; larger ly is rejected with KPZ_EINVAL).  Calls that overlap
+ * on one device share the workspace and race: serialise them (one stream,
+ * or events). */
+int kpz_roughness_noalloc(const uint32_t* lattice, int lx, int ly, int encoding, int64_t* out, void* stream);
 /* height moments <h^k>, k = 1..7 (Roughness::h[] after normAndSetW2,
  * kpz/roughness.cpp:38-43, kpz/schedulerService.h:52-71): per-row double
  * sums on the device, rows added in long double; the 7 values the
