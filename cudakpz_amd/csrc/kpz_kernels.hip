@@ -2648,28 +2648,70 @@ __global__ __launch_bounds__(256) void batch_rowstats_kernel(const BatchRep* __r
 		row_stats_row<ENC_LOCAL, false>(rep.lat, nullptr, rep.lx, rep.ly, (int)w.y, threadIdx.x & 63, o, true);
 }
 
-// exact combination per replica (one thread per replica), int128 sums
-__global__ void batch_combine_kernel(const BatchRep* __restrict__ reps, int nreps, const int64_t* __restrict__ scratch,
-		int64_t* __restrict__ out)
+// exact combination per replica: one workgroup of BC_WG lanes per replica.
+// Lane k takes a contiguous run of rows and sums them relative to the run's
+// start (row y starts at the previous row's start plus its change, as in
+// row_part_kernel); a workgroup scan of the runs' height changes gives every
+// run's true start H, and sum h = A1 + H L n, sum h^2 = A2 + 2 H A1 + H^2 L n
+// (int128) are added over the workgroup.
+constexpr int BC_WG = 256;
+__global__ __launch_bounds__(BC_WG) void batch_combine_kernel(const BatchRep* __restrict__ reps, int nreps,
+		const int64_t* __restrict__ scratch, int64_t* __restrict__ out)
 {
-	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	constexpr int NW = BC_WG / 64;
+	__shared__ int64_t wd[NW];
+	__shared__ __int128 w1[NW], w2[NW];
+	const int r = blockIdx.x;
 	if(r >= nreps)
-		return;
+		return; // uniform per workgroup
 	const BatchRep& rep = reps[r];
-	const int64_t L = 1ll << rep.lx;
-	const int64_t* s = scratch + 4 * rep.rowBase;
-	int64_t prev = 0;
-	__int128 s1 = 0, s2 = 0;
-	for(int y = 0; y < (1 << rep.ly); ++y, s += 4)
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	const int64_t L = 1ll << rep.lx, ny = 1ll << rep.ly;
+	const int64_t per = (ny + BC_WG - 1) / BC_WG;
+	const int64_t y0 = ((int64_t)threadIdx.x * per < ny) ? (int64_t)threadIdx.x * per : ny;
+	const int64_t y1 = (y0 + per < ny) ? y0 + per : ny;
+	const int64_t* s = scratch + 4 * (rep.rowBase + y0);
+	int64_t rel = 0; // start height of the next row, relative to the run's start
+	__int128 a1 = 0, a2 = 0;
+	for(int64_t y = y0; y < y1; ++y, s += 4)
 	{
-		const int64_t hs = prev + (s[3] ? 1 : -1);
-		s1 += (__int128)L * hs + s[0];
-		s2 += (__int128)(L * hs) * hs + (__int128)(2 * hs) * s[0] + s[1];
-		prev = hs + s[2];
+		const int64_t hs = rel + (s[3] ? 1 : -1);
+		a1 += (__int128)L * hs + s[0];
+		a2 += (__int128)(L * hs) * hs + (__int128)(2 * hs) * s[0] + s[1];
+		rel = hs + s[2];
 	}
-	out[3 * r] = (int64_t)s1;
-	out[3 * r + 1] = (int64_t)(uint64_t)s2;
-	out[3 * r + 2] = (int64_t)(s2 >> 64);
+	const int64_t n = y1 - y0;
+	const int64_t inc = wave_incl_scan(rel, lane);
+	if(lane == 63)
+		wd[wv] = inc;
+	__syncthreads();
+	int64_t H = inc - rel;
+#pragma unroll
+	for(int k = 0; k < NW; ++k)
+		if(k < wv)
+			H += wd[k];
+	const __int128 s1 = a1 + (__int128)H * L * n;
+	const __int128 s2 = a2 + (__int128)2 * H * a1 + (__int128)H * H * L * n;
+	const __int128 t1 = wave_sum128(s1), t2 = wave_sum128(s2);
+	if(lane == 0)
+	{
+		w1[wv] = t1;
+		w2[wv] = t2;
+	}
+	__syncthreads();
+	if(threadIdx.x == 0)
+	{
+		__int128 S1 = 0, S2 = 0;
+#pragma unroll
+		for(int k = 0; k < NW; ++k)
+		{
+			S1 += w1[k];
+			S2 += w2[k];
+		}
+		out[3 * r] = (int64_t)S1;
+		out[3 * r + 1] = (int64_t)(uint64_t)S2;
+		out[3 * r + 2] = (int64_t)(S2 >> 64);
+	}
 }
 
 template<int LBX, int LBY, int UPD>
@@ -3129,7 +3171,7 @@ hipError_t launch_batch_roughness(const BatchRep* reps, int nreps, const uint2* 
 	if(nrows > 0)
 		hipLaunchKernelGGL(batch_rowstats_kernel, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0, stream, reps, rows, nrows, scratch);
 	if(nreps > 0)
-		hipLaunchKernelGGL(batch_combine_kernel, dim3((unsigned)((nreps + 63) / 64)), dim3(64), 0, stream, reps, nreps, scratch, out);
+		hipLaunchKernelGGL(batch_combine_kernel, dim3((unsigned)nreps), dim3(BC_WG), 0, stream, reps, nreps, scratch, out);
 	return hipGetLastError();
 }
 
