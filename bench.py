@@ -95,6 +95,41 @@ def run_dt(K, args, ev):
     return s, total_ms, mcs_ms, w2
 
 
+def run_reductions(K, lx, seed, reps=5):
+    """Exact w2 and autocorrelator samples on the lattice of a run (the C2 DT
+    lattice, ENC_LOCAL, and the C3 SCABit lattice, ENC_CBBIT) after a few MCS
+    with a tag taken: the best wall time of `reps` calls of roughness() /
+    correlate() (each queues the reductions, copies 7 int64 back and waits).
+    Algorithmic bytes: the lattice once (N/4 bytes) for w2, lattice and
+    snapshot (N/2) for the correlator."""
+    N = 1 << (2 * lx)
+    out = {}
+    for name, cls, pq in (("dt_enc_local", K.SchedulerDT, None), ("scabit_enc_cbbit", K.SchedulerSCABit, (0.95, 0.05))):
+        s = cls(lx, lx)
+        s.supplySeed(seed)
+        s.initHomogeneous()
+        run = s.mcs
+        if pq:
+            s.setPQ(*pq)
+            run = s.mcsPQSyncRng
+        run(2)
+        s.correlateTag()
+        run(2)
+        s.sync()
+        r = {}
+        for what, fn, nbytes in (("w2", s.roughness, N // 4), ("correlate", s.correlate, N // 2)):
+            best = float("inf")
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                fn()
+                best = min(best, time.perf_counter() - t0)
+            r[what] = {"ms": best * 1e3, "hbm_GBs_algorithmic": nbytes / best / 1e9,
+                       "hbm_frac": nbytes / best / 1e9 / HBM_PEAK_GBS}
+        out[name] = r
+        s.close()
+    return out
+
+
 def run_scabit(K, lx, p, q, steps, seed, ev, cls=None, kpz=False, rng=0, threads=0):
     """steps MCS of mcsPQSyncRng (kpz: mcs) after 2 warm-up MCS, hipEvent time.
     rng = TinyMT32: the engine's default worker count, parameter set t of
@@ -539,6 +574,7 @@ def main():
                          "hbm_frac": ups * bpu / 1e9 / HBM_PEAK_GBS, "ms_per_mcs": ms / steps}
             print(f"# {name}: {ms / steps:.3f} ms/MCS", file=sys.stderr, flush=True)
         line["secondary"] = sec
+        line["reductions"] = run_reductions(K, args.lx, args.seed)
     if args.batch:
         line["batched"] = run_batch_bench(K, ev, args.batch, args.serial_mcs)
     if args.serve > 0:

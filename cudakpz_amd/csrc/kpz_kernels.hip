@@ -257,6 +257,136 @@ __device__ __forceinline__ void dt_writeback_x16(uint32_t* __restrict__ lat, int
 	*reinterpret_cast<uint4*>(rp + c0) = q;
 }
 
+// The first two delta swaps of unzip_bytes: every byte x0 y0 x1 y1 x2 y2 x3 y3
+// (one site row of a word's 4 sites) becomes x0 x1 x2 x3 y0 y1 y2 y3;
+// join_nibbles is its inverse.
+__device__ __forceinline__ uint32_t split_nibbles(uint32_t x)
+{
+	uint32_t t;
+	t = (x ^ (x >> 1)) & 0x22222222u; x ^= t ^ (t << 1);
+	t = (x ^ (x >> 2)) & 0x0C0C0C0Cu; x ^= t ^ (t << 2);
+	return x;
+}
+__device__ __forceinline__ uint32_t join_nibbles(uint32_t x)
+{
+	uint32_t t;
+	t = (x ^ (x >> 2)) & 0x0C0C0C0Cu; x ^= t ^ (t << 2);
+	t = (x ^ (x >> 1)) & 0x22222222u; x ^= t ^ (t << 1);
+	return x;
+}
+
+// The bits of the 5 x 5 words of a random-origin 16 x 16 block that the
+// block owns, for each origin offset (shx, shy) inside its first word (entry
+// 4 shy + shx): X of rows 0..15 at columns 0..16 (16 = the halo column), Y of
+// rows 0..16 (16 = the halo row) at columns 0..15; bit 8 rr + 2 j (X) and
+// + 1 (Y) of word k of word row wk is site (4 k + j - shx, 4 wk + rr - shy).
+struct DtShiftMasks {
+	uint32_t m[16][25];
+};
+constexpr DtShiftMasks make_dt_shift_masks()
+{
+	DtShiftMasks t{};
+	for(int sh = 0; sh < 16; ++sh)
+		for(int wk = 0; wk < 5; ++wk)
+			for(int k = 0; k < 5; ++k)
+			{
+				uint32_t v = 0;
+				for(int rr = 0; rr < 4; ++rr)
+					for(int j = 0; j < 4; ++j)
+					{
+						const int r = 4 * wk + rr - (sh >> 2), x = 4 * k + j - (sh & 3);
+						if(r >= 0 && r < 16 && x >= 0 && x <= 16)
+							v |= 1u << (8 * rr + 2 * j);
+						if(r >= 0 && r <= 16 && x >= 0 && x <= 15)
+							v |= 1u << (8 * rr + 2 * j + 1);
+					}
+				t.m[sh][5 * wk + k] = v;
+			}
+	return t;
+}
+__constant__ DtShiftMasks kDtShiftMasks = make_dt_shift_masks();
+
+// Random-origin DT (DTr) with 16 x 16-site blocks: the block starts shx sites
+// into its first word and shy site rows into its first word row (uniform per
+// launch), so it spans 5 x 5 words (20 x 20 sites).  Stage: per site row the
+// bytes of words 0-3 are gathered and unzipped as in dt_stage_x16, word 4's
+// byte (sites 16-19, split into nibbles once per word row) is appended, and
+// the 20-site X and Y rows are shifted down by shx.
+template<class Tile>
+__device__ __forceinline__ void dt_stage_x16s(const uint32_t* __restrict__ lat, int XW, int mXW, int mYW, int c0,
+                                              int wr0, int shx, int shy, uint32_t (&raw)[5][5], const Tile& my)
+{
+#pragma unroll
+	for(int wk = 0; wk < 5; ++wk)
+	{
+		const uint32_t* rp = lat + (size_t)((wr0 + wk) & mYW) * XW;
+#pragma unroll
+		for(int k = 0; k < 5; ++k)
+			raw[wk][k] = rp[(c0 + k) & mXW];
+	}
+#pragma unroll
+	for(int wk = 0; wk < 5; ++wk)
+	{
+		const uint32_t u4 = split_nibbles(raw[wk][4]);
+		const uint32_t u4y = u4 >> 4; // byte rr: the y nibble of u4's byte rr in bits 0-3
+#pragma unroll
+		for(int rr = 0; rr < 4; ++rr)
+		{
+			const int r = 4 * wk + rr - shy; // launch-uniform
+			if(r < 0 || r > 16)
+				continue;
+			const uint32_t b = __builtin_amdgcn_perm(raw[wk][1], raw[wk][0], perm_sel(rr, 4 + rr, 12, 12))
+			                 | __builtin_amdgcn_perm(raw[wk][3], raw[wk][2], perm_sel(12, 12, rr, 4 + rr));
+			const uint32_t u = unzip_bytes(b);
+			const uint32_t x20 = __builtin_amdgcn_perm(u4, u, perm_sel(0, 2, 4 + rr, 12)) & 0xFFFFFu;
+			const uint32_t y20 = __builtin_amdgcn_perm(u4y, u, perm_sel(1, 3, 4 + rr, 12)) & 0xFFFFFu;
+			if(r < 16)
+				my.setX(r, (x20 >> shx) & 0x1FFFFu);
+			my.setY(r, (y20 >> shx) & 0xFFFFu);
+		}
+	}
+}
+
+// Inverse of dt_stage_x16s: every site row is shifted up by shx and zipped
+// back (sites 0-15 as in dt_writeback_x16, sites 16-19 as nibble pairs that
+// are joined per word row), and each of the 25 words is merged into the word
+// read at staging under the bits of that word the block owns
+// (kDtShiftMasks).  No other lane of the phase writes these words.
+template<class Tile>
+__device__ __forceinline__ void dt_writeback_x16s(uint32_t* __restrict__ lat, int XW, int mXW, int mYW, int c0,
+                                                  int wr0, int shx, int shy, const uint32_t (&raw)[5][5],
+                                                  const Tile& my)
+{
+	const uint32_t* m = kDtShiftMasks.m[4 * shy + shx];
+#pragma unroll
+	for(int wk = 0; wk < 5; ++wk)
+	{
+		uint32_t B[4], Q = 0;
+#pragma unroll
+		for(int rr = 0; rr < 4; ++rr)
+		{
+			const int r = 4 * wk + rr - shy; // launch-uniform
+			const uint32_t tx = (r >= 0 && r < 16) ? my.X(r) : 0u;
+			const uint32_t ty = (r >= 0 && r <= 16) ? my.Y(r) : 0u;
+			const uint32_t x20 = tx << shx, y20 = ty << shx;
+			B[rr] = zip_bytes(__builtin_amdgcn_perm(y20, x20, perm_sel(0, 4, 1, 5)));
+			Q |= (((x20 >> 16) & 0xFu) | ((y20 >> 12) & 0xF0u)) << (8 * rr);
+		}
+		const uint32_t T0 = __builtin_amdgcn_perm(B[1], B[0], perm_sel(0, 4, 2, 6));
+		const uint32_t T1 = __builtin_amdgcn_perm(B[1], B[0], perm_sel(1, 5, 3, 7));
+		const uint32_t T2 = __builtin_amdgcn_perm(B[3], B[2], perm_sel(0, 4, 2, 6));
+		const uint32_t T3 = __builtin_amdgcn_perm(B[3], B[2], perm_sel(1, 5, 3, 7));
+		const uint32_t v[5] = {__builtin_amdgcn_perm(T2, T0, perm_sel(0, 1, 4, 5)),
+		                       __builtin_amdgcn_perm(T3, T1, perm_sel(0, 1, 4, 5)),
+		                       __builtin_amdgcn_perm(T2, T0, perm_sel(2, 3, 6, 7)),
+		                       __builtin_amdgcn_perm(T3, T1, perm_sel(2, 3, 6, 7)), join_nibbles(Q)};
+		uint32_t* rp = lat + (size_t)((wr0 + wk) & mYW) * XW;
+#pragma unroll
+		for(int k = 0; k < 5; ++k)
+			rp[(c0 + k) & mXW] = __builtin_amdgcn_bitop3_b32(raw[wk][k], v[k], m[5 * wk + k], 0xD8); // (raw & ~m) | (v & m)
+	}
+}
+
 // ======================================================================
 // TinyMT32 worker streams: the 127-bit generator of
 // inc/tinyMT/tinymt32.h:70-107 (transition, tempering) and
@@ -444,9 +574,12 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 	// Static 16-wide blocks take the byte-gather path; shifted origins (DTr)
 	// and narrower blocks the general nibble path below.
 	constexpr bool X16 = !RO && LBX == 4;
+	constexpr bool X16S = RO && LBX == 4 && LBY == 4; // DTr, 16 x 16 blocks
 	uint32_t raw[NWY][NWX];
 	if constexpr(X16)
 		dt_stage_x16<NWY>(lat, XW, mXW, mYW, c0, wr0, raw, my);
+	else if constexpr(X16S)
+		dt_stage_x16s(lat, XW, mXW, mYW, c0, wr0, shx, shy, raw, my);
 	else
 	{
 #pragma unroll
@@ -604,6 +737,11 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 	if constexpr(X16)
 	{
 		dt_writeback_x16<NWY>(lat, XW, mXW, mYW, c0, wr0, raw, my);
+		return;
+	}
+	if constexpr(X16S)
+	{
+		dt_writeback_x16s(lat, XW, mXW, mYW, c0, wr0, shx, shy, raw, my);
 		return;
 	}
 	constexpr uint32_t SEGX = MX, SEGY = MY;
@@ -2007,6 +2145,115 @@ __global__ __launch_bounds__(256) void row_stats_local_kernel(const uint32_t* __
 	}
 }
 
+// Row statistics of an ENC_CBBIT lattice (roughness), table-driven as above.
+// Site row y's x slopes are bit i of the row's word j in two quadrants
+// (kpz/systemSize.h:143-165): the even sites x = 64j + 2i in quadrant
+// 2 (y & 1), the odd sites x = 64j + 2i + 1 in quadrant 2 (1 - (y & 1)).  A
+// byte holding a nibble of each (4 even sites in bits 0-3, the 4 odd sites
+// between them in bits 4-7) indexes the 8-site walk table; the bytes of a
+// word pair (e, o) are the bytes of (e & 0x0F0F0F0F) | (o << 4 & 0xF0F0F0F0)
+// (nibbles 0, 2, 4, 6) and of (e >> 4 & 0x0F0F0F0F) | (o & 0xF0F0F0F0)
+// (nibbles 1, 3, 5, 7).  One wave per site row, CB_RS_K word pairs per lane
+// and round (sums per lane and round stay below 2^31).
+constexpr int CB_RS_K = 4;
+
+__global__ __launch_bounds__(256) void row_stats_cbbit_kernel(const uint32_t* __restrict__ lat, int lx, int ly,
+		int64_t* __restrict__ out)
+{
+	__shared__ uint32_t tab[256];
+	{
+		// entry of index i: bit j = site 2j, bit 4 + j = site 2j + 1 (j < 4);
+		// (q + 8) | (A + 36) << 5 | B << 12
+		const int i = threadIdx.x;
+		int p = 0, A = 0, B = 0;
+#pragma unroll
+		for(int k = 0; k < 8; ++k)
+		{
+			p += ((i >> ((k & 1) ? 4 + (k >> 1) : (k >> 1))) & 1) ? 1 : -1;
+			A += p;
+			B += p * p;
+		}
+		tab[i] = (uint32_t)(p + 8) | ((uint32_t)(A + 36) << 5) | ((uint32_t)B << 12);
+	}
+	__syncthreads();
+	const int y = blockIdx.x * 4 + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
+	if(y >= (1 << ly))
+		return; // uniform per wave; no barrier below
+	const int WR = 1 << (lx - 6); // words per row and quadrant
+	const size_t sw = (size_t)1 << (lx + ly - 4);
+	const size_t rowW = (size_t)y << (lx - 6);
+	const uint32_t* __restrict__ ev = lat + rowW + ((y & 1) ? (sw >> 1) : 0);
+	const uint32_t* __restrict__ od = lat + rowW + ((y & 1) ? 0 : (sw >> 1));
+	const int K = WR < CB_RS_K ? WR : CB_RS_K;
+	const int span = 64 * K;
+	int64_t S1 = 0, S2 = 0;
+	int base = 0;
+	for(int w0 = 0; w0 < WR; w0 += span)
+	{
+		const int wl = w0 + lane * K;
+		const bool live = wl < WR;
+		int c = 0, s1 = 0, s2 = 0;
+		auto chunk = [&](uint32_t idx) {
+			const uint32_t e = tab[idx];
+			const int q = (int)(e & 31u) - 8, A = (int)((e >> 5) & 127u) - 36, B = (int)(e >> 12);
+			s1 += 8 * c + A;
+			s2 += 8 * c * c + 2 * c * A + B;
+			c += q;
+		};
+		auto pair = [&](uint32_t e, uint32_t o) {
+			const uint32_t lo = __builtin_amdgcn_bitop3_b32(e, o << 4, 0x0F0F0F0Fu, 0xE4); // nibbles 0, 2, 4, 6
+			const uint32_t hi = __builtin_amdgcn_bitop3_b32(e >> 4, o, 0x0F0F0F0Fu, 0xE4); // nibbles 1, 3, 5, 7
+#pragma unroll
+			for(int k = 0; k < 4; ++k)
+			{
+				chunk((lo >> (8 * k)) & 0xFFu);
+				chunk((hi >> (8 * k)) & 0xFFu);
+			}
+		};
+		if(live && K == CB_RS_K)
+		{
+			const uint4 a = *reinterpret_cast<const uint4*>(ev + wl);
+			const uint4 b = *reinterpret_cast<const uint4*>(od + wl);
+			pair(a.x, b.x);
+			pair(a.y, b.y);
+			pair(a.z, b.z);
+			pair(a.w, b.w);
+		}
+		else if(live)
+		{
+			for(int k = 0; k < K; ++k)
+				pair(ev[wl + k], od[wl + k]);
+		}
+		const int n = live ? 64 * K : 0;
+		int incl = c;
+#pragma unroll
+		for(int o = 1; o < 64; o <<= 1)
+		{
+			const int u = __shfl_up(incl, o, 64);
+			if(lane >= o)
+				incl += u;
+		}
+		const int64_t C = (int64_t)base + (incl - c);
+		S1 += (int64_t)n * C + s1;
+		S2 += (int64_t)n * C * C + 2 * C * s1 + s2;
+		base += __shfl(incl, 63, 64);
+	}
+	const int64_t L = (int64_t)1 << lx;
+	const int64_t sp = wave_sum(S1), spp = wave_sum(S2);
+	if(lane == 0)
+	{
+		const int64_t D = base;
+		const int64_t s0 = (ev[0] & 1u) ? 1 : -1;
+		int64_t* o = out + (size_t)y * ROW_STATS;
+		o[0] = sp - L * s0 + D;
+		o[1] = spp - 2 * s0 * sp + L + D * D;
+		o[2] = D;
+		o[3] = ybit0<ENC_CBBIT>(lat, lx, ly, y);
+		o[4] = o[5] = o[6] = o[7] = o[8] = o[9] = 0;
+	}
+}
+
 // The same against a snapshot (correlateTag), 4 sites per lookup: a raw
 // byte indexes the walk table of one lattice ((q + 4) | (A + 10) << 4 |
 // B << 9 over its 4 x slopes, y bits ignored), and the two bytes' x slopes
@@ -3011,6 +3258,9 @@ hipError_t launch_row_stats(const uint32_t* lat, const uint32_t* snap, int enc, 
 	{
 		if(snap)
 			hipLaunchKernelGGL((row_stats_kernel<ENC_CBBIT, true>), grid, dim3(256), 0, stream, lat, snap, lx, ly, out);
+		else if(lx >= 6)
+			// one wave per site row, the 8-site walk table
+			hipLaunchKernelGGL(row_stats_cbbit_kernel, grid, dim3(256), 0, stream, lat, lx, ly, out);
 		else
 			hipLaunchKernelGGL((row_stats_kernel<ENC_CBBIT, false>), grid, dim3(256), 0, stream, lat, snap, lx, ly, out);
 	}
