@@ -27,7 +27,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "site updates/s (DT, SCABit @65536²); batched replica-MCS/s vs serial"
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
+VALU_SIMDS = 256 * 4       # MI355X: 256 CUs x 4 SIMDs of 16 lanes
+CLOCK_HZ = 2.4e9           # peak engine clock
+VALU_PEAK_LANE_OPS = VALU_SIMDS * 16 * CLOCK_HZ
 
 
 class HipEvents:
@@ -436,17 +438,18 @@ def cpu_baseline(lx=12):
                       f"{vl3_all['lattice']} {vl3_all['mcs']} + {vl3_one['mcs']} MCS"}
 
 
-def kernel_roofline(K, key, avg_launch_ms, algo_bytes, fast, slow):
+def kernel_roofline(K, key, avg_launch_ms, algo_bytes, fast, slow, mix=None):
     """Roofline object of one kernel.  The kernels here are bound by integer
     VALU issue, not HBM, so the binding roofline is VALU: achieved = the
     lane operations of one launch (rocprof SQ_INSTS_VALU per launch x 64
     lanes, profiles/traffic.json) / the live average launch time; peak = the
     highest issue rate of any VALU instruction measured on this device in this
     run (v_xor_b32 class, kpz_valu_peak), which no instruction mix can
-    exceed.  Beside it: the HBM roofline (algorithmic bytes / launch time vs
-    8 TB/s), the counted HBM traffic (FETCH_SIZE + WRITE_SIZE, calibrated),
-    and the issue-time bound of the instruction mix (slow-class instructions
-    issue at about half the fast-class rate)."""
+    exceed.  Beside it: the measured rate of the kernel's own instruction mix
+    (kpz_valu_mix_rate; a fast-class instruction next to slow-class ones
+    issues at about the slow rate, so this is the bound the kernel can
+    reach), the HBM roofline (algorithmic bytes / launch time vs 8 TB/s) and
+    the counted HBM traffic (FETCH_SIZE + WRITE_SIZE, calibrated)."""
     prof = {}
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
@@ -464,16 +467,22 @@ def kernel_roofline(K, key, avg_launch_ms, algo_bytes, fast, slow):
         out["achieved"] = lane_ops / t / 1e12
         out["frac"] = out["achieved"] / out["peak"]
         out["lane_ops_per_launch"] = lane_ops
-        mix = prof.get("issue_mix")
         if mix:
-            # issue-time lower bound from the compiled loop's instruction mix
-            # (per unit: slow-class, v_mad_u64_u32 at 1.4 slow slots, fast-class;
-            # everything outside the loop counted as fast class)
-            units = prof["units_per_launch"]
-            rest = lane_ops / units - (mix["slow"] + mix["mad64"] + mix["fast"])
-            bound = units * ((mix["slow"] + 1.4 * mix["mad64"]) / slow + (mix["fast"] + max(rest, 0.0)) / fast)
-            out["issue_bound_ms"] = bound * 1e3
-            out["issue_bound_frac"] = bound / t
+            # the same 8-instruction mix as the kernel's update loop (slow-class
+            # share), measured on this device in this run: a fast-class
+            # instruction next to slow-class ones issues at about the slow rate
+            out["mix_peak"] = {"slow_of_8": mix[0], "T_lane_ops": mix[1] / 1e12,
+                               "frac": out["achieved"] / (mix[1] / 1e12)}
+        if prof.get("valu_dual_issue_quads_per_launch") is not None and prof.get("units_per_launch"):
+            # VALU issue time of one launch: one quad-cycle (a wave64 instruction
+            # on a 16-lane SIMD) per instruction, minus the quad-cycles that
+            # issued two (SQ_ACTIVE_INST_VALU2), v_mad_u64_u32 at 1.4 (its
+            # measured rate), over the 1024 SIMDs of the 256 CUs at the
+            # 2.4 GHz peak engine clock
+            quads = (prof["valu_insts_per_launch"] - prof["valu_dual_issue_quads_per_launch"]
+                     + 0.4 * prof["units_per_launch"] * prof.get("mad64_per_unit", 0) / 64.0)
+            busy = quads * 4.0 / (VALU_SIMDS * CLOCK_HZ)
+            out["valu_issue_busy"] = {"quad_cycles_per_launch": quads, "issue_ms": busy * 1e3, "frac": busy / t}
     return out
 
 
@@ -511,8 +520,12 @@ def main():
     bytes_per_launch = 0.5 * N / 4  # 0.5 B per site update, N/4 updates per phase
     value = N * args.steps / (total_ms * 1e-3)
     slow, fast = K.valu_peak()
+    # both hot loops are 3 of 8 slow-class instructions: DT 8 of 20 per update
+    # (v_mad_u64_u32, v_mul_lo_u32, v_add3_u32), per-site SCABit 3 of 8 per
+    # site (v_mad_u64_u32, 2 v_mad_u32_u24)
+    mix = (3, K.valu_mix_rate(3))
     roof = kernel_roofline(K, f"dt_phase_kernel<4,4,KPZ,static>@{1 << args.lx}", avg_launch_ms,
-                           bytes_per_launch, fast, slow)
+                           bytes_per_launch, fast, slow, mix)
     line = {
         "metric": METRIC,
         "value": value,
@@ -548,7 +561,8 @@ def main():
                 # the autocorrelator configuration's kernel: one launch per
                 # half-sweep, 2 launches per MCS, 2^31 site updates each
                 line["roofline_c3"] = kernel_roofline(
-                    K, f"cbbit_half_kernel<u64,per-site>@{1 << args.lx}", ms / steps / 2, 1.0 * N / 2, fast, slow)
+                    K, f"cbbit_half_kernel<u64,per-site>@{1 << args.lx}", ms / steps / 2, 1.0 * N / 2, fast, slow,
+                    mix)
         # the reference's default "dt" scheduler: DT with a random origin per
         # MCS (SchedulerCPUDTr semantics), p = 1, q = 0, 0.5 B per update
         ms = run_scabit(K, args.lx, 1.0, 0.0, 100, args.seed, ev, K.SchedulerDTr, kpz=True)

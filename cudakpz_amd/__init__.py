@@ -787,3 +787,15 @@ def valu_peak(device=-1, reps=3):
     slow, fast = C.c_double(), C.c_double()
     _check(L.kpz_valu_peak(int(device), int(reps), C.byref(slow), C.byref(fast)))
     return slow.value, fast.value
+
+
+def valu_mix_rate(slow_of_8, device=-1, reps=3):
+    """Measured issue rate (lane operations per second) of blocks of 8
+    independent VALU instructions, `slow_of_8` (0, 2, 3, 4, 5 or 8) of them
+    v_mul_lo_u32 and the rest v_xor_b32: the VALU roofline of a kernel whose
+    instruction stream has that slow-class fraction."""
+    L = lib()
+    L.kpz_valu_mix_rate.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    r = C.c_double()
+    _check(L.kpz_valu_mix_rate(int(device), int(reps), int(slow_of_8), C.byref(r)))
+    return r.value

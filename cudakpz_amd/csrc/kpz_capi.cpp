@@ -497,8 +497,45 @@ int kpz_valu_peak(int device, int reps, double* slow_class, double* fast_class)
 			}
 			*(fast ? fast_class : slow_class) = best;
 		}
-		hipEventDestroy(e0);
-		hipEventDestroy(e1);
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+	});
+}
+
+int kpz_valu_mix_rate(int device, int reps, int slow_of_8, double* lane_ops_per_s)
+{
+	if(!lane_ops_per_s || reps < 1)
+		return fail(KPZ_EINVAL, "null output or reps < 1");
+	if(slow_of_8 != 0 && (slow_of_8 < 2 || slow_of_8 > 5) && slow_of_8 != 8)
+		return fail(KPZ_EINVAL, "slow_of_8 must be 0, 2, 3, 4, 5 or 8");
+	return guard([&] {
+		if(device >= 0)
+			kpz::check(hipSetDevice(device), "hipSetDevice");
+		int dev = 0, cus = 0;
+		kpz::check(hipGetDevice(&dev), "hipGetDevice");
+		kpz::check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+		const unsigned blocks = (unsigned)(cus > 0 ? cus : 1) * 32u; // 8 waves per SIMD
+		const int iters = 1024;
+		kpz::DevBuf out((size_t)blocks * 256 * 4);
+		hipEvent_t e0, e1;
+		kpz::check(hipEventCreate(&e0), "event");
+		kpz::check(hipEventCreate(&e1), "event");
+		double best = 0;
+		for(int r = 0; r <= reps; ++r) // the first launch warms up
+		{
+			kpz::check(hipEventRecord(e0, nullptr), "event");
+			kpz::check(kpz::launch_valu_mix(out.as<uint32_t>(), blocks, iters, slow_of_8, nullptr), "valu mix");
+			kpz::check(hipEventRecord(e1, nullptr), "event");
+			kpz::check(hipEventSynchronize(e1), "event sync");
+			float ms = 0;
+			kpz::check(hipEventElapsedTime(&ms, e0, e1), "event time");
+			const double ops = (double)blocks * 256.0 * iters * 8.0;
+			if(r > 0 && ms > 0)
+				best = std::max(best, ops / (ms * 1e-3));
+		}
+		*lane_ops_per_s = best;
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
 	});
 }
 

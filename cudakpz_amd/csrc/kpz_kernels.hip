@@ -67,7 +67,6 @@ __device__ __forceinline__ uint32_t hi24_step(uint32_t h, uint32_t aLo, uint32_t
 }
 
 __device__ __forceinline__ uint32_t shflw(uint32_t v, int src, int width) { return (uint32_t)__shfl((int)v, src, width); }
-__device__ __forceinline__ uint64_t shflw(uint64_t v, int src, int width) { return shfl64(v, src, width); }
 
 // Acceptance bits of 32 consecutive draws, draw k -> bit k.  The difference
 // d = r - t of a draw's r < 2^23 (or 2^24) and a threshold t <= 2^23 (2^24)
@@ -2767,6 +2766,42 @@ __global__ __launch_bounds__(256) void valu_probe_kernel(uint32_t* __restrict__ 
 	out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3 ^ a4 ^ a5 ^ a6 ^ a7;
 }
 
+// Issue rate of an instruction mix: blocks of 8 independent VALU instructions,
+// NS of them v_mul_lo_u32 (slow class) spread evenly among v_xor_b32 (fast
+// class).  On gfx950 runs of fast-class instructions issue at about twice the
+// rate of slow-class ones, but a fast-class instruction next to a slow-class
+// one issues at about the slow rate, so a kernel's bound is the rate of its
+// own mix, not the fast-class peak.
+#define KPZ_VS(i) "v_mul_lo_u32 %" #i ", %" #i ", %8\n\t"
+#define KPZ_VF(i) "v_xor_b32 %" #i ", %8, %" #i "\n\t"
+#define KPZ_MIX_ASM(body) \
+	asm volatile(body : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5), "+v"(a6), "+v"(a7) : "v"(kv))
+template<int NS>
+__global__ __launch_bounds__(256) void valu_mix_kernel(uint32_t* __restrict__ out, int iters, uint32_t k1)
+{
+	uint32_t a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, a4 = a0 + 4, a5 = a0 + 5, a6 = a0 + 6, a7 = a0 + 7;
+	const uint32_t kv = (k1 ^ threadIdx.x) | 1u;
+	for(int it = 0; it < iters; ++it)
+	{
+		if constexpr(NS == 2)
+			KPZ_MIX_ASM(KPZ_VS(0) KPZ_VF(1) KPZ_VF(2) KPZ_VF(3) KPZ_VS(4) KPZ_VF(5) KPZ_VF(6) KPZ_VF(7));
+		else if constexpr(NS == 3)
+			KPZ_MIX_ASM(KPZ_VS(0) KPZ_VF(1) KPZ_VF(2) KPZ_VS(3) KPZ_VF(4) KPZ_VF(5) KPZ_VS(6) KPZ_VF(7));
+		else if constexpr(NS == 4)
+			KPZ_MIX_ASM(KPZ_VS(0) KPZ_VF(1) KPZ_VS(2) KPZ_VF(3) KPZ_VS(4) KPZ_VF(5) KPZ_VS(6) KPZ_VF(7));
+		else if constexpr(NS == 5)
+			KPZ_MIX_ASM(KPZ_VS(0) KPZ_VS(1) KPZ_VF(2) KPZ_VS(3) KPZ_VS(4) KPZ_VF(5) KPZ_VS(6) KPZ_VF(7));
+		else if constexpr(NS == 8)
+			KPZ_MIX_ASM(KPZ_VS(0) KPZ_VS(1) KPZ_VS(2) KPZ_VS(3) KPZ_VS(4) KPZ_VS(5) KPZ_VS(6) KPZ_VS(7));
+		else
+			KPZ_MIX_ASM(KPZ_VF(0) KPZ_VF(1) KPZ_VF(2) KPZ_VF(3) KPZ_VF(4) KPZ_VF(5) KPZ_VF(6) KPZ_VF(7));
+	}
+	out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3 ^ a4 ^ a5 ^ a6 ^ a7;
+}
+#undef KPZ_MIX_ASM
+#undef KPZ_VF
+#undef KPZ_VS
+
 __global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n)
 {
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -3431,6 +3466,21 @@ hipError_t launch_valu_probe(uint32_t* out, unsigned blocks, int iters, bool fas
 		hipLaunchKernelGGL(valu_probe_kernel<1>, dim3(blocks), dim3(256), 0, stream, out, iters, 0x9E3779B9u, 0x7F4A7C15u);
 	else
 		hipLaunchKernelGGL(valu_probe_kernel<0>, dim3(blocks), dim3(256), 0, stream, out, iters, 0x9E3779B9u, 0x7F4A7C15u);
+	return hipGetLastError();
+}
+
+hipError_t launch_valu_mix(uint32_t* out, unsigned blocks, int iters, int slow, hipStream_t stream)
+{
+	switch(slow)
+	{
+		case 0: hipLaunchKernelGGL(valu_mix_kernel<0>, dim3(blocks), dim3(256), 0, stream, out, iters, 0x7F4A7C15u); break;
+		case 2: hipLaunchKernelGGL(valu_mix_kernel<2>, dim3(blocks), dim3(256), 0, stream, out, iters, 0x7F4A7C15u); break;
+		case 3: hipLaunchKernelGGL(valu_mix_kernel<3>, dim3(blocks), dim3(256), 0, stream, out, iters, 0x7F4A7C15u); break;
+		case 4: hipLaunchKernelGGL(valu_mix_kernel<4>, dim3(blocks), dim3(256), 0, stream, out, iters, 0x7F4A7C15u); break;
+		case 5: hipLaunchKernelGGL(valu_mix_kernel<5>, dim3(blocks), dim3(256), 0, stream, out, iters, 0x7F4A7C15u); break;
+		case 8: hipLaunchKernelGGL(valu_mix_kernel<8>, dim3(blocks), dim3(256), 0, stream, out, iters, 0x7F4A7C15u); break;
+		default: return hipErrorInvalidValue;
+	}
 	return hipGetLastError();
 }
 

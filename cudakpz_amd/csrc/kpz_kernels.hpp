@@ -223,5 +223,8 @@ hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s
 // integer VALU issue-rate probe: blocks x 256 lanes x iters x 8 v_xad_u32
 // (fast: v_xor_b32); out needs blocks * 256 u32
 hipError_t launch_valu_probe(uint32_t* out, unsigned blocks, int iters, bool fast, hipStream_t stream);
+// blocks of 8 independent instructions, `slow` (0, 2, 3, 4, 5 or 8) of them
+// v_mul_lo_u32 and the rest v_xor_b32, 8 chains per lane
+hipError_t launch_valu_mix(uint32_t* out, unsigned blocks, int iters, int slow, hipStream_t stream);
 
 } // namespace kpz

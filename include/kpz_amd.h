@@ -223,6 +223,14 @@ void* kpz_stream(const kpz_scheduler* h);
  * gfx950; fast_class (may be NULL): v_xor_b32, the rate of plain xor, add
  * and bitop3, about twice as high. */
 int kpz_valu_peak(int device, int reps, double* slow_class, double* fast_class);
+/* Measurement utility, no counterpart in the reference: the issue rate of an
+ * instruction mix, lane operations per second, of blocks of 8 independent
+ * VALU instructions of which slow_of_8 (0, 2, 3, 4, 5 or 8) are v_mul_lo_u32
+ * (slow class) and the rest v_xor_b32 (fast class), 8 waves per SIMD.  On
+ * gfx950 a fast-class instruction next to slow-class ones issues at about the
+ * slow rate, so this, at a kernel's own slow fraction, is the VALU roofline
+ * of that kernel. */
+int kpz_valu_mix_rate(int device, int reps, int slow_of_8, double* lane_ops_per_s);
 
 /* ---------------------------------------------------------------------
  * Batched replicas (no counterpart in the reference: CudaKpz runs one

@@ -50,6 +50,17 @@ def test_null_arguments_are_rejected(lib):
     assert b"null" in lib.kpz_last_error()
 
 
+def test_measurement_utilities_reject_bad_arguments(lib):
+    lib.kpz_last_error.restype = C.c_char_p
+    r = C.c_double()
+    for bad in (1, 6, 7, 9, -1):
+        assert lib.kpz_valu_mix_rate(-1, 1, bad, C.byref(r)) == -1  # rejected before any device call
+        assert b"slow_of_8" in lib.kpz_last_error()
+    assert lib.kpz_valu_mix_rate(-1, 0, 3, C.byref(r)) == -1
+    assert lib.kpz_valu_mix_rate(-1, 1, 3, None) == -1
+    assert lib.kpz_valu_peak(-1, 1, None, None) == -1
+
+
 def test_python_mirror_fails_loudly_without_library(monkeypatch, tmp_path):
     import cudakpz_amd as K
     monkeypatch.setattr(K, "LIB_PATH", str(tmp_path / "missing.so"))
