@@ -393,25 +393,34 @@ __device__ __forceinline__ void dt_writeback_x16s(uint32_t* __restrict__ lat, in
 // planes of T words: status[0..3], mat1, mat2, tmat (the layout of
 // TinyMT32::d_Random, kpz/GPUTinyMT.cu:38-61, and of its `.tinyMT` dataset).
 // ======================================================================
+// v + v as v_add_u32: left shifts issue at half the rate of adds on gfx950,
+// and the compiler turns v + v written in C into a shift
+__device__ __forceinline__ uint32_t twice(uint32_t v)
+{
+	uint32_t r;
+	asm("v_add_u32 %0, %1, %1" : "=v"(r) : "v"(v));
+	return r;
+}
+
 struct TinyMT {
 	uint32_t s0, s1, s2, s3, m1, m2, tm;
 
+	// the transition with three-input logic ops: 0x6A is (a & b) ^ c, 0x96 a ^ b ^ c
 	__device__ __forceinline__ void next()
 	{
-		uint32_t y = s3;
-		uint32_t x = (s0 & 0x7FFFFFFFu) ^ s1 ^ s2;
-		x ^= x << 1;
-		y ^= (y >> 1) ^ x;
+		uint32_t x = __builtin_amdgcn_bitop3_b32(s0, 0x7FFFFFFFu, s1, 0x6A) ^ s2;
+		x ^= twice(x);
+		const uint32_t y = __builtin_amdgcn_bitop3_b32(s3, s3 >> 1, x, 0x96);
 		const uint32_t m = 0u - (y & 1u);
 		s0 = s1;
-		s1 = s2 ^ (m & m1);
-		s2 = x ^ (y << 10) ^ (m & m2);
+		s1 = __builtin_amdgcn_bitop3_b32(m, m1, s2, 0x6A);
+		s2 = __builtin_amdgcn_bitop3_b32(m, m2, x, 0x6A) ^ (y << 10);
 		s3 = y;
 	}
 	__device__ __forceinline__ uint32_t temper() const
 	{
 		const uint32_t t1 = s0 + (s2 >> 8);
-		return (s3 ^ t1) ^ ((0u - (t1 & 1u)) & tm);
+		return __builtin_amdgcn_bitop3_b32(0u - (t1 & 1u), tm, s3, 0x6A) ^ t1;
 	}
 	__device__ __forceinline__ uint32_t u32()
 	{
@@ -1426,34 +1435,32 @@ __global__ __launch_bounds__(256) void cbbit_tmt_gen_kernel(CbHalfArgs a, int lC
 		}
 		else
 		{
-			// acceptance bits: the sign of rf - t (both at most 2^24) funnel-shifted
-			// into an accumulator per 32 sites, which ends bit-reversed; the
-			// disorder mode keeps both classes' bits and selects per site
+			// acceptance bits: the sign of rf - t (both at most 2^24) of draw k on
+			// bit k of the word (SignBits32); the disorder mode keeps both
+			// classes' bits and selects per site
 			W pb = 0, qb = 0, pb1 = 0, qb1 = 0;
-#pragma unroll
-			for(int h = 0; h < WB / 32; ++h)
-			{
-				uint32_t pa = 0, qa = 0, pa1 = 0, qa1 = 0;
-#pragma unroll 8
-				for(int k = 0; k < 32; ++k)
-				{
+			static_for<WB / 32>([&](auto hc) {
+				constexpr int h = decltype(hc)::value;
+				SignBits32 pa, qa, pa1, qa1;
+				static_for<32>([&](auto kc) {
+					constexpr int k = decltype(kc)::value;
 					const uint32_t rf = g.u32() >> 8;
-					pa = __builtin_amdgcn_alignbit(pa, rf - a.tp, 31);
-					qa = __builtin_amdgcn_alignbit(qa, rf - a.tq, 31);
-					if(MODE == 2)
+					pa.template put<k>(rf - a.tp);
+					qa.template put<k>(rf - a.tq);
+					if constexpr(MODE == 2)
 					{
-						pa1 = __builtin_amdgcn_alignbit(pa1, rf - a.tp1, 31);
-						qa1 = __builtin_amdgcn_alignbit(qa1, rf - a.tq1, 31);
+						pa1.template put<k>(rf - a.tp1);
+						qa1.template put<k>(rf - a.tq1);
 					}
-				}
-				pb |= (W)__brev(pa) << (32 * h);
-				qb |= (W)__brev(qa) << (32 * h);
-				if(MODE == 2)
+				});
+				pb |= (W)pa.bits() << (32 * h);
+				qb |= (W)qa.bits() << (32 * h);
+				if constexpr(MODE == 2)
 				{
-					pb1 |= (W)__brev(pa1) << (32 * h);
-					qb1 |= (W)__brev(qa1) << (32 * h);
+					pb1 |= (W)pa1.bits() << (32 * h);
+					qb1 |= (W)qa1.bits() << (32 * h);
 				}
-			}
+			});
 			if(MODE == 2)
 			{
 				const W cl = dis[row + xw]; // class: the site's x-slope bit of the map
