@@ -14,8 +14,9 @@
 //    crosses lanes with wave shuffles.  Per-site random numbers come from
 //    each worker's stream via precomputed affine jump-ahead, so any lane
 //    order reproduces the CPU counterpart's sequential draw order.
-//  * Reductions: one wave per site row computes exact int64 partial sums
-//    relative to the row's start height; the host combines rows in int128.
+//  * Reductions: one wave per site row (or word row) computes exact int64
+//    sums relative to the row's start height from LDS walk tables (8 sites
+//    per lookup); two small passes combine the rows in int128 on the device.
 #include "kpz_kernels.hpp"
 
 #include <mutex>
@@ -635,7 +636,7 @@ __device__ __forceinline__ void dt_block(const DtPhaseArgs& a, int64_t i, int64_
 		// class rows of the disorder map (c(m_disorder): the site's 2 bits,
 		// kpz/schedulerCPUBaseThreadedRng.inl:50-75): byte rr of a word is
 		// one site row, 2 bits per site, so a row is 2*BX bits
-		constexpr uint32_t DM = (BX == 16) ? 0xFFFFFFFFu : ((1u << (2 * BX)) - 1u);
+		constexpr uint32_t DM = (uint32_t)((1ull << (2 * BX)) - 1u);
 #pragma unroll
 		for(int wk = 0; wk < NWY; ++wk)
 		{
@@ -2260,131 +2261,301 @@ __global__ __launch_bounds__(256) void row_stats_cbbit_kernel(const uint32_t* __
 	}
 }
 
-// The same against a snapshot (correlateTag), 4 sites per lookup: a raw
-// byte indexes the walk table of one lattice ((q + 4) | (A + 10) << 4 |
-// B << 9 over its 4 x slopes, y bits ignored), and the two bytes' x slopes
-// interleaved index the cross table (sum of P P' over the 4 sites, + 16).
-// Cross sums: sum (C + c)(C' + c') = n C C' + C s1' + C' s1 + s11, and the
-// heights' convention gives S11 = sum P P' - s0' sum P - s0 sum P' + L s0 s0'
-// + D D'.  tcs (+1 per equal slope bit, -1 per different one) is summed per
-// word into the word row's first row (the host adds the rows).
-constexpr int RSC_K = 8; // words per lane and round
+// The correlator against a snapshot, 8 sites per lookup.  A byte of each
+// lattice (8 consecutive x slopes of a row) indexes the 8-site walk tables of
+// that lattice, as in the roughness kernels, and the two bytes together index
+// the cross table (the sum of P P' over the 8 sites of the two walks from 0).
+// Byte layouts: ENC_LOCAL bit 2k = site k, bit 2k+1 = site 4+k (k < 4);
+// ENC_CBBIT bit j = site 2j, bit 4+j = site 2j+1.  The walk table packs a
+// chunk's end q, sum A and square sum B in one u32 ((q + 8) | (A + 36) << 5 |
+// B << 12); the cross table is 65536 int16.  The 129 KiB of tables fill most
+// of a CU's LDS, so a workgroup of 1024 lanes builds them
+// once and its 16 waves then take units (ENC_LOCAL: word rows of 4 site rows;
+// ENC_CBBIT: site rows) in a grid-stride loop.  Sums per lane and round stay
+// below 2^31 (at most 256 sites per lane and round).
+constexpr int CORR_WG = 1024;
 
-__global__ __launch_bounds__(256) void row_corr_local_kernel(const uint32_t* __restrict__ lat,
-		const uint32_t* __restrict__ snap, int lx, int ly, int64_t* __restrict__ out)
+template<int ENC>
+__device__ __forceinline__ int corr_site_bit(uint32_t idx, int i)
 {
-	__shared__ uint32_t own[256], cross[256];
+	if(ENC == ENC_LOCAL)
+		return (int)(idx >> (i < 4 ? 2 * i : 2 * (i - 4) + 1)) & 1;
+	return (int)(idx >> ((i & 1) ? 4 + (i >> 1) : (i >> 1))) & 1;
+}
+
+struct CorrTables {
+	uint32_t* own; // the 8-site walk of a byte, packed
+	int16_t* X;    // [v | w << 8]: sum of P P' of the two walks
+};
+
+// running sums of one site row over a lane's chunks, relative to the lane's
+// first site: c, cs the walks' ends so far
+struct CorrAcc {
+	int c = 0, cs = 0, s1 = 0, s2 = 0, t1 = 0, t2 = 0, x11 = 0;
+	// chunk of bytes v (this lattice), w (snapshot); xa = the cross entry's byte
+	// address (2 (v | w << 8))
+	__device__ __forceinline__ void chunk(const CorrTables& T, uint32_t v, uint32_t w, uint32_t xa)
 	{
-		const int i = threadIdx.x;
-		int p = 0, A = 0, B = 0, p1 = 0, p2 = 0, X = 0;
-#pragma unroll
-		for(int k = 0; k < 4; ++k)
-		{
-			p += ((i >> (2 * k)) & 1) ? 1 : -1;
-			A += p;
-			B += p * p;
-			p1 += ((i >> (2 * k)) & 1) ? 1 : -1;
-			p2 += ((i >> (2 * k + 1)) & 1) ? 1 : -1;
-			X += p1 * p2;
-		}
-		own[i] = (uint32_t)(p + 4) | ((uint32_t)(A + 10) << 4) | ((uint32_t)B << 9);
-		cross[i] = (uint32_t)(X + 16);
+		const uint32_t e = T.own[v], f = T.own[w];
+		const int X = *reinterpret_cast<const int16_t*>(reinterpret_cast<const char*>(T.X) + xa);
+		const int q = (int)(e & 31u) - 8, A = (int)((e >> 5) & 127u) - 36, B = (int)(e >> 12);
+		const int qs = (int)(f & 31u) - 8, As = (int)((f >> 5) & 127u) - 36, Bs = (int)(f >> 12);
+		s1 += 8 * c + A;
+		s2 += c * (8 * c + 2 * A) + B;
+		t1 += 8 * cs + As;
+		t2 += cs * (8 * cs + 2 * As) + Bs;
+		x11 += c * (8 * cs + As) + cs * A + X;
+		c += q;
+		cs += qs;
+		// one running value per sum: without this the compiler regroups the
+		// additions of all the round's chunks into trees whose terms wait in
+		// registers
+		asm("" : "+v"(c), "+v"(cs), "+v"(s1), "+v"(s2), "+v"(t1), "+v"(t2), "+v"(x11));
 	}
-	__syncthreads();
-	const int yw = blockIdx.x * 4 + (threadIdx.x >> 6);
-	const int lane = threadIdx.x & 63;
-	if(yw >= (1 << (ly - 2)))
-		return; // uniform per wave; no barrier below
+};
+
+// the lane's round sums into the row totals: a wave scan of the chunk ends
+// gives every lane's offsets C, Cs; n sites per lane this round
+struct CorrRow {
+	int64_t S1 = 0, S2 = 0, T1 = 0, T2 = 0, S11 = 0;
+	int base = 0, baseS = 0;
+	__device__ __forceinline__ void add(const CorrAcc& a, int n, int lane)
+	{
+		int incl = a.c, incs = a.cs;
+#pragma unroll
+		for(int o = 1; o < 64; o <<= 1)
+		{
+			const int u = __shfl_up(incl, o, 64), us = __shfl_up(incs, o, 64);
+			if(lane >= o)
+			{
+				incl += u;
+				incs += us;
+			}
+		}
+		const int64_t C = (int64_t)base + (incl - a.c), Cs = (int64_t)baseS + (incs - a.cs);
+		S1 += (int64_t)n * C + a.s1;
+		S2 += (int64_t)n * C * C + 2 * C * a.s1 + a.s2;
+		T1 += (int64_t)n * Cs + a.t1;
+		T2 += (int64_t)n * Cs * Cs + 2 * Cs * a.t1 + a.t2;
+		S11 += (int64_t)n * C * Cs + C * a.t1 + Cs * a.s1 + a.x11;
+		base += __shfl(incl, 63, 64);
+		baseS += __shfl(incs, 63, 64);
+	}
+	// lane 0: the row's ROW_STATS values (heights of Correlator::set: the walk
+	// starts at site 1 and site 0 comes last, s0 / s0s its x slopes)
+	__device__ __forceinline__ void write(int64_t* o, int lx, int64_t s0, int64_t s0s, int64_t y0, int64_t y0s,
+	                                      int64_t tcs)
+	{
+		const int64_t L = (int64_t)1 << lx;
+		const int64_t sp = wave_sum(S1), spp = wave_sum(S2), tp = wave_sum(T1), tpp = wave_sum(T2),
+		              x = wave_sum(S11);
+		if((threadIdx.x & 63) == 0)
+		{
+			const int64_t D = base, Ds = baseS;
+			o[0] = sp - L * s0 + D;
+			o[1] = spp - 2 * s0 * sp + L + D * D;
+			o[2] = D;
+			o[3] = y0;
+			o[4] = tp - L * s0s + Ds;
+			o[5] = Ds;
+			o[6] = y0s;
+			o[7] = x - s0s * sp - s0 * tp + L * s0 * s0s + D * Ds;
+			o[8] = tcs;
+			o[9] = tpp - 2 * s0s * tp + L + Ds * Ds;
+		}
+	}
+};
+
+// four chunks from a word V of this lattice's bytes and a word W of the
+// snapshot's (byte k of each: chunk k); acc(k) is the accumulator of chunk k
+template<class Acc>
+__device__ __forceinline__ void corr_chunks4(const CorrTables& T, uint32_t V, uint32_t W, Acc&& acc)
+{
+	// the cross entries' byte addresses, two per v_perm: [v_k, w_k, v_k+1, w_k+1]
+	// doubled
+	const uint32_t p01 = __builtin_amdgcn_perm(W, V, 0x05010400u), p23 = __builtin_amdgcn_perm(W, V, 0x07030602u);
+	const uint32_t x0 = twice(p01) & 0x1FFFEu, x1 = (p01 >> 15) & 0x1FFFEu;
+	const uint32_t x2 = twice(p23) & 0x1FFFEu, x3 = (p23 >> 15) & 0x1FFFEu;
+	acc(0).chunk(T, V & 0xFFu, W & 0xFFu, x0);
+	acc(1).chunk(T, (V >> 8) & 0xFFu, (W >> 8) & 0xFFu, x1);
+	acc(2).chunk(T, (V >> 16) & 0xFFu, (W >> 16) & 0xFFu, x2);
+	acc(3).chunk(T, V >> 24, W >> 24, x3);
+}
+
+constexpr int CORR_LOCAL_K = 16; // words per lane and round (ENC_LOCAL): 64 sites of each row
+constexpr int CORR_CB_K = 4;     // word positions per lane and round (ENC_CBBIT): 256 sites
+
+// ENC_LOCAL word row yw (site rows 4 yw .. 4 yw + 3); needs 2^(lx-2) >= 2
+__device__ __forceinline__ void corr_unit_local(const CorrTables& T, const uint32_t* __restrict__ lat,
+		const uint32_t* __restrict__ snap, int lx, int ly, int yw, int lane, int64_t* __restrict__ out)
+{
 	const int XW = 1 << (lx - 2);
-	const int K = XW < RSC_K ? XW : RSC_K;
+	const int K = XW < CORR_LOCAL_K ? XW : CORR_LOCAL_K; // even
 	const int span = 64 * K;
 	const size_t rowOff = (size_t)yw << (lx - 2);
 	const uint32_t* __restrict__ ra = lat + rowOff;
 	const uint32_t* __restrict__ rb = snap + rowOff;
-	int64_t S1[4] = {0, 0, 0, 0}, S2[4] = {0, 0, 0, 0}, T1[4] = {0, 0, 0, 0}, T2[4] = {0, 0, 0, 0};
-	int64_t S11[4] = {0, 0, 0, 0};
-	int base[4] = {0, 0, 0, 0}, baseS[4] = {0, 0, 0, 0};
+	CorrRow R[4];
 	int tcs = 0;
 	for(int w0 = 0; w0 < XW; w0 += span)
 	{
 		const int wl = w0 + lane * K;
 		const bool live = wl < XW;
-		int c[4] = {0, 0, 0, 0}, cs[4] = {0, 0, 0, 0};
-		int s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0}, t1[4] = {0, 0, 0, 0}, t2[4] = {0, 0, 0, 0};
-		int x11[4] = {0, 0, 0, 0};
-		if(live)
+		CorrAcc A[4];
+		// 8 sites of each row r from the word pair (a, b): byte r of a holds
+		// sites 0..3 on its even bits, byte r of b sites 4..7 (moved to the odd
+		// bits), so byte r of V = (a & 0x55..) | (2b & 0xAA..) is row r's chunk
+		auto pair = [&](uint32_t a, uint32_t b, uint32_t as, uint32_t bs) {
+			tcs += 2 * (__popc(~(a ^ as)) + __popc(~(b ^ bs))) - 64; // +1 per equal bit, -1 per different one
+			const uint32_t V = __builtin_amdgcn_bitop3_b32(a, twice(b), 0x55555555u, 0xE4);
+			const uint32_t W = __builtin_amdgcn_bitop3_b32(as, twice(bs), 0x55555555u, 0xE4);
+			corr_chunks4(T, V, W, [&](int r) -> CorrAcc& { return A[r]; });
+		};
+		if(live && K == CORR_LOCAL_K)
 		{
-			for(int k = 0; k < K; ++k)
-			{
-				const uint32_t a = ra[wl + k], b = rb[wl + k];
-				tcs += 2 * __popc(~(a ^ b)) - 32;
 #pragma unroll
-				for(int r = 0; r < 4; ++r)
-				{
-					const uint32_t v = (a >> (8 * r)) & 0xFFu, w = (b >> (8 * r)) & 0xFFu;
-					const uint32_t e = own[v], f = own[w];
-					const int X = (int)cross[(v & 0x55u) | ((w << 1) & 0xAAu)] - 16;
-					const int q = (int)(e & 15u) - 4, A = (int)((e >> 4) & 31u) - 10, B = (int)(e >> 9);
-					const int qs = (int)(f & 15u) - 4, As = (int)((f >> 4) & 31u) - 10, Bs = (int)(f >> 9);
-					s1[r] += 4 * c[r] + A;
-					s2[r] += 4 * c[r] * c[r] + 2 * c[r] * A + B;
-					t1[r] += 4 * cs[r] + As;
-					t2[r] += 4 * cs[r] * cs[r] + 2 * cs[r] * As + Bs;
-					x11[r] += 4 * c[r] * cs[r] + c[r] * As + cs[r] * A + X;
-					c[r] += q;
-					cs[r] += qs;
-				}
+			for(int k = 0; k < CORR_LOCAL_K; k += 4)
+			{
+				const uint4 a = *reinterpret_cast<const uint4*>(ra + wl + k);
+				const uint4 b = *reinterpret_cast<const uint4*>(rb + wl + k);
+				pair(a.x, a.y, b.x, b.y);
+				pair(a.z, a.w, b.z, b.w);
 			}
+		}
+		else if(live)
+		{
+			for(int k = 0; k < K; k += 2)
+				pair(ra[wl + k], ra[wl + k + 1], rb[wl + k], rb[wl + k + 1]);
 		}
 		const int n = live ? 4 * K : 0;
 #pragma unroll
 		for(int r = 0; r < 4; ++r)
-		{
-			int incl = c[r], incs = cs[r];
-#pragma unroll
-			for(int o = 1; o < 64; o <<= 1)
-			{
-				const int u = __shfl_up(incl, o, 64), us = __shfl_up(incs, o, 64);
-				if(lane >= o)
-				{
-					incl += u;
-					incs += us;
-				}
-			}
-			const int64_t C = (int64_t)base[r] + (incl - c[r]), Cs = (int64_t)baseS[r] + (incs - cs[r]);
-			S1[r] += (int64_t)n * C + s1[r];
-			S2[r] += (int64_t)n * C * C + 2 * C * s1[r] + s2[r];
-			T1[r] += (int64_t)n * Cs + t1[r];
-			T2[r] += (int64_t)n * Cs * Cs + 2 * Cs * t1[r] + t2[r];
-			S11[r] += (int64_t)n * C * Cs + C * t1[r] + Cs * s1[r] + x11[r];
-			base[r] += __shfl(incl, 63, 64);
-			baseS[r] += __shfl(incs, 63, 64);
-		}
+			R[r].add(A[r], n, lane);
 	}
-	const int64_t L = (int64_t)1 << lx;
+	// tcs counts both slope bits of all 4 site rows: the word row's first row
+	// carries it (the host adds the rows)
 	const int64_t tcsRow = wave_sum(tcs);
 #pragma unroll
 	for(int r = 0; r < 4; ++r)
 	{
-		const int64_t sp = wave_sum(S1[r]), spp = wave_sum(S2[r]);
-		const int64_t tp = wave_sum(T1[r]), tpp = wave_sum(T2[r]), x = wave_sum(S11[r]);
-		if(lane == 0)
+		const int y = 4 * yw + r;
+		const int64_t s0 = ((ra[0] >> (8 * r)) & 1u) ? 1 : -1, s0s = ((rb[0] >> (8 * r)) & 1u) ? 1 : -1;
+		R[r].write(out + (size_t)y * ROW_STATS, lx, s0, s0s, ybit0<ENC_LOCAL>(lat, lx, ly, y),
+		           ybit0<ENC_LOCAL>(snap, lx, ly, y), r == 0 ? tcsRow : 0);
+	}
+}
+
+// ENC_CBBIT site row y: x slopes of the even sites in quadrant 2 (y & 1), of
+// the odd sites in quadrant 2 (1 - (y & 1)), y slopes in quadrants 1 and 3
+// (kpz/systemSize.h:143-165); tcs counts all four quadrants' bits of the row
+__device__ __forceinline__ void corr_unit_cbbit(const CorrTables& T, const uint32_t* __restrict__ lat,
+		const uint32_t* __restrict__ snap, int lx, int ly, int y, int lane, int64_t* __restrict__ out)
+{
+	const int WR = 1 << (lx - 6); // words per row and quadrant
+	const size_t q = (size_t)1 << (lx + ly - 6);
+	const size_t rowW = (size_t)y << (lx - 6);
+	const size_t oe = rowW + ((y & 1) ? 2 * q : 0), oo = rowW + ((y & 1) ? 0 : 2 * q);
+	const int K = WR < CORR_CB_K ? WR : CORR_CB_K;
+	const int span = 64 * K;
+	CorrRow R;
+	int tcs = 0;
+	for(int w0 = 0; w0 < WR; w0 += span)
+	{
+		const int wl = w0 + lane * K;
+		const bool live = wl < WR;
+		CorrAcc A;
+		// word position j: 64 sites; (e, o) the even / odd sites' x slopes;
+		// byte k of lo holds sites 16k .. 16k+7, byte k of hi the next 8
+		auto word = [&](uint32_t e, uint32_t o, uint32_t es, uint32_t os, uint32_t y1, uint32_t y3, uint32_t y1s,
+		                uint32_t y3s) {
+			tcs += 2 * (__popc(~(e ^ es)) + __popc(~(o ^ os)) + __popc(~(y1 ^ y1s)) + __popc(~(y3 ^ y3s))) - 128;
+			const uint32_t lo = __builtin_amdgcn_bitop3_b32(e, o << 4, 0x0F0F0F0Fu, 0xE4);   // nibbles 0, 2, 4, 6
+			const uint32_t hi = __builtin_amdgcn_bitop3_b32(e >> 4, o, 0x0F0F0F0Fu, 0xE4);   // nibbles 1, 3, 5, 7
+			const uint32_t los = __builtin_amdgcn_bitop3_b32(es, os << 4, 0x0F0F0F0Fu, 0xE4);
+			const uint32_t his = __builtin_amdgcn_bitop3_b32(es >> 4, os, 0x0F0F0F0Fu, 0xE4);
+			// chunk order along the row: lo byte 0, hi byte 0, lo byte 1, ...
+			const uint32_t V0 = __builtin_amdgcn_perm(hi, lo, 0x05010400u), V1 = __builtin_amdgcn_perm(hi, lo, 0x07030602u);
+			const uint32_t W0 = __builtin_amdgcn_perm(his, los, 0x05010400u), W1 = __builtin_amdgcn_perm(his, los, 0x07030602u);
+			corr_chunks4(T, V0, W0, [&](int) -> CorrAcc& { return A; });
+			corr_chunks4(T, V1, W1, [&](int) -> CorrAcc& { return A; });
+		};
+		if(live && K == CORR_CB_K)
 		{
-			const int y = 4 * yw + r;
-			const int64_t D = base[r], Ds = baseS[r];
-			const int64_t s0 = ((ra[0] >> (8 * r)) & 1u) ? 1 : -1, s0s = ((rb[0] >> (8 * r)) & 1u) ? 1 : -1;
-			int64_t* o = out + (size_t)y * ROW_STATS;
-			o[0] = sp - L * s0 + D;
-			o[1] = spp - 2 * s0 * sp + L + D * D;
-			o[2] = D;
-			o[3] = ybit0<ENC_LOCAL>(lat, lx, ly, y);
-			o[4] = tp - L * s0s + Ds;
-			o[5] = Ds;
-			o[6] = ybit0<ENC_LOCAL>(snap, lx, ly, y);
-			o[7] = x - s0s * sp - s0 * tp + L * s0 * s0s + D * Ds;
-			o[8] = (r == 0) ? tcsRow : 0;
-			o[9] = tpp - 2 * s0s * tp + L + Ds * Ds;
+			const uint4 e = *reinterpret_cast<const uint4*>(lat + oe + wl);
+			const uint4 o = *reinterpret_cast<const uint4*>(lat + oo + wl);
+			const uint4 es = *reinterpret_cast<const uint4*>(snap + oe + wl);
+			const uint4 os = *reinterpret_cast<const uint4*>(snap + oo + wl);
+			const uint4 y1 = *reinterpret_cast<const uint4*>(lat + q + rowW + wl);
+			const uint4 y3 = *reinterpret_cast<const uint4*>(lat + 3 * q + rowW + wl);
+			const uint4 y1s = *reinterpret_cast<const uint4*>(snap + q + rowW + wl);
+			const uint4 y3s = *reinterpret_cast<const uint4*>(snap + 3 * q + rowW + wl);
+			word(e.x, o.x, es.x, os.x, y1.x, y3.x, y1s.x, y3s.x);
+			word(e.y, o.y, es.y, os.y, y1.y, y3.y, y1s.y, y3s.y);
+			word(e.z, o.z, es.z, os.z, y1.z, y3.z, y1s.z, y3s.z);
+			word(e.w, o.w, es.w, os.w, y1.w, y3.w, y1s.w, y3s.w);
 		}
+		else if(live)
+		{
+			for(int k = 0; k < K; ++k)
+			{
+				const size_t j = (size_t)(wl + k);
+				word(lat[oe + j], lat[oo + j], snap[oe + j], snap[oo + j], lat[q + rowW + j], lat[3 * q + rowW + j],
+				     snap[q + rowW + j], snap[3 * q + rowW + j]);
+			}
+		}
+		R.add(A, live ? 64 * K : 0, lane);
+	}
+	const int64_t tcsRow = wave_sum(tcs);
+	const int64_t s0 = (lat[oe] & 1u) ? 1 : -1, s0s = (snap[oe] & 1u) ? 1 : -1;
+	R.write(out + (size_t)y * ROW_STATS, lx, s0, s0s, ybit0<ENC_CBBIT>(lat, lx, ly, y),
+	        ybit0<ENC_CBBIT>(snap, lx, ly, y), tcsRow);
+}
+
+template<int ENC>
+__global__ __launch_bounds__(CORR_WG) void row_corr_table_kernel(const uint32_t* __restrict__ lat,
+		const uint32_t* __restrict__ snap, int lx, int ly, int64_t* __restrict__ out)
+{
+	__shared__ uint32_t tOwn[256];
+	__shared__ int16_t tX[1 << 16];
+	{
+		// the walk of this lane's own byte v, then the cross sums against the
+		// bytes w = w0, w0 + 4, ... (64 of them)
+		const uint32_t v = threadIdx.x & 255u;
+		int a[8];
+		int p = 0, A = 0, B = 0;
+#pragma unroll
+		for(int i = 0; i < 8; ++i)
+		{
+			p += corr_site_bit<ENC>(v, i) ? 1 : -1;
+			a[i] = p;
+			A += p;
+			B += p * p;
+		}
+		if(threadIdx.x < 256)
+			tOwn[v] = (uint32_t)(p + 8) | ((uint32_t)(A + 36) << 5) | ((uint32_t)B << 12);
+		for(uint32_t w = threadIdx.x >> 8; w < 256; w += CORR_WG / 256)
+		{
+			int b = 0, X = 0;
+#pragma unroll
+			for(int i = 0; i < 8; ++i)
+			{
+				b += corr_site_bit<ENC>(w, i) ? 1 : -1;
+				X += a[i] * b;
+			}
+			tX[v | (w << 8)] = (int16_t)X;
+		}
+	}
+	__syncthreads();
+	const CorrTables T{tOwn, tX};
+	const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+	const int units = (ENC == ENC_LOCAL) ? (1 << (ly - 2)) : (1 << ly);
+	// no barrier below: waves leave the loop independently
+	for(int u = (int)blockIdx.x * (CORR_WG / 64) + wave; u < units; u += (int)gridDim.x * (CORR_WG / 64))
+	{
+		if(ENC == ENC_LOCAL)
+			corr_unit_local(T, lat, snap, lx, ly, u, lane, out);
+		else
+			corr_unit_cbbit(T, lat, snap, lx, ly, u, lane, out);
 	}
 }
 
@@ -3278,14 +3449,28 @@ hipError_t launch_consistency(const uint32_t* lat, int enc, int lx, int ly, int*
 	return hipGetLastError();
 }
 
+namespace {
+// workgroups of row_corr_table_kernel: one per CU (its tables fill the CU's
+// LDS), fewer when the lattice has fewer units than 16 per CU
+unsigned corr_grid(int64_t units)
+{
+	int dev = 0, cus = 256;
+	if(hipGetDevice(&dev) != hipSuccess ||
+	   hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+		cus = 256;
+	const int64_t need = (units + CORR_WG / 64 - 1) / (CORR_WG / 64);
+	return (unsigned)(need < cus ? need : cus);
+}
+} // namespace
+
 hipError_t launch_row_stats(const uint32_t* lat, const uint32_t* snap, int enc, int lx, int ly,
 		int64_t* out, hipStream_t stream)
 {
 	const dim3 grid((unsigned)(((1 << ly) + 3) / 4));
 	if(enc == ENC_LOCAL)
 	{
-		if(snap && ly >= 2)
-			hipLaunchKernelGGL(row_corr_local_kernel, dim3((unsigned)(((1 << (ly - 2)) + 3) / 4)), dim3(256), 0,
+		if(snap && lx >= 3 && ly >= 2)
+			hipLaunchKernelGGL(row_corr_table_kernel<ENC_LOCAL>, dim3(corr_grid(1ll << (ly - 2))), dim3(CORR_WG), 0,
 			                   stream, lat, snap, lx, ly, out);
 		else if(snap)
 			hipLaunchKernelGGL((row_stats_kernel<ENC_LOCAL, true>), grid, dim3(256), 0, stream, lat, snap, lx, ly, out);
@@ -3298,7 +3483,10 @@ hipError_t launch_row_stats(const uint32_t* lat, const uint32_t* snap, int enc, 
 	}
 	else
 	{
-		if(snap)
+		if(snap && lx >= 6)
+			hipLaunchKernelGGL(row_corr_table_kernel<ENC_CBBIT>, dim3(corr_grid(1ll << ly)), dim3(CORR_WG), 0, stream,
+			                   lat, snap, lx, ly, out);
+		else if(snap)
 			hipLaunchKernelGGL((row_stats_kernel<ENC_CBBIT, true>), grid, dim3(256), 0, stream, lat, snap, lx, ly, out);
 		else if(lx >= 6)
 			// one wave per site row, the 8-site walk table
