@@ -219,9 +219,9 @@ void* kpz_stream(const kpz_scheduler* h);
  * issue rates of the device, the roofline that bounds the DT and per-site
  * SCA kernels, in lane operations per second (best of `reps` launches of 8
  * independent chains per lane on 32 x 256-lane blocks per CU).  slow_class:
- * v_xad_u32, the rate of shifts, multiplies, add3, bfe, perm and alignbit on
- * gfx950; fast_class (may be NULL): v_xor_b32, the rate of plain xor, add
- * and bitop3, about twice as high. */
+ * v_xad_u32, the rate of left shifts, multiplies, add3, bfe, perm and
+ * alignbit on gfx950; fast_class (may be NULL): v_xor_b32, the rate of plain
+ * xor, add, bitop3 and right shifts, about twice as high. */
 int kpz_valu_peak(int device, int reps, double* slow_class, double* fast_class);
 /* Measurement utility, no counterpart in the reference: the issue rate of an
  * instruction mix, lane operations per second, of blocks of 8 independent
