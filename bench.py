@@ -102,8 +102,10 @@ def run_reductions(K, lx, seed, reps=5):
     lattice, ENC_LOCAL, and the C3 SCABit lattice, ENC_CBBIT) after a few MCS
     with a tag taken: the best wall time of `reps` calls of roughness() /
     correlate() (each queues the reductions, copies 7 int64 back and waits).
-    Algorithmic bytes: the lattice once (N/4 bytes) for w2, lattice and
-    snapshot (N/2) for the correlator."""
+    Algorithmic bytes: for w2 the x slopes, all of an ENC_LOCAL lattice's
+    words (N/4 bytes: x and y bits interleave) but only the two x quadrants of
+    an ENC_CBBIT one (N/8); for the correlator every bit of lattice and
+    snapshot (N/2: its tcs term compares the y slopes too)."""
     N = 1 << (2 * lx)
     out = {}
     for name, cls, pq in (("dt_enc_local", K.SchedulerDT, None), ("scabit_enc_cbbit", K.SchedulerSCABit, (0.95, 0.05))):
@@ -119,7 +121,8 @@ def run_reductions(K, lx, seed, reps=5):
         run(2)
         s.sync()
         r = {}
-        for what, fn, nbytes in (("w2", s.roughness, N // 4), ("correlate", s.correlate, N // 2)):
+        w2bytes = N // 8 if cls is K.SchedulerSCABit else N // 4
+        for what, fn, nbytes in (("w2", s.roughness, w2bytes), ("correlate", s.correlate, N // 2)):
             best = float("inf")
             for _ in range(reps):
                 t0 = time.perf_counter()

@@ -2385,10 +2385,16 @@ __device__ __forceinline__ void corr_chunks4(const CorrTables& T, uint32_t V, ui
 constexpr int CORR_LOCAL_K = 16; // words per lane and round (ENC_LOCAL): 64 sites of each row
 constexpr int CORR_CB_K = 4;     // word positions per lane and round (ENC_CBBIT): 256 sites
 
-// ENC_LOCAL word row yw (site rows 4 yw .. 4 yw + 3); needs 2^(lx-2) >= 2
+// ENC_LOCAL word row yw (site rows 4 yw .. 4 yw + 3); needs 2^(lx-2) >= 2.
+// A round loads all of a lane's K words of both lattices first, so that a
+// wave's loads use whole cache lines while they are resident, keeps per word
+// pair only the row bytes V (this lattice) and W (snapshot), and then walks
+// the four site rows one after the other (one row's running sums live at a
+// time).
 __device__ __forceinline__ void corr_unit_local(const CorrTables& T, const uint32_t* __restrict__ lat,
 		const uint32_t* __restrict__ snap, int lx, int ly, int yw, int lane, int64_t* __restrict__ out)
 {
+	constexpr int NP = CORR_LOCAL_K / 2; // word pairs per lane and round
 	const int XW = 1 << (lx - 2);
 	const int K = XW < CORR_LOCAL_K ? XW : CORR_LOCAL_K; // even
 	const int span = 64 * K;
@@ -2401,36 +2407,58 @@ __device__ __forceinline__ void corr_unit_local(const CorrTables& T, const uint3
 	{
 		const int wl = w0 + lane * K;
 		const bool live = wl < XW;
-		CorrAcc A[4];
+		uint32_t V[NP], W[NP];
 		// 8 sites of each row r from the word pair (a, b): byte r of a holds
 		// sites 0..3 on its even bits, byte r of b sites 4..7 (moved to the odd
 		// bits), so byte r of V = (a & 0x55..) | (2b & 0xAA..) is row r's chunk
-		auto pair = [&](uint32_t a, uint32_t b, uint32_t as, uint32_t bs) {
+		auto pair = [&](int p, uint32_t a, uint32_t b, uint32_t as, uint32_t bs) {
 			tcs += 2 * (__popc(~(a ^ as)) + __popc(~(b ^ bs))) - 64; // +1 per equal bit, -1 per different one
-			const uint32_t V = __builtin_amdgcn_bitop3_b32(a, twice(b), 0x55555555u, 0xE4);
-			const uint32_t W = __builtin_amdgcn_bitop3_b32(as, twice(bs), 0x55555555u, 0xE4);
-			corr_chunks4(T, V, W, [&](int r) -> CorrAcc& { return A[r]; });
+			V[p] = __builtin_amdgcn_bitop3_b32(a, twice(b), 0x55555555u, 0xE4);
+			W[p] = __builtin_amdgcn_bitop3_b32(as, twice(bs), 0x55555555u, 0xE4);
 		};
 		if(live && K == CORR_LOCAL_K)
 		{
+			uint4 a[NP / 2], b[NP / 2];
 #pragma unroll
-			for(int k = 0; k < CORR_LOCAL_K; k += 4)
+			for(int k = 0; k < NP / 2; ++k)
 			{
-				const uint4 a = *reinterpret_cast<const uint4*>(ra + wl + k);
-				const uint4 b = *reinterpret_cast<const uint4*>(rb + wl + k);
-				pair(a.x, a.y, b.x, b.y);
-				pair(a.z, a.w, b.z, b.w);
+				a[k] = *reinterpret_cast<const uint4*>(ra + wl + 4 * k);
+				b[k] = *reinterpret_cast<const uint4*>(rb + wl + 4 * k);
+			}
+#pragma unroll
+			for(int k = 0; k < NP / 2; ++k)
+			{
+				pair(2 * k, a[k].x, a[k].y, b[k].x, b[k].y);
+				pair(2 * k + 1, a[k].z, a[k].w, b[k].z, b[k].w);
 			}
 		}
-		else if(live)
+		else
 		{
-			for(int k = 0; k < K; k += 2)
-				pair(ra[wl + k], ra[wl + k + 1], rb[wl + k], rb[wl + k + 1]);
+#pragma unroll
+			for(int p = 0; p < NP; ++p)
+			{
+				V[p] = W[p] = 0;
+				if(live && 2 * p < K)
+					pair(p, ra[wl + 2 * p], ra[wl + 2 * p + 1], rb[wl + 2 * p], rb[wl + 2 * p + 1]);
+			}
 		}
-		const int n = live ? 4 * K : 0;
 #pragma unroll
 		for(int r = 0; r < 4; ++r)
-			R[r].add(A[r], n, lane);
+		{
+			CorrAcc A;
+#pragma unroll
+			for(int p = 0; p < NP; ++p)
+			{
+				if(2 * p >= K) // uniform: K < CORR_LOCAL_K only on rows under 64 sites
+					break;
+				// cross entry byte address 2 (v | w << 8): bytes r of V and W side by side
+				const uint32_t xa = twice(__builtin_amdgcn_perm(W[p], V[p], perm_sel(r, 4 + r, 12, 12)));
+				A.chunk(T, (V[p] >> (8 * r)) & 0xFFu, (W[p] >> (8 * r)) & 0xFFu, xa);
+			}
+			if(!live)
+				A = CorrAcc{};
+			R[r].add(A, live ? 4 * K : 0, lane);
+		}
 	}
 	// tcs counts both slope bits of all 4 site rows: the word row's first row
 	// carries it (the host adds the rows)
@@ -2550,7 +2578,8 @@ __global__ __launch_bounds__(CORR_WG) void row_corr_table_kernel(const uint32_t*
 	const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
 	const int units = (ENC == ENC_LOCAL) ? (1 << (ly - 2)) : (1 << ly);
 	// no barrier below: waves leave the loop independently
-	for(int u = (int)blockIdx.x * (CORR_WG / 64) + wave; u < units; u += (int)gridDim.x * (CORR_WG / 64))
+	constexpr int WAVES = CORR_WG / 64;
+	for(int u = (int)blockIdx.x * WAVES + wave; u < units; u += (int)gridDim.x * WAVES)
 	{
 		if(ENC == ENC_LOCAL)
 			corr_unit_local(T, lat, snap, lx, ly, u, lane, out);
@@ -3484,7 +3513,8 @@ hipError_t launch_row_stats(const uint32_t* lat, const uint32_t* snap, int enc, 
 	else
 	{
 		if(snap && lx >= 6)
-			hipLaunchKernelGGL(row_corr_table_kernel<ENC_CBBIT>, dim3(corr_grid(1ll << ly)), dim3(CORR_WG), 0, stream,
+			hipLaunchKernelGGL(row_corr_table_kernel<ENC_CBBIT>, dim3(corr_grid(1ll << ly)), dim3(CORR_WG), 0,
+			                   stream,
 			                   lat, snap, lx, ly, out);
 		else if(snap)
 			hipLaunchKernelGGL((row_stats_kernel<ENC_CBBIT, true>), grid, dim3(256), 0, stream, lat, snap, lx, ly, out);

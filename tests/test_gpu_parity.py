@@ -117,8 +117,11 @@ def test_scabit_pq_half_zero_uses_bitparallel_path():
     assert_same(s, lat, st, m)
 
 
+# (13, 6) and (15, 5): rows long enough that the correlator's lanes take
+# several rounds of words each
 @pytest.mark.parametrize("cls,enc,lx,ly", [(K.SchedulerDT, 0, 9, 8), (K.SchedulerSCABit, 1, 9, 8),
-                                           (K.SchedulerDT, 0, 5, 4), (K.SchedulerSCABit, 1, 6, 3)])
+                                           (K.SchedulerDT, 0, 5, 4), (K.SchedulerSCABit, 1, 6, 3),
+                                           (K.SchedulerDT, 0, 13, 6), (K.SchedulerSCABit, 1, 15, 5)])
 def test_roughness_and_correlate_parity(cls, enc, lx, ly):
     kw = {"layout": (2, 2)} if cls is K.SchedulerDT else {}
     s = cls(lx, ly, **kw)
