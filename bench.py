@@ -489,6 +489,51 @@ def kernel_roofline(K, key, avg_launch_ms, algo_bytes, fast, slow, mix=None):
     return out
 
 
+def run_c3(K, lx, seed, mcs, ev):
+    """The autocorrelator configuration end to end: SchedulerSCABit 2^lx x 2^lx,
+    flat IC, p = .95, q = .05, `mcs` MCS with correlator tags at MCS 10, 100
+    and 1000, and at every sample time (kpz/sampling.h:84-100) and tag time the
+    exact w2 and the autocorrelator of every live tag, as the kpzScaling event
+    loop does (kpz/kpzScaling.cpp:186-208: mcs to the next event, then
+    correlate()), with the device reductions in place of popSystem and the CPU
+    correlator.  hipEvent time of the whole loop on the engine's stream."""
+    s = K.SchedulerSCABit(lx, lx)
+    s.supplySeed(seed)
+    s.initHomogeneous()
+    s.setPQ(0.95, 0.05)
+    tags = [t for t in (10, 100, 1000) if t < mcs]
+    events = sorted(set(sample_times(mcs)) | set(tags) | {mcs})
+    stream = s.stream
+    e0, e1, a, b = ev.event(), ev.event(), ev.event(), ev.event()
+    s.sync()
+    mcs_ms, ncorr, done, last = 0.0, 0, 0, None
+    ev.record(e0, stream)
+    for t in events:
+        ev.record(a, stream)
+        s.mcsPQSyncRng(t - done)
+        ev.record(b, stream)
+        mcs_ms += ev.ms(a, b)
+        done = t
+        if t in tags:
+            s.correlateTag()
+        r = s.roughness()
+        res = s.correlateAll()
+        ncorr += len(res)
+        last = (r, res)
+    ev.record(e1, stream)
+    total_ms = ev.ms(e0, e1)
+    N = 1 << (2 * lx)
+    out = {"workload": f"SchedulerSCABit {1 << lx}x{1 << lx} ENC_CBBIT, flat IC, p=.95 q=.05, {mcs} MCS, "
+                       f"correlator tags at MCS {', '.join(map(str, tags))}, exact w2 and every live tag's "
+                       "autocorrelator at each sample and tag time",
+           "events": len(events), "correlator_evaluations": ncorr, "device_ms": total_ms, "mcs_ms": mcs_ms,
+           "site_updates_per_s": N * mcs / (total_ms * 1e-3), "analysis_share": 1.0 - mcs_ms / total_ms,
+           "w2_final": last[0].w2,
+           "c_h_final": {str(m): c.ch for m, c in last[1]}}
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -504,6 +549,8 @@ def main():
     ap.add_argument("--serial-mcs", type=int, default=10, help="MCS of the serial per-replica baseline")
     ap.add_argument("--serve", type=float, default=2.0, metavar="SECONDS",
                     help="serving loop: Poisson arrivals at 200/s for SECONDS (0: skip)")
+    ap.add_argument("--c3-mcs", type=int, default=10000, metavar="MCS",
+                    help="the autocorrelator configuration end to end for MCS (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="K",
                     help="after the run, K lattice copies (1 GiB read + 1 GiB write each) whose "
                          "FETCH_SIZE/WRITE_SIZE calibrate the counters")
@@ -592,6 +639,8 @@ def main():
             print(f"# {name}: {ms / steps:.3f} ms/MCS", file=sys.stderr, flush=True)
         line["secondary"] = sec
         line["reductions"] = run_reductions(K, args.lx, args.seed)
+    if args.c3_mcs > 0:
+        line["c3_end_to_end"] = run_c3(K, args.lx, args.seed, args.c3_mcs, ev)
     if args.batch:
         line["batched"] = run_batch_bench(K, ev, args.batch, args.serial_mcs)
     if args.serve > 0:
