@@ -804,50 +804,64 @@ __global__ __launch_bounds__(DT_WG) void dt_phase_kernel(DtPhaseArgs a, int64_t 
 
 // SchedulerCPUDTr<TinyMT32> (SchedulerCPUDTr::Func_MCS::exec,
 // kpz/schedulerCPUDTr.cpp:147-175: worker t updates blocks t, t+T, ... of the
-// domain set in order with its own stream).  One lane per block: block i is
-// the j = i / T-th block of worker i % T, whose stream is jumped j blocks of
-// draws ahead (a.tmtJ[i] = x^(j * draws per block) mod the orbit polynomial);
-// the lane of a worker's last block writes the worker's state to a.tmtOut.
-// With T >= blocks every worker has one block at most and its lane works in
-// place.
+// domain set in order with its own stream).  With T >= blocks every worker
+// has one block at most and the lane of block i works in place on stream i.
+// Otherwise lane c runs a chain of C = a.tmtChain consecutive blocks of worker
+// t = c % T, blocks t + T (k C + m) with k = c / T, m < C, in order with one
+// stream jumped to the chain's start (a.tmtJ[c] = x^(k C draws per block)
+// mod the orbit polynomial): a jump costs 128 recurrence steps, so long
+// walks share it over C blocks; the lane of a worker's last block writes the
+// worker's state to a.tmtOut.
 template<int LBX, int LBY, int UPD>
 __global__ __launch_bounds__(DT_WG) void dt_tmt_kernel(DtPhaseArgs a, int64_t nb, uint32_t* __restrict__ g)
 {
 	__shared__ uint32_t tile[(2 * (1 << LBY) + 1) * DT_WG];
 	__shared__ uint32_t tileD[(UPD == 2) ? (1 << LBY) * DT_WG : 1];
-	const int64_t i = (int64_t)blockIdx.x * DT_WG + threadIdx.x;
-	if(i >= nb)
-		return; // no barrier in this kernel; workers without a block keep their stream
-	const int64_t t = (a.T >= nb) ? i : i % a.T;
-	const int64_t j = (a.T >= nb) ? 0 : i / a.T;
+	const int64_t c = (int64_t)blockIdx.x * DT_WG + threadIdx.x;
+	const DtTile my{tile + threadIdx.x, tile + (1 << LBY) * DT_WG + threadIdx.x, tile, threadIdx.x * 4};
+	uint32_t* const myD = (UPD == 2) ? tileD + threadIdx.x : nullptr;
 	DtTmt cur;
+	if(a.T >= nb)
+	{
+		if(c >= nb)
+			return; // no barrier in this kernel; workers without a block keep their stream
+		cur.g = TinyMT::load(g, a.T, c);
+		dt_block<LBX, LBY, UPD, true>(a, c, nb, my, myD, cur);
+		cur.g.store(g, a.T, c);
+		return;
+	}
+	const int C = a.tmtChain;
+	const int64_t t = c % a.T, k = c / a.T;
+	int64_t i = t + a.T * (k * C);
+	if(i >= nb)
+		return; // the chain starts past the worker's last block
 	cur.g = TinyMT::load(g, a.T, t);
-	if(j > 0)
+	if(k > 0)
 	{
 		uint32_t st[4] = {cur.g.s0, cur.g.s1, cur.g.s2, cur.g.s3};
-		tmt_jump4(st, a.tmtJ[i], a.tmtOrb[t], cur.g.m1, cur.g.m2);
+		tmt_jump4(st, a.tmtJ[c], a.tmtOrb[t], cur.g.m1, cur.g.m2);
 		cur.g.s0 = st[0];
 		cur.g.s1 = st[1];
 		cur.g.s2 = st[2];
 		cur.g.s3 = st[3];
 	}
-	const DtTile my{tile + threadIdx.x, tile + (1 << LBY) * DT_WG + threadIdx.x, tile, threadIdx.x * 4};
-	dt_block<LBX, LBY, UPD, true>(a, i, nb, my, (UPD == 2) ? tileD + threadIdx.x : nullptr, cur);
-	if(a.T >= nb)
-		cur.g.store(g, a.T, t);
-	else if(i + a.T >= nb)
-		cur.g.store(a.tmtOut, a.T, t);
+	for(int m = 0; m < C && i < nb; ++m, i += a.T)
+	{
+		dt_block<LBX, LBY, UPD, true>(a, i, nb, my, myD, cur);
+		if(i + a.T >= nb)
+			cur.g.store(a.tmtOut, a.T, t);
+	}
 }
 
-// tmtJ for dt_tmt_kernel: block i gets x^((i / T) * drawsPerBlock) mod the
-// orbit polynomial of worker i % T
-__global__ __launch_bounds__(256) void tmt_block_polys_kernel(const TmtOrbit* __restrict__ orb, int64_t T, int64_t nb,
-		uint64_t drawsPerBlock, Poly128* __restrict__ out)
+// tmtJ for dt_tmt_kernel: lane c gets x^((c / T) * drawsPerEntry) mod the
+// orbit polynomial of worker c % T (drawsPerEntry: the draws of a chain)
+__global__ __launch_bounds__(256) void tmt_block_polys_kernel(const TmtOrbit* __restrict__ orb, int64_t T, int64_t n,
+		uint64_t drawsPerEntry, Poly128* __restrict__ out)
 {
 	const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-	if(i >= nb)
+	if(i >= n)
 		return;
-	out[i] = poly_xpow((uint64_t)(i / T) * drawsPerBlock, orb[i % T]);
+	out[i] = poly_xpow((uint64_t)(i / T) * drawsPerEntry, orb[i % T]);
 }
 
 // ======================================================================
@@ -3347,13 +3361,13 @@ hipError_t launch_cb_local_tmt_gen(const CbLocalArgs& a, const uint32_t* gin, ui
 	return hipGetLastError();
 }
 
-hipError_t launch_tmt_block_polys(const TmtOrbit* orb, int64_t T, int64_t nb, uint64_t drawsPerBlock, Poly128* out,
+hipError_t launch_tmt_block_polys(const TmtOrbit* orb, int64_t T, int64_t n, uint64_t drawsPerEntry, Poly128* out,
 		hipStream_t stream)
 {
-	if(!orb || !out || T < 1 || nb < 1)
+	if(!orb || !out || T < 1 || n < 1)
 		return hipErrorInvalidValue;
-	hipLaunchKernelGGL(tmt_block_polys_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, stream, orb, T, nb,
-	                   drawsPerBlock, out);
+	hipLaunchKernelGGL(tmt_block_polys_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, orb, T, n,
+	                   drawsPerEntry, out);
 	return hipGetLastError();
 }
 
@@ -3370,9 +3384,11 @@ hipError_t launch_dt_phase(const DtPhaseArgs& a, hipStream_t stream)
 	if(!dt_layout_supported(a.lbx, a.lby) || a.T < 1 || (a.tmt && !a.random_origin))
 		return hipErrorInvalidValue;
 	const int64_t nb = 1ll << (a.lx - a.lbx + a.ly - a.lby - 2);
-	if(a.tmt && a.T < nb && (!a.tmtOut || !a.tmtJ || !a.tmtOrb))
+	const bool chains = a.tmt && a.T < nb;
+	if(chains && (!a.tmtOut || !a.tmtJ || !a.tmtOrb || a.tmtChain < 1))
 		return hipErrorInvalidValue;
-	const dim3 grid((unsigned)((nb + DT_WG - 1) / DT_WG));
+	const int64_t lanes = chains ? dt_tmt_lanes(nb, a.T, a.tmtChain) : nb;
+	const dim3 grid((unsigned)((lanes + DT_WG - 1) / DT_WG));
 	switch(a.lbx)
 	{
 		case 2: return dt_dispatch_y<2>(a, nb, grid, stream);

@@ -31,15 +31,20 @@ struct DtPhaseArgs {
 	uint32_t* tmt;          // TinyMT32 streams (7 planes of T words; random origin only;
 	                        // thresholds then on the 2^24 scale), or null for SLCG64
 	// TinyMT32 with fewer streams than blocks: the states after the phase go
-	// to tmtOut, block i's stream is jumped by tmtJ[i] (launch_tmt_block_polys)
+	// to tmtOut; lane c runs the tmtChain consecutive blocks t + T (k C + m),
+	// m < C = tmtChain, of worker t = c % T, k = c / T, with the stream jumped
+	// by tmtJ[c] = x^(k C draws per block) (launch_tmt_block_polys)
 	uint32_t* tmtOut;
 	const Poly128* tmtJ;
 	const TmtOrbit* tmtOrb;
+	int tmtChain;
 };
+// lanes of a TinyMT32 DT phase with T < nb streams and chains of C blocks
+inline int64_t dt_tmt_lanes(int64_t nb, int64_t T, int C) { return T * (((nb + T - 1) / T + C - 1) / C); }
 hipError_t launch_dt_phase(const DtPhaseArgs& a, hipStream_t stream);
-// tmtJ for a DT phase with T < nb streams: block i -> x^((i / T) * drawsPerBlock)
-// mod the orbit polynomial of stream i % T
-hipError_t launch_tmt_block_polys(const TmtOrbit* orb, int64_t T, int64_t nb, uint64_t drawsPerBlock, Poly128* out,
+// tmtJ for a DT phase with T < nb streams: entry c < n -> x^((c / T) * drawsPerEntry)
+// mod the orbit polynomial of stream c % T (drawsPerEntry: the draws of a chain)
+hipError_t launch_tmt_block_polys(const TmtOrbit* orb, int64_t T, int64_t n, uint64_t drawsPerEntry, Poly128* out,
 		hipStream_t stream);
 
 // ---- TinyMT32 stream sets: tinymt32_init of every stream from the seed in

@@ -497,14 +497,21 @@ void SchedulerDT::run(int n, int upd)
 	a.jumps = tinyMT() ? nullptr : jumps(du);
 	if(tinyMT() && T_ < nb_)
 	{
-		// blocks start mid-stream: jump tables per block (draws per block = du << (lbx + lby))
+		// blocks start mid-stream: a lane runs a chain of C consecutive blocks of
+		// one worker from a jump to the chain's start (draws per block = du <<
+		// (lbx + lby)); chains of 4 where they still leave >= 2^19 lanes (8
+		// waves per SIMD), single blocks on smaller lattices
 		a.tmtOrb = tmtOrbits();
+		const int C = (nb_ >= (1ll << 21) && nb_ >= 16 * T_) ? 4 : 1;
+		a.tmtChain = C;
 		const uint64_t dpb = (uint64_t)du << (lbx_ + lby_);
-		auto& tab = tmtTab_[dpb];
+		const int64_t lanes = dt_tmt_lanes(nb_, T_, C);
+		auto& tab = tmtTab_[(dpb << 4) | (uint64_t)C];
 		if(!tab)
 		{
-			tab = std::make_unique<DevBuf>((size_t)nb_ * sizeof(Poly128));
-			check(launch_tmt_block_polys(a.tmtOrb, T_, nb_, dpb, tab->as<Poly128>(), stream_), "tmt_block_polys");
+			tab = std::make_unique<DevBuf>((size_t)lanes * sizeof(Poly128));
+			check(launch_tmt_block_polys(a.tmtOrb, T_, lanes, dpb * (uint64_t)C, tab->as<Poly128>(), stream_),
+			      "tmt_block_polys");
 		}
 		a.tmtJ = tab->as<Poly128>();
 	}

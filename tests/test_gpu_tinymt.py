@@ -238,6 +238,19 @@ def test_state_file_resumes_bit_exact(tmp_path):
     assert a.master_draw() == b.master_draw()
 
 
+@pytest.mark.parametrize("T", [32, 96])
+def test_dtr_chains_of_blocks_against_oracle(T):
+    # 16384 x 8192 in 4x4 blocks: 2^21 blocks per domain set, so a lane runs 4
+    # consecutive blocks of one worker from a single jump; with 96 workers the
+    # blocks do not divide evenly and the last chain of a worker is short
+    s, m = engine(K.SchedulerDTr, 14, 13, 91, T, layout=(2, 2))
+    s.mcs(1)
+    master, st = oracle(s, 91, m)
+    lat = O.flat(14, 13, O.ENC_LOCAL)
+    O.dtr_mcs_tmt(lat, 14, 13, 2, 2, st, master, 1)
+    check(s, lat, st, master)
+
+
 @pytest.mark.parametrize("T", [1024, 300])
 def test_medium_dtr_against_oracle(T):
     # 1024^2, 16x16 blocks, 1024 blocks per domain set: one block per worker,
