@@ -21,8 +21,10 @@ Batch::Batch(const std::vector<ReplicaSpec>& specs, const BatchTuning& tune, int
 	check(hipGetDevice(&device_), "hipGetDevice");
 	check(hipStreamCreateWithFlags(&s0_, hipStreamNonBlocking), "hipStreamCreate");
 	check(hipStreamCreateWithFlags(&s1_, hipStreamNonBlocking), "hipStreamCreate");
+	check(hipStreamCreateWithFlags(&s2_, hipStreamNonBlocking), "hipStreamCreate");
 	check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreate");
 	check(hipEventCreateWithFlags(&join_, hipEventDisableTiming), "hipEventCreate");
+	check(hipEventCreateWithFlags(&join2_, hipEventDisableTiming), "hipEventCreate");
 	for(auto& e : copied_)
 		check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
 
@@ -144,22 +146,44 @@ void Batch::buildTables()
 		std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
 			return reps_[a].spec.lx + reps_[a].spec.ly > reps_[b].spec.lx + reps_[b].spec.ly;
 		});
-	// DT lanes, one class per (kind, mode) so every wave is uniform
-	std::vector<uint2> lanes;
-	for(int kind : {BK_DT, BK_DTR})
-		for(int mode : {0, 1})
-		{
-			for(int r : order)
+	// DT lane groups: with concurrent == 2 each DT replica goes whole to the
+	// group with fewer lanes so far (in launch order), else all to group 0
+	std::vector<int> group(R, 0);
+	if(tune_.concurrent >= 2)
+	{
+		int64_t load[2] = {0, 0};
+		for(int r : order)
+			if(reps_[r].dev.kind != BK_SCABIT)
 			{
-				const BatchRep& d = reps_[r].dev;
-				if(d.kind != kind || d.mode != mode)
-					continue;
-				for(int64_t b = 0; b < reps_[r].T; ++b)
-					lanes.push_back(make_uint2((unsigned)r, (unsigned)b));
+				group[r] = load[1] < load[0] ? 1 : 0;
+				load[group[r]] += reps_[r].T;
 			}
-			while(lanes.size() % 64)
-				lanes.push_back(make_uint2(0xFFFFFFFFu, 0));
+	}
+	// DT lanes of each group, one class per (kind, mode) so every wave is uniform
+	std::vector<uint2> lanes;
+	for(int g = 0; g < 2; ++g)
+	{
+		dtOff_[g] = (int64_t)lanes.size();
+		for(int kind : {BK_DT, BK_DTR})
+		{
+			if(kind == BK_DTR)
+				dtStatic_[g] = (int64_t)lanes.size() - dtOff_[g];
+			for(int mode : {0, 1})
+			{
+				for(int r : order)
+				{
+					const BatchRep& d = reps_[r].dev;
+					if(d.kind != kind || d.mode != mode || group[r] != g)
+						continue;
+					for(int64_t b = 0; b < reps_[r].T; ++b)
+						lanes.push_back(make_uint2((unsigned)r, (unsigned)b));
+				}
+				while(lanes.size() % 64)
+					lanes.push_back(make_uint2(0xFFFFFFFFu, 0));
+			}
 		}
+		dtLanes_[g] = (int64_t)lanes.size() - dtOff_[g];
+	}
 	// SCABit waves: 64 >> lG rows of one replica each
 	std::vector<uint2> waves;
 	for(int r : order)
@@ -181,7 +205,6 @@ void Batch::buildTables()
 			rows.push_back(make_uint2((unsigned)r, (unsigned)y));
 		rowBase += 1 << reps_[r].dev.ly;
 	}
-	nDtLanes_ = (int64_t)lanes.size();
 	nCbWaves_ = (int64_t)waves.size();
 	nRows_ = (int64_t)rows.size();
 
@@ -221,6 +244,8 @@ Batch::~Batch()
 		(void)hipStreamSynchronize(s0_);
 	if(s1_)
 		(void)hipStreamSynchronize(s1_);
+	if(s2_)
+		(void)hipStreamSynchronize(s2_);
 	for(auto& kv : graphs_)
 	{
 		if(kv.second.exec)
@@ -241,6 +266,10 @@ Batch::~Batch()
 		(void)hipEventDestroy(fork_);
 	if(join_)
 		(void)hipEventDestroy(join_);
+	if(join2_)
+		(void)hipEventDestroy(join2_);
+	if(s2_)
+		(void)hipStreamDestroy(s2_);
 	if(s1_)
 		(void)hipStreamDestroy(s1_);
 	if(s0_)
@@ -288,19 +317,27 @@ Batch::Chunk& Batch::graphFor(int b, int len)
 		return it->second;
 	Chunk c;
 	hipStream_t cbStream = tune_.concurrent ? s1_ : s0_;
+	const bool split = dtLanes_[1] > 0; // second DT group on s2_
 	check(hipStreamBeginCapture(s0_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-	if(tune_.concurrent)
-	{
+	if(tune_.concurrent || split)
 		check(hipEventRecord(fork_, s0_), "fork");
+	if(tune_.concurrent)
 		check(hipStreamWaitEvent(s1_, fork_, 0), "fork wait");
+	if(split)
+		check(hipStreamWaitEvent(s2_, fork_, 0), "fork wait");
+	BatchDtArgs dg[2];
+	for(int g = 0; g < 2; ++g)
+	{
+		BatchDtArgs& da = dg[g];
+		da = BatchDtArgs{};
+		da.reps = repTable_.as<BatchRep>();
+		da.lanes = laneTable_.as<uint2>() + dtOff_[g];
+		da.order = orderBuf_[b].as<uint8_t>();
+		da.shift = shiftBuf_[b].as<uint16_t>();
+		da.nreps = size();
+		da.nlanes = dtLanes_[g];
+		da.nstatic = dtStatic_[g];
 	}
-	BatchDtArgs da{};
-	da.reps = repTable_.as<BatchRep>();
-	da.lanes = laneTable_.as<uint2>();
-	da.order = orderBuf_[b].as<uint8_t>();
-	da.shift = shiftBuf_[b].as<uint16_t>();
-	da.nreps = size();
-	da.nlanes = nDtLanes_;
 	BatchCbArgs ca{};
 	ca.reps = repTable_.as<BatchRep>();
 	ca.waves = waveTable_.as<uint2>();
@@ -308,12 +345,14 @@ Batch::Chunk& Batch::graphFor(int b, int len)
 	for(int m = 0; m < len; ++m)
 	{
 		if(hasDt_)
-			for(int d = 0; d < 4; ++d)
-			{
-				da.m = m;
-				da.d = d;
-				check(launch_batch_dt(da, s0_), "batch_dt");
-			}
+			for(int g = 0; g < 2; ++g)
+				for(int d = 0; d < 4; ++d)
+				{
+					dg[g].m = m;
+					dg[g].d = d;
+					if(dg[g].nlanes > 0)
+						check(launch_batch_dt(dg[g], g ? s2_ : s0_), "batch_dt");
+				}
 		if(hasCb_)
 			for(int s = 0; s < 2; ++s)
 			{
@@ -326,6 +365,11 @@ Batch::Chunk& Batch::graphFor(int b, int len)
 	{
 		check(hipEventRecord(join_, s1_), "join");
 		check(hipStreamWaitEvent(s0_, join_, 0), "join wait");
+	}
+	if(split)
+	{
+		check(hipEventRecord(join2_, s2_), "join");
+		check(hipStreamWaitEvent(s0_, join2_, 0), "join wait");
 	}
 	check(hipStreamEndCapture(s0_, &c.graph), "hipStreamEndCapture");
 	check(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
@@ -421,9 +465,12 @@ double autotuneBatch(const std::vector<ReplicaSpec>& specs, int mcs, int device,
 		t.chunk = c;
 		tryOne(t);
 	}
+	for(int c = 0; c <= 2; ++c)
 	{
 		BatchTuning t = best;
-		t.concurrent ^= 1;
+		if(t.concurrent == c)
+			continue;
+		t.concurrent = c;
 		tryOne(t);
 	}
 	for(int l = 2; l <= 6; ++l)

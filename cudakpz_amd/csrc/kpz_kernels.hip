@@ -3036,11 +3036,15 @@ __global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst, cons
 // One DT phase of every DT/DTr replica: lane -> {replica, block}.  The host
 // keeps each wave within one (kind, mode) class, so the dispatch below is
 // wave-uniform.  Block layout 16 x 16 sites.
+// RO: the DTr lanes [nstatic, nlanes), else the static DT lanes [0, nstatic):
+// one kernel per kind, so that the static DT lanes keep the register budget
+// (and occupancy) of the static update loop rather than the DTr staging's
+template<bool RO>
 __global__ __launch_bounds__(DT_WG) void batch_dt_kernel(BatchDtArgs a)
 {
 	__shared__ uint32_t tile[33 * DT_WG]; // X plane (16 rows), Y plane (17 rows)
-	const int64_t lane = (int64_t)blockIdx.x * DT_WG + threadIdx.x;
-	if(lane >= a.nlanes)
+	const int64_t lane = (int64_t)blockIdx.x * DT_WG + threadIdx.x + (RO ? a.nstatic : 0);
+	if(lane >= (RO ? a.nlanes : a.nstatic))
 		return;
 	const uint2 w = a.lanes[lane];
 	if(w.x == 0xFFFFFFFFu)
@@ -3064,7 +3068,7 @@ __global__ __launch_bounds__(DT_WG) void batch_dt_kernel(BatchDtArgs a)
 	const int64_t nb = 1ll << (rep.lx + rep.ly - 10);
 	p.T = nb;
 	const DtTile my{tile + threadIdx.x, tile + 16 * DT_WG + threadIdx.x, tile, threadIdx.x * 4};
-	if(rep.kind == BK_DTR)
+	if constexpr(RO)
 	{
 		const uint16_t sh = a.shift[mi];
 		p.sx = sh & 0xFF;
@@ -3677,9 +3681,14 @@ hipError_t launch_cb_local_half(const CbLocalArgs& a, hipStream_t stream)
 
 hipError_t launch_batch_dt(const BatchDtArgs& a, hipStream_t stream)
 {
-	if(a.nlanes <= 0)
-		return hipSuccess;
-	hipLaunchKernelGGL(batch_dt_kernel, dim3((unsigned)((a.nlanes + DT_WG - 1) / DT_WG)), dim3(DT_WG), 0, stream, a);
+	if(a.nstatic < 0 || a.nstatic > a.nlanes)
+		return hipErrorInvalidValue;
+	if(a.nstatic > 0)
+		hipLaunchKernelGGL(batch_dt_kernel<false>, dim3((unsigned)((a.nstatic + DT_WG - 1) / DT_WG)), dim3(DT_WG), 0,
+		                   stream, a);
+	if(a.nlanes > a.nstatic)
+		hipLaunchKernelGGL(batch_dt_kernel<true>, dim3((unsigned)((a.nlanes - a.nstatic + DT_WG - 1) / DT_WG)),
+		                   dim3(DT_WG), 0, stream, a);
 	return hipGetLastError();
 }
 

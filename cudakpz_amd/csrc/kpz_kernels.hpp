@@ -200,6 +200,8 @@ struct BatchDtArgs {
 	const uint16_t* shift;  // [m * nreps + r]: DTr origin shift, sx | sy << 8
 	int nreps;
 	int64_t nlanes;
+	int64_t nstatic;        // lanes [0, nstatic) hold static DT replicas, the rest DTr
+	                        // (the tables put static DT first, padded to whole waves)
 	int m, d;               // MCS within the chunk, position in the domain order
 	const int* limit;       // optional [r]: replica r only runs MCS m < limit[r]
 };

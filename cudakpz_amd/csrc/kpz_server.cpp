@@ -340,6 +340,9 @@ void Server::rebuild(PinnedStage& stage)
 	std::vector<uint2> lanes, waves;
 	auto runs = [](const Job* j) { return j && j->planned < (uint32_t)j->spec.mcs; };
 	for(int kind : {BK_DT, BK_DTR})
+	{
+		if(kind == BK_DTR)
+			nStatic_ = (int64_t)lanes.size();
 		for(int mode : {0, 1})
 		{
 			for(int sl = 0; sl < maxSlots_; ++sl)
@@ -353,6 +356,7 @@ void Server::rebuild(PinnedStage& stage)
 			while(lanes.size() % 64)
 				lanes.push_back(make_uint2(0xFFFFFFFFu, 0));
 		}
+	}
 	for(int sl = 0; sl < maxSlots_; ++sl)
 	{
 		Job* j = slots_[(size_t)sl];
@@ -530,6 +534,7 @@ bool Server::enqueue(Tick& t)
 	da.shift = shiftBuf_.as<uint16_t>();
 	da.nreps = maxSlots_;
 	da.nlanes = nLanes_;
+	da.nstatic = nStatic_;
 	da.limit = limitBuf_.as<int>();
 	BatchCbArgs ca{};
 	ca.reps = repTable_.as<BatchRep>();

@@ -153,7 +153,7 @@ private:
 	DevBuf repTable_, laneTable_, waveTable_, rowTable_, rowScratch_, outBuf_, orderBuf_, shiftBuf_, identity_, sampleReps_,
 			limitBuf_;
 	std::vector<BatchRep> hostReps_;
-	int64_t nLanes_ = 0, nWaves_ = 0;
+	int64_t nLanes_ = 0, nStatic_ = 0, nWaves_ = 0;
 	Tick ticks_[2];
 	int cur_ = 0;
 	double gpuMs_ = 0, spanMs_ = 0;

@@ -254,11 +254,12 @@ typedef struct kpz_batch_tuning {
 	int max_lanes_log2; /* SCABit lanes per row = min(words per row, 2^this), 0..6 */
 	int pq_words64;     /* SCABit per-site replicas on 64-bit (1) or 32-bit (0) words */
 	int chunk;          /* MCS per captured graph */
-	int concurrent;     /* DT and SCABit work on two streams */
+	int concurrent;     /* 0: one stream; 1: DT and SCABit work on two streams; 2: also the
+	                       DT replicas in two groups of equal lane count on two streams */
 	int sort_by_size;   /* large replicas first */
 } kpz_batch_tuning;
 
-/* tuning = NULL: defaults {6, 1, 50, 1, 1} */
+/* tuning = NULL: defaults {6, 1, 50, 2, 1} */
 int kpz_batch_create(const kpz_replica* reps, int n, const kpz_batch_tuning* tuning, int device, kpz_batch** out);
 int kpz_batch_destroy(kpz_batch* b);
 int kpz_batch_init_flat(kpz_batch* b);

@@ -51,7 +51,8 @@ def run_batch(tuning, n, chunks):
 
 
 @pytest.mark.parametrize("tuning", [None, K.BatchTuning(2, 0, 3, 0, 0), K.BatchTuning(6, 1, 2, 1, 1),
-                                    K.BatchTuning(4, 0, 7, 1, 0)])
+                                    K.BatchTuning(4, 0, 7, 1, 0), K.BatchTuning(6, 1, 2, 2, 1),
+                                    K.BatchTuning(3, 0, 5, 2, 0)])
 def test_batch_equals_standalone(tuning):
     n = 7
     b = run_batch(tuning, n, [3, 4])
@@ -63,6 +64,22 @@ def test_batch_equals_standalone(tuning):
         r = s.roughness()
         assert (rough[i].sum_h, rough[i].sum_h2) == (r.sum_h, r.sum_h2)
         s.close()
+
+
+@pytest.mark.parametrize("concurrent", [0, 2])
+def test_dt_only_batch_in_groups(concurrent):
+    """DT/DTr replicas only: one stream, or two lane-balanced groups on two streams."""
+    reps = [r for r in MIX if r[0] != K.KPZ_SCABIT]
+    b = K.Batch(reps, K.BatchTuning(6, 1, 4, concurrent, 1))
+    b.initHomogeneous()
+    b.mcs(3)
+    b.mcs(6)
+    for i, rep in enumerate(reps):
+        s = alone(rep, 9)
+        assert np.array_equal(b.popSystem(i), s.popSystem()), f"replica {i} {rep}"
+        assert np.array_equal(b.streams(i), s.streams()), f"replica {i} streams"
+        s.close()
+    b.close()
 
 
 def test_batch_matches_oracle_directly():
