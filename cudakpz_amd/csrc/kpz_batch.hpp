@@ -35,8 +35,9 @@ struct BatchTuning {
 	int pqWords64 = 1;     // SCABit per-site replicas: 64-bit (1) or 32-bit (0) words
 	int chunk = 50;        // MCS per captured graph
 	int concurrent = 2;    // 0: one stream; 1: DT and SCABit chains on two streams;
-	                       // 2: also the DT replicas in two groups of equal lane count on
-	                       // two streams, so one group's phase tail overlaps the other's
+	                       // 2: also the DT and the SCABit replicas each in two groups of
+	                       // equal site count on streams of their own, so one group's
+	                       // launch tail overlaps the other's next launch
 	int sortBySize = 1;    // large replicas first in the launch order
 };
 
@@ -82,17 +83,20 @@ private:
 	std::vector<Rep> reps_;
 	BatchTuning tune_;
 	int device_ = 0;
-	hipStream_t s0_ = nullptr, s1_ = nullptr, s2_ = nullptr;
-	hipEvent_t fork_ = nullptr, join_ = nullptr, join2_ = nullptr;
+	// s0_: DT group 0; s2_: DT group 1; s1_: SCABit group 0; s3_: SCABit group 1
+	hipStream_t s0_ = nullptr, s1_ = nullptr, s2_ = nullptr, s3_ = nullptr;
+	hipEvent_t fork_ = nullptr, join_[3] = {nullptr, nullptr, nullptr};
 	DevBuf latArena_, stArena_, tabArena_, repTable_, laneTable_, waveTable_, rowTable_, rowScratch_, outBuf_;
 	DevBuf orderBuf_[2], shiftBuf_[2];
 	uint8_t* orderHost_[2] = {nullptr, nullptr};
 	uint16_t* shiftHost_[2] = {nullptr, nullptr};
 	hipEvent_t copied_[2] = {nullptr, nullptr};
-	// DT lane groups (the second is empty unless concurrent == 2): lanes
-	// [dtOff_[g], dtOff_[g] + dtLanes_[g]) of the lane table, static DT first
+	// replica groups (the second ones are empty unless concurrent == 2): DT
+	// lanes [dtOff_[g], dtOff_[g] + dtLanes_[g]) of the lane table, static DT
+	// first; SCABit waves [cbOff_[g], cbOff_[g] + cbWaves_[g]) of the wave table
 	int64_t dtOff_[2] = {0, 0}, dtLanes_[2] = {0, 0}, dtStatic_[2] = {0, 0};
-	int64_t nCbWaves_ = 0, nRows_ = 0;
+	int64_t cbOff_[2] = {0, 0}, cbWaves_[2] = {0, 0};
+	int64_t nRows_ = 0;
 	bool hasDt_ = false, hasCb_ = false;
 	uint64_t sites_ = 0;
 	unsigned int mcs_ = 0;

@@ -132,6 +132,9 @@ Server::Server(size_t hbmBudget, int maxSlots, int maxTickMcs, int device)
 		check(hipSetDevice(device), "hipSetDevice");
 	check(hipGetDevice(&device_), "hipGetDevice");
 	check(hipStreamCreateWithFlags(&s0_, hipStreamNonBlocking), "hipStreamCreate");
+	check(hipStreamCreateWithFlags(&s1_, hipStreamNonBlocking), "hipStreamCreate");
+	check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreate");
+	check(hipEventCreateWithFlags(&join_, hipEventDisableTiming), "hipEventCreate");
 	arena_.init(hbmBudget, 4096);
 	slots_.assign((size_t)maxSlots_, nullptr);
 	hostReps_.assign((size_t)maxSlots_, BatchRep{});
@@ -155,6 +158,8 @@ Server::~Server()
 {
 	if(s0_)
 		(void)hipStreamSynchronize(s0_);
+	if(s1_)
+		(void)hipStreamSynchronize(s1_);
 	for(Job* j : queue_)
 		delete j;
 	for(Job* j : slots_)
@@ -168,6 +173,12 @@ Server::~Server()
 	}
 	if(first_)
 		(void)hipEventDestroy(first_);
+	if(fork_)
+		(void)hipEventDestroy(fork_);
+	if(join_)
+		(void)hipEventDestroy(join_);
+	if(s1_)
+		(void)hipStreamDestroy(s1_);
 	if(s0_)
 		(void)hipStreamDestroy(s0_);
 }
@@ -188,7 +199,7 @@ void Server::ensure(DevBuf& b, size_t bytes)
 {
 	if(b.bytes() < bytes)
 	{
-		check(hipStreamSynchronize(s0_), "grow sync");
+		check(hipStreamSynchronize(s0_), "grow sync"); // s1_'s work is joined into s0_
 		b.alloc(std::max(bytes, 2 * b.bytes()));
 	}
 }
@@ -333,29 +344,53 @@ bool Server::admit(Job& j, PinnedStage& stage)
 	return true;
 }
 
-// device tables of the jobs that still have MCS to run: DT lanes (one per
-// block) grouped by (kind, mode), SCABit waves (rows)
+// device tables of the jobs that still have MCS to run, per launch chain: DT
+// lanes (one per block) grouped by (kind, mode), SCABit waves (rows).  Each
+// running job goes to the chain with fewer sites so far, in slot order.
 void Server::rebuild(PinnedStage& stage)
 {
 	std::vector<uint2> lanes, waves;
 	auto runs = [](const Job* j) { return j && j->planned < (uint32_t)j->spec.mcs; };
-	for(int kind : {BK_DT, BK_DTR})
-	{
-		if(kind == BK_DTR)
-			nStatic_ = (int64_t)lanes.size();
-		for(int mode : {0, 1})
+	uint64_t load[2] = {0, 0};
+	for(Job* j : slots_)
+		if(runs(j))
 		{
-			for(int sl = 0; sl < maxSlots_; ++sl)
-			{
-				Job* j = slots_[(size_t)sl];
-				if(!runs(j) || j->dev.kind != kind || j->dev.mode != mode)
-					continue;
-				for(int64_t b = 0; b < j->T; ++b)
-					lanes.push_back(make_uint2((unsigned)sl, (unsigned)b));
-			}
-			while(lanes.size() % 64)
-				lanes.push_back(make_uint2(0xFFFFFFFFu, 0));
+			j->group = load[1] < load[0] ? 1 : 0;
+			load[j->group] += 1ull << (j->spec.lx + j->spec.ly);
 		}
+	for(int g = 0; g < 2; ++g)
+	{
+		laneOff_[g] = (int64_t)lanes.size();
+		for(int kind : {BK_DT, BK_DTR})
+		{
+			if(kind == BK_DTR)
+				nStatic_[g] = (int64_t)lanes.size() - laneOff_[g];
+			for(int mode : {0, 1})
+			{
+				for(int sl = 0; sl < maxSlots_; ++sl)
+				{
+					Job* j = slots_[(size_t)sl];
+					if(!runs(j) || j->group != g || j->dev.kind != kind || j->dev.mode != mode)
+						continue;
+					for(int64_t b = 0; b < j->T; ++b)
+						lanes.push_back(make_uint2((unsigned)sl, (unsigned)b));
+				}
+				while(lanes.size() % 64)
+					lanes.push_back(make_uint2(0xFFFFFFFFu, 0));
+			}
+		}
+		nLanes_[g] = (int64_t)lanes.size() - laneOff_[g];
+		waveOff_[g] = (int64_t)waves.size();
+		for(int sl = 0; sl < maxSlots_; ++sl)
+		{
+			Job* j = slots_[(size_t)sl];
+			if(!runs(j) || j->group != g || j->dev.kind != BK_SCABIT)
+				continue;
+			const int rpw = 64 >> j->dev.lG;
+			for(int y = 0; y < (1 << j->dev.ly); y += rpw)
+				waves.push_back(make_uint2((unsigned)sl, (unsigned)y));
+		}
+		nWaves_[g] = (int64_t)waves.size() - waveOff_[g];
 	}
 	for(int sl = 0; sl < maxSlots_; ++sl)
 	{
@@ -363,17 +398,10 @@ void Server::rebuild(PinnedStage& stage)
 		hostReps_[(size_t)sl] = runs(j) ? j->dev : BatchRep{};
 		if(j)
 			j->inTable = runs(j);
-		if(!runs(j) || j->dev.kind != BK_SCABIT)
-			continue;
-		const int rpw = 64 >> j->dev.lG;
-		for(int y = 0; y < (1 << j->dev.ly); y += rpw)
-			waves.push_back(make_uint2((unsigned)sl, (unsigned)y));
 	}
-	nLanes_ = (int64_t)lanes.size();
-	nWaves_ = (int64_t)waves.size();
-	if(nLanes_)
+	if(!lanes.empty())
 		ensure(laneTable_, lanes.size() * sizeof(uint2));
-	if(nWaves_)
+	if(!waves.empty())
 		ensure(waveTable_, waves.size() * sizeof(uint2));
 	auto put = [&](DevBuf& dst, const void* src, size_t bytes, const char* what) {
 		void* p = stage.take(bytes);
@@ -381,15 +409,16 @@ void Server::rebuild(PinnedStage& stage)
 		check(hipMemcpyAsync(dst.get(), p, bytes, hipMemcpyHostToDevice, s0_), what);
 	};
 	put(repTable_, hostReps_.data(), sizeof(BatchRep) * hostReps_.size(), "reps");
-	if(nLanes_)
+	if(!lanes.empty())
 		put(laneTable_, lanes.data(), lanes.size() * sizeof(uint2), "lanes");
-	if(nWaves_)
+	if(!waves.empty())
 		put(waveTable_, waves.data(), waves.size() * sizeof(uint2), "waves");
 	dirty_ = false;
 }
 
 // plan the next tick and queue all of its device work; nothing here waits
-// for the tick that may still be running (all work is ordered on s0_)
+// for the tick that may still be running (all work is ordered on s0_, into
+// which each tick joins its s1_ chain)
 bool Server::enqueue(Tick& t)
 {
 	t.stage.reset();
@@ -486,10 +515,12 @@ bool Server::enqueue(Tick& t)
 		if(j && limit[j->slot] > 0)
 			for(size_t s = j->nextPlan; s < j->samples.size() && j->samples[s] <= j->planned + (uint32_t)limit[j->slot]; ++s)
 				t.plan.push_back({(int)(j->samples[s] - j->planned) - 1, j});
-	std::stable_sort(t.plan.begin(), t.plan.end(), [](const Sample& a, const Sample& b) { return a.m < b.m; });
+	std::stable_sort(t.plan.begin(), t.plan.end(), [](const Sample& a, const Sample& b) {
+		return a.m != b.m ? a.m < b.m : a.job->group < b.job->group;
+	});
 	const size_t np = t.plan.size();
 	std::vector<int64_t> rowStart(np + 1);
-	std::vector<size_t> groupStart; // plan index where each m-group starts
+	std::vector<size_t> groupStart; // plan index where each (m, chain) group starts
 	int64_t nrows = 0;
 	for(size_t i = 0; i < np; ++i)
 	{
@@ -502,10 +533,10 @@ bool Server::enqueue(Tick& t)
 	int64_t rb = 0, maxGroupRows = 0;
 	for(size_t i = 0; i < np; ++i)
 	{
-		if(i == 0 || t.plan[i].m != t.plan[i - 1].m)
+		if(i == 0 || t.plan[i].m != t.plan[i - 1].m || t.plan[i].job->group != t.plan[i - 1].job->group)
 		{
 			groupStart.push_back(i);
-			rb = 0; // scratch rows restart per group (groups run in stream order)
+			rb = 0; // scratch rows restart per group (a chain's groups run in its stream order)
 		}
 		sreps[i] = t.plan[i].job->dev;
 		sreps[i].rowBase = rb;
@@ -521,52 +552,71 @@ bool Server::enqueue(Tick& t)
 	{
 		ensure(sampleReps_, np * sizeof(BatchRep));
 		ensure(rowTable_, (size_t)nrows * sizeof(uint2));
-		ensure(rowScratch_, (size_t)maxGroupRows * 4 * sizeof(int64_t));
+		ensure(rowScratch_, 2 * (size_t)maxGroupRows * 4 * sizeof(int64_t)); // one region per chain
 		ensure(outBuf_, np * 3 * sizeof(int64_t));
 		check(hipMemcpyAsync(sampleReps_.get(), sreps, np * sizeof(BatchRep), hipMemcpyHostToDevice, s0_), "sample reps");
 		check(hipMemcpyAsync(rowTable_.get(), srows, (size_t)nrows * sizeof(uint2), hipMemcpyHostToDevice, s0_), "rows");
 	}
 
-	BatchDtArgs da{};
-	da.reps = repTable_.as<BatchRep>();
-	da.lanes = laneTable_.as<uint2>();
-	da.order = orderBuf_.as<uint8_t>();
-	da.shift = shiftBuf_.as<uint16_t>();
-	da.nreps = maxSlots_;
-	da.nlanes = nLanes_;
-	da.nstatic = nStatic_;
-	da.limit = limitBuf_.as<int>();
-	BatchCbArgs ca{};
-	ca.reps = repTable_.as<BatchRep>();
-	ca.waves = waveTable_.as<uint2>();
-	ca.nwaves = nWaves_;
-	ca.limit = limitBuf_.as<int>();
+	BatchDtArgs da[2];
+	BatchCbArgs ca[2];
+	for(int c = 0; c < 2; ++c)
+	{
+		da[c] = BatchDtArgs{};
+		da[c].reps = repTable_.as<BatchRep>();
+		da[c].lanes = laneTable_.as<uint2>() + laneOff_[c];
+		da[c].order = orderBuf_.as<uint8_t>();
+		da[c].shift = shiftBuf_.as<uint16_t>();
+		da[c].nreps = maxSlots_;
+		da[c].nlanes = nLanes_[c];
+		da[c].nstatic = nStatic_[c];
+		da[c].limit = limitBuf_.as<int>();
+		ca[c] = BatchCbArgs{};
+		ca[c].reps = repTable_.as<BatchRep>();
+		ca[c].waves = waveTable_.as<uint2>() + waveOff_[c];
+		ca[c].nwaves = nWaves_[c];
+		ca[c].limit = limitBuf_.as<int>();
+	}
+	const hipStream_t chain[2] = {s0_, s1_};
+	const bool two = nLanes_[1] > 0 || nWaves_[1] > 0;
+	if(two)
+	{
+		check(hipEventRecord(fork_, s0_), "fork");
+		check(hipStreamWaitEvent(s1_, fork_, 0), "fork wait");
+	}
 	size_t g = 0;
 	for(int m = 0; m < k; ++m)
+		for(int c = 0; c < 2; ++c)
+		{
+			da[c].m = m;
+			ca[c].m = m;
+			for(int d = 0; d < 4 && da[c].nlanes > 0; ++d)
+			{
+				da[c].d = d;
+				check(launch_batch_dt(da[c], chain[c]), "batch_dt");
+			}
+			for(int s = 0; s < 2 && ca[c].nwaves > 0; ++s)
+			{
+				ca[c].signum = s;
+				ca[c].buf = s;
+				check(launch_batch_cbbit(ca[c], chain[c]), "batch_cbbit");
+			}
+			// exact roughness of the chain's jobs whose sample time is this MCS
+			if(g + 1 < groupStart.size() && t.plan[groupStart[g]].m == m && t.plan[groupStart[g]].job->group == c)
+			{
+				const size_t i0 = groupStart[g], i1 = groupStart[g + 1];
+				check(launch_batch_roughness(sampleReps_.as<BatchRep>() + i0, (int)(i1 - i0),
+				                             rowTable_.as<uint2>() + rowStart[i0], rowStart[i1] - rowStart[i0],
+				                             rowScratch_.as<int64_t>() + (size_t)c * maxGroupRows * 4,
+				                             outBuf_.as<int64_t>() + 3 * i0, chain[c]),
+				      "batch_roughness");
+				++g;
+			}
+		}
+	if(two)
 	{
-		da.m = m;
-		ca.m = m;
-		for(int d = 0; d < 4; ++d)
-		{
-			da.d = d;
-			check(launch_batch_dt(da, s0_), "batch_dt");
-		}
-		for(int s = 0; s < 2; ++s)
-		{
-			ca.signum = s;
-			ca.buf = s;
-			check(launch_batch_cbbit(ca, s0_), "batch_cbbit");
-		}
-		// exact roughness of the jobs whose sample time is this MCS
-		if(g + 1 < groupStart.size() && t.plan[groupStart[g]].m == m)
-		{
-			const size_t i0 = groupStart[g], i1 = groupStart[g + 1];
-			check(launch_batch_roughness(sampleReps_.as<BatchRep>() + i0, (int)(i1 - i0),
-			                             rowTable_.as<uint2>() + rowStart[i0], rowStart[i1] - rowStart[i0],
-			                             rowScratch_.as<int64_t>(), outBuf_.as<int64_t>() + 3 * i0, s0_),
-			      "batch_roughness");
-			++g;
-		}
+		check(hipEventRecord(join_, s1_), "join");
+		check(hipStreamWaitEvent(s0_, join_, 0), "join wait");
 	}
 	if(np)
 		check(hipMemcpyAsync(t.out, outBuf_.get(), np * 3 * sizeof(int64_t), hipMemcpyDeviceToHost, s0_), "roughness D2H");

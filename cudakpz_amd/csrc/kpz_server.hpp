@@ -116,6 +116,7 @@ private:
 		std::vector<uint32_t> samples;
 		size_t nextPlan = 0;         // next sample time to plan
 		bool inTable = false;        // in the current device tables
+		int group = 0;               // launch chain (stream) in the current tables
 		JobResult res;
 		BatchRep dev{};
 	};
@@ -142,7 +143,11 @@ private:
 	BuddyArena arena_;
 	int maxSlots_, maxTick_;
 	int device_ = 0;
-	hipStream_t s0_ = nullptr;
+	// a tick's launches run in two chains of jobs of about equal site count,
+	// s0_ and s1_, forked after the tick's uploads and joined before its
+	// downloads, so one chain's launch tails overlap the other's next launch
+	hipStream_t s0_ = nullptr, s1_ = nullptr;
+	hipEvent_t fork_ = nullptr, join_ = nullptr;
 	std::deque<Job*> queue_;
 	std::vector<Job*> slots_;
 	std::deque<JobResult> done_;
@@ -153,7 +158,10 @@ private:
 	DevBuf repTable_, laneTable_, waveTable_, rowTable_, rowScratch_, outBuf_, orderBuf_, shiftBuf_, identity_, sampleReps_,
 			limitBuf_;
 	std::vector<BatchRep> hostReps_;
-	int64_t nLanes_ = 0, nStatic_ = 0, nWaves_ = 0;
+	// per chain: DT lanes [laneOff_[g], + nLanes_[g]) (static DT first,
+	// nStatic_[g] of them) and SCABit waves [waveOff_[g], + nWaves_[g])
+	int64_t laneOff_[2] = {0, 0}, nLanes_[2] = {0, 0}, nStatic_[2] = {0, 0};
+	int64_t waveOff_[2] = {0, 0}, nWaves_[2] = {0, 0};
 	Tick ticks_[2];
 	int cur_ = 0;
 	double gpuMs_ = 0, spanMs_ = 0;
