@@ -202,17 +202,19 @@ void Batch::buildTables()
 		}
 		cbWaves_[g] = (int64_t)waves.size() - cbOff_[g];
 	}
-	// all rows, for the reductions
+	// all rows, for the reductions: waves of 64 / batch_row_lanes(lx) rows
 	std::vector<uint2> rows;
 	int64_t rowBase = 0;
 	for(int r = 0; r < R; ++r)
 	{
 		reps_[r].dev.rowBase = rowBase;
-		for(int y = 0; y < (1 << reps_[r].dev.ly); ++y)
+		const int rpw = 64 / batch_row_lanes(reps_[r].dev.lx);
+		for(int y = 0; y < (1 << reps_[r].dev.ly); y += rpw)
 			rows.push_back(make_uint2((unsigned)r, (unsigned)y));
 		rowBase += 1 << reps_[r].dev.ly;
 	}
-	nRows_ = (int64_t)rows.size();
+	nRows_ = rowBase;
+	nRowWaves_ = (int64_t)rows.size();
 
 	std::vector<BatchRep> table(R);
 	for(int r = 0; r < R; ++r)
@@ -411,7 +413,7 @@ void Batch::mcs(int n)
 
 void Batch::roughness(int64_t* out)
 {
-	check(launch_batch_roughness(repTable_.as<BatchRep>(), size(), rowTable_.as<uint2>(), nRows_,
+	check(launch_batch_roughness(repTable_.as<BatchRep>(), size(), rowTable_.as<uint2>(), nRowWaves_,
 			rowScratch_.as<int64_t>(), outBuf_.as<int64_t>(), s0_), "batch_roughness");
 	check(hipMemcpyAsync(out, outBuf_.get(), (size_t)size() * 3 * 8, hipMemcpyDeviceToHost, s0_), "roughness D2H");
 	sync();

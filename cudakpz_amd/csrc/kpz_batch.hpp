@@ -96,7 +96,7 @@ private:
 	// first; SCABit waves [cbOff_[g], cbOff_[g] + cbWaves_[g]) of the wave table
 	int64_t dtOff_[2] = {0, 0}, dtLanes_[2] = {0, 0}, dtStatic_[2] = {0, 0};
 	int64_t cbOff_[2] = {0, 0}, cbWaves_[2] = {0, 0};
-	int64_t nRows_ = 0;
+	int64_t nRows_ = 0, nRowWaves_ = 0; // site rows of all replicas, row-statistics waves
 	bool hasDt_ = false, hasCb_ = false;
 	uint64_t sites_ = 0;
 	unsigned int mcs_ = 0;

@@ -3140,19 +3140,80 @@ __global__ __launch_bounds__(256) void batch_cbbit_kernel(BatchCbArgs a)
 }
 
 // per-row roughness statistics of every replica (one wave per row)
-__global__ __launch_bounds__(256) void batch_rowstats_kernel(const BatchRep* __restrict__ reps, const uint2* __restrict__ rows,
-		int64_t nrows, int64_t* __restrict__ scratch)
+// The first four row statistics of site row y (as row_stats_row with short4)
+// from an aligned segment of G lanes (G a power of two dividing the row's
+// 64-site chunk count, or 1): segment lane sl takes chunks sl, sl + G, ...;
+// the scans and sums stay inside the segment, so the 64 / G segments of a
+// wave handle 64 / G rows at once.
+template<int ENC>
+__device__ __forceinline__ void row_stats_seg(const uint32_t* __restrict__ lat, int lx, int ly, int y, int lane, int G,
+		int64_t* __restrict__ o)
+{
+	const int sl = lane & (G - 1), s0 = lane - sl;
+	const int nbits = (lx >= 6) ? 64 : (1 << lx);
+	const int nch = (lx >= 6) ? (1 << (lx - 6)) : 1;
+	int64_t S1 = 0, S2 = 0, run = 0;
+	int d0 = 0;
+	for(int base = 0; base < nch; base += G)
+	{
+		const int j = base + sl;
+		int q = 0;
+		int64_t sq = 0, sq2 = 0;
+		const uint64_t b = xbits<ENC>(lat, lx, ly, j, y);
+		const int n = (j == 0) ? nbits - 1 : nbits;
+		chunk_prefix(b, nbits, j == 0, q, sq, sq2);
+		if(j == 0)
+			d0 = (b & 1) ? 1 : -1;
+		int64_t incl = q;
+		for(int k = 1; k < G; k <<= 1)
+		{
+			const int64_t u = (int64_t)shfl64((uint64_t)incl, sl >= k ? lane - k : lane, 64);
+			if(sl >= k)
+				incl += u;
+		}
+		const int64_t c = run + incl - q;
+		S1 += n * c + sq;
+		S2 += n * c * c + 2 * c * sq + sq2;
+		run += (int64_t)shfl64((uint64_t)incl, s0 + G - 1, 64);
+	}
+	for(int k = G >> 1; k > 0; k >>= 1)
+	{
+		S1 += (int64_t)shfl64((uint64_t)S1, lane ^ k, 64);
+		S2 += (int64_t)shfl64((uint64_t)S2, lane ^ k, 64);
+	}
+	d0 = __shfl(d0, s0, 64);
+	if(sl == 0)
+	{
+		// site x = 0 sits at the end of the row walk: P_0 = D (all x slopes)
+		const int64_t D = run + d0;
+		o[0] = S1 + D;
+		o[1] = S2 + D * D;
+		o[2] = D;
+		o[3] = ybit0<ENC>(lat, lx, ly, y);
+	}
+}
+
+// waves[] = {replica, first row}: the wave's 64 / G segments of G lanes take
+// consecutive rows (segments past the replica's last row leave; the rest only
+// exchange values inside their own segment)
+__global__ __launch_bounds__(256) void batch_rowstats_kernel(const BatchRep* __restrict__ reps, const uint2* __restrict__ waves,
+		int64_t nwaves, int64_t* __restrict__ scratch)
 {
 	const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-	if(gw >= nrows)
+	if(gw >= nwaves)
 		return;
-	const uint2 w = rows[gw];
+	const uint2 w = waves[gw];
 	const BatchRep& rep = reps[w.x];
-	int64_t* o = scratch + 4 * (rep.rowBase + w.y);
+	const int lane = threadIdx.x & 63;
+	const int G = rep.lx >= 12 ? 64 : (rep.lx >= 6 ? 1 << (rep.lx - 6) : 1);
+	const int y = (int)w.y + lane / G;
+	if(y >= (1 << rep.ly))
+		return;
+	int64_t* o = scratch + 4 * (rep.rowBase + y);
 	if(rep.kind == BK_SCABIT)
-		row_stats_row<ENC_CBBIT, false>(rep.lat, nullptr, rep.lx, rep.ly, (int)w.y, threadIdx.x & 63, o, true);
+		row_stats_seg<ENC_CBBIT>(rep.lat, rep.lx, rep.ly, y, lane, G, o);
 	else
-		row_stats_row<ENC_LOCAL, false>(rep.lat, nullptr, rep.lx, rep.ly, (int)w.y, threadIdx.x & 63, o, true);
+		row_stats_seg<ENC_LOCAL>(rep.lat, rep.lx, rep.ly, y, lane, G, o);
 }
 
 // exact combination per replica: one workgroup of BC_WG lanes per replica.
@@ -3700,11 +3761,12 @@ hipError_t launch_batch_cbbit(const BatchCbArgs& a, hipStream_t stream)
 	return hipGetLastError();
 }
 
-hipError_t launch_batch_roughness(const BatchRep* reps, int nreps, const uint2* rows, int64_t nrows,
+hipError_t launch_batch_roughness(const BatchRep* reps, int nreps, const uint2* waves, int64_t nwaves,
 		int64_t* scratch, int64_t* out, hipStream_t stream)
 {
-	if(nrows > 0)
-		hipLaunchKernelGGL(batch_rowstats_kernel, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0, stream, reps, rows, nrows, scratch);
+	if(nwaves > 0)
+		hipLaunchKernelGGL(batch_rowstats_kernel, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, stream, reps, waves, nwaves,
+		                   scratch);
 	if(nreps > 0)
 		hipLaunchKernelGGL(batch_combine_kernel, dim3((unsigned)nreps), dim3(BC_WG), 0, stream, reps, nreps, scratch, out);
 	return hipGetLastError();

@@ -218,9 +218,14 @@ struct BatchCbArgs {
 };
 hipError_t launch_batch_cbbit(const BatchCbArgs& a, hipStream_t stream);
 
-// exact roughness of every replica: rows[] = per-wave {replica, y} over all
-// rows, scratch = 4 int64 per row, out[3 r..] = {sum h, sum h^2 lo, hi}
-hipError_t launch_batch_roughness(const BatchRep* reps, int nreps, const uint2* rows, int64_t nrows,
+// lanes per site row in the batched row statistics: one per 64-site chunk, at
+// most 64; a wave takes 64 / batch_row_lanes(lx) consecutive rows of a replica
+inline int batch_row_lanes(int lx) { return lx >= 12 ? 64 : (lx >= 6 ? 1 << (lx - 6) : 1); }
+
+// exact roughness of every replica: waves[] = per-wave {replica, first row},
+// covering every row of every replica (64 / batch_row_lanes(lx) rows per
+// wave), scratch = 4 int64 per row, out[3 r..] = {sum h, sum h^2 lo, hi}
+hipError_t launch_batch_roughness(const BatchRep* reps, int nreps, const uint2* waves, int64_t nwaves,
 		int64_t* scratch, int64_t* out, hipStream_t stream);
 
 // ---- utilities -------------------------------------------------------------

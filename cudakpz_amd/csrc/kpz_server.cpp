@@ -519,13 +519,15 @@ bool Server::enqueue(Tick& t)
 		return a.m != b.m ? a.m < b.m : a.job->group < b.job->group;
 	});
 	const size_t np = t.plan.size();
+	// row-statistics waves (64 / batch_row_lanes(lx) rows each) of every sample
 	std::vector<int64_t> rowStart(np + 1);
 	std::vector<size_t> groupStart; // plan index where each (m, chain) group starts
 	int64_t nrows = 0;
 	for(size_t i = 0; i < np; ++i)
 	{
 		rowStart[i] = nrows;
-		nrows += 1 << t.plan[i].job->spec.ly;
+		const int rpw = 64 / batch_row_lanes(t.plan[i].job->spec.lx);
+		nrows += ((1 << t.plan[i].job->spec.ly) + rpw - 1) / rpw;
 	}
 	rowStart[np] = nrows;
 	BatchRep* sreps = (BatchRep*)t.stage.take(std::max<size_t>(np, 1) * sizeof(BatchRep));
@@ -540,10 +542,10 @@ bool Server::enqueue(Tick& t)
 		}
 		sreps[i] = t.plan[i].job->dev;
 		sreps[i].rowBase = rb;
-		const int ny = 1 << sreps[i].ly;
-		for(int y = 0; y < ny; ++y)
-			srows[rowStart[i] + y] = make_uint2((unsigned)(i - groupStart.back()), (unsigned)y);
-		rb += ny;
+		const int ny = 1 << sreps[i].ly, rpw = 64 / batch_row_lanes(sreps[i].lx);
+		for(int y = 0, k = 0; y < ny; y += rpw, ++k)
+			srows[rowStart[i] + k] = make_uint2((unsigned)(i - groupStart.back()), (unsigned)y);
+		rb += ny; // scratch: 4 int64 per site row
 		maxGroupRows = std::max(maxGroupRows, rb);
 	}
 	groupStart.push_back(np);

@@ -23,6 +23,9 @@ MIX = [
     (K.KPZ_SCABIT, 8, 8, 0.5, 0.0, 110),
     (K.KPZ_DT, 5, 5, 0.9, 0.1, 111),
     (K.KPZ_SCABIT, 10, 10, 0.6, 0.4, 112),
+    # rows of 64 and 128 chunks: one or two passes of a full wave per row in the row statistics
+    (K.KPZ_DT, 12, 6, 1.0, 0.0, 113),
+    (K.KPZ_SCABIT, 13, 6, 0.5, 0.5, 114),
 ]
 
 CLS = {K.KPZ_DT: K.SchedulerDT, K.KPZ_DTR: K.SchedulerDTr, K.KPZ_SCABIT: K.SchedulerSCABit}

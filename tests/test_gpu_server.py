@@ -42,6 +42,8 @@ JOBS = [
     (K.KPZ_SCABIT, 6, 6, 90, 1.0, 0.0, 16),
     (K.KPZ_SCABIT, 10, 8, 12, 0.55, 0.45, 17),
     (K.KPZ_DTR, 6, 6, 50, 1.0, 0.0, 18),
+    (K.KPZ_SCABIT, 13, 6, 20, 0.7, 0.3, 19),
+    (K.KPZ_DT, 12, 5, 15, 1.0, 0.0, 20),
 ]
 
 
