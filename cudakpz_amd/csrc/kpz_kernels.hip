@@ -1691,6 +1691,21 @@ __device__ __forceinline__ uint64_t xbits(const uint32_t* lat, int lx, int ly, i
 		const size_t base = ((size_t)(y >> 2) << (lx - 2)) + (size_t)j * 16;
 		const int sh = (y & 3) << 3;
 		uint64_t b = 0;
+		if(nw == 16)
+		{
+			// 16 words at a multiple of 16 words from the lattice start: four 16-byte loads
+			const uint4* p = reinterpret_cast<const uint4*>(lat + base);
+#pragma unroll
+			for(int k4 = 0; k4 < 4; ++k4)
+			{
+				const uint4 v = p[k4];
+				const uint32_t ws[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+				for(int kk = 0; kk < 4; ++kk)
+					b |= (uint64_t)(even16((ws[kk] >> sh) & 0xFFu) & 0xFu) << (16 * k4 + 4 * kk);
+			}
+			return b;
+		}
 		for(int k = 0; k < nw; ++k)
 		{
 			const uint32_t byte = (lat[base + k] >> sh) & 0xFFu;
@@ -3140,14 +3155,67 @@ __global__ __launch_bounds__(256) void batch_cbbit_kernel(BatchCbArgs a)
 }
 
 // per-row roughness statistics of every replica (one wave per row)
+// Walk of one byte of x slopes (bit i set: step +1, else -1; bit 0 first):
+// end height q, sum A and square sum B of the heights after each of the 8
+// steps, packed (q + 8) | (A + 36) << 5 | B << 12 (q in -8..8, A in -36..36,
+// B <= 204)
+struct ByteWalk {
+	uint32_t e[256];
+};
+constexpr ByteWalk make_byte_walk()
+{
+	ByteWalk t{};
+	for(int v = 0; v < 256; ++v)
+	{
+		int q = 0, a = 0, b = 0;
+		for(int i = 0; i < 8; ++i)
+		{
+			q += ((v >> i) & 1) ? 1 : -1;
+			a += q;
+			b += q * q;
+		}
+		t.e[v] = (uint32_t)(q + 8) | ((uint32_t)(a + 36) << 5) | ((uint32_t)b << 12);
+	}
+	return t;
+}
+__constant__ ByteWalk kByteWalk = make_byte_walk();
+
+// chunk_prefix from the byte walks (tab = kByteWalk in LDS): nbits a multiple
+// of 8.  Skipping bit 0 (step s0) lowers every later height by s0, so with
+// the full walk's end Q, sum S1 and square sum S2 over N steps the walk from
+// bit 1 has end Q - s0, sum S1 - N s0 and square sum S2 - 2 s0 S1 + N.
+__device__ __forceinline__ void chunk_prefix_tab(const uint32_t* tab, uint64_t bits, int nbits, bool skip0, int& q,
+		int64_t& sq, int64_t& sq2)
+{
+	int c = 0, s1 = 0, s2 = 0;
+	for(int t = 0; t < nbits; t += 8)
+	{
+		const uint32_t e = tab[(uint32_t)(bits >> t) & 0xFFu];
+		const int qe = (int)(e & 31u) - 8, a = (int)((e >> 5) & 127u) - 36, b = (int)(e >> 12);
+		s1 += 8 * c + a;
+		s2 += 8 * c * c + 2 * c * a + b;
+		c += qe;
+	}
+	if(skip0)
+	{
+		const int s0 = (bits & 1) ? 1 : -1;
+		s2 += nbits - 2 * s0 * s1;
+		s1 -= nbits * s0;
+		c -= s0;
+	}
+	q = c;
+	sq = s1;
+	sq2 = s2;
+}
+
 // The first four row statistics of site row y (as row_stats_row with short4)
 // from an aligned segment of G lanes (G a power of two dividing the row's
 // 64-site chunk count, or 1): segment lane sl takes chunks sl, sl + G, ...;
 // the scans and sums stay inside the segment, so the 64 / G segments of a
-// wave handle 64 / G rows at once.
+// wave handle 64 / G rows at once.  tab: kByteWalk in LDS.
 template<int ENC>
 __device__ __forceinline__ void row_stats_seg(const uint32_t* __restrict__ lat, int lx, int ly, int y, int lane, int G,
-		int64_t* __restrict__ o)
+		const uint32_t* tab, int64_t* __restrict__ o)
 {
 	const int sl = lane & (G - 1), s0 = lane - sl;
 	const int nbits = (lx >= 6) ? 64 : (1 << lx);
@@ -3161,7 +3229,7 @@ __device__ __forceinline__ void row_stats_seg(const uint32_t* __restrict__ lat, 
 		int64_t sq = 0, sq2 = 0;
 		const uint64_t b = xbits<ENC>(lat, lx, ly, j, y);
 		const int n = (j == 0) ? nbits - 1 : nbits;
-		chunk_prefix(b, nbits, j == 0, q, sq, sq2);
+		chunk_prefix_tab(tab, b, nbits, j == 0, q, sq, sq2);
 		if(j == 0)
 			d0 = (b & 1) ? 1 : -1;
 		int64_t incl = q;
@@ -3199,6 +3267,9 @@ __device__ __forceinline__ void row_stats_seg(const uint32_t* __restrict__ lat, 
 __global__ __launch_bounds__(256) void batch_rowstats_kernel(const BatchRep* __restrict__ reps, const uint2* __restrict__ waves,
 		int64_t nwaves, int64_t* __restrict__ scratch)
 {
+	__shared__ uint32_t tab[256];
+	tab[threadIdx.x] = kByteWalk.e[threadIdx.x]; // blockDim.x == 256
+	__syncthreads();
 	const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
 	if(gw >= nwaves)
 		return;
@@ -3211,9 +3282,9 @@ __global__ __launch_bounds__(256) void batch_rowstats_kernel(const BatchRep* __r
 		return;
 	int64_t* o = scratch + 4 * (rep.rowBase + y);
 	if(rep.kind == BK_SCABIT)
-		row_stats_seg<ENC_CBBIT>(rep.lat, rep.lx, rep.ly, y, lane, G, o);
+		row_stats_seg<ENC_CBBIT>(rep.lat, rep.lx, rep.ly, y, lane, G, tab, o);
 	else
-		row_stats_seg<ENC_LOCAL>(rep.lat, rep.lx, rep.ly, y, lane, G, o);
+		row_stats_seg<ENC_LOCAL>(rep.lat, rep.lx, rep.ly, y, lane, G, tab, o);
 }
 
 // exact combination per replica: one workgroup of BC_WG lanes per replica.
