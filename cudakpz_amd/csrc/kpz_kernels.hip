@@ -3154,7 +3154,7 @@ __global__ __launch_bounds__(256) void batch_cbbit_kernel(BatchCbArgs a)
 	}
 }
 
-// per-row roughness statistics of every replica (one wave per row)
+// ---- per-row roughness statistics of every replica ----
 // Walk of one byte of x slopes (bit i set: step +1, else -1; bit 0 first):
 // end height q, sum A and square sum B of the heights after each of the 8
 // steps, packed (q + 8) | (A + 36) << 5 | B << 12 (q in -8..8, A in -36..36,
