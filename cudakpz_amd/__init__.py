@@ -475,7 +475,7 @@ class BatchTuning(C.Structure):
 
     @classmethod
     def default(cls):
-        return cls(6, 1, 50, 2, 1)
+        return cls(3, 1, 50, 2, 1)
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -31,7 +31,7 @@ struct ReplicaSpec {
 
 // performance knobs; none of them changes any replica's result
 struct BatchTuning {
-	int maxLanesLog2 = 6;  // SCABit: lanes per row = min(words per row, 2^this)
+	int maxLanesLog2 = 3;  // SCABit: lanes per row = min(words per row, 2^this)
 	int pqWords64 = 1;     // SCABit per-site replicas: 64-bit (1) or 32-bit (0) words
 	int chunk = 50;        // MCS per captured graph
 	int concurrent = 2;    // 0: one stream; 1: DT and SCABit chains on two streams;

@@ -259,7 +259,7 @@ typedef struct kpz_batch_tuning {
 	int sort_by_size;   /* large replicas first */
 } kpz_batch_tuning;
 
-/* tuning = NULL: defaults {6, 1, 50, 2, 1} */
+/* tuning = NULL: defaults {3, 1, 50, 2, 1} */
 int kpz_batch_create(const kpz_replica* reps, int n, const kpz_batch_tuning* tuning, int device, kpz_batch** out);
 int kpz_batch_destroy(kpz_batch* b);
 int kpz_batch_init_flat(kpz_batch* b);
