@@ -99,7 +99,8 @@ public:
 	size_t hbmCapacity() const { return arena_.capacity(); }
 	uint64_t siteUpdates() const { return siteUpdates_; }
 	// device time spent in ticks and the span from the first tick's start to
-	// the last finished tick's end, in seconds (hipEvents on the server stream)
+	// the last finished tick's end, in seconds (hipEvents on s0_, into which
+	// each tick's second chain joins)
 	void busy(double& gpuSeconds, double& spanSeconds) const;
 	double now() const;
 

@@ -314,7 +314,8 @@ int kpz_server_stats(kpz_server* s, uint64_t out[5]);
 /* the server's clock (s since creation; monotonic) */
 double kpz_server_now(const kpz_server* s);
 /* device time of the server: gpu_seconds = time the device spent in ticks
- * (hipEvents around each tick's work on the server stream), span_seconds =
+ * (hipEvents around each tick's work on the server's main stream; a tick's
+ * second launch chain forks from and joins into it), span_seconds =
  * from the first tick's start to the end of the last finished tick; their
  * ratio is the fraction of time the device was busy.  Ticks are pipelined:
  * kpz_server_step queues the next tick before it waits for the previous one. */
